@@ -1,0 +1,157 @@
+"""Exact diagonalisation in the Fock basis (Jordan-Wigner), oracle pin.
+
+TEST INFRASTRUCTURE ONLY.  Independent of every SU(2) routine: it builds the
+Hamiltonian of the same lattice / couplings (oracle/lattice.py) on bit
+strings, with modes ordered (site 1 up, site 1 down, site 2 up, ...) along the
+DMRG path, restricted to fixed N and S_z = 0, and projects onto total S = 0
+with the Casimir S^2.
+"""
+from itertools import combinations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def _popcount(a):
+    a = a.astype(np.int64)
+    c = np.zeros_like(a)
+    while np.any(a):
+        c += a & 1
+        a >>= 1
+    return c
+
+
+class FockSector:
+    def __init__(self, nsite, n_elec, kind):
+        self.nsite = nsite
+        self.kind = kind
+        states = []
+        half = n_elec // 2
+        for ups in combinations(range(nsite), half):
+            su = sum(1 << (2 * i) for i in ups)
+            for dns in combinations(range(nsite), n_elec - half):
+                if kind == "tj" and set(ups) & set(dns):
+                    continue
+                states.append(su + sum(1 << (2 * i + 1) for i in dns))
+        states.sort()
+        self.states = np.array(states, dtype=np.int64)
+        self.index = {int(s): i for i, s in enumerate(states)}
+
+    def apply(self, ops, vec_states=None):
+        """Apply a product of fermion operators (list of (mode, dagger)),
+        rightmost first, to every basis state.  Returns (src_idx, dst_idx, sign)
+        for the states where the result stays in the sector."""
+        s = self.states.copy()
+        sign = np.ones(len(s))
+        ok = np.ones(len(s), dtype=bool)
+        for mode, dag in reversed(ops):
+            bit = (s >> mode) & 1
+            if dag:
+                ok &= bit == 0
+            else:
+                ok &= bit == 1
+            below = s & ((1 << mode) - 1)
+            sign *= np.where(_popcount(below) % 2 == 1, -1.0, 1.0)
+            s = s ^ (1 << mode)
+        src = np.nonzero(ok)[0]
+        dst, keep = [], []
+        for i in src:
+            j = self.index.get(int(s[i]))
+            if j is not None:
+                dst.append(j)
+                keep.append(i)
+        keep = np.array(keep, dtype=np.int64)
+        return keep, np.array(dst, dtype=np.int64), sign[keep]
+
+
+def hamiltonian(model, positions, bonds, n_elec):
+    """Sparse H on the ordered site list ``positions`` (lattice positions in
+    fermion order) with ``bonds`` = list of (pu, pv, hop, J, V), hop = amp of
+    c^dag_pu c_pv (h.c. added)."""
+    order = {p: i for i, p in enumerate(positions)}
+    fs = FockSector(len(positions), n_elec, model.kind)
+    dim = len(fs.states)
+    rows, cols, vals = [], [], []
+
+    def add(ops, c):
+        src, dst, sg = fs.apply(ops)
+        rows.extend(dst.tolist())
+        cols.extend(src.tolist())
+        vals.extend((c * sg).tolist())
+
+    for (pu, pv, hop, J, V) in bonds:
+        i, j = order[pu], order[pv]
+        if hop != 0:
+            for sgm in (0, 1):
+                add([(2 * i + sgm, True), (2 * j + sgm, False)], hop)
+                add([(2 * j + sgm, True), (2 * i + sgm, False)], np.conj(hop))
+        if J != 0:
+            # S_i.S_j = Sz Sz + (S+_i S-_j + S-_i S+_j)/2
+            for a in (0, 1):
+                for b in (0, 1):
+                    sa = 0.5 if a == 0 else -0.5
+                    sb = 0.5 if b == 0 else -0.5
+                    add([(2 * i + a, True), (2 * i + a, False), (2 * j + b, True), (2 * j + b, False)], J * sa * sb)
+            add([(2 * i, True), (2 * i + 1, False), (2 * j + 1, True), (2 * j, False)], 0.5 * J)
+            add([(2 * i + 1, True), (2 * i, False), (2 * j, True), (2 * j + 1, False)], 0.5 * J)
+        if V != 0:
+            for a in (0, 1):
+                for b in (0, 1):
+                    add([(2 * i + a, True), (2 * i + a, False), (2 * j + b, True), (2 * j + b, False)], V)
+    if model.kind == "hubbard" and model.U != 0:
+        for i in range(len(positions)):
+            add([(2 * i, True), (2 * i, False), (2 * i + 1, True), (2 * i + 1, False)], model.U)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim), dtype=np.complex128)
+    return H, fs
+
+
+def casimir(fs):
+    """Total S^2 on the S_z = 0 sector."""
+    n = fs.nsite
+    rows, cols, vals = [], [], []
+    dim = len(fs.states)
+
+    def add(ops, c):
+        src, dst, sg = fs.apply(ops)
+        rows.extend(dst.tolist())
+        cols.extend(src.tolist())
+        vals.extend((c * sg).tolist())
+
+    for i in range(n):
+        for j in range(n):
+            for a in (0, 1):
+                for b in (0, 1):
+                    sa = 0.5 if a == 0 else -0.5
+                    sb = 0.5 if b == 0 else -0.5
+                    add([(2 * i + a, True), (2 * i + a, False), (2 * j + b, True), (2 * j + b, False)], sa * sb)
+            add([(2 * i, True), (2 * i + 1, False), (2 * j + 1, True), (2 * j, False)], 0.5)
+            add([(2 * i + 1, True), (2 * i, False), (2 * j, True), (2 * j + 1, False)], 0.5)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim), dtype=np.complex128)
+
+
+def singlet_spectrum(model, positions, bonds, n_elec, k=None):
+    """Eigenvalues of H restricted to total S = 0 (all of them when the
+    sector is small, else the lowest k via H + 50 S^2)."""
+    H, fs = hamiltonian(model, positions, bonds, n_elec)
+    S2 = casimir(fs)
+    dim = H.shape[0]
+    if k is None and dim <= 3000:
+        w, v = np.linalg.eigh(H.toarray())
+        s2 = np.real(np.einsum("ij,ij->j", v.conj(), S2.toarray() @ v))
+        # degenerate multiplets mix S values; diagonalise S2 within each
+        # degenerate H eigenspace
+        out = []
+        i = 0
+        while i < dim:
+            j = i
+            while j + 1 < dim and abs(w[j + 1] - w[i]) < 1e-8:
+                j += 1
+            blk = v[:, i:j + 1]
+            ss = np.linalg.eigvalsh(blk.conj().T @ (S2 @ blk))
+            out.extend([w[i]] * int(np.sum(np.abs(ss) < 1e-6)))
+            i = j + 1
+        return np.array(out)
+    kk = k or 4
+    w = spla.eigsh(H + 50.0 * S2, k=kk, which="SA", return_eigenvectors=False)
+    return np.sort(np.real(w))

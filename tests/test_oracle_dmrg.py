@@ -1,0 +1,125 @@
+"""Pins for the oracle's sweep machinery: with no truncation every finite
+sweep step equals the exact ground state (growth, renormalisation, wave
+transformation); Lanczos against numpy eigh; the kept-count rule on
+synthetic spectra."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dmrg, driver, ed
+from oracle import lattice as lat
+
+
+def _ed_gs(m):
+    sg = dmrg.StepGeometry(m, m.nsite // 2 - 1, m.nsite - m.nsite // 2 - 1)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    return ed.singlet_spectrum(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)[0]
+
+
+@pytest.mark.parametrize("kind,Lx,Ly,flux,N", [("hubbard", 4, 2, 0.15, 6), ("tj", 4, 2, 0.2, 6),
+                                               ("hubbard", 3, 2, 0.0, 4)])
+def test_untruncated_sweeps_equal_ed(kind, Lx, Ly, flux, N):
+    m = lat.Model(kind=kind, Lx=Lx, Ly=Ly, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=N)
+    e0 = _ed_gs(m)
+    recs = driver.run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    sweep = [r for r in recs if r.phase == "sweep1"]
+    assert sweep
+    for r in sweep:
+        assert abs(r.energy - e0) < 1e-10 * abs(e0), (r.sys_len, r.energy, e0)
+        # the transformed guess is already the ground state
+        assert r.iters <= 3
+
+
+def test_wave_transformation_is_isometry_and_preserves_energy():
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=4.0, t=1.0, flux=0.1, n_elec=6)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=0, tol=1e-12))
+    run.run()
+    # redo one LR step by hand from the stored state
+    n = m.nsite
+    label = 2
+    sysb = run.sys_blocks[label]
+    envb = run.env_blocks[n - label - 2]
+    E, psi, it, se, ee, H = run._solve_with_guess(sysb, envb, m.n_elec, 1e-13, None, None, "LR")
+    rho = dmrg.density_matrix(H.sb, psi, "sys")
+    tr = dmrg.truncation(rho, 10 ** 6, 10 ** 6, 0.0)
+    newsys = dmrg.truncate_block(se, tr.trun, m, label + 1)
+    sg2 = dmrg.StepGeometry(m, label + 1, n - label - 3)
+    se2 = dmrg.enlarge_in_geometry(newsys, m, "sys", sg2)
+    ee2 = dmrg.enlarge_in_geometry(run.env_blocks[n - label - 3], m, "env", sg2)
+    H2 = dmrg.Superblock(se2, ee2, sg2, m, m.n_elec)
+    psi2 = dmrg.wave_trans_LR(H.sb, se, ee, psi, tr.trun, run.env_trun[n - label - 2], se2, ee2, H2.sb)
+    assert abs(np.vdot(psi2, psi2).real - 1.0) < 1e-12
+    e2 = np.vdot(psi2, H2.matvec(psi2)).real
+    assert abs(e2 - E) < 1e-9
+
+
+def test_lanczos_matches_eigh():
+    rng = np.random.default_rng(3)
+    n = 60
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A = A + A.conj().T
+    v0 = dmrg.start_vector(n, 7)
+    E, psi, it = dmrg.lanczos(lambda v: A @ v, v0, 1e-12, 100)
+    w, V = np.linalg.eigh(A)
+    assert abs(E - w[0]) < 1e-9
+    assert abs(abs(np.vdot(V[:, 0], psi)) - 1.0) < 1e-6
+
+
+def test_lanczos_stopping_rule_beta():
+    # an eigenvector start stops at beta <= lanzero after one product
+    A = np.diag(np.arange(1.0, 9.0)).astype(complex)
+    v0 = np.zeros(8, complex)
+    v0[0] = 1.0
+    E, psi, it = dmrg.lanczos(lambda v: A @ v, v0, 1e-10, 100)
+    assert it == 1 and abs(E - 1.0) < 1e-15
+
+
+def _rho_from_spectrum(blocks):
+    rho = []
+    for a, vals in enumerate(blocks):
+        rho.append((a, np.diag(np.asarray(vals, dtype=complex))))
+    return rho
+
+
+def test_truncation_keeps_kept_max_when_tail_large():
+    rho = _rho_from_spectrum([[0.4, 0.1, 0.05], [0.3, 0.1, 0.05]])
+    tr = dmrg.truncation(rho, 2, 3, 1e-18)
+    assert tr.num_state == 3
+    assert tr.kept_per_sector == [2, 1]
+    assert abs(tr.err - 0.2) < 1e-15
+
+
+def test_truncation_small_dim_keeps_all():
+    rho = _rho_from_spectrum([[0.5], [0.5]])
+    tr = dmrg.truncation(rho, 4, 8, 1e-18)
+    assert tr.num_state == 2 and tr.err == 0.0
+
+
+def test_truncation_stops_when_tail_below_threshold():
+    rho = _rho_from_spectrum([[0.6, 1e-20, 0.0], [0.4, 0.0]])
+    tr = dmrg.truncation(rho, 1, 4, float(np.float32(1e-18)))
+    assert tr.num_state == 2
+    assert tr.kept_per_sector == [1, 1]
+
+
+def test_truncation_tie_break_is_deterministic():
+    # equal weights in two sectors: the earlier sector wins the last slot
+    rho = _rho_from_spectrum([[0.25, 0.25], [0.25, 0.25]])
+    tr = dmrg.truncation(rho, 1, 3, 1e-18)
+    assert tr.num_state == 3
+    assert tr.kept_per_sector == [2, 1]
+
+
+def test_entropy_of_uniform_spectrum():
+    rho = _rho_from_spectrum([[0.25, 0.25], [0.25, 0.25]])
+    tr = dmrg.truncation(rho, 4, 4, 1e-18)
+    assert abs(tr.entropy - math.log(4)) < 1e-14
+
+
+def test_start_vector_is_deterministic_and_normalised():
+    a = dmrg.start_vector(1000, 5)
+    b = dmrg.start_vector(1000, 5)
+    assert np.array_equal(a, b)
+    assert abs(np.vdot(a, a).real - 1.0) < 1e-14
+    assert np.all(a.real > 0) and np.all(a.imag > 0)
