@@ -1,0 +1,131 @@
+"""ctypes binding of libhhdmrg.so (the C ABI in include/hhdmrg.h).
+
+The library is built in-tree (``python -m hofstadter_hubbard_tsc_amd.build``);
+loading fails loudly when it is missing -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhhdmrg.so")
+
+i32, i64, f64, u64 = C.c_int32, C.c_int64, C.c_double, C.c_uint64
+vp = C.c_void_p
+P = C.POINTER
+
+
+class ModelParams(C.Structure):
+    _fields_ = [("kind", i32), ("Lx", i32), ("Ly", i32), ("lattice", i32), ("periodic_y", i32),
+                ("n_elec", i32), ("t", f64), ("U", f64), ("J", f64), ("flux", f64)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("kept_max", i32), ("kept_max_warm", i32), ("n_sweeps", i32), ("max_iter", i32),
+                ("tol", f64), ("warm_tol_factor", f64), ("first_tol_factor", f64), ("seed", u64),
+                ("precision", i32), ("reserved", i32)]
+
+
+class StepRecord(C.Structure):
+    _fields_ = [("phase", i32), ("sys_len", i32), ("env_len", i32), ("iters", i32), ("num_state", i32),
+                ("pad", i32), ("dim", i64), ("energy", f64), ("trunc_err", f64), ("entropy", f64),
+                ("seconds", f64), ("matvec_seconds", f64), ("matvec_flops", f64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("matvecs", i64), ("matvec_ms", f64), ("stage1_ms", f64), ("stage2_ms", f64),
+                ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64)]
+
+
+# numpy views of the engine's task structs (hh_core.hpp: Term, Tile)
+TERM_DTYPE = np.dtype([("cr", "<f8"), ("ci", "<f8"), ("a_off", "<i8"), ("b_off", "<i8"), ("a_ld", "<i4"),
+                       ("b_ld", "<i4"), ("K", "<i4"), ("a_buf", "u1"), ("b_buf", "u1"), ("a_op", "u1"),
+                       ("b_op", "u1")], align=True)
+TILE_DTYPE = np.dtype([("c_off", "<i8"), ("c_ld", "<i4"), ("M", "<i4"), ("N", "<i4"), ("t0", "<i4"),
+                       ("t1", "<i4"), ("c_buf", "<i4"), ("accumulate", "<i4")], align=True)
+
+# (name, restype, argtypes): every entry point of include/hhdmrg.h
+SIGNATURES = [
+    ("hh_ctx_create", i32, [i32, P(vp)]),
+    ("hh_ctx_destroy", i32, [vp]),
+    ("hh_last_error", C.c_char_p, [vp]),
+    ("hh_set_model", i32, [vp, P(ModelParams)]),
+    ("hh_dev_alloc", i32, [vp, i64, P(vp)]),
+    ("hh_dev_free", i32, [vp, vp]),
+    ("hh_memcpy_h2d", i32, [vp, vp, vp, i64]),
+    ("hh_memcpy_d2h", i32, [vp, vp, vp, i64]),
+    ("hh_block_site", i32, [vp, i32, P(vp)]),
+    ("hh_block_create", i32, [vp, i32, i32, i32, P(i32), P(i32), P(i32), i32, P(i32), P(vp)]),
+    ("hh_block_destroy", i32, [vp]),
+    ("hh_block_info", i32, [vp, P(i64), P(i32), P(i32), P(i32)]),
+    ("hh_block_sectors", i32, [vp, P(i32), P(i32), P(i32), P(i64)]),
+    ("hh_block_sites", i32, [vp, P(i32)]),
+    ("hh_block_layout", i32, [vp, i32, P(i32), P(i32), P(i32), P(i64)]),
+    ("hh_block_op_offset", i32, [vp, i32, i32, P(i64)]),
+    ("hh_block_upload", i32, [vp, vp, vp]),
+    ("hh_block_download", i32, [vp, vp, vp]),
+    ("hh_superblock_bind", i32, [vp, vp, vp, i32, P(vp)]),
+    ("hh_superblock_destroy", i32, [vp]),
+    ("hh_superblock_info", i32, [vp, P(i64), P(i32), P(f64), P(f64)]),
+    ("hh_superblock_pairs", i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i64)]),
+    ("hh_superblock_enlarged", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32)]),
+    ("hh_model_wave", i32, [vp, vp, vp, vp]),
+    ("hh_lanczos", i32, [vp, vp, vp, f64, i32, P(f64), P(i32)]),
+    ("hh_truncate", i32, [vp, vp, i32, vp, i32, i32, f64, P(vp), P(i32), P(f64), P(f64)]),
+    ("hh_trunc_identity", i32, [vp, vp, i32, P(vp)]),
+    ("hh_trunc_destroy", i32, [vp]),
+    ("hh_trunc_kept", i32, [vp, P(i32), P(i32), P(i32)]),
+    ("hh_grow_block", i32, [vp, vp, i32, vp, P(vp)]),
+    ("hh_wave_trans", i32, [vp, i32, vp, vp, vp, vp, vp, vp]),
+    ("hh_dmrg_run", i32, [vp, P(Schedule), P(StepRecord), i32, P(i32)]),
+    ("hh_set_profiling", i32, [vp, i32]),
+    ("hh_get_stats", i32, [vp, P(Stats)]),
+    ("hh_reset_stats", i32, [vp]),
+    ("hh_superblock_tasks", i32, [vp, i32, P(vp), P(i32), P(vp), P(i32), P(i64)]),
+    ("hh_plan_matvec", i32, [P(ModelParams), i32, i32, P(i32), P(i32), P(i32), i32, P(i32), i32, i32, P(i32),
+                             P(i32), P(i32), i32, P(i32), i32, P(vp)]),
+    ("hh_plan_grow", i32, [P(ModelParams), i32, i32, i32, P(i32), P(i32), P(i32), i32, P(i32), i32, i32, i32,
+                           P(i32), P(i32), P(i64), P(i32), P(vp)]),
+    ("hh_plan_destroy", i32, [vp]),
+    ("hh_plan_tasks", i32, [vp, i32, P(vp), P(i32), P(vp), P(i32), P(i64)]),
+    ("hh_plan_block", i32, [vp, i32, P(i64), P(i32), P(i32), P(i32), P(i32), P(i64), P(i32), P(i32)]),
+    ("hh_plan_layout", i32, [vp, i32, i32, P(i32), P(i32), P(i32), P(i64)]),
+    ("hh_plan_op_offset", i32, [vp, i32, i32, i32, P(i64)]),
+    ("hh_plan_super", i32, [vp, P(i64), P(i32), P(i32), P(i32), P(i32), P(i32), P(i64)]),
+    ("hh_run_tasks", i32, [vp, vp, i32, vp, i32, P(vp), i32]),
+    ("hh_start_vector", i32, [vp, vp, i64, u64]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libhhdmrg.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libhhdmrg.so is not built: run `python -m hofstadter_hubbard_tsc_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ip(a):
+    """ctypes pointer to a contiguous numpy array."""
+    if a is None:
+        return None
+    t = {np.dtype("int32"): i32, np.dtype("int64"): i64, np.dtype("float64"): f64}[a.dtype]
+    return a.ctypes.data_as(P(t))
+
+
+def model_params(m):
+    """ModelParams from an object with kind/Lx/Ly/lattice/periodic_y/n_elec/t/U/J/flux."""
+    kind = {"hubbard": 0, "tj": 1}.get(m.kind, m.kind)
+    lattice = {"square": 0, "triangular": 1}.get(m.lattice, m.lattice)
+    return ModelParams(kind, m.Lx, m.Ly, lattice, int(m.periodic_y), m.n_elec, m.t, m.U, m.J, m.flux)

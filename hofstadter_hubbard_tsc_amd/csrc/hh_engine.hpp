@@ -1,0 +1,52 @@
+// Device engine: grouped complex GEMM over tile lists (FP64 / FP32 MFMA on
+// gfx950) and the deterministic vector kernels of the Lanczos solver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hh_core.hpp"
+
+namespace hh {
+
+struct BufPtrs { double2* p[8]; };
+struct WorkItem { int32_t tile, r0, c0, pad; };
+
+// Tile list resident on the device, launched as one grouped kernel.
+struct DevTasks {
+  Tile* tiles = nullptr;
+  Term* terms = nullptr;
+  WorkItem* work = nullptr;
+  int ntiles = 0, nterms = 0, nwork = 0;
+  double flops_alg = 0.0, flops_exec = 0.0;
+  void upload(const TaskList& tl, hipStream_t s);
+  void release();
+  ~DevTasks() { release(); }
+};
+
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+// C tiles = sum of terms; one work item (64x64 macro tile) per workgroup.
+void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision = PREC_FP64);
+
+// vector kernels (complex128 vectors of length n)
+void dev_zero(double2* p, int64_t n, hipStream_t s);
+void dev_scale(double2* p, double a, int64_t n, hipStream_t s);
+void dev_copy(double2* dst, const double2* src, int64_t n, hipStream_t s);
+// w = w - a*v1 - b*v2 (v2 may be null)
+void dev_lanczos_update(double2* w, const double2* v1, double a, const double2* v2, double b, int64_t n, hipStream_t s);
+// out[j] = Re <V_j, w>, j < nv, deterministic two-pass reduction; V_j = V + j*stride
+void dev_rdots(const double2* V, int64_t stride, int nv, const double2* w, int64_t n, double* out_dev,
+               double* partial_dev, hipStream_t s);
+// w = beta*w + sum_j c_j V_j  (c on the device)
+void dev_combine(double2* w, double beta, const double2* V, int64_t stride, int nv, const double* c_dev, int64_t n,
+                 hipStream_t s);
+int rdot_partials();   // partial-sum slots per vector
+
+void check_hip(hipError_t e, const char* what);
+// counter-based start vector generated on the device (same stream as the host)
+void dev_start_vector(double2* p, int64_t n, uint64_t seed, hipStream_t s);
+
+}  // namespace hh

@@ -1,0 +1,1281 @@
+// Runtime objects behind the C ABI (include/hhdmrg.h): context, device-resident
+// blocks, bound superblocks, truncations, the Lanczos solver and the sweep
+// driver.  All arithmetic runs in the HIP engine (hh_engine.hip) and in
+// rocSOLVER's Hermitian eigensolver; the host only compiles task lists,
+// applies the integer kept-count rule and steers the sweep.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hhdmrg.h"
+#include "hh_core.hpp"
+#include "hh_engine.hpp"
+
+using namespace hh;
+
+namespace {
+
+struct HHError : std::runtime_error {
+  int code;
+  HHError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+inline void need(bool ok, const std::string& msg) {
+  if (!ok) throw HHError(1, msg);
+}
+
+// device allocation owned by one object
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    if (b <= bytes && p) return;
+    reset();
+    if (b == 0) b = 16;
+    check_hip(hipMalloc(&p, b), "hipMalloc");
+    bytes = b;
+  }
+  double2* c() const { return (double2*)p; }
+};
+
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ----------------------------------------------------------------------------
+// symmetric tridiagonal eigenproblem (DSTEV stand-in): implicit-shift QL
+// (the textbook tql2 method); ascending eigenvalues, optional first vector.
+// ----------------------------------------------------------------------------
+void tridiag_ql(std::vector<double>& d, std::vector<double> e, std::vector<double>* z0) {
+  const int n = (int)d.size();
+  std::vector<double> Z;
+  if (z0) {
+    Z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+  }
+  e.resize(n);
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    while (true) {
+      int m;
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-300 || std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) throw HHError(-3, "tridiagonal QL did not converge");
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i;
+      for (i = m - 1; i >= l; --i) {
+        double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        if (z0)
+          for (int k = 0; k < n; ++k) {
+            double& zi1 = Z[(size_t)(i + 1) * n + k];
+            double& zi = Z[(size_t)i * n + k];
+            const double t = zi1;
+            zi1 = s * zi + c * t;
+            zi = c * zi - s * t;
+          }
+      }
+      if (r == 0.0 && i >= l) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  // sort ascending (with vectors)
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  std::vector<double> ds(n);
+  for (int i = 0; i < n; ++i) ds[i] = d[idx[i]];
+  if (z0) {
+    z0->assign(n, 0.0);
+    for (int k = 0; k < n; ++k) (*z0)[k] = Z[(size_t)idx[0] * n + k];
+  }
+  d = ds;
+}
+
+}  // namespace
+
+// ============================================================================
+// objects
+// ============================================================================
+struct hh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle rb = nullptr;
+  ModelParams model;
+  bool model_set = false;
+  int precision = PREC_FP64;
+  std::string err;
+  // Lanczos workspace
+  DevMem V, dots, partial, scratch_w;
+  // profiling
+  bool profile = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hh_stats stats{};
+  std::vector<DevMem*> user_allocs;
+};
+
+struct hh_block {
+  BlockDesc d;
+  DevMem arena;
+};
+
+struct hh_superblock {
+  const hh_block* sys = nullptr;
+  const hh_block* env = nullptr;
+  int N = 0;
+  Geometry g;
+  MatvecPlan plan;
+  DevTasks merge, st1, st2;
+  DevMem merged, X;
+};
+
+struct hh_trunc {
+  int side = 0;
+  EnlBasis eb;
+  std::vector<KeptSector> kept;
+  std::vector<int> kept_per_rho;   // per rho block (super-basis order)
+  DevMem evec;
+  int num_state = 0;
+  double err = 0.0, entropy = 0.0;
+};
+
+namespace {
+
+BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* env, double2* X, double2* mrg,
+                double2* aux, double2* aux2) {
+  BufPtrs b;
+  b.p[B_IN] = in;
+  b.p[B_OUT] = out;
+  b.p[B_SYS] = sys ? sys->arena.c() : nullptr;
+  b.p[B_ENV] = env ? env->arena.c() : nullptr;
+  b.p[B_X] = X;
+  b.p[B_MRG] = mrg;
+  b.p[B_AUX] = aux;
+  b.p[B_AUX2] = aux2;
+  return b;
+}
+
+cplx site_val(int model_kind, int kind, int sp, int s) {
+  const double r2 = std::sqrt(2.0);
+  if (kind == OP_CDAG) {
+    if (sp == 1 && s == 0) return -r2;
+    if (model_kind == HUBBARD && sp == 2 && s == 1) return r2;
+    return 0.0;
+  }
+  if (kind == OP_S) return (sp == 1 && s == 1) ? std::sqrt(1.5) : 0.0;
+  if (kind == OP_N) {
+    if (sp != s) return 0.0;
+    if (s == 1) return r2;
+    if (s == 2) return 2.0;
+  }
+  return 0.0;
+}
+
+hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>& sec, const std::vector<int>& sites) {
+  auto* b = new hh_block();
+  b->d.side = side;
+  b->d.length = length;
+  b->d.sec = sec;
+  b->d.sites = sites;
+  layout_block(b->d, n_op_kinds(ctx->model.kind));
+  b->arena.alloc(sizeof(double2) * (size_t)std::max<int64_t>(b->d.size, 1));
+  return b;
+}
+
+void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
+  BufPtrs b = bufs_of((double2*)in, out, sb->sys, sb->env, sb->X.c(), sb->merged.c(), nullptr, nullptr);
+  const double t0 = ctx->profile ? wall() : 0.0;
+  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
+  launch_tiles(sb->st1, b, ctx->stream, ctx->precision);
+  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[1], ctx->stream), "event");
+  launch_tiles(sb->st2, b, ctx->stream, ctx->precision);
+  if (ctx->profile) {
+    check_hip(hipEventRecord(ctx->ev[2], ctx->stream), "event");
+    check_hip(hipEventSynchronize(ctx->ev[2]), "event sync");
+    float a = 0.f, c = 0.f;
+    check_hip(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]), "elapsed");
+    check_hip(hipEventElapsedTime(&c, ctx->ev[1], ctx->ev[2]), "elapsed");
+    ctx->stats.stage1_ms += a;
+    ctx->stats.stage2_ms += c;
+    ctx->stats.matvec_ms += (wall() - t0) * 1e3;
+  }
+  ctx->stats.matvecs += 1;
+  ctx->stats.flops_alg += sb->plan.stage2.flops_alg;
+  ctx->stats.flops_exec += sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
+  ctx->stats.bytes_alg += sb->plan.bytes_alg;
+}
+
+double rdot(hh_ctx* ctx, const double2* a, const double2* b, int64_t n) {
+  double r = 0.0;
+  dev_rdots(a, 0, 1, b, n, (double*)ctx->dots.p, (double*)ctx->partial.p, ctx->stream);
+  check_hip(hipMemcpyAsync(&r, ctx->dots.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+  check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  return r;
+}
+
+void ensure_ws(hh_ctx* ctx, int64_t n, int nvec) {
+  ctx->V.alloc(sizeof(double2) * (size_t)n * (size_t)(nvec + 1));
+  ctx->dots.alloc(sizeof(double) * (nvec + 2));
+  ctx->partial.alloc(sizeof(double) * (size_t)rdot_partials() * (nvec + 2));
+}
+
+// Lanczos, lanczos_memo_new (code/lanczos.f90:4-176), levels = 1.
+void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter, double* e0, int* iters,
+             double* mv_seconds) {
+  const int64_t n = sb->plan.sb.dim;
+  need(n > 0, "empty superblock");
+  need(max_iter >= 1, "max_iter must be >= 1");
+  ensure_ws(ctx, n, max_iter + 1);
+  double2* V = ctx->V.c();
+  double2* w = V + (size_t)max_iter * n;   // last slot is the work vector
+  auto vec = [&](int j) { return V + (size_t)j * n; };
+  double mv = 0.0;
+  auto hpsi = [&](const double2* x, double2* y) {
+    const double t = wall();
+    matvec(ctx, sb, x, y);
+    check_hip(hipStreamSynchronize(ctx->stream), "matvec");
+    mv += wall() - t;
+  };
+  // v1 = psi / |psi|
+  dev_copy(vec(0), psi, n, ctx->stream);
+  const double nrm0 = std::sqrt(rdot(ctx, vec(0), vec(0), n));
+  need(nrm0 > 0.0, "zero start vector");
+  dev_scale(vec(0), 1.0 / nrm0, n, ctx->stream);
+  hpsi(vec(0), w);
+  std::vector<double> alpha{rdot(ctx, vec(0), w, n)}, beta;
+  double ritz = alpha[0];
+  int last = 1;
+  std::vector<double> cg(max_iter + 1);
+  while (last < max_iter) {
+    dev_lanczos_update(w, vec(last - 1), alpha[last - 1], last >= 2 ? vec(last - 2) : nullptr,
+                       last >= 2 ? beta[last - 2] : 0.0, n, ctx->stream);
+    const double b = std::sqrt(rdot(ctx, w, w, n));
+    beta.push_back(b);
+    if (std::fabs(b) <= lanzero) break;
+    dev_scale(w, 1.0 / b, n, ctx->stream);
+    // classical Gram-Schmidt with real coefficients (Gram_Schmidt_process,
+    // code/space_info.f90:1176-1214): c_j = Re<v_j,w>/Re<v_j,v_j>
+    dev_rdots(V, n, last, w, n, (double*)ctx->dots.p, (double*)ctx->partial.p, ctx->stream);
+    check_hip(hipMemcpyAsync(cg.data(), ctx->dots.p, sizeof(double) * last, hipMemcpyDeviceToHost, ctx->stream),
+              "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+    for (int j = 0; j < last; ++j) cg[j] = -cg[j];   // Lanczos vectors are unit norm
+    check_hip(hipMemcpyAsync(ctx->dots.p, cg.data(), sizeof(double) * last, hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+    dev_combine(w, 1.0, V, n, last, (const double*)ctx->dots.p, n, ctx->stream);
+    const double nw = std::sqrt(rdot(ctx, w, w, n));
+    dev_scale(w, 1.0 / nw, n, ctx->stream);
+    dev_copy(vec(last), w, n, ctx->stream);
+    ++last;
+    hpsi(vec(last - 1), w);
+    alpha.push_back(rdot(ctx, vec(last - 1), w, n));
+    std::vector<double> d(alpha), e(beta);
+    e.resize(d.size());
+    tridiag_ql(d, e, nullptr);
+    if (std::fabs(d[0] - ritz) <= lanzero || last == max_iter) {
+      ritz = d[0];
+      break;
+    }
+    ritz = d[0];
+  }
+  // Ritz vector from the stored Lanczos vectors
+  std::vector<double> d(alpha), e(beta), z;
+  e.resize(d.size());
+  tridiag_ql(d, e, &z);
+  check_hip(hipMemcpyAsync(ctx->dots.p, z.data(), sizeof(double) * last, hipMemcpyHostToDevice, ctx->stream), "h2d");
+  dev_combine(psi, 0.0, V, n, last, (const double*)ctx->dots.p, n, ctx->stream);
+  const double np = std::sqrt(rdot(ctx, psi, psi, n));
+  dev_scale(psi, 1.0 / np, n, ctx->stream);
+  check_hip(hipStreamSynchronize(ctx->stream), "lanczos");
+  *e0 = d[0];
+  *iters = last;
+  if (mv_seconds) *mv_seconds = mv;
+}
+
+// Reduced density matrix + Get_truncation_operator (code/get_den.f90:184-428)
+hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi, int kept_min, int kept_max,
+                   double trun_err) {
+  const SuperBasis& S = sb->plan.sb;
+  std::vector<int> rsec, rdim;
+  std::vector<int64_t> roff;
+  TaskList tl = compile_density(S, side, rsec, roff, rdim);
+  int64_t total = 0;
+  for (int d : rdim) total += (int64_t)d * d;
+  auto* t = new hh_trunc();
+  t->side = side;
+  t->eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
+  t->evec.alloc(sizeof(double2) * (size_t)std::max<int64_t>(total, 1));
+  DevTasks dt;
+  dt.upload(tl, ctx->stream);
+  launch_tiles(dt, bufs_of((double2*)psi, t->evec.c(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
+               ctx->stream, PREC_FP64);
+  int64_t nev = 0;
+  int dmax = 1;
+  for (int d : rdim) {
+    nev += d;
+    dmax = std::max(dmax, d);
+  }
+  DevMem evals, work, info;
+  evals.alloc(sizeof(double) * (size_t)std::max<int64_t>(nev, 1));
+  work.alloc(sizeof(double) * (size_t)dmax);
+  info.alloc(sizeof(rocblas_int) * rdim.size() + 16);
+  int64_t eo = 0;
+  for (size_t i = 0; i < rdim.size(); ++i) {
+    rocblas_status st = rocsolver_zheevd(ctx->rb, rocblas_evect_original, rocblas_fill_upper, rdim[i],
+                                         (rocblas_double_complex*)(t->evec.c() + roff[i]), rdim[i],
+                                         (double*)evals.p + eo, (double*)work.p, (rocblas_int*)info.p + i);
+    if (st != rocblas_status_success) {
+      delete t;
+      throw HHError(-2, "rocsolver_zheevd failed");
+    }
+    eo += rdim[i];
+  }
+  std::vector<double> ev(nev);
+  std::vector<rocblas_int> inf(rdim.size());
+  check_hip(hipMemcpyAsync(ev.data(), evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+  check_hip(hipMemcpyAsync(inf.data(), info.p, sizeof(rocblas_int) * rdim.size(), hipMemcpyDeviceToHost, ctx->stream),
+            "d2h");
+  check_hip(hipStreamSynchronize(ctx->stream), "eig");
+  for (auto x : inf)
+    if (x != 0) {
+      delete t;
+      throw HHError(-2, "rocsolver_zheevd did not converge");
+    }
+  // global descending sort, ties by position in the concatenated ascending
+  // per-sector spectra (deterministic stand-in for QuickSort_Double)
+  std::vector<int64_t> order(nev);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    if (ev[a] != ev[b]) return ev[a] > ev[b];
+    return a < b;
+  });
+  std::vector<double> sv(nev);
+  double tot = 0.0;
+  for (int64_t i = 0; i < nev; ++i) {
+    sv[i] = ev[order[i]];
+    tot += sv[i];
+  }
+  for (auto& x : sv) x /= tot;
+  int64_t num;
+  double err;
+  if (nev <= kept_min) {
+    num = nev;
+    err = 0.0;
+  } else {
+    err = 0.0;
+    for (int64_t i = kept_min; i < nev; ++i) err += sv[i];
+    num = kept_min;
+    while (err > trun_err) {
+      ++num;
+      err -= sv[num - 1];
+      if (err <= trun_err || num >= kept_max || num >= nev) break;
+    }
+  }
+  double ent = 0.0;
+  for (double x : sv)
+    if (x >= 1.0e-15) ent -= x * std::log(x);
+  std::vector<int> keep(rdim.size(), 0);
+  {
+    std::vector<int> owner(nev);
+    int64_t o = 0;
+    for (size_t i = 0; i < rdim.size(); ++i)
+      for (int k = 0; k < rdim[i]; ++k) owner[o++] = (int)i;
+    for (int64_t i = 0; i < num; ++i) keep[owner[order[i]]]++;
+  }
+  for (size_t i = 0; i < rdim.size(); ++i) {
+    if (keep[i] == 0) continue;
+    KeptSector ks;
+    ks.a = rsec[i];
+    ks.keep = keep[i];
+    ks.u_off = roff[i] + (int64_t)(rdim[i] - keep[i]) * rdim[i];
+    ks.u_ld = rdim[i];
+    t->kept.push_back(ks);
+  }
+  t->kept_per_rho = keep;
+  t->num_state = (int)num;
+  t->err = err;
+  t->entropy = ent;
+  return t;
+}
+
+hh_trunc* trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side) {
+  auto* t = new hh_trunc();
+  t->side = side;
+  t->eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
+  int64_t total = 0;
+  for (auto& s : t->eb.sec) total += (int64_t)s.dim * s.dim;
+  std::vector<double2> h((size_t)std::max<int64_t>(total, 1), make_double2(0.0, 0.0));
+  int64_t o = 0;
+  for (int a = 0; a < (int)t->eb.sec.size(); ++a) {
+    const int d = t->eb.sec[a].dim;
+    for (int i = 0; i < d; ++i) h[o + (int64_t)i * d + i] = make_double2(1.0, 0.0);
+    t->kept.push_back({a, d, o, d});
+    t->num_state += d;
+    o += (int64_t)d * d;
+  }
+  t->evec.alloc(sizeof(double2) * h.size());
+  check_hip(hipMemcpyAsync(t->evec.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream), "h2d");
+  check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  return t;
+}
+
+hh_block* grow(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t) {
+  need(t->side == side, "truncation side mismatch");
+  const hh_block* blk = side == 0 ? sb->sys : sb->env;
+  GrowPlan gp = compile_grow(ctx->model, sb->g, blk->d, t->kept);
+  auto* nb = new hh_block();
+  nb->d = gp.out;
+  nb->arena.alloc(sizeof(double2) * (size_t)std::max<int64_t>(nb->d.size, 1));
+  DevMem mrg, X;
+  mrg.alloc(sizeof(double2) * (size_t)std::max<int64_t>(gp.merged_size, 1));
+  X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(gp.x_size, 1));
+  DevTasks a, b, c;
+  a.upload(gp.merge_tasks, ctx->stream);
+  b.upload(gp.stage1, ctx->stream);
+  c.upload(gp.stage2, ctx->stream);
+  BufPtrs bp = bufs_of(nullptr, nb->arena.c(), blk, nullptr, X.c(), mrg.c(), t->evec.c(), nullptr);
+  launch_tiles(a, bp, ctx->stream, PREC_FP64);
+  launch_tiles(b, bp, ctx->stream, PREC_FP64);
+  launch_tiles(c, bp, ctx->stream, PREC_FP64);
+  check_hip(hipStreamSynchronize(ctx->stream), "grow");
+  return nb;
+}
+
+void wave_trans(hh_ctx* ctx, int dir, hh_superblock* osb, const double2* psi_old, const hh_trunc* st,
+                const hh_trunc* et, hh_superblock* nsb, double2* psi_new) {
+  TransPlan tp = dir == 0 ? compile_wave_trans_lr(osb->plan.sb, osb->plan.sbs, osb->plan.ebs, st->kept, et->kept,
+                                                  nsb->plan.sbs, nsb->plan.ebs, nsb->plan.sb)
+                          : compile_wave_trans_rl(osb->plan.sb, osb->plan.sbs, osb->plan.ebs, st->kept, et->kept,
+                                                  nsb->plan.sbs, nsb->plan.ebs, nsb->plan.sb);
+  DevMem X;
+  X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(tp.x_size, 1));
+  DevTasks a, b;
+  a.upload(tp.stage1, ctx->stream);
+  b.upload(tp.stage2, ctx->stream);
+  BufPtrs bp = bufs_of((double2*)psi_old, psi_new, nullptr, nullptr, X.c(), nullptr, st->evec.c(), et->evec.c());
+  launch_tiles(a, bp, ctx->stream, PREC_FP64);
+  launch_tiles(b, bp, ctx->stream, PREC_FP64);
+  ensure_ws(ctx, 1, 1);
+  const double n2 = rdot(ctx, psi_new, psi_new, nsb->plan.sb.dim);
+  if (n2 > 0.0) dev_scale(psi_new, 1.0 / std::sqrt(n2), nsb->plan.sb.dim, ctx->stream);
+  check_hip(hipStreamSynchronize(ctx->stream), "wave_trans");
+}
+
+hh_superblock* bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N) {
+  need(sys->d.side == 0 && env->d.side == 1, "bind expects a sys block and an env block");
+  auto* s = new hh_superblock();
+  s->sys = sys;
+  s->env = env;
+  s->N = N;
+  s->g = make_geometry(ctx->model, sys->d.length, env->d.length);
+  s->plan = compile_matvec(ctx->model, s->g, sys->d, env->d, N);
+  s->merged.alloc(sizeof(double2) * (size_t)std::max<int64_t>(s->plan.merged_size, 1));
+  s->X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(s->plan.x_size, 1));
+  s->merge.upload(s->plan.merge_tasks, ctx->stream);
+  s->st1.upload(s->plan.stage1, ctx->stream);
+  s->st2.upload(s->plan.stage2, ctx->stream);
+  launch_tiles(s->merge, bufs_of(nullptr, nullptr, sys, env, nullptr, s->merged.c(), nullptr, nullptr), ctx->stream,
+               PREC_FP64);
+  check_hip(hipStreamSynchronize(ctx->stream), "bind");
+  return s;
+}
+
+hh_block* site_block(hh_ctx* ctx, int side) {
+  const ModelParams& m = ctx->model;
+  auto st = site_sectors(m.kind);
+  std::vector<int> sites = boundary_sites(m, 1);
+  hh_block* b = new_block(ctx, side, 1, st, sites);
+  std::vector<double2> h((size_t)b->d.size, make_double2(0.0, 0.0));
+  if (m.kind == HUBBARD) h[b->d.h_off[2]] = make_double2(m.U, 0.0);
+  const int nk = n_op_kinds(m.kind);
+  for (size_t si = 0; si < b->d.sites.size(); ++si)
+    for (int k = 0; k < nk; ++k)
+      for (auto& sub : b->d.lay[k].subs) {
+        cplx v = site_val(m.kind, k, sub.bra, sub.ket);
+        h[b->d.op_off[si][k] + sub.off] = make_double2(v.real(), v.imag());
+      }
+  check_hip(hipMemcpyAsync(b->arena.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+            "h2d");
+  check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  return b;
+}
+
+// ----------------------------------------------------------------------------
+// sweep driver: Get_Ground_State (code/quantum.f90:179-288)
+// ----------------------------------------------------------------------------
+struct Driver {
+  hh_ctx* ctx;
+  hh_schedule S;
+  std::map<int, std::unique_ptr<hh_block>> sysb, envb;
+  std::map<int, std::unique_ptr<hh_trunc>> syst, envt;
+  std::vector<hh_step_record> recs;
+  DevMem psi, psi2;
+  int64_t psi_dim = 0;
+  uint64_t step = 0;
+  // previous step (for the wave transformation)
+  std::unique_ptr<hh_superblock> prev_sb;
+  std::unique_ptr<hh_block> prev_hold_sys, prev_hold_env;
+
+  static constexpr double TRUN_ERR = 1.0000000180025095e-18;   // float32(1e-18) widened
+
+  int kmin(int kmax) const { return std::min(3 * kmax / 4, 3600); }
+
+  void record(int phase, hh_superblock* sb, double E, int it, const hh_trunc* t, double secs, double mvs, int64_t mv0,
+              double fl0) {
+    hh_step_record r{};
+    r.phase = phase;
+    r.sys_len = sb->sys->d.length;
+    r.env_len = sb->env->d.length;
+    r.iters = it;
+    r.num_state = t ? t->num_state : 0;
+    r.dim = sb->plan.sb.dim;
+    r.energy = E;
+    r.trunc_err = t ? t->err : 0.0;
+    r.entropy = t ? t->entropy : 0.0;
+    r.seconds = secs;
+    r.matvec_seconds = mvs;
+    r.matvec_flops = ctx->stats.flops_alg - fl0;
+    (void)mv0;
+    recs.push_back(r);
+  }
+
+  double2* psi_buf(int64_t n, DevMem& m) {
+    m.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
+    return m.c();
+  }
+
+  void warm_up() {
+    const ModelParams& m = ctx->model;
+    const int kmax = S.kept_max_warm > 0 ? S.kept_max_warm : S.kept_max;
+    const double lanzero = S.tol * S.warm_tol_factor;
+    const double terr = TRUN_ERR * 100;
+    sysb[1].reset(site_block(ctx, 0));
+    envb[1].reset(site_block(ctx, 1));
+    int label = 1;
+    while (label + 1 <= m.nsite() / 2) {
+      const double t0 = wall();
+      const int64_t mv0 = ctx->stats.matvecs;
+      const double fl0 = ctx->stats.flops_alg;
+      const int N = electrons_for_length(m, 2 * label + 2);
+      std::unique_ptr<hh_superblock> sb(bind(ctx, sysb[label].get(), envb[label].get(), N));
+      const int64_t n = sb->plan.sb.dim;
+      double2* p = psi_buf(n, psi);
+      dev_start_vector(p, n, S.seed + step, ctx->stream);
+      ++step;
+      double E, mvs;
+      int it;
+      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      std::unique_ptr<hh_trunc> ts, te;
+      if (sb->plan.sbs.total_dim() <= kmax) ts.reset(trunc_identity(ctx, sb.get(), 0));
+      else ts.reset(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
+      if (sb->plan.ebs.total_dim() <= kmax) te.reset(trunc_identity(ctx, sb.get(), 1));
+      else te.reset(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      std::unique_ptr<hh_block> ns(grow(ctx, sb.get(), 0, ts.get()));
+      std::unique_ptr<hh_block> ne(grow(ctx, sb.get(), 1, te.get()));
+      record(0, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
+      sysb[label + 1] = std::move(ns);
+      envb[label + 1] = std::move(ne);
+      syst[label + 1] = std::move(ts);
+      envt[label + 1] = std::move(te);
+      ++label;
+    }
+  }
+
+  // one finite-sweep step; returns the bound superblock (kept for the next
+  // step's wave transformation)
+  void sweep_lr(int bg, int end, double lanzero, double terr, int kmax, bool fresh, int phase) {
+    const ModelParams& m = ctx->model;
+    const int ns_ = m.nsite();
+    int label = bg;
+    bool first = true;
+    while (label + 1 <= end) {
+      const double t0 = wall();
+      const int64_t mv0 = ctx->stats.matvecs;
+      const double fl0 = ctx->stats.flops_alg;
+      hh_block* sb_sys = sysb.at(label).get();
+      hh_block* sb_env = envb.at(ns_ - label - 2).get();
+      std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
+      const int64_t n = sb->plan.sb.dim;
+      double2* p;
+      if (first) {
+        if (fresh || psi_dim != n) {
+          p = psi_buf(n, psi);
+          dev_start_vector(p, n, S.seed + step, ctx->stream);
+        } else {
+          p = psi.c();
+        }
+      } else {
+        double2* np = psi_buf(n, psi2);
+        wave_trans(ctx, 0, prev_sb.get(), psi.c(), syst.at(label).get(), envt.at(sb_env->d.length + 1).get(),
+                   sb.get(), np);
+        std::swap(psi.p, psi2.p);
+        std::swap(psi.bytes, psi2.bytes);
+        p = psi.c();
+      }
+      ++step;
+      double E, mvs;
+      int it;
+      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      psi_dim = n;
+      std::unique_ptr<hh_trunc> ts(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
+      std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 0, ts.get()));
+      record(phase, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
+      // the previous superblock references blocks that stay stored
+      prev_sb = std::move(sb);
+      sysb[label + 1] = std::move(nb);
+      syst[label + 1] = std::move(ts);
+      first = false;
+      ++label;
+    }
+  }
+
+  void sweep_rl(int bg, int end, double lanzero, double terr, int kmax, int phase) {
+    const ModelParams& m = ctx->model;
+    const int ns_ = m.nsite();
+    int label = bg;
+    bool first = true;
+    while (label - 1 >= end) {
+      const double t0 = wall();
+      const int64_t mv0 = ctx->stats.matvecs;
+      const double fl0 = ctx->stats.flops_alg;
+      hh_block* sb_sys = sysb.at(label - 1).get();
+      hh_block* sb_env = envb.at(ns_ - label - 1).get();
+      std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
+      const int64_t n = sb->plan.sb.dim;
+      double2* p;
+      if (first) {
+        if (psi_dim != n) {
+          p = psi_buf(n, psi);
+          dev_start_vector(p, n, S.seed + step, ctx->stream);
+        } else {
+          p = psi.c();
+        }
+      } else {
+        double2* np = psi_buf(n, psi2);
+        wave_trans(ctx, 1, prev_sb.get(), psi.c(), syst.at(sb_sys->d.length + 1).get(),
+                   envt.at(ns_ - label - 1).get(), sb.get(), np);
+        std::swap(psi.p, psi2.p);
+        std::swap(psi.bytes, psi2.bytes);
+        p = psi.c();
+      }
+      ++step;
+      double E, mvs;
+      int it;
+      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      psi_dim = n;
+      std::unique_ptr<hh_trunc> te(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 1, te.get()));
+      record(phase, sb.get(), E, it, te.get(), wall() - t0, mvs, mv0, fl0);
+      prev_sb = std::move(sb);
+      envb[ns_ - label] = std::move(nb);
+      envt[ns_ - label] = std::move(te);
+      first = false;
+      --label;
+    }
+  }
+
+  void run() {
+    const int n = ctx->model.nsite();
+    const int kw = S.kept_max_warm > 0 ? S.kept_max_warm : S.kept_max;
+    warm_up();
+    sweep_lr(n / 2 - 1, n - 2, S.tol * S.first_tol_factor, TRUN_ERR * 15, kw, true, 1);
+    prev_sb.reset();
+    sweep_rl(n - 2, 1, S.tol * S.first_tol_factor, TRUN_ERR * 15, kw, 1);
+    for (int k = 0; k < S.n_sweeps; ++k) {
+      prev_sb.reset();
+      sweep_lr(1, n - 2, S.tol, TRUN_ERR, S.kept_max, false, k + 2);
+      prev_sb.reset();
+      sweep_rl(n - 2, 1, S.tol, TRUN_ERR, S.kept_max, k + 2);
+    }
+    prev_sb.reset();
+  }
+};
+
+// message of a failed hh_ctx_create (the context no longer exists)
+std::string& create_error() {
+  static thread_local std::string s;
+  return s;
+}
+
+// C-ABI guard: exceptions -> status codes
+template <typename F>
+int guard(hh_ctx* ctx, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const HHError& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int hh_ctx_create(int device, hh_ctx** out) {
+  if (!out) return 1;
+  auto* c = new hh_ctx();
+  c->device = device;
+  int rc = guard(c, [&] {
+    int n = 0;
+    check_hip(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw HHError(-1, "no such HIP device");
+    check_hip(hipSetDevice(device), "hipSetDevice");
+    check_hip(hipStreamCreate(&c->stream), "hipStreamCreate");
+    if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw HHError(-2, "rocblas_create_handle");
+    rocblas_set_stream(c->rb, c->stream);
+    for (auto& e : c->ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+  });
+  if (rc != 0) {
+    create_error() = c->err;
+    hh_ctx_destroy(c);
+    *out = nullptr;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int hh_ctx_destroy(hh_ctx* ctx) {
+  if (!ctx) return 0;
+  for (auto* m : ctx->user_allocs) delete m;
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->rb) rocblas_destroy_handle(ctx->rb);
+  ctx->V.reset();
+  ctx->dots.reset();
+  ctx->partial.reset();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+const char* hh_last_error(const hh_ctx* ctx) { return ctx ? ctx->err.c_str() : create_error().c_str(); }
+
+int hh_set_model(hh_ctx* ctx, const hh_model_params* p) {
+  return guard(ctx, [&] {
+    need(p != nullptr, "null model");
+    need(p->kind == HUBBARD || p->kind == TJ, "model kind must be 0 (Hubbard) or 1 (t-J)");
+    need(p->Lx >= 2 && p->Ly >= 1, "lattice must have Lx >= 2, Ly >= 1");
+    need(p->n_elec >= 0 && p->n_elec <= 2 * p->Lx * p->Ly && p->n_elec % 2 == 0, "n_elec must be even (S = 0)");
+    need(!(p->lattice == TRIANGULAR && p->flux != 0.0), "flux on the triangular lattice is not supported");
+    ModelParams m;
+    m.kind = p->kind;
+    m.Lx = p->Lx;
+    m.Ly = p->Ly;
+    m.lattice = p->lattice;
+    m.periodic_y = p->periodic_y;
+    m.n_elec = p->n_elec;
+    m.t = p->t;
+    m.U = p->U;
+    m.J = p->J;
+    m.flux = p->flux;
+    ctx->model = m;
+    ctx->model_set = true;
+  });
+}
+
+int hh_dev_alloc(hh_ctx* ctx, int64_t bytes, void** dev) {
+  return guard(ctx, [&] {
+    need(dev && bytes >= 0, "bad allocation request");
+    check_hip(hipMalloc(dev, (size_t)std::max<int64_t>(bytes, 16)), "hipMalloc");
+  });
+}
+int hh_dev_free(hh_ctx* ctx, void* dev) {
+  return guard(ctx, [&] { check_hip(hipFree(dev), "hipFree"); });
+}
+int hh_memcpy_h2d(hh_ctx* ctx, void* dev, const void* host, int64_t bytes) {
+  return guard(ctx, [&] {
+    check_hip(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+int hh_memcpy_d2h(hh_ctx* ctx, void* host, const void* dev, int64_t bytes) {
+  return guard(ctx, [&] {
+    check_hip(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+
+int hh_block_site(hh_ctx* ctx, int side, hh_block** out) {
+  return guard(ctx, [&] {
+    need(ctx->model_set, "hh_set_model first");
+    need(side == 0 || side == 1, "side must be 0 or 1");
+    *out = site_block(ctx, side);
+  });
+}
+
+int hh_block_create(hh_ctx* ctx, int side, int length, int nsec, const int32_t* N, const int32_t* S2,
+                    const int32_t* dim, int nsites, const int32_t* sites, hh_block** out) {
+  return guard(ctx, [&] {
+    need(ctx->model_set, "hh_set_model first");
+    need(side == 0 || side == 1, "side must be 0 or 1");
+    need(nsec >= 0 && length >= 1, "bad block shape");
+    std::vector<Sector> sec;
+    for (int i = 0; i < nsec; ++i) {
+      need(dim[i] >= 0 && S2[i] >= 0 && (N[i] - S2[i]) % 2 == 0, "sector (N, 2S) parity / dim error");
+      sec.push_back({N[i], S2[i], dim[i]});
+    }
+    std::vector<int> st;
+    for (int i = 0; i < nsites; ++i) {
+      need(sites[i] >= 1 && sites[i] <= length, "boundary site outside the block");
+      st.push_back(sites[i]);
+    }
+    *out = new_block(ctx, side, length, sec, st);
+  });
+}
+
+int hh_block_destroy(hh_block* b) {
+  delete b;
+  return 0;
+}
+
+int hh_block_info(const hh_block* b, int64_t* arena, int32_t* nsec, int32_t* nsites, int32_t* length) {
+  if (!b) return 1;
+  if (arena) *arena = b->d.size;
+  if (nsec) *nsec = (int32_t)b->d.sec.size();
+  if (nsites) *nsites = (int32_t)b->d.sites.size();
+  if (length) *length = b->d.length;
+  return 0;
+}
+int hh_block_sectors(const hh_block* b, int32_t* N, int32_t* S2, int32_t* dim, int64_t* h_off) {
+  if (!b) return 1;
+  for (size_t i = 0; i < b->d.sec.size(); ++i) {
+    if (N) N[i] = b->d.sec[i].N;
+    if (S2) S2[i] = b->d.sec[i].S2;
+    if (dim) dim[i] = b->d.sec[i].dim;
+    if (h_off) h_off[i] = b->d.h_off[i];
+  }
+  return 0;
+}
+int hh_block_sites(const hh_block* b, int32_t* sites) {
+  if (!b) return 1;
+  for (size_t i = 0; i < b->d.sites.size(); ++i) sites[i] = b->d.sites[i];
+  return 0;
+}
+int hh_block_layout(const hh_block* b, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off) {
+  if (!b || kind < 0 || kind > 2) return 1;
+  const OpLayout& L = b->d.lay[kind];
+  if (nsub) *nsub = (int32_t)L.subs.size();
+  for (size_t i = 0; i < L.subs.size(); ++i) {
+    if (bra) bra[i] = L.subs[i].bra;
+    if (ket) ket[i] = L.subs[i].ket;
+    if (off) off[i] = L.subs[i].off;
+  }
+  return 0;
+}
+int hh_block_op_offset(const hh_block* b, int site_index, int kind, int64_t* off) {
+  if (!b || site_index < 0 || site_index >= (int)b->d.sites.size() || kind < 0 || kind > 2) return 1;
+  *off = b->d.op_off[site_index][kind];
+  return *off < 0 ? 1 : 0;
+}
+int hh_block_upload(hh_ctx* ctx, hh_block* b, const double* host) {
+  return guard(ctx, [&] {
+    check_hip(hipMemcpyAsync(b->arena.p, host, sizeof(double2) * b->d.size, hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+int hh_block_download(hh_ctx* ctx, const hh_block* b, double* host) {
+  return guard(ctx, [&] {
+    check_hip(hipMemcpyAsync(host, b->arena.p, sizeof(double2) * b->d.size, hipMemcpyDeviceToHost, ctx->stream),
+              "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+
+int hh_superblock_bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N, hh_superblock** out) {
+  return guard(ctx, [&] {
+    need(ctx->model_set, "hh_set_model first");
+    need(sys && env && out, "null argument");
+    *out = bind(ctx, sys, env, N);
+  });
+}
+int hh_superblock_destroy(hh_superblock* sb) {
+  delete sb;
+  return 0;
+}
+int hh_superblock_info(const hh_superblock* sb, int64_t* dim, int32_t* npairs, double* flops_alg,
+                       double* flops_exec) {
+  if (!sb) return 1;
+  if (dim) *dim = sb->plan.sb.dim;
+  if (npairs) *npairs = (int32_t)sb->plan.sb.pairs.size();
+  if (flops_alg) *flops_alg = sb->plan.stage2.flops_alg;
+  if (flops_exec) *flops_exec = sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
+  return 0;
+}
+int hh_superblock_pairs(const hh_superblock* sb, int32_t* sys_sec, int32_t* env_sec, int32_t* rows, int32_t* cols,
+                        int64_t* off) {
+  if (!sb) return 1;
+  const SuperBasis& S = sb->plan.sb;
+  for (size_t i = 0; i < S.pairs.size(); ++i) {
+    if (sys_sec) sys_sec[i] = S.pairs[i].first;
+    if (env_sec) env_sec[i] = S.pairs[i].second;
+    if (rows) rows[i] = S.rows[i];
+    if (cols) cols[i] = S.cols[i];
+    if (off) off[i] = S.off[i];
+  }
+  return 0;
+}
+int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim) {
+  if (!sb) return 1;
+  const EnlBasis& E = side == 0 ? sb->plan.sbs : sb->plan.ebs;
+  if (nsec) *nsec = (int32_t)E.sec.size();
+  for (size_t i = 0; i < E.sec.size(); ++i) {
+    if (N) N[i] = E.sec[i].N;
+    if (S2) S2[i] = E.sec[i].S2;
+    if (dim) dim[i] = E.sec[i].dim;
+  }
+  return 0;
+}
+
+int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in, double* psi_out) {
+  return guard(ctx, [&] {
+    need(psi_in && psi_out && psi_in != psi_out, "model_wave needs distinct input and output vectors");
+    matvec(ctx, sb, (const double2*)psi_in, (double2*)psi_out);
+    check_hip(hipStreamSynchronize(ctx->stream), "model_wave");
+  });
+}
+
+int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi, double lanzero, int max_iter, double* e0, int* iters) {
+  return guard(ctx, [&] {
+    need(psi && e0 && iters, "null argument");
+    lanczos(ctx, sb, (double2*)psi, lanzero, max_iter, e0, iters, nullptr);
+  });
+}
+
+int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi, int kept_min, int kept_max,
+                double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy) {
+  return guard(ctx, [&] {
+    need(side == 0 || side == 1, "side must be 0 or 1");
+    need(kept_min >= 0 && kept_max >= 1, "bad kept range");
+    hh_trunc* t = truncate(ctx, sb, side, (const double2*)psi, kept_min, kept_max, trun_err);
+    *out = t;
+    if (num_state) *num_state = t->num_state;
+    if (err) *err = t->err;
+    if (entropy) *entropy = t->entropy;
+  });
+}
+int hh_trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side, hh_trunc** out) {
+  return guard(ctx, [&] { *out = trunc_identity(ctx, sb, side); });
+}
+int hh_trunc_destroy(hh_trunc* t) {
+  delete t;
+  return 0;
+}
+int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* keep) {
+  if (!t) return 1;
+  if (nsec) *nsec = (int32_t)t->kept.size();
+  for (size_t i = 0; i < t->kept.size(); ++i) {
+    if (sector) sector[i] = t->kept[i].a;
+    if (keep) keep[i] = t->kept[i].keep;
+  }
+  return 0;
+}
+
+int hh_grow_block(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t, hh_block** out) {
+  return guard(ctx, [&] { *out = grow(ctx, sb, side, t); });
+}
+
+int hh_wave_trans(hh_ctx* ctx, int dir, hh_superblock* old_sb, const double* psi_old, const hh_trunc* sys_t,
+                  const hh_trunc* env_t, hh_superblock* new_sb, double* psi_new) {
+  return guard(ctx, [&] {
+    need(dir == 0 || dir == 1, "dir must be 0 (LR) or 1 (RL)");
+    wave_trans(ctx, dir, old_sb, (const double2*)psi_old, sys_t, env_t, new_sb, (double2*)psi_new);
+  });
+}
+
+int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_rec, int* nrec) {
+  return guard(ctx, [&] {
+    need(ctx->model_set, "hh_set_model first");
+    need(s && s->kept_max >= 1 && s->max_iter >= 2, "bad schedule");
+    need(s->precision == PREC_FP64, "only the FP64 form (precision 0) is available in hh_dmrg_run");
+    need(ctx->model.nsite() >= 4, "lattice too small for two-site DMRG");
+    Driver d;
+    d.ctx = ctx;
+    d.S = *s;
+    ctx->precision = s->precision;
+    d.run();
+    const int n = (int)std::min<size_t>(d.recs.size(), (size_t)std::max(max_rec, 0));
+    if (rec) std::memcpy(rec, d.recs.data(), sizeof(hh_step_record) * n);
+    if (nrec) *nrec = (int)d.recs.size();
+  });
+}
+
+int hh_set_profiling(hh_ctx* ctx, int profile) {
+  ctx->profile = profile != 0;
+  return 0;
+}
+int hh_get_stats(hh_ctx* ctx, hh_stats* out) {
+  if (!ctx || !out) return 1;
+  *out = ctx->stats;
+  return 0;
+}
+int hh_reset_stats(hh_ctx* ctx) {
+  if (!ctx) return 1;
+  ctx->stats = hh_stats{};
+  return 0;
+}
+
+int hh_superblock_tasks(const hh_superblock* sb, int stage, const void** tiles, int32_t* ntiles, const void** terms,
+                        int32_t* nterms, int64_t* scratch) {
+  if (!sb) return 1;
+  const TaskList* tl = stage == 0 ? &sb->plan.merge_tasks : stage == 1 ? &sb->plan.stage1 : &sb->plan.stage2;
+  if (tiles) *tiles = tl->tiles.data();
+  if (ntiles) *ntiles = (int32_t)tl->tiles.size();
+  if (terms) *terms = tl->terms.data();
+  if (nterms) *nterms = (int32_t)tl->terms.size();
+  if (scratch) *scratch = stage == 0 ? sb->plan.merged_size : sb->plan.x_size;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// host-only plans
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct hh_plan {
+  BlockDesc blk[3];
+  MatvecPlan mv;
+  GrowPlan gp;
+  bool is_grow = false;
+  std::string err;
+};
+
+namespace {
+ModelParams to_model(const hh_model_params* p) {
+  ModelParams m;
+  m.kind = p->kind;
+  m.Lx = p->Lx;
+  m.Ly = p->Ly;
+  m.lattice = p->lattice;
+  m.periodic_y = p->periodic_y;
+  m.n_elec = p->n_elec;
+  m.t = p->t;
+  m.U = p->U;
+  m.J = p->J;
+  m.flux = p->flux;
+  return m;
+}
+BlockDesc make_desc(const ModelParams& m, int side, int len, int nsec, const int32_t* N, const int32_t* S2,
+                    const int32_t* dim, int nsites, const int32_t* sites) {
+  BlockDesc d;
+  d.side = side;
+  d.length = len;
+  for (int i = 0; i < nsec; ++i) d.sec.push_back({N[i], S2[i], dim[i]});
+  for (int i = 0; i < nsites; ++i) d.sites.push_back(sites[i]);
+  layout_block(d, n_op_kinds(m.kind));
+  return d;
+}
+}  // namespace
+
+extern "C" {
+
+int hh_plan_matvec(const hh_model_params* mp, int sys_len, int sys_nsec, const int32_t* sys_N, const int32_t* sys_S2,
+                   const int32_t* sys_dim, int sys_nsites, const int32_t* sys_sites, int env_len, int env_nsec,
+                   const int32_t* env_N, const int32_t* env_S2, const int32_t* env_dim, int env_nsites,
+                   const int32_t* env_sites, int N, hh_plan** out) {
+  try {
+    ModelParams m = to_model(mp);
+    auto* p = new hh_plan();
+    p->blk[0] = make_desc(m, 0, sys_len, sys_nsec, sys_N, sys_S2, sys_dim, sys_nsites, sys_sites);
+    p->blk[1] = make_desc(m, 1, env_len, env_nsec, env_N, env_S2, env_dim, env_nsites, env_sites);
+    Geometry g = make_geometry(m, sys_len, env_len);
+    p->mv = compile_matvec(m, g, p->blk[0], p->blk[1], N);
+    *out = p;
+    return 0;
+  } catch (const std::exception& e) {
+    create_error() = e.what();
+    return 1;
+  }
+}
+
+int hh_plan_grow(const hh_model_params* mp, int side, int length, int nsec, const int32_t* Nq, const int32_t* S2q,
+                 const int32_t* dimq, int nsites, const int32_t* sites, int ns, int ne, int nkept,
+                 const int32_t* kept_a, const int32_t* kept_n, const int64_t* u_off, const int32_t* u_ld,
+                 hh_plan** out) {
+  try {
+    ModelParams m = to_model(mp);
+    auto* p = new hh_plan();
+    p->is_grow = true;
+    p->blk[side] = make_desc(m, side, length, nsec, Nq, S2q, dimq, nsites, sites);
+    Geometry g = make_geometry(m, ns, ne);
+    std::vector<KeptSector> kept;
+    for (int i = 0; i < nkept; ++i) kept.push_back({kept_a[i], kept_n[i], u_off[i], u_ld[i]});
+    p->gp = compile_grow(m, g, p->blk[side], kept);
+    p->blk[2] = p->gp.out;
+    *out = p;
+    return 0;
+  } catch (const std::exception& e) {
+    create_error() = e.what();
+    return 1;
+  }
+}
+
+int hh_plan_destroy(hh_plan* p) {
+  delete p;
+  return 0;
+}
+
+int hh_plan_tasks(const hh_plan* p, int stage, const void** tiles, int32_t* ntiles, const void** terms,
+                  int32_t* nterms, int64_t* scratch) {
+  if (!p || stage < 0 || stage > 2) return 1;
+  const TaskList* tl;
+  int64_t sc;
+  if (p->is_grow) {
+    tl = stage == 0 ? &p->gp.merge_tasks : stage == 1 ? &p->gp.stage1 : &p->gp.stage2;
+    sc = stage == 0 ? p->gp.merged_size : p->gp.x_size;
+  } else {
+    tl = stage == 0 ? &p->mv.merge_tasks : stage == 1 ? &p->mv.stage1 : &p->mv.stage2;
+    sc = stage == 0 ? p->mv.merged_size : p->mv.x_size;
+  }
+  if (tiles) *tiles = tl->tiles.data();
+  if (ntiles) *ntiles = (int32_t)tl->tiles.size();
+  if (terms) *terms = tl->terms.data();
+  if (nterms) *nterms = (int32_t)tl->terms.size();
+  if (scratch) *scratch = sc;
+  return 0;
+}
+
+int hh_plan_block(const hh_plan* p, int which, int64_t* arena, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim,
+                  int64_t* h_off, int32_t* nsites, int32_t* sites) {
+  if (!p || which < 0 || which > 2) return 1;
+  const BlockDesc& d = p->blk[which];
+  if (arena) *arena = d.size;
+  if (nsec) *nsec = (int32_t)d.sec.size();
+  for (size_t i = 0; i < d.sec.size(); ++i) {
+    if (N) N[i] = d.sec[i].N;
+    if (S2) S2[i] = d.sec[i].S2;
+    if (dim) dim[i] = d.sec[i].dim;
+    if (h_off) h_off[i] = d.h_off[i];
+  }
+  if (nsites) *nsites = (int32_t)d.sites.size();
+  if (sites)
+    for (size_t i = 0; i < d.sites.size(); ++i) sites[i] = d.sites[i];
+  return 0;
+}
+
+int hh_plan_layout(const hh_plan* p, int which, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off) {
+  if (!p || which < 0 || which > 2 || kind < 0 || kind > 2) return 1;
+  const OpLayout& L = p->blk[which].lay[kind];
+  if (nsub) *nsub = (int32_t)L.subs.size();
+  for (size_t i = 0; i < L.subs.size(); ++i) {
+    if (bra) bra[i] = L.subs[i].bra;
+    if (ket) ket[i] = L.subs[i].ket;
+    if (off) off[i] = L.subs[i].off;
+  }
+  return 0;
+}
+
+int hh_plan_op_offset(const hh_plan* p, int which, int site_index, int kind, int64_t* off) {
+  if (!p || which < 0 || which > 2) return 1;
+  const BlockDesc& d = p->blk[which];
+  if (site_index < 0 || site_index >= (int)d.sites.size() || kind < 0 || kind > 2) return 1;
+  *off = d.op_off[site_index][kind];
+  return *off < 0 ? 1 : 0;
+}
+
+int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,
+                  int32_t* cols, int64_t* off) {
+  if (!p || p->is_grow) return 1;
+  const SuperBasis& S = p->mv.sb;
+  if (dim) *dim = S.dim;
+  if (npairs) *npairs = (int32_t)S.pairs.size();
+  for (size_t i = 0; i < S.pairs.size(); ++i) {
+    if (sys_sec) sys_sec[i] = S.pairs[i].first;
+    if (env_sec) env_sec[i] = S.pairs[i].second;
+    if (rows) rows[i] = S.rows[i];
+    if (cols) cols[i] = S.cols[i];
+    if (off) off[i] = S.off[i];
+  }
+  return 0;
+}
+
+int hh_run_tasks(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
+                 void* const* bufs_dev, int precision) {
+  return guard(ctx, [&] {
+    need(ntiles >= 0 && nterms >= 0 && bufs_dev, "bad task list");
+    need(precision == PREC_FP64 || precision == PREC_FP32, "precision must be 0 or 1");
+    TaskList tl;
+    tl.tiles.assign((const Tile*)tiles, (const Tile*)tiles + ntiles);
+    tl.terms.assign((const Term*)terms, (const Term*)terms + nterms);
+    for (const Tile& t : tl.tiles) {
+      need(t.t0 >= 0 && t.t1 <= nterms && t.t0 <= t.t1 && t.M >= 0 && t.N >= 0, "tile out of range");
+      need(t.c_buf >= 0 && t.c_buf < 8 && bufs_dev[t.c_buf], "tile buffer slot");
+      for (int k = t.t0; k < t.t1; ++k) {
+        const Term& m = tl.terms[k];
+        need(m.a_buf < 8 && bufs_dev[m.a_buf] && m.a_op <= 3 && m.b_op <= 3 && m.K >= 0, "term operand");
+        if (m.K > 0) need(m.b_buf < 8 && bufs_dev[m.b_buf], "term operand");
+      }
+    }
+    DevTasks dt;
+    dt.upload(tl, ctx->stream);
+    BufPtrs b;
+    for (int i = 0; i < 8; ++i) b.p[i] = (double2*)bufs_dev[i];
+    launch_tiles(dt, b, ctx->stream, precision);
+    check_hip(hipStreamSynchronize(ctx->stream), "hh_run_tasks");
+  });
+}
+
+int hh_start_vector(hh_ctx* ctx, double* psi, int64_t n, uint64_t seed) {
+  return guard(ctx, [&] {
+    dev_start_vector((double2*)psi, n, seed, ctx->stream);
+    check_hip(hipStreamSynchronize(ctx->stream), "start");
+    // normalise (the host-side stream is normalised the same way)
+    ensure_ws(ctx, 1, 1);
+    const double n2 = rdot(ctx, (double2*)psi, (double2*)psi, n);
+    dev_scale((double2*)psi, 1.0 / std::sqrt(n2), n, ctx->stream);
+    check_hip(hipStreamSynchronize(ctx->stream), "start");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+"""Host-side mirror of the reference inner-loop interface, over the C ABI.
+
+Each method names the Fortran subroutine it stands in for
+(cfengno1/Hofstadter_Hubbard_TSC, paths relative to the reference tree).
+Device vectors are torch complex128 CUDA tensors (HIP on ROCm); all compute
+runs in libhhdmrg.so on the context's stream.  There is no CPU fallback: every
+call raises when the library or the GPU is missing.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+class HHError(RuntimeError):
+    pass
+
+
+def _check(ctx_ptr, rc, what):
+    if rc != 0:
+        lib = L.load()
+        msg = lib.hh_last_error(ctx_ptr)
+        raise HHError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def _dptr(t):
+    """Device address of a contiguous torch tensor."""
+    if not t.is_cuda or not t.is_contiguous():
+        raise HHError("expected a contiguous device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+class Block:
+    """A device-resident block (Total_Model, code/pubdata.f90:167-181)."""
+
+    def __init__(self, ctx, ptr):
+        self.ctx = ctx
+        self.ptr = C.c_void_p(ptr) if not isinstance(ptr, C.c_void_p) else ptr
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().hh_block_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def info(self):
+        lib = L.load()
+        a, ns, nst, ln = L.i64(), L.i32(), L.i32(), L.i32()
+        lib.hh_block_info(self.ptr, C.byref(a), C.byref(ns), C.byref(nst), C.byref(ln))
+        return a.value, ns.value, nst.value, ln.value
+
+    def sectors(self):
+        lib = L.load()
+        _, ns, _, _ = self.info()
+        N, S2, d = (np.zeros(ns, np.int32) for _ in range(3))
+        h = np.zeros(ns, np.int64)
+        lib.hh_block_sectors(self.ptr, L.ip(N), L.ip(S2), L.ip(d), L.ip(h))
+        return N, S2, d, h
+
+    def download(self):
+        lib = L.load()
+        a, _, _, _ = self.info()
+        out = np.zeros(max(a, 1), np.complex128)
+        _check(self.ctx.ptr, lib.hh_block_download(self.ctx.ptr, self.ptr, out.ctypes.data_as(C.c_void_p)),
+               "hh_block_download")
+        return out[:a]
+
+    def upload(self, arena):
+        lib = L.load()
+        arena = np.ascontiguousarray(arena, dtype=np.complex128)
+        _check(self.ctx.ptr, lib.hh_block_upload(self.ctx.ptr, self.ptr, arena.ctypes.data_as(C.c_void_p)),
+               "hh_block_upload")
+
+
+class Superblock:
+    """hmerge + compiled model_wave tasks (code/mat_vec.f90:3476-3852)."""
+
+    def __init__(self, ctx, ptr, sys, env):
+        self.ctx = ctx
+        self.ptr = ptr
+        self._keep = (sys, env)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().hh_superblock_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def info(self):
+        lib = L.load()
+        d, n, fa, fe = L.i64(), L.i32(), L.f64(), L.f64()
+        lib.hh_superblock_info(self.ptr, C.byref(d), C.byref(n), C.byref(fa), C.byref(fe))
+        return {"dim": d.value, "npairs": n.value, "flops_alg": fa.value, "flops_exec": fe.value}
+
+    @property
+    def dim(self):
+        return self.info()["dim"]
+
+    def pairs(self):
+        lib = L.load()
+        n = self.info()["npairs"]
+        s, e, r, c = (np.zeros(n, np.int32) for _ in range(4))
+        o = np.zeros(n, np.int64)
+        lib.hh_superblock_pairs(self.ptr, L.ip(s), L.ip(e), L.ip(r), L.ip(c), L.ip(o))
+        return s, e, r, c, o
+
+    def enlarged(self, side):
+        lib = L.load()
+        n = L.i32()
+        lib.hh_superblock_enlarged(self.ptr, side, C.byref(n), None, None, None)
+        N, S2, d = (np.zeros(n.value, np.int32) for _ in range(3))
+        lib.hh_superblock_enlarged(self.ptr, side, C.byref(n), L.ip(N), L.ip(S2), L.ip(d))
+        return N, S2, d
+
+
+class Trunc:
+    """Truncation operator (Get_truncation_operator, code/get_den.f90:184-428)."""
+
+    def __init__(self, ctx, ptr, num_state=0, err=0.0, entropy=0.0):
+        self.ctx = ctx
+        self.ptr = ptr
+        self.num_state = num_state
+        self.err = err
+        self.entropy = entropy
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().hh_trunc_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def kept(self):
+        lib = L.load()
+        n = L.i32()
+        lib.hh_trunc_kept(self.ptr, C.byref(n), None, None)
+        a, k = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        lib.hh_trunc_kept(self.ptr, C.byref(n), L.ip(a), L.ip(k))
+        return a, k
+
+
+@dataclass
+class StepRecord:
+    phase: int
+    sys_len: int
+    env_len: int
+    iters: int
+    num_state: int
+    dim: int
+    energy: float
+    trunc_err: float
+    entropy: float
+    seconds: float
+    matvec_seconds: float
+    matvec_flops: float
+
+
+class Context:
+    """One GPU, one stream (hh_ctx)."""
+
+    def __init__(self, device=0):
+        lib = L.load()
+        p = C.c_void_p()
+        rc = lib.hh_ctx_create(device, C.byref(p))
+        if rc != 0:
+            raise HHError("hh_ctx_create failed (%d): %s" % (rc, lib.hh_last_error(None).decode()))
+        self.ptr = p
+
+    def close(self):
+        if self.ptr:
+            L.load().hh_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rc(self, rc, what):
+        _check(self.ptr, rc, what)
+
+    # -- model (code/quantum.f90:93-160, code/warm_up.f90:347-666) ---------------
+    def set_model(self, model):
+        self.model = model
+        mp = L.model_params(model)
+        self._rc(L.load().hh_set_model(self.ptr, C.byref(mp)), "hh_set_model")
+
+    # -- blocks --------------------------------------------------------------
+    def site_block(self, side):
+        """Get_hamiltonian_site (code/warm_up.f90:1208-1244)."""
+        p = C.c_void_p()
+        self._rc(L.load().hh_block_site(self.ptr, side, C.byref(p)), "hh_block_site")
+        return Block(self, p)
+
+    def create_block(self, side, length, sectors, sites):
+        N = np.array([q[0] for q in sectors], np.int32)
+        S2 = np.array([q[1] for q in sectors], np.int32)
+        d = np.array([q[2] for q in sectors], np.int32)
+        st = np.array(sites, np.int32)
+        p = C.c_void_p()
+        self._rc(L.load().hh_block_create(self.ptr, side, length, len(N), L.ip(N), L.ip(S2), L.ip(d), len(st),
+                                          L.ip(st), C.byref(p)), "hh_block_create")
+        return Block(self, p)
+
+    def layout(self, block, kind):
+        lib = L.load()
+        n = L.i32()
+        lib.hh_block_layout(block.ptr, kind, C.byref(n), None, None, None)
+        bra, ket = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        off = np.zeros(n.value, np.int64)
+        lib.hh_block_layout(block.ptr, kind, C.byref(n), L.ip(bra), L.ip(ket), L.ip(off))
+        return bra, ket, off
+
+    def op_offset(self, block, site_index, kind):
+        o = L.i64()
+        rc = L.load().hh_block_op_offset(block.ptr, site_index, kind, C.byref(o))
+        return o.value if rc == 0 else -1
+
+    # -- superblock ----------------------------------------------------------
+    def bind(self, sys, env, N):
+        """hmerge + compile model_wave (code/mat_vec.f90:3623-3852)."""
+        p = C.c_void_p()
+        self._rc(L.load().hh_superblock_bind(self.ptr, sys.ptr, env.ptr, N, C.byref(p)), "hh_superblock_bind")
+        return Superblock(self, p, sys, env)
+
+    def model_wave(self, sb, psi_in, psi_out):
+        """model_wave (code/mat_vec.f90:3476): psi_out = H psi_in."""
+        _sync()
+        self._rc(L.load().hh_model_wave(self.ptr, sb.ptr, _dptr(psi_in), _dptr(psi_out)), "hh_model_wave")
+
+    def lanczos(self, sb, psi, lanzero, max_iter=100):
+        """lanczos_memo_new (code/lanczos.f90:4-176), levels=1; psi updated in place."""
+        _sync()
+        e, it = L.f64(), L.i32()
+        self._rc(L.load().hh_lanczos(self.ptr, sb.ptr, _dptr(psi), lanzero, max_iter, C.byref(e), C.byref(it)),
+                 "hh_lanczos")
+        return e.value, it.value
+
+    def truncate(self, sb, side, psi, kept_min, kept_max, trun_err):
+        """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90)."""
+        _sync()
+        p = C.c_void_p()
+        ns, er, en = L.i32(), L.f64(), L.f64()
+        self._rc(L.load().hh_truncate(self.ptr, sb.ptr, side, _dptr(psi), kept_min, kept_max, trun_err, C.byref(p),
+                                      C.byref(ns), C.byref(er), C.byref(en)), "hh_truncate")
+        return Trunc(self, p, ns.value, er.value, en.value)
+
+    def trunc_identity(self, sb, side):
+        p = C.c_void_p()
+        self._rc(L.load().hh_trunc_identity(self.ptr, sb.ptr, side, C.byref(p)), "hh_trunc_identity")
+        return Trunc(self, p)
+
+    def grow(self, sb, side, trunc):
+        """Get_Hamiltonian_Sys/Env + update_trun_* (code/get_ham.f90:4-659)."""
+        p = C.c_void_p()
+        self._rc(L.load().hh_grow_block(self.ptr, sb.ptr, side, trunc.ptr, C.byref(p)), "hh_grow_block")
+        return Block(self, p)
+
+    def wave_trans(self, direction, old_sb, psi_old, sys_t, env_t, new_sb, psi_new):
+        """wave_trans_LR (direction 0) / wave_trans_RL (1), code/warm_up.f90:1855-2170."""
+        _sync()
+        self._rc(L.load().hh_wave_trans(self.ptr, direction, old_sb.ptr, _dptr(psi_old), sys_t.ptr, env_t.ptr,
+                                        new_sb.ptr, _dptr(psi_new)), "hh_wave_trans")
+
+    def start_vector(self, psi, seed):
+        _sync()
+        self._rc(L.load().hh_start_vector(self.ptr, _dptr(psi), psi.numel(), seed), "hh_start_vector")
+
+    # -- whole run (Get_Ground_State, code/quantum.f90:179-288) ---------------
+    def dmrg_run(self, kept_max, n_sweeps, tol, kept_max_warm=0, max_iter=100, seed=1234,
+                 warm_tol_factor=1000.0, first_tol_factor=15.0, precision=0):
+        s = L.Schedule(kept_max, kept_max_warm, n_sweeps, max_iter, tol, warm_tol_factor, first_tol_factor, seed,
+                       precision, 0)
+        cap = 4096
+        recs = (L.StepRecord * cap)()
+        n = L.i32()
+        self._rc(L.load().hh_dmrg_run(self.ptr, C.byref(s), recs, cap, C.byref(n)), "hh_dmrg_run")
+        out = []
+        for r in recs[:min(n.value, cap)]:
+            out.append(StepRecord(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy,
+                                  r.trunc_err, r.entropy, r.seconds, r.matvec_seconds, r.matvec_flops))
+        return out
+
+    # -- profiling -----------------------------------------------------------
+    def set_profiling(self, on):
+        L.load().hh_set_profiling(self.ptr, 1 if on else 0)
+
+    def stats(self):
+        s = L.Stats()
+        L.load().hh_get_stats(self.ptr, C.byref(s))
+        return {f: getattr(s, f) for f, _ in L.Stats._fields_}
+
+    def reset_stats(self):
+        L.load().hh_reset_stats(self.ptr)

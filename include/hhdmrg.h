@@ -1,0 +1,222 @@
+/*
+ * hhdmrg.h -- C ABI of the MI355X-native SU(2) DMRG inner loop.
+ *
+ * Drop-in boundary for the inner loop of cfengno1/Hofstadter_Hubbard_TSC
+ * (Fortran).  Every entry point names the reference subroutine it replaces
+ * (paths relative to the reference tree).  The arguments are plain pointers
+ * and sizes; complex arrays are interleaved (re, im) doubles, column-major,
+ * as Fortran's double complex.  "dev" pointers are device (HBM) addresses on
+ * the context's GPU; everything else is host memory borrowed for the call.
+ *
+ * Return codes: 0 ok, > 0 quantum-number / shape / argument error,
+ * < 0 HIP or rocSOLVER error.  hh_last_error() gives the message.  No entry
+ * point aborts the process (the reference stops on QN mismatches,
+ * code/mat_vec.f90:987-990).
+ *
+ * Threading: one context per host thread; calls on a context serialise on
+ * its HIP stream.  There is no global mutable state.
+ */
+#ifndef HHDMRG_H
+#define HHDMRG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hh_ctx hh_ctx;
+typedef struct hh_block hh_block;
+typedef struct hh_superblock hh_superblock;
+typedef struct hh_trunc hh_trunc;
+
+/* Model and lattice: replaces the hard-coded couplings and lattice setup of
+ * program Quantum (code/quantum.f90:93-160), dmrg_setup
+ * (code/warm_up.f90:347-666) and get_phase (code/quantum.f90:293-438; here the
+ * Landau gauge x bonds real, y bonds t*exp(i 2 pi flux x)). */
+typedef struct {
+  int32_t kind;        /* 0 Hubbard, 1 t-J */
+  int32_t Lx, Ly;
+  int32_t lattice;     /* 0 square, 1 triangular */
+  int32_t periodic_y;
+  int32_t n_elec;
+  double t, U, J, flux;
+} hh_model_params;
+
+/* Sweep schedule of Get_Ground_State (code/quantum.f90:179-288). */
+typedef struct {
+  int32_t kept_max;          /* finite-sweep kept_max */
+  int32_t kept_max_warm;     /* warm-up / first sweep kept_max (0: kept_max) */
+  int32_t n_sweeps;
+  int32_t max_iter;          /* Lanczos vectors (reference: 100) */
+  double tol;                /* lanerr of the finite sweeps */
+  double warm_tol_factor;    /* reference: 1000 */
+  double first_tol_factor;   /* reference: 15 */
+  uint64_t seed;
+  int32_t precision;         /* 0 FP64 MFMA (plain form) */
+  int32_t reserved;
+} hh_schedule;
+
+/* One DMRG step (a line of Energy_All.dat + Num_State.dat + Entropy.dat,
+ * code/lanczos.f90:170-174, code/get_den.f90:336-360). */
+typedef struct {
+  int32_t phase;             /* 0 warm-up, 1 first sweep, k+1 sweep k */
+  int32_t sys_len, env_len;
+  int32_t iters;
+  int32_t num_state;
+  int32_t pad;
+  int64_t dim;               /* superblock dimension (multiplet space) */
+  double energy;
+  double trunc_err;
+  double entropy;
+  double seconds;            /* wall time of the step */
+  double matvec_seconds;     /* wall time inside H.psi */
+  double matvec_flops;       /* reference-order algorithmic flops of H.psi */
+} hh_step_record;
+
+/* ---- context ---------------------------------------------------------- */
+int hh_ctx_create(int device, hh_ctx** out);
+int hh_ctx_destroy(hh_ctx* ctx);
+const char* hh_last_error(const hh_ctx* ctx);
+/* couplings / lattice (code/quantum.f90:93-160, code/warm_up.f90:347-666) */
+int hh_set_model(hh_ctx* ctx, const hh_model_params* p);
+
+/* ---- device memory helpers for callers without their own allocator ----- */
+int hh_dev_alloc(hh_ctx* ctx, int64_t bytes, void** dev);
+int hh_dev_free(hh_ctx* ctx, void* dev);
+int hh_memcpy_h2d(hh_ctx* ctx, void* dev, const void* host, int64_t bytes);
+int hh_memcpy_d2h(hh_ctx* ctx, void* host, const void* dev, int64_t bytes);
+
+/* ---- blocks (Total_Model, code/pubdata.f90:167-181) --------------------- */
+/* Single-site block with H = U*double (Get_hamiltonian_site,
+ * code/warm_up.f90:1208-1244 and warm_up_point :1301-1325). side 0 sys, 1 env. */
+int hh_block_site(hh_ctx* ctx, int side, hh_block** out);
+/* Empty block with the given sectors (N, 2S, dim) and boundary sites (block
+ * index 1..length); the caller uploads H and the operators
+ * (model_from_disk, code/space_info.f90:1569-1619). */
+int hh_block_create(hh_ctx* ctx, int side, int length, int nsec, const int32_t* N, const int32_t* S2,
+                    const int32_t* dim, int nsites, const int32_t* sites, hh_block** out);
+int hh_block_destroy(hh_block* b);
+/* sizes: arena length (complex elements), sector count, boundary-site count */
+int hh_block_info(const hh_block* b, int64_t* arena, int32_t* nsec, int32_t* nsites, int32_t* length);
+int hh_block_sectors(const hh_block* b, int32_t* N, int32_t* S2, int32_t* dim, int64_t* h_off);
+int hh_block_sites(const hh_block* b, int32_t* sites);
+/* layout of operator kind (0 c^dag, 1 S, 2 n): sub-blocks (bra, ket, offset);
+ * pass NULL arrays to query nsub.  op_off[site][kind] offsets via hh_block_op_offset. */
+int hh_block_layout(const hh_block* b, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off);
+int hh_block_op_offset(const hh_block* b, int site_index, int kind, int64_t* off);
+int hh_block_upload(hh_ctx* ctx, hh_block* b, const double* host);
+int hh_block_download(hh_ctx* ctx, const hh_block* b, double* host);
+
+/* ---- superblock: hmerge + task compilation (code/mat_vec.f90:3623-3852) -- */
+/* Binds sys and env (borrowed until destroy) at total N and S = 0; builds the
+ * enlarged bases (Get_basis, code/update.f90:11-112), the super basis
+ * (Get_super_basis, :118-171), the merged boundary operators and the task
+ * lists of H.psi, and uploads them.  Geometry = dmrg_setup(sys len, env len). */
+int hh_superblock_bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N, hh_superblock** out);
+int hh_superblock_destroy(hh_superblock* sb);
+/* psi length (complex) and number of (sys, env) sector pairs */
+int hh_superblock_info(const hh_superblock* sb, int64_t* dim, int32_t* npairs, double* flops_alg,
+                       double* flops_exec);
+/* per pair: sys enlarged sector, env enlarged sector, rows, cols, offset */
+int hh_superblock_pairs(const hh_superblock* sb, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,
+                        int32_t* cols, int64_t* off);
+/* enlarged basis of one side: sectors (N, 2S, dim) */
+int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim);
+
+/* H.psi: replaces model_wave(sys, env, sys_bs, env_bs, invec, outvec)
+ * (code/mat_vec.f90:3476); psi_out is overwritten. */
+int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in_dev, double* psi_out_dev);
+
+/* Lanczos: replaces lanczos_memo_new (code/lanczos.f90:4-176) with levels=1:
+ * psi_dev holds the start vector on entry and the ground state on exit. */
+int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi_dev, double lanzero, int max_iter, double* e0,
+               int* iters);
+
+/* Reduced density matrix + Get_truncation_operator
+ * (code/get_den.f90:4-99, :184-428): side 0 sys (psi psi^dag), 1 env. */
+int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, int kept_min, int kept_max,
+                double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy);
+/* identity truncation of the warm-up (code/warm_up.f90:1421-1432) */
+int hh_trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side, hh_trunc** out);
+int hh_trunc_destroy(hh_trunc* t);
+/* kept multiplets per enlarged sector (enlarged-basis order of that side) */
+int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* keep);
+
+/* Block growth + renormalisation: replaces Get_Hamiltonian_Sys/Env
+ * (code/get_ham.f90:4-659) and update_trun_dia/ndia (code/update.f90:1387-1605):
+ * the side's block of sb grown by its centre site and truncated with t. */
+int hh_grow_block(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t, hh_block** out);
+
+/* psi prediction between steps: replaces wave_trans_LR / wave_trans_RL
+ * (code/warm_up.f90:1855-2170).  dir 0 = LR, 1 = RL.  sys_t / env_t are the
+ * truncations of the old sys (LR: this step's; RL: the one that built the old
+ * sys block) and of the env (LR: the one that built the old env block; RL:
+ * this step's).  psi_new_dev is overwritten and normalised. */
+int hh_wave_trans(hh_ctx* ctx, int dir, hh_superblock* old_sb, const double* psi_old_dev, const hh_trunc* sys_t,
+                  const hh_trunc* env_t, hh_superblock* new_sb, double* psi_new_dev);
+
+/* Whole ground-state search with the reference schedule (Get_Ground_State,
+ * code/quantum.f90:179-288): warm-up, centre-out half sweep, n_sweeps sweeps.
+ * Fills up to max_rec records; *nrec = steps done. */
+int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_rec, int* nrec);
+
+/* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
+typedef struct {
+  int64_t matvecs;           /* H.psi calls since the last reset */
+  double matvec_ms;          /* wall time of those calls */
+  double stage1_ms;          /* kernel time of the stage-1 (A.psi) launches */
+  double stage2_ms;          /* kernel time of the stage-2 (all terms) launches */
+  double flops_alg;          /* reference-order algorithmic flops of those calls */
+  double flops_exec;         /* flops the engine executed */
+  double bytes_alg;          /* compulsory bytes: psi in/out + operator sub-blocks */
+} hh_stats;
+/* profile = 1 records HIP events around every H.psi launch (adds a sync). */
+int hh_set_profiling(hh_ctx* ctx, int profile);
+int hh_get_stats(hh_ctx* ctx, hh_stats* out);
+int hh_reset_stats(hh_ctx* ctx);
+
+/* ---- host-logic inspection (no GPU work) -------------------------------- */
+/* Task lists of a bound superblock: stage 0 merges, 1 stage-1 products,
+ * 2 H.psi terms.  Arrays as in the engine: tiles (c_off, c_ld, M, N, t0, t1,
+ * c_buf, accumulate) and terms (see hh_core.hpp).  Pointers stay valid until
+ * the superblock is destroyed. */
+int hh_superblock_tasks(const hh_superblock* sb, int stage, const void** tiles, int32_t* ntiles,
+                        const void** terms, int32_t* nterms, int64_t* scratch);
+/* Host-only plans: the same task compilers without a device, for checking
+ * the host logic.  Blocks are described by sectors (N, 2S, dim) and boundary
+ * sites; which = 0 sys / 1 env / 2 grown block. */
+typedef struct hh_plan hh_plan;
+int hh_plan_matvec(const hh_model_params* m, int sys_len, int sys_nsec, const int32_t* sys_N, const int32_t* sys_S2,
+                   const int32_t* sys_dim, int sys_nsites, const int32_t* sys_sites, int env_len, int env_nsec,
+                   const int32_t* env_N, const int32_t* env_S2, const int32_t* env_dim, int env_nsites,
+                   const int32_t* env_sites, int N, hh_plan** out);
+/* Growth plan of one block (side 0/1) in the geometry (ns, ne) with kept
+ * sectors (enlarged index, keep, U offset, U leading dimension). */
+int hh_plan_grow(const hh_model_params* m, int side, int length, int nsec, const int32_t* Nq, const int32_t* S2q,
+                 const int32_t* dimq, int nsites, const int32_t* sites, int ns, int ne, int nkept,
+                 const int32_t* kept_a, const int32_t* kept_n, const int64_t* u_off, const int32_t* u_ld,
+                 hh_plan** out);
+int hh_plan_destroy(hh_plan* p);
+int hh_plan_tasks(const hh_plan* p, int stage, const void** tiles, int32_t* ntiles, const void** terms,
+                  int32_t* nterms, int64_t* scratch);
+int hh_plan_block(const hh_plan* p, int which, int64_t* arena, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim,
+                  int64_t* h_off, int32_t* nsites, int32_t* sites);
+int hh_plan_layout(const hh_plan* p, int which, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off);
+int hh_plan_op_offset(const hh_plan* p, int which, int site_index, int kind, int64_t* off);
+int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,
+                  int32_t* cols, int64_t* off);
+
+/* Run one tile list on the device: bufs_dev[8] are the buffer slots the
+ * terms address (hh_core.hpp: Buf); precision 0 FP64, 1 FP32.  The engine
+ * entry point behind every task list above, exposed for direct use. */
+int hh_run_tasks(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
+                 void* const* bufs_dev, int precision);
+
+/* Counter-based start vector (splitmix64 stream) written to psi_dev. */
+int hh_start_vector(hh_ctx* ctx, double* psi_dev, int64_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HHDMRG_H */
