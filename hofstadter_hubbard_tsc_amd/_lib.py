@@ -35,7 +35,8 @@ class StepRecord(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("matvecs", i64), ("matvec_ms", f64), ("stage1_ms", f64), ("stage2_ms", f64),
-                ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64)]
+                ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64), ("stage1_flops_alg", f64),
+                ("stage2_flops_alg", f64)]
 
 
 # numpy views of the engine's task structs (hh_core.hpp: Term, Tile)
@@ -79,6 +80,13 @@ SIGNATURES = [
     ("hh_grow_block", i32, [vp, vp, i32, vp, P(vp)]),
     ("hh_wave_trans", i32, [vp, i32, vp, vp, vp, vp, vp, vp]),
     ("hh_dmrg_run", i32, [vp, P(Schedule), P(StepRecord), i32, P(i32)]),
+    ("hh_driver_create", i32, [vp, P(Schedule), P(vp)]),
+    ("hh_driver_destroy", i32, [vp]),
+    ("hh_driver_warmup", i32, [vp, P(StepRecord), i32, P(i32)]),
+    ("hh_driver_sweep", i32, [vp, i32, i32, i32, f64, f64, i32, i32, i32, P(StepRecord), i32, P(i32)]),
+    ("hh_driver_reset_transform", i32, [vp]),
+    ("hh_driver_block_count", i32, [vp, i32, P(i32)]),
+    ("hh_driver_block", i32, [vp, i32, i32, P(vp)]),
     ("hh_set_profiling", i32, [vp, i32]),
     ("hh_get_stats", i32, [vp, P(Stats)]),
     ("hh_reset_stats", i32, [vp]),

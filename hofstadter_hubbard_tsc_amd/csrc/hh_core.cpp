@@ -561,15 +561,17 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
                     } else {
                       xo = itx->second;
                     }
+                    // reference association (mat_vec.f90:1749-1770): first GEMM
+                    // counted on stage 1, second on stage 2
                     if (!adj) {
                       out.add(ko, mk_term(coef, B_X, xo + (int64_t)cs.pos * xm, xm, OP_NN, Qe.buf, b_off, b_ld, b_op, me));
-                      P.stage2.flops_alg += 8.0 * ((double)mbo * me * mb + (double)mbo * meo * me);
-                      P.stage2.flops_exec += 8.0 * mbo * meo * me;
+                      P.stage1.flops_alg += 8.0 * (double)mbo * me * mb;
+                      P.stage2.flops_alg += 8.0 * (double)mbo * meo * me;
                     } else {
                       out.add(ki, mk_term(std::conj(coef), B_X, xo + (int64_t)cpos_out * xm, xm, OP_NN, Qe.buf, b_off,
                                           b_ld, adj_op(b_op), meo));
-                      P.stage2.flops_alg += 8.0 * ((double)mb * meo * mbo + (double)mb * me * meo);
-                      P.stage2.flops_exec += 8.0 * mb * me * meo;
+                      P.stage1.flops_alg += 8.0 * (double)mb * meo * mbo;
+                      P.stage2.flops_alg += 8.0 * (double)mb * me * meo;
                     }
                   }
                 } else if (Ps.block) {

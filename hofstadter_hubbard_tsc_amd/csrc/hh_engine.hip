@@ -78,17 +78,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Acc;
+// Operand layouts of both 16x16x4 MFMAs: lane l holds A[l%16][l/16] and
+// B[l/16][l%16].  Output element v of lane l sits in column l%16 and row
+// 4*(l/16)+v for f32 but row (l/16)+4*v for f64 (checked on gfx950).
 template <> struct Acc<double> {
   typedef d4 V;
   static __device__ __forceinline__ V mfma(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 template <> struct Acc<float> {
   typedef f4 V;
   static __device__ __forceinline__ V mfma(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
 __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t idx, bool conj) {
@@ -100,7 +105,7 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // ----------------------------------------------------------------------------
 // grouped tile kernel
 // ----------------------------------------------------------------------------
-template <typename T>
+template <typename T, int TAG>
 __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, BufPtrs bufs) {
   typedef typename Acc<T>::V V;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int r = wr + a * 16 + lk * 4 + v, c = wc + b * 16 + li;
+            const int r = wr + a * 16 + Acc<T>::row(lk, v), c = wc + b * 16 + li;
             if (r < Mv && c < Nv) {
               double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
               exR[a][b][v] += tm.cr * x.x - tm.ci * x.y;
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = wr + a * 16 + lk * 4 + v, c = wc + b * 16 + li;
+        const int r = wr + a * 16 + Acc<T>::row(lk, v), c = wc + b * 16 + li;
         if (r < Mv && c < Nv) {
           double re = (double)accR[a][b][v], im = (double)accI[a][b][v];
           if (has_ex) {
@@ -246,12 +251,18 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
       }
 }
 
-void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision) {
+template <typename T>
+static void launch_t(const DevTasks& t, const BufPtrs& b, hipStream_t s, int tag) {
+  const dim3 g(t.nwork), blk(256);
+  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
+  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
+  else hipLaunchKernelGGL((k_tiles<T, 0>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
+}
+
+void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag) {
   if (t.nwork == 0) return;
-  if (precision == PREC_FP32)
-    hipLaunchKernelGGL(k_tiles<float>, dim3(t.nwork), dim3(256), 0, s, t.tiles, t.terms, t.work, b);
-  else
-    hipLaunchKernelGGL(k_tiles<double>, dim3(t.nwork), dim3(256), 0, s, t.tiles, t.terms, t.work, b);
+  if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
+  else launch_t<double>(t, b, s, tag);
   check_hip(hipGetLastError(), "k_tiles launch");
 }
 

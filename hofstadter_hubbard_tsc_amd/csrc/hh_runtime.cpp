@@ -228,9 +228,9 @@ void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
   BufPtrs b = bufs_of((double2*)in, out, sb->sys, sb->env, sb->X.c(), sb->merged.c(), nullptr, nullptr);
   const double t0 = ctx->profile ? wall() : 0.0;
   if (ctx->profile) check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
-  launch_tiles(sb->st1, b, ctx->stream, ctx->precision);
+  launch_tiles(sb->st1, b, ctx->stream, ctx->precision, 1);
   if (ctx->profile) check_hip(hipEventRecord(ctx->ev[1], ctx->stream), "event");
-  launch_tiles(sb->st2, b, ctx->stream, ctx->precision);
+  launch_tiles(sb->st2, b, ctx->stream, ctx->precision, 2);
   if (ctx->profile) {
     check_hip(hipEventRecord(ctx->ev[2], ctx->stream), "event");
     check_hip(hipEventSynchronize(ctx->ev[2]), "event sync");
@@ -242,7 +242,9 @@ void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
     ctx->stats.matvec_ms += (wall() - t0) * 1e3;
   }
   ctx->stats.matvecs += 1;
-  ctx->stats.flops_alg += sb->plan.stage2.flops_alg;
+  ctx->stats.flops_alg += sb->plan.stage1.flops_alg + sb->plan.stage2.flops_alg;
+  ctx->stats.stage1_flops_alg += sb->plan.stage1.flops_alg;
+  ctx->stats.stage2_flops_alg += sb->plan.stage2.flops_alg;
   ctx->stats.flops_exec += sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
   ctx->stats.bytes_alg += sb->plan.bytes_alg;
 }
@@ -639,7 +641,7 @@ struct Driver {
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
       const int64_t n = sb->plan.sb.dim;
       double2* p;
-      if (first) {
+      if (first && !prev_sb) {
         if (fresh || psi_dim != n) {
           p = psi_buf(n, psi);
           dev_start_vector(p, n, S.seed + step, ctx->stream);
@@ -685,7 +687,7 @@ struct Driver {
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
       const int64_t n = sb->plan.sb.dim;
       double2* p;
-      if (first) {
+      if (first && !prev_sb) {
         if (psi_dim != n) {
           p = psi_buf(n, psi);
           dev_start_vector(p, n, S.seed + step, ctx->stream);
@@ -949,7 +951,7 @@ int hh_superblock_info(const hh_superblock* sb, int64_t* dim, int32_t* npairs, d
   if (!sb) return 1;
   if (dim) *dim = sb->plan.sb.dim;
   if (npairs) *npairs = (int32_t)sb->plan.sb.pairs.size();
-  if (flops_alg) *flops_alg = sb->plan.stage2.flops_alg;
+  if (flops_alg) *flops_alg = sb->plan.stage1.flops_alg + sb->plan.stage2.flops_alg;
   if (flops_exec) *flops_exec = sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
   return 0;
 }
@@ -1049,6 +1051,80 @@ int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_
     if (rec) std::memcpy(rec, d.recs.data(), sizeof(hh_step_record) * n);
     if (nrec) *nrec = (int)d.recs.size();
   });
+}
+
+}  // extern "C"
+
+struct hh_driver {
+  Driver d;
+  size_t taken = 0;
+};
+
+extern "C" {
+
+int hh_driver_create(hh_ctx* ctx, const hh_schedule* s, hh_driver** out) {
+  return guard(ctx, [&] {
+    need(ctx->model_set, "hh_set_model first");
+    need(s && s->kept_max >= 1 && s->max_iter >= 2, "bad schedule");
+    need(s->precision == PREC_FP64, "only the FP64 form (precision 0) is available");
+    auto* h = new hh_driver();
+    h->d.ctx = ctx;
+    h->d.S = *s;
+    *out = h;
+  });
+}
+
+int hh_driver_destroy(hh_driver* h) {
+  delete h;
+  return 0;
+}
+
+static void take_records(hh_driver* h, hh_step_record* rec, int max_rec, int* nrec) {
+  const size_t n = h->d.recs.size() - h->taken;
+  const size_t c = std::min<size_t>(n, (size_t)std::max(max_rec, 0));
+  if (rec && c) std::memcpy(rec, h->d.recs.data() + h->taken, sizeof(hh_step_record) * c);
+  if (nrec) *nrec = (int)n;
+  h->taken = h->d.recs.size();
+}
+
+int hh_driver_warmup(hh_driver* h, hh_step_record* rec, int max_rec, int* nrec) {
+  return guard(h->d.ctx, [&] {
+    h->d.ctx->precision = h->d.S.precision;
+    h->d.warm_up();
+    take_records(h, rec, max_rec, nrec);
+  });
+}
+
+int hh_driver_sweep(hh_driver* h, int dir, int bg, int end, double lanzero, double trun_err, int kept_max, int fresh,
+                    int phase, hh_step_record* rec, int max_rec, int* nrec) {
+  return guard(h->d.ctx, [&] {
+    const int n = h->d.ctx->model.nsite();
+    need(dir == 0 || dir == 1, "dir must be 0 (LR) or 1 (RL)");
+    need(bg >= 1 && bg <= n - 1 && end >= 1 && end <= n - 1, "sweep bounds outside the lattice");
+    if (dir == 0) h->d.sweep_lr(bg, end, lanzero, trun_err, kept_max, fresh != 0, phase);
+    else h->d.sweep_rl(bg, end, lanzero, trun_err, kept_max, phase);
+    take_records(h, rec, max_rec, nrec);
+  });
+}
+
+int hh_driver_reset_transform(hh_driver* h) {
+  h->d.prev_sb.reset();
+  return 0;
+}
+
+int hh_driver_block_count(const hh_driver* h, int side, int32_t* count) {
+  if (!h || !count) return 1;
+  *count = (int32_t)(side == 0 ? h->d.sysb.size() : h->d.envb.size());
+  return 0;
+}
+
+int hh_driver_block(const hh_driver* h, int side, int length, const hh_block** out) {
+  if (!h || !out) return 1;
+  const auto& m = side == 0 ? h->d.sysb : h->d.envb;
+  auto it = m.find(length);
+  if (it == m.end()) return 1;
+  *out = it->second.get();
+  return 0;
 }
 
 int hh_set_profiling(hh_ctx* ctx, int profile) {

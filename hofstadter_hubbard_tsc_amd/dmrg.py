@@ -43,10 +43,11 @@ class Block:
     def __init__(self, ctx, ptr):
         self.ctx = ctx
         self.ptr = C.c_void_p(ptr) if not isinstance(ptr, C.c_void_p) else ptr
+        self.ptr_owned = True
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and self.ptr_owned:
                 L.load().hh_block_destroy(self.ptr)
         except Exception:
             pass
@@ -56,6 +57,13 @@ class Block:
         a, ns, nst, ln = L.i64(), L.i32(), L.i32(), L.i32()
         lib.hh_block_info(self.ptr, C.byref(a), C.byref(ns), C.byref(nst), C.byref(ln))
         return a.value, ns.value, nst.value, ln.value
+
+    def sites(self):
+        lib = L.load()
+        _, _, nst, _ = self.info()
+        out = np.zeros(nst, np.int32)
+        lib.hh_block_sites(self.ptr, L.ip(out))
+        return out
 
     def sectors(self):
         lib = L.load()
@@ -302,3 +310,55 @@ class Context:
 
     def reset_stats(self):
         L.load().hh_reset_stats(self.ptr)
+
+
+class Driver:
+    """Step-wise sweep driver (hh_driver): warm_up_point, left_to_right and
+    right_to_left of code/warm_up.f90:1284-1849, one call at a time, with all
+    blocks and truncations resident on the device."""
+
+    def __init__(self, ctx, kept_max, tol, kept_max_warm=0, max_iter=100, seed=1234, warm_tol_factor=1000.0,
+                 first_tol_factor=15.0, precision=0):
+        self.ctx = ctx
+        s = L.Schedule(kept_max, kept_max_warm, 0, max_iter, tol, warm_tol_factor, first_tol_factor, seed,
+                       precision, 0)
+        p = C.c_void_p()
+        ctx._rc(L.load().hh_driver_create(ctx.ptr, C.byref(s), C.byref(p)), "hh_driver_create")
+        self.ptr = p
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().hh_driver_destroy(self.ptr)
+        except Exception:
+            pass
+
+    @staticmethod
+    def _recs(buf, n):
+        return [StepRecord(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy, r.trunc_err,
+                           r.entropy, r.seconds, r.matvec_seconds, r.matvec_flops) for r in buf[:n]]
+
+    def warmup(self):
+        cap = 4096
+        buf = (L.StepRecord * cap)()
+        n = L.i32()
+        self.ctx._rc(L.load().hh_driver_warmup(self.ptr, buf, cap, C.byref(n)), "hh_driver_warmup")
+        return self._recs(buf, min(n.value, cap))
+
+    def sweep(self, direction, bg, end, lanzero, trun_err, kept_max, fresh=False, phase=1):
+        cap = 4096
+        buf = (L.StepRecord * cap)()
+        n = L.i32()
+        self.ctx._rc(L.load().hh_driver_sweep(self.ptr, direction, bg, end, lanzero, trun_err, kept_max,
+                                              1 if fresh else 0, phase, buf, cap, C.byref(n)), "hh_driver_sweep")
+        return self._recs(buf, min(n.value, cap))
+
+    def block(self, side, length):
+        """Borrowed handle of a stored block (do not destroy)."""
+        p = C.c_void_p()
+        rc = L.load().hh_driver_block(self.ptr, side, length, C.byref(p))
+        if rc != 0:
+            raise HHError("no stored block of length %d" % length)
+        b = Block(self.ctx, p)
+        b.ptr_owned = False
+        return b

@@ -161,6 +161,25 @@ int hh_wave_trans(hh_ctx* ctx, int dir, hh_superblock* old_sb, const double* psi
  * Fills up to max_rec records; *nrec = steps done. */
 int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_rec, int* nrec);
 
+/* ---- step-wise driver: the sweep routines one call at a time ----------- */
+typedef struct hh_driver hh_driver;
+int hh_driver_create(hh_ctx* ctx, const hh_schedule* s, hh_driver** out);
+int hh_driver_destroy(hh_driver* d);
+/* warm_up_point(1, Nsite/2) (code/warm_up.f90:1284-1491) */
+int hh_driver_warmup(hh_driver* d, hh_step_record* rec, int max_rec, int* nrec);
+/* left_to_right (dir 0, code/warm_up.f90:1495-1671) or right_to_left (dir 1,
+ * :1675-1849) from label bg to end; fresh = 1 starts the first step from a
+ * new random vector instead of the carried / transformed one.  Blocks and
+ * truncations stay resident on the device between calls. */
+int hh_driver_sweep(hh_driver* d, int dir, int bg, int end, double lanzero, double trun_err, int kept_max, int fresh,
+                    int phase, hh_step_record* rec, int max_rec, int* nrec);
+/* forget the previous superblock (next first step does not transform) */
+int hh_driver_reset_transform(hh_driver* d);
+int hh_driver_block_count(const hh_driver* d, int side, int32_t* count);
+/* borrowed handle of a stored block (side 0 sys / 1 env, by length); valid
+ * until the driver replaces or frees it */
+int hh_driver_block(const hh_driver* d, int side, int length, const hh_block** out);
+
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {
   int64_t matvecs;           /* H.psi calls since the last reset */
@@ -170,6 +189,8 @@ typedef struct {
   double flops_alg;          /* reference-order algorithmic flops of those calls */
   double flops_exec;         /* flops the engine executed */
   double bytes_alg;          /* compulsory bytes: psi in/out + operator sub-blocks */
+  double stage1_flops_alg;   /* part of flops_alg done by the stage-1 launches */
+  double stage2_flops_alg;   /* part of flops_alg done by the stage-2 launches */
 } hh_stats;
 /* profile = 1 records HIP events around every H.psi launch (adds a sync). */
 int hh_set_profiling(hh_ctx* ctx, int profile);
