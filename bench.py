@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the SU(2) DMRG inner loop is measured on a single MI355X "
@@ -124,8 +125,8 @@ def main():
     drv = Driver(ctx, kept_max=args.m, tol=args.tol, kept_max_warm=args.m, seed=20240601)
     t_prep = time.time()
     warm = drv.warmup()
-    print("# warm-up: %d steps, %.1f s, E0 = %.10f" % (len(warm), time.time() - t_prep, warm[-1].energy),
-          file=sys.stderr, flush=True)
+    print("# warm-up: %d steps, %.1f s, %d Lanczos vectors, E0 = %.10f"
+          % (len(warm), time.time() - t_prep, sum(r.iters for r in warm), warm[-1].energy), file=sys.stderr, flush=True)
     bg = n // 2 - 1
     kmin = min(3 * args.m // 4, 3600)
     if args.warmup > 0:
@@ -167,6 +168,16 @@ def main():
             "kernel": "k_tiles<double, 2> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
             "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1)}
+    if args.dump_structure and recs:
+        ls = recs[-1].sys_len + 1
+        dump = {}
+        for side, length in ((0, ls), (1, n - ls - 2)):
+            b = drv.block(side, length)
+            N_, S2_, d_, _ = b.sectors()
+            dump["sys" if side == 0 else "env"] = {"length": length, "N": N_.tolist(), "S2": S2_.tolist(),
+                                                   "dim": d_.tolist(), "sites": b.sites().tolist()}
+        with open(args.dump_structure, "w") as f:
+            json.dump(dump, f)
     cpu = None
     if not args.no_cpu:
         try:

@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <stdexcept>
 
+#include <cmath>
+
 #include "hh_engine.hpp"
 
 namespace hh {
@@ -20,18 +22,25 @@ void check_hip(hipError_t e, const char* what) {
 }
 
 // ----------------------------------------------------------------------------
-// task upload
+// task upload: 32x32 macro tiles, one per wave; long term lists are split
+// over several waves and reduced in a fixed order
 // ----------------------------------------------------------------------------
-static constexpr int BM = 64, BN = 64, KC = 16, LPAD = 16;
+static constexpr int MT = 32;          // macro tile edge (one wave)
+static constexpr int WAVES = 4;        // independent waves per workgroup
+static constexpr int TARGET_ITEMS = 8192;
 
 void DevTasks::release() {
   if (tiles) (void)hipFree(tiles);
   if (terms) (void)hipFree(terms);
   if (work) (void)hipFree(work);
+  if (red) (void)hipFree(red);
+  if (partial) (void)hipFree(partial);
   tiles = nullptr;
   terms = nullptr;
   work = nullptr;
-  ntiles = nterms = nwork = 0;
+  red = nullptr;
+  partial = nullptr;
+  ntiles = nterms = nwork = nred = nslots = 0;
 }
 
 void DevTasks::upload(const TaskList& tl, hipStream_t s) {
@@ -40,22 +49,61 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
   nterms = (int)tl.terms.size();
   flops_alg = tl.flops_alg;
   flops_exec = tl.flops_exec;
-  std::vector<WorkItem> w;
-  // largest tiles first: long work items start early (simple LPT ordering)
-  std::vector<std::pair<double, int>> cost;
+  // work per macro tile = sum of max(K, 1) over its terms
+  double total = 0.0;
+  std::vector<double> tw(ntiles, 0.0);
   for (int i = 0; i < ntiles; ++i) {
     const Tile& t = tl.tiles[i];
-    double c = 1.0;
+    double c = 0.0;
     for (int k = t.t0; k < t.t1; ++k) c += (double)std::max(tl.terms[k].K, 1);
-    cost.push_back({-c * std::min(t.M, BM) * std::min(t.N, BN), i});
+    tw[i] = c;
+    total += c * (double)((t.M + MT - 1) / MT) * (double)((t.N + MT - 1) / MT);
   }
-  std::stable_sort(cost.begin(), cost.end());
-  for (auto& ci : cost) {
-    const Tile& t = tl.tiles[ci.second];
-    for (int r = 0; r < t.M; r += BM)
-      for (int c = 0; c < t.N; c += BN) w.push_back({ci.second, r, c, 0});
+  const double target = std::max(64.0, total / TARGET_ITEMS);
+  std::vector<std::pair<double, WorkItem>> items;
+  std::vector<ReduceItem> reds;
+  int slot = 0;
+  for (int i = 0; i < ntiles; ++i) {
+    const Tile& t = tl.tiles[i];
+    // contiguous term groups of roughly equal work
+    std::vector<int> cuts{t.t0};
+    const int groups = std::max(1, std::min(32, (int)std::ceil(tw[i] / target)));
+    if (groups > 1) {
+      double acc = 0.0;
+      const double per = tw[i] / groups;
+      for (int k = t.t0; k < t.t1; ++k) {
+        acc += (double)std::max(tl.terms[k].K, 1);
+        if (acc >= per * (double)cuts.size() && k + 1 < t.t1 && (int)cuts.size() < groups) cuts.push_back(k + 1);
+      }
+    }
+    cuts.push_back(t.t1);
+    const int g = (int)cuts.size() - 1;
+    for (int r = 0; r < t.M; r += MT)
+      for (int c = 0; c < t.N; c += MT) {
+        if (g == 1) {
+          items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}});
+        } else {
+          for (int q = 0; q < g; ++q) {
+            double wq = 0.0;
+            for (int k = cuts[q]; k < cuts[q + 1]; ++k) wq += (double)std::max(tl.terms[k].K, 1);
+            items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}});
+          }
+          reds.push_back(ReduceItem{i, r, c, slot, g, 0});
+          slot += g;
+        }
+      }
   }
+  // longest work items first (LPT)
+  std::stable_sort(items.begin(), items.end(),
+                   [](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
+                     return a.first > b.first;
+                   });
+  std::vector<WorkItem> w;
+  w.reserve(items.size());
+  for (auto& it : items) w.push_back(it.second);
   nwork = (int)w.size();
+  nred = (int)reds.size();
+  nslots = slot;
   if (ntiles) {
     check_hip(hipMalloc(&tiles, sizeof(Tile) * ntiles), "hipMalloc tiles");
     check_hip(hipMemcpyAsync(tiles, tl.tiles.data(), sizeof(Tile) * ntiles, hipMemcpyHostToDevice, s), "copy tiles");
@@ -67,6 +115,11 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
   if (nwork) {
     check_hip(hipMalloc(&work, sizeof(WorkItem) * nwork), "hipMalloc work");
     check_hip(hipMemcpyAsync(work, w.data(), sizeof(WorkItem) * nwork, hipMemcpyHostToDevice, s), "copy work");
+  }
+  if (nred) {
+    check_hip(hipMalloc(&red, sizeof(ReduceItem) * nred), "hipMalloc red");
+    check_hip(hipMemcpyAsync(red, reds.data(), sizeof(ReduceItem) * nred, hipMemcpyHostToDevice, s), "copy red");
+    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * MT * MT), "hipMalloc partial");
   }
   check_hip(hipStreamSynchronize(s), "task upload");
 }
@@ -103,22 +156,23 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 }
 
 // ----------------------------------------------------------------------------
-// grouped tile kernel
+// grouped tile kernel: every wave owns one 32x32 macro tile and streams the
+// MFMA operands straight from HBM / L2 (lane l loads A[l%16][k] and
+// B[k][l%16] for its k = l/16 of each k-step), so waves never wait on each
+// other: no LDS, no barriers.
 // ----------------------------------------------------------------------------
 template <typename T, int TAG>
-__global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
-                                               const WorkItem* __restrict__ work, BufPtrs bufs) {
+__global__ __launch_bounds__(256, 3) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
+                                               const WorkItem* __restrict__ work, int nwork, BufPtrs bufs,
+                                               double2* __restrict__ partial) {
   typedef typename Acc<T>::V V;
-  __shared__ T Lr[KC][BM + LPAD], Li[KC][BM + LPAD];
-  __shared__ T Rr[KC][BN + LPAD], Ri[KC][BN + LPAD];
-
-  const WorkItem w = work[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = blockIdx.x * WAVES + wave;
+  if (wi >= nwork) return;
+  const WorkItem w = work[wi];
   const Tile tile = tiles[w.tile];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int r0 = w.r0, c0 = w.c0;
-  const int Mv = min(BM, tile.M - r0), Nv = min(BN, tile.N - c0);
+  const int Mv = min(MT, tile.M - r0), Nv = min(MT, tile.N - c0);
   const int li = lane & 15, lk = lane >> 4;
 
   V accR[2][2], accI[2][2];
@@ -129,21 +183,11 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
       accR[a][b] = V{0, 0, 0, 0};
       accI[a][b] = V{0, 0, 0, 0};
     }
-  // elementwise (K == 0) contributions accumulate in double
-  double exR[2][2][4], exI[2][2][4];
-  bool has_ex = false;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) exR[a][b][v] = exI[a][b][v] = 0.0;
-
-  for (int ti = tile.t0; ti < tile.t1; ++ti) {
+  for (int ti = w.t0; ti < w.t1; ++ti) {
     const Term tm = terms[ti];
     const double2* __restrict__ A = bufs.p[tm.a_buf];
     if (tm.K == 0) {
-      has_ex = true;
+      // elementwise term, added straight into the accumulators
       const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -151,11 +195,11 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int r = wr + a * 16 + Acc<T>::row(lk, v), c = wc + b * 16 + li;
+            const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
             if (r < Mv && c < Nv) {
               double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
-              exR[a][b][v] += tm.cr * x.x - tm.ci * x.y;
-              exI[a][b][v] += tm.cr * x.y + tm.ci * x.x;
+              accR[a][b][v] += (T)(tm.cr * x.x - tm.ci * x.y);
+              accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
             }
           }
       continue;
@@ -165,67 +209,60 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
     const bool a_conj = (tm.a_op == OP_C || tm.a_op == OP_R);
     const bool b_kfast = (tm.b_op == OP_NN || tm.b_op == OP_R);
     const bool b_conj = (tm.b_op == OP_C || tm.b_op == OP_R);
-    for (int kb = 0; kb < tm.K; kb += KC) {
-      // stage op(A) * coef  (BM x KC)
-#pragma unroll
-      for (int j = 0; j < (BM * KC) / 256; ++j) {
-        const int idx = tid + 256 * j;
-        int r, k;
-        if (a_rowfast) { r = idx % BM; k = idx / BM; }
-        else { k = idx % KC; r = idx / KC; }
-        double2 x = make_double2(0.0, 0.0);
-        if (r < Mv && kb + k < tm.K) {
-          const int64_t o = a_rowfast ? tm.a_off + (int64_t)(r0 + r) + (int64_t)(kb + k) * tm.a_ld
-                                      : tm.a_off + (int64_t)(kb + k) + (int64_t)(r0 + r) * tm.a_ld;
-          x = ld_op(A, o, a_conj);
-        }
-        Lr[k][r] = (T)(tm.cr * x.x - tm.ci * x.y);
-        Li[k][r] = (T)(tm.cr * x.y + tm.ci * x.x);
+    // per-lane element strides: A(r, k) and B(k, c)
+    const int64_t a_rs = a_rowfast ? 1 : tm.a_ld, a_ks = a_rowfast ? tm.a_ld : 1;
+    const int64_t b_ks = b_kfast ? 1 : tm.b_ld, b_cs = b_kfast ? tm.b_ld : 1;
+    const bool rv0 = li < Mv, rv1 = 16 + li < Mv;
+    const bool cv0 = li < Nv, cv1 = 16 + li < Nv;
+    const int64_t a0 = tm.a_off + (int64_t)(r0 + li) * a_rs;
+    const int64_t a1 = a0 + 16 * a_rs;
+    const int64_t b0 = tm.b_off + (int64_t)(c0 + li) * b_cs;
+    const int64_t b1 = b0 + 16 * b_cs;
+    // operands of k-step kb are loaded one step ahead of their MFMAs
+    double2 na0 = make_double2(0.0, 0.0), na1 = na0, nb0 = na0, nb1 = na0;
+    {
+      const int k = lk;
+      const bool kv = k < tm.K;
+      if (kv && rv0) na0 = ld_op(A, a0 + k * a_ks, a_conj);
+      if (kv && rv1) na1 = ld_op(A, a1 + k * a_ks, a_conj);
+      if (kv && cv0) nb0 = ld_op(B, b0 + k * b_ks, b_conj);
+      if (kv && cv1) nb1 = ld_op(B, b1 + k * b_ks, b_conj);
+    }
+    for (int kb = 0; kb < tm.K; kb += 4) {
+      const double2 xa0 = na0, xa1 = na1, xb0 = nb0, xb1 = nb1;
+      {
+        const int k = kb + 4 + lk;
+        const bool kv = k < tm.K;
+        na0 = make_double2(0.0, 0.0);
+        na1 = na0;
+        nb0 = na0;
+        nb1 = na0;
+        if (kv && rv0) na0 = ld_op(A, a0 + k * a_ks, a_conj);
+        if (kv && rv1) na1 = ld_op(A, a1 + k * a_ks, a_conj);
+        if (kv && cv0) nb0 = ld_op(B, b0 + k * b_ks, b_conj);
+        if (kv && cv1) nb1 = ld_op(B, b1 + k * b_ks, b_conj);
       }
-      // stage op(B)  (KC x BN)
+      T ar[2], ai[2], br[2], bi[2];
+      ar[0] = (T)(tm.cr * xa0.x - tm.ci * xa0.y);
+      ai[0] = (T)(tm.cr * xa0.y + tm.ci * xa0.x);
+      ar[1] = (T)(tm.cr * xa1.x - tm.ci * xa1.y);
+      ai[1] = (T)(tm.cr * xa1.y + tm.ci * xa1.x);
+      br[0] = (T)xb0.x;
+      bi[0] = (T)xb0.y;
+      br[1] = (T)xb1.x;
+      bi[1] = (T)xb1.y;
 #pragma unroll
-      for (int j = 0; j < (BN * KC) / 256; ++j) {
-        const int idx = tid + 256 * j;
-        int k, c;
-        if (b_kfast) { k = idx % KC; c = idx / KC; }
-        else { c = idx % BN; k = idx / BN; }
-        double2 x = make_double2(0.0, 0.0);
-        if (c < Nv && kb + k < tm.K) {
-          const int64_t o = b_kfast ? tm.b_off + (int64_t)(kb + k) + (int64_t)(c0 + c) * tm.b_ld
-                                    : tm.b_off + (int64_t)(c0 + c) + (int64_t)(kb + k) * tm.b_ld;
-          x = ld_op(B, o, b_conj);
-        }
-        Rr[k][c] = (T)x.x;
-        Ri[k][c] = (T)x.y;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < KC; kk += 4) {
-        T ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          ar[a] = Lr[kk + lk][wr + a * 16 + li];
-          ai[a] = Li[kk + lk][wr + a * 16 + li];
-        }
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          br[b] = Rr[kk + lk][wc + b * 16 + li];
-          bi[b] = Ri[kk + lk][wc + b * 16 + li];
+          accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
+          accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
+          accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
+          accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
         }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
-            accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
-            accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
-            accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
-          }
-      }
-      __syncthreads();
     }
   }
-  // write the macro tile once
+  // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -233,30 +270,56 @@ __global__ __launch_bounds__(256) void k_tiles(const Tile* __restrict__ tiles, c
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = wr + a * 16 + Acc<T>::row(lk, v), c = wc + b * 16 + li;
+        const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
         if (r < Mv && c < Nv) {
           double re = (double)accR[a][b][v], im = (double)accI[a][b][v];
-          if (has_ex) {
-            re += exR[a][b][v];
-            im += exI[a][b][v];
+          if (w.slot >= 0) {
+            partial[(int64_t)w.slot * MT * MT + r + c * MT] = make_double2(re, im);
+          } else {
+            const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
+            if (tile.accumulate) {
+              double2 old = C[o];
+              re += old.x;
+              im += old.y;
+            }
+            C[o] = make_double2(re, im);
           }
-          const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
-          if (tile.accumulate) {
-            double2 old = C[o];
-            re += old.x;
-            im += old.y;
-          }
-          C[o] = make_double2(re, im);
         }
       }
 }
 
+// C (+)= sum over the slots of a split macro tile, slot order fixed
+__global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, const ReduceItem* __restrict__ red,
+                                                BufPtrs bufs, const double2* __restrict__ partial) {
+  const ReduceItem ri = red[blockIdx.x];
+  const Tile tile = tiles[ri.tile];
+  const int Mv = min(MT, tile.M - ri.r0), Nv = min(MT, tile.N - ri.c0);
+  double2* __restrict__ C = bufs.p[tile.c_buf];
+  for (int e = threadIdx.x; e < MT * MT; e += 256) {
+    const int r = e % MT, c = e / MT;
+    if (r >= Mv || c >= Nv) continue;
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < ri.nslot; ++q) {
+      const double2 x = partial[(int64_t)(ri.slot0 + q) * MT * MT + e];
+      re += x.x;
+      im += x.y;
+    }
+    const int64_t o = tile.c_off + (int64_t)(ri.r0 + r) + (int64_t)(ri.c0 + c) * tile.c_ld;
+    if (tile.accumulate) {
+      const double2 old = C[o];
+      re += old.x;
+      im += old.y;
+    }
+    C[o] = make_double2(re, im);
+  }
+}
+
 template <typename T>
 static void launch_t(const DevTasks& t, const BufPtrs& b, hipStream_t s, int tag) {
-  const dim3 g(t.nwork), blk(256);
-  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
-  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
-  else hipLaunchKernelGGL((k_tiles<T, 0>), g, blk, 0, s, t.tiles, t.terms, t.work, b);
+  const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
+  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else hipLaunchKernelGGL((k_tiles<T, 0>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
 }
 
 void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag) {
@@ -264,6 +327,10 @@ void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precis
   if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
   else launch_t<double>(t, b, s, tag);
   check_hip(hipGetLastError(), "k_tiles launch");
+  if (t.nred) {
+    hipLaunchKernelGGL(k_reduce, dim3(t.nred), dim3(256), 0, s, t.tiles, t.red, b, t.partial);
+    check_hip(hipGetLastError(), "k_reduce launch");
+  }
 }
 
 // ----------------------------------------------------------------------------

@@ -12,14 +12,21 @@
 namespace hh {
 
 struct BufPtrs { double2* p[8]; };
-struct WorkItem { int32_t tile, r0, c0, pad; };
+// One wave's share of the work: a 32x32 macro tile of output region `tile`
+// and the term range [t0, t1); slot >= 0 sends the partial sum to the
+// partial buffer (tiles whose term list is split across several waves).
+struct WorkItem { int32_t tile, r0, c0, t0, t1, slot; };
+// Deterministic reduction of split macro tiles: C = sum of nslot partials.
+struct ReduceItem { int32_t tile, r0, c0, slot0, nslot, pad; };
 
 // Tile list resident on the device, launched as one grouped kernel.
 struct DevTasks {
   Tile* tiles = nullptr;
   Term* terms = nullptr;
   WorkItem* work = nullptr;
-  int ntiles = 0, nterms = 0, nwork = 0;
+  ReduceItem* red = nullptr;
+  double2* partial = nullptr;
+  int ntiles = 0, nterms = 0, nwork = 0, nred = 0, nslots = 0;
   double flops_alg = 0.0, flops_exec = 0.0;
   void upload(const TaskList& tl, hipStream_t s);
   void release();
