@@ -106,6 +106,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the SU(2) DMRG inner loop is measured on a single MI355X "
@@ -116,6 +119,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
+        return
+    os.environ.setdefault("HHDMRG_VERBOSE", "1")   # progress line per DMRG step on stderr
     from hofstadter_hubbard_tsc_amd import Context, Driver
 
     model = workload(args.m)
@@ -153,21 +166,22 @@ def main():
     mv_time = sum(r.matvec_seconds for r in recs)
     mv_flops = sum(r.matvec_flops for r in recs)
     gflops = mv_flops / mv_time / 1e9 if mv_time > 0 else 0.0
-    # dominant kernel: H.psi stage 2 (k_tiles<double, 2>), HIP events on the
-    # library stream around every launch
+    # dominant kernel: H.psi stage 2, launched as k_tiles<double, 4> while
+    # profiling is on; HIP events on the library stream around every launch
     s2_launches = st["matvecs"]
     s2_avg_ms = st["stage2_ms"] / max(s2_launches, 1)
     s2_tflops = st["stage2_flops_alg"] / max(st["stage2_ms"], 1e-30) / 1e9
     s1_tflops = st["stage1_flops_alg"] / max(st["stage1_ms"], 1e-30) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_stage2.json")
+    tpath = os.path.join(ROOT, "profiles", "hpsi_stage2_tj_cyl32x4_m%d_traffic.json" % args.m)
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "k_tiles<double, 2> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
+            "kernel": "hh::k_tiles<double, 4> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
-            "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1)}
+            "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1),
+            "reduce_ms_per_matvec": st["reduce_ms"] / max(s2_launches, 1)}
     if args.dump_structure and recs:
         ls = recs[-1].sys_len + 1
         dump = {}

@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhhdmrg.so")
+LIB_PATH = os.environ.get("HHDMRG_LIB", os.path.join(HERE, "libhhdmrg.so"))
 
 i32, i64, f64, u64 = C.c_int32, C.c_int64, C.c_double, C.c_uint64
 vp = C.c_void_p
@@ -36,7 +36,7 @@ class StepRecord(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("matvecs", i64), ("matvec_ms", f64), ("stage1_ms", f64), ("stage2_ms", f64),
                 ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64), ("stage1_flops_alg", f64),
-                ("stage2_flops_alg", f64)]
+                ("stage2_flops_alg", f64), ("reduce_ms", f64)]
 
 
 # numpy views of the engine's task structs (hh_core.hpp: Term, Tile)

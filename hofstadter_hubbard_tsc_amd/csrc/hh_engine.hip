@@ -319,14 +319,19 @@ static void launch_t(const DevTasks& t, const BufPtrs& b, hipStream_t s, int tag
   const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
   if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
   else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 3) hipLaunchKernelGGL((k_tiles<T, 3>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 4) hipLaunchKernelGGL((k_tiles<T, 4>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
   else hipLaunchKernelGGL((k_tiles<T, 0>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
 }
 
-void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag) {
+void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
+  if (t.nwork > 0) {
+    if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
+    else launch_t<double>(t, b, s, tag);
+    check_hip(hipGetLastError(), "k_tiles launch");
+  }
+  if (mid) check_hip(hipEventRecord(mid, s), "event");
   if (t.nwork == 0) return;
-  if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
-  else launch_t<double>(t, b, s, tag);
-  check_hip(hipGetLastError(), "k_tiles launch");
   if (t.nred) {
     hipLaunchKernelGGL(k_reduce, dim3(t.nred), dim3(256), 0, s, t.tiles, t.red, b, t.partial);
     check_hip(hipGetLastError(), "k_reduce launch");

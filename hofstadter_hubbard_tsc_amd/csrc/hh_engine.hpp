@@ -35,10 +35,13 @@ struct DevTasks {
 
 enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
 
-// C tiles = sum of terms; one work item (64x64 macro tile) per workgroup.
+// C tiles = sum of terms; one wave per 32x32 macro tile and term range.
 // tag names the launch site so profiles separate them: 0 generic, 1 H.psi
-// stage 1 (A.psi products), 2 H.psi stage 2 (all output tiles).
-void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision = PREC_FP64, int tag = 0);
+// stage 1 (A.psi products), 2 H.psi stage 2 (all output tiles), 3 and 4 the
+// same two inside a profiled region.  `mid`, when given, is recorded between
+// the tile kernel and the split-sum reduction.
+void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision = PREC_FP64, int tag = 0,
+                  hipEvent_t mid = nullptr);
 
 // vector kernels (complex128 vectors of length n)
 void dev_zero(double2* p, int64_t n, hipStream_t s);

@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -151,7 +153,7 @@ struct hh_ctx {
   DevMem V, dots, partial, scratch_w;
   // profiling
   bool profile = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hh_stats stats{};
   std::vector<DevMem*> user_allocs;
 };
@@ -226,19 +228,22 @@ hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>
 
 void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
   BufPtrs b = bufs_of((double2*)in, out, sb->sys, sb->env, sb->X.c(), sb->merged.c(), nullptr, nullptr);
-  const double t0 = ctx->profile ? wall() : 0.0;
-  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
-  launch_tiles(sb->st1, b, ctx->stream, ctx->precision, 1);
-  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[1], ctx->stream), "event");
-  launch_tiles(sb->st2, b, ctx->stream, ctx->precision, 2);
-  if (ctx->profile) {
+  if (!ctx->profile) {
+    launch_tiles(sb->st1, b, ctx->stream, ctx->precision, 1);
+    launch_tiles(sb->st2, b, ctx->stream, ctx->precision, 2);
+  } else {
+    const double t0 = wall();
+    check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
+    launch_tiles(sb->st1, b, ctx->stream, ctx->precision, 3, ctx->ev[1]);
     check_hip(hipEventRecord(ctx->ev[2], ctx->stream), "event");
-    check_hip(hipEventSynchronize(ctx->ev[2]), "event sync");
-    float a = 0.f, c = 0.f;
-    check_hip(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]), "elapsed");
-    check_hip(hipEventElapsedTime(&c, ctx->ev[1], ctx->ev[2]), "elapsed");
-    ctx->stats.stage1_ms += a;
-    ctx->stats.stage2_ms += c;
+    launch_tiles(sb->st2, b, ctx->stream, ctx->precision, 4, ctx->ev[3]);
+    check_hip(hipEventRecord(ctx->ev[4], ctx->stream), "event");
+    check_hip(hipEventSynchronize(ctx->ev[4]), "event sync");
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; ++i) check_hip(hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]), "elapsed");
+    ctx->stats.stage1_ms += ms[0];
+    ctx->stats.reduce_ms += ms[1] + ms[3];
+    ctx->stats.stage2_ms += ms[2];
     ctx->stats.matvec_ms += (wall() - t0) * 1e3;
   }
   ctx->stats.matvecs += 1;
@@ -581,6 +586,13 @@ struct Driver {
     r.matvec_flops = ctx->stats.flops_alg - fl0;
     (void)mv0;
     recs.push_back(r);
+    // HHDMRG_VERBOSE=1: one progress line per step on stderr
+    static const bool verbose = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
+    if (verbose) {
+      std::fprintf(stderr, "# step phase=%d sys=%d env=%d dim=%lld iters=%d kept=%d E=%.12f %.2fs\n", r.phase,
+                   r.sys_len, r.env_len, (long long)r.dim, r.iters, r.num_state, r.energy, r.seconds);
+      std::fflush(stderr);
+    }
   }
 
   double2* psi_buf(int64_t n, DevMem& m) {

@@ -184,15 +184,19 @@ int hh_driver_block(const hh_driver* d, int side, int length, const hh_block** o
 typedef struct {
   int64_t matvecs;           /* H.psi calls since the last reset */
   double matvec_ms;          /* wall time of those calls */
-  double stage1_ms;          /* kernel time of the stage-1 (A.psi) launches */
-  double stage2_ms;          /* kernel time of the stage-2 (all terms) launches */
+  double stage1_ms;          /* kernel time of the stage-1 (A.psi) tile launches */
+  double stage2_ms;          /* kernel time of the stage-2 (all terms) tile launches */
   double flops_alg;          /* reference-order algorithmic flops of those calls */
   double flops_exec;         /* flops the engine executed */
   double bytes_alg;          /* compulsory bytes: psi in/out + operator sub-blocks */
   double stage1_flops_alg;   /* part of flops_alg done by the stage-1 launches */
   double stage2_flops_alg;   /* part of flops_alg done by the stage-2 launches */
+  double reduce_ms;          /* kernel time of the split-sum reductions of both stages */
 } hh_stats;
-/* profile = 1 records HIP events around every H.psi launch (adds a sync). */
+/* profile = 1 records HIP events around every H.psi launch (adds a sync).
+ * While it is on, the H.psi tile kernels run as their own instantiations
+ * (k_tiles<double, 3> stage 1, k_tiles<double, 4> stage 2, identical code),
+ * so a kernel-trace profile separates the timed launches from all others. */
 int hh_set_profiling(hh_ctx* ctx, int profile);
 int hh_get_stats(hh_ctx* ctx, hh_stats* out);
 int hh_reset_stats(hh_ctx* ctx);
