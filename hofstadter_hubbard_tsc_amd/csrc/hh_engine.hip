@@ -1,10 +1,11 @@
 // Grouped complex GEMM over tile lists and Lanczos vector kernels for gfx950.
 //
-// One workgroup (4 wave64) owns a 64x64 macro tile of one output region and
-// walks every term that lands on it (output-stationary, no atomics, fixed
-// summation order => deterministic).  Complex products run as four real
-// 16x16x4 MFMAs per k-step on LDS-staged operand tiles; the term coefficient
-// is folded into the left operand while it is staged.
+// One wave64 owns a 32x32 macro tile of one output region and walks every
+// term that lands on it, or a contiguous share of those terms whose partial
+// sum a second kernel adds in a fixed order (output-stationary, no atomics,
+// deterministic).  Complex products run as four real 16x16x4 FP64 (or FP32)
+// MFMAs per k-step; the term coefficient and the operand's conjugation are
+// applied to the registers between load and MFMA.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,9 +26,18 @@ void check_hip(hipError_t e, const char* what) {
 // task upload: 32x32 macro tiles, one per wave; long term lists are split
 // over several waves and reduced in a fixed order
 // ----------------------------------------------------------------------------
-static constexpr int MT = 32;          // macro tile edge (one wave)
+static constexpr int WA = 2, WB = 2;   // 16x16 MFMA sub-tiles per wave: rows, columns
+static constexpr int MTR = 16 * WA;     // macro tile rows (one wave)
+static constexpr int MTC = 16 * WB;     // macro tile columns
 static constexpr int WAVES = 4;        // independent waves per workgroup
-static constexpr int TARGET_ITEMS = 8192;
+// Term lists are cut so that the launch holds about this many equal work
+// items: enough that longest-first dispatch leaves a short tail.
+static constexpr int TARGET_ITEMS = 65536;
+// Macro tiles are listed in bands of SWZ macro rows, column by column, and
+// runs of XCHUNK consecutive workgroups are kept on one XCD: neighbouring
+// items then share operand panels in that XCD's L2.
+static constexpr int SWZ = 4;
+static constexpr int XCHUNK = 8;
 
 void DevTasks::release() {
   if (tiles) (void)hipFree(tiles);
@@ -57,7 +67,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
     double c = 0.0;
     for (int k = t.t0; k < t.t1; ++k) c += (double)std::max(tl.terms[k].K, 1);
     tw[i] = c;
-    total += c * (double)((t.M + MT - 1) / MT) * (double)((t.N + MT - 1) / MT);
+    total += c * (double)((t.M + MTR - 1) / MTR) * (double)((t.N + MTC - 1) / MTC);
   }
   const double target = std::max(64.0, total / TARGET_ITEMS);
   std::vector<std::pair<double, WorkItem>> items;
@@ -78,20 +88,25 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
     }
     cuts.push_back(t.t1);
     const int g = (int)cuts.size() - 1;
-    for (int r = 0; r < t.M; r += MT)
-      for (int c = 0; c < t.N; c += MT) {
-        if (g == 1) {
-          items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}});
-        } else {
-          for (int q = 0; q < g; ++q) {
-            double wq = 0.0;
-            for (int k = cuts[q]; k < cuts[q + 1]; ++k) wq += (double)std::max(tl.terms[k].K, 1);
-            items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}});
+    // macro tiles in bands of SWZ macro rows, column by column inside a band
+    const int nr = (t.M + MTR - 1) / MTR, nc = (t.N + MTC - 1) / MTC;
+    const int band = SWZ;
+    for (int rb = 0; rb < nr; rb += band)
+      for (int ci = 0; ci < nc; ++ci)
+        for (int ri = rb; ri < std::min(nr, rb + band); ++ri) {
+          const int r = ri * MTR, c = ci * MTC;
+          if (g == 1) {
+            items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}});
+          } else {
+            for (int q = 0; q < g; ++q) {
+              double wq = 0.0;
+              for (int k = cuts[q]; k < cuts[q + 1]; ++k) wq += (double)std::max(tl.terms[k].K, 1);
+              items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}});
+            }
+            reds.push_back(ReduceItem{i, r, c, slot, g, 0});
+            slot += g;
           }
-          reds.push_back(ReduceItem{i, r, c, slot, g, 0});
-          slot += g;
         }
-      }
   }
   // longest work items first (LPT)
   std::stable_sort(items.begin(), items.end(),
@@ -119,7 +134,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
   if (nred) {
     check_hip(hipMalloc(&red, sizeof(ReduceItem) * nred), "hipMalloc red");
     check_hip(hipMemcpyAsync(red, reds.data(), sizeof(ReduceItem) * nred, hipMemcpyHostToDevice, s), "copy red");
-    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * MT * MT), "hipMalloc partial");
+    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * MTR * MTC), "hipMalloc partial");
   }
   check_hip(hipStreamSynchronize(s), "task upload");
 }
@@ -159,27 +174,39 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // grouped tile kernel: every wave owns one 32x32 macro tile and streams the
 // MFMA operands straight from HBM / L2 (lane l loads A[l%16][k] and
 // B[k][l%16] for its k = l/16 of each k-step), so waves never wait on each
-// other: no LDS, no barriers.
+// other: no LDS, no barriers.  Two waves per SIMD (194 VGPRs): the register
+// double buffer of the operand pipeline does not fit three.
 // ----------------------------------------------------------------------------
 template <typename T, int TAG>
-__global__ __launch_bounds__(256, 3) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
+__global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, int nwork, BufPtrs bufs,
                                                double2* __restrict__ partial) {
   typedef typename Acc<T>::V V;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wi = blockIdx.x * WAVES + wave;
+  int wg = blockIdx.x;
+  {
+    // workgroups reach the 8 XCDs round-robin; give each XCD runs of XCHUNK
+    // consecutive workgroups (a bijection on the full blocks of 8*XCHUNK)
+    const int full = ((int)gridDim.x / (8 * XCHUNK)) * 8 * XCHUNK;
+    if (wg < full) {
+      const int x = wg & 7, j = wg >> 3;
+      wg = (j / XCHUNK) * 8 * XCHUNK + x * XCHUNK + (j % XCHUNK);
+    }
+  }
+  // wave-uniform: lets the compiler keep the work item and terms in SGPRs
+  const int wi = __builtin_amdgcn_readfirstlane(wg * WAVES + wave);
   if (wi >= nwork) return;
   const WorkItem w = work[wi];
   const Tile tile = tiles[w.tile];
   const int r0 = w.r0, c0 = w.c0;
-  const int Mv = min(MT, tile.M - r0), Nv = min(MT, tile.N - c0);
+  const int Mv = min(16 * WA, tile.M - r0), Nv = min(16 * WB, tile.N - c0);
   const int li = lane & 15, lk = lane >> 4;
 
-  V accR[2][2], accI[2][2];
+  V accR[WA][WB], accI[WA][WB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < WB; ++b) {
       accR[a][b] = V{0, 0, 0, 0};
       accI[a][b] = V{0, 0, 0, 0};
     }
@@ -190,9 +217,9 @@ __global__ __launch_bounds__(256, 3) void k_tiles(const Tile* __restrict__ tiles
       // elementwise term, added straight into the accumulators
       const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < WA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WB; ++b)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
@@ -209,72 +236,83 @@ __global__ __launch_bounds__(256, 3) void k_tiles(const Tile* __restrict__ tiles
     const bool a_conj = (tm.a_op == OP_C || tm.a_op == OP_R);
     const bool b_kfast = (tm.b_op == OP_NN || tm.b_op == OP_R);
     const bool b_conj = (tm.b_op == OP_C || tm.b_op == OP_R);
-    // per-lane element strides: A(r, k) and B(k, c)
-    const int64_t a_rs = a_rowfast ? 1 : tm.a_ld, a_ks = a_rowfast ? tm.a_ld : 1;
-    const int64_t b_ks = b_kfast ? 1 : tm.b_ld, b_cs = b_kfast ? tm.b_ld : 1;
-    const bool rv0 = li < Mv, rv1 = 16 + li < Mv;
-    const bool cv0 = li < Nv, cv1 = 16 + li < Nv;
-    const int64_t a0 = tm.a_off + (int64_t)(r0 + li) * a_rs;
-    const int64_t a1 = a0 + 16 * a_rs;
-    const int64_t b0 = tm.b_off + (int64_t)(c0 + li) * b_cs;
-    const int64_t b1 = b0 + 16 * b_cs;
-    // operands of k-step kb are loaded one step ahead of their MFMAs
-    double2 na0 = make_double2(0.0, 0.0), na1 = na0, nb0 = na0, nb1 = na0;
-    {
-      const int k = lk;
-      const bool kv = k < tm.K;
-      if (kv && rv0) na0 = ld_op(A, a0 + k * a_ks, a_conj);
-      if (kv && rv1) na1 = ld_op(A, a1 + k * a_ks, a_conj);
-      if (kv && cv0) nb0 = ld_op(B, b0 + k * b_ks, b_conj);
-      if (kv && cv1) nb1 = ld_op(B, b1 + k * b_ks, b_conj);
-    }
-    for (int kb = 0; kb < tm.K; kb += 4) {
-      const double2 xa0 = na0, xa1 = na1, xb0 = nb0, xb1 = nb1;
-      {
-        const int k = kb + 4 + lk;
-        const bool kv = k < tm.K;
-        na0 = make_double2(0.0, 0.0);
-        na1 = na0;
-        nb0 = na0;
-        nb1 = na0;
-        if (kv && rv0) na0 = ld_op(A, a0 + k * a_ks, a_conj);
-        if (kv && rv1) na1 = ld_op(A, a1 + k * a_ks, a_conj);
-        if (kv && cv0) nb0 = ld_op(B, b0 + k * b_ks, b_conj);
-        if (kv && cv1) nb1 = ld_op(B, b1 + k * b_ks, b_conj);
+    // operand bases are wave-uniform; lanes keep 32-bit element offsets
+    // (one operand sub-block is far below 2^31 elements).  Every load is
+    // unconditional: rows / columns past the region edge read the last valid
+    // row / column (their results are never stored) and k past K reads k = K-1,
+    // zeroed in the A operand at use.  No branch or wait sits next to a load.
+    const double2* __restrict__ Ab = A + tm.a_off;
+    const double2* __restrict__ Bb = B + tm.b_off;
+    const int a_rs = a_rowfast ? 1 : tm.a_ld, a_ks = a_rowfast ? tm.a_ld : 1;
+    const int b_ks = b_kfast ? 1 : tm.b_ld, b_cs = b_kfast ? tm.b_ld : 1;
+    const int K = tm.K, kmax = K - 1;
+    const double sa = a_conj ? -1.0 : 1.0, sb = b_conj ? -1.0 : 1.0;
+    const double cr = tm.cr, ci = tm.ci;
+    uint32_t pa[WA], pb[WB];   // k-independent part of each lane's offset
+#pragma unroll
+    for (int a = 0; a < WA; ++a) pa[a] = (uint32_t)(min(r0 + 16 * a + li, r0 + Mv - 1) * a_rs);
+#pragma unroll
+    for (int b = 0; b < WB; ++b) pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
+    // software pipeline over two register buffers: the operands of k-step kb
+    // are converted to MFMA inputs, the loads of step kb+4 go out into the
+    // other buffer, then the MFMAs of step kb run while those loads fly
+    double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
+    auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], int k0) {
+      const uint32_t kc = (uint32_t)min(k0 + lk, kmax);
+#pragma unroll
+      for (int a = 0; a < WA; ++a) xa[a] = Ab[pa[a] + kc * (uint32_t)a_ks];
+#pragma unroll
+      for (int b = 0; b < WB; ++b) xb[b] = Bb[pb[b] + kc * (uint32_t)b_ks];
+    };
+    auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB],
+                    int kb) {
+      // only the last k-step of a term can hold k >= K
+      const double kf = (kb + 4 <= K || kb + lk < K) ? 1.0 : 0.0;
+      T ar[WA], ai[WA], br[WB], bi[WB];
+#pragma unroll
+      for (int a = 0; a < WA; ++a) {
+        const double xr = kf * xa[a].x, xi = kf * sa * xa[a].y;
+        ar[a] = (T)(cr * xr - ci * xi);
+        ai[a] = (T)(cr * xi + ci * xr);
       }
-      T ar[2], ai[2], br[2], bi[2];
-      ar[0] = (T)(tm.cr * xa0.x - tm.ci * xa0.y);
-      ai[0] = (T)(tm.cr * xa0.y + tm.ci * xa0.x);
-      ar[1] = (T)(tm.cr * xa1.x - tm.ci * xa1.y);
-      ai[1] = (T)(tm.cr * xa1.y + tm.ci * xa1.x);
-      br[0] = (T)xb0.x;
-      bi[0] = (T)xb0.y;
-      br[1] = (T)xb1.x;
-      bi[1] = (T)xb1.y;
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < WB; ++b) {
+        br[b] = (T)xb[b].x;
+        bi[b] = (T)(sb * xb[b].y);
+      }
+      // unconditional (k is clamped), so the wait counts are path-independent
+      load(ya, yb, kb + 4);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+      for (int a = 0; a < WA; ++a)
+#pragma unroll
+        for (int b = 0; b < WB; ++b) {
           accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
           accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
           accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
           accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
         }
+    };
+    load(xa0, xb0, 0);
+    for (int kb = 0; kb < K; kb += 8) {
+      step(xa0, xb0, xa1, xb1, kb);
+      if (kb + 4 >= K) break;
+      step(xa1, xb1, xa0, xb0, kb + 4);
     }
   }
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < WB; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
         if (r < Mv && c < Nv) {
           double re = (double)accR[a][b][v], im = (double)accI[a][b][v];
           if (w.slot >= 0) {
-            partial[(int64_t)w.slot * MT * MT + r + c * MT] = make_double2(re, im);
+            partial[(int64_t)w.slot * MTR * MTC + r + c * MTR] = make_double2(re, im);
           } else {
             const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
             if (tile.accumulate) {
@@ -293,14 +331,14 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
                                                 BufPtrs bufs, const double2* __restrict__ partial) {
   const ReduceItem ri = red[blockIdx.x];
   const Tile tile = tiles[ri.tile];
-  const int Mv = min(MT, tile.M - ri.r0), Nv = min(MT, tile.N - ri.c0);
+  const int Mv = min(MTR, tile.M - ri.r0), Nv = min(MTC, tile.N - ri.c0);
   double2* __restrict__ C = bufs.p[tile.c_buf];
-  for (int e = threadIdx.x; e < MT * MT; e += 256) {
-    const int r = e % MT, c = e / MT;
+  for (int e = threadIdx.x; e < MTR * MTC; e += 256) {
+    const int r = e % MTR, c = e / MTR;
     if (r >= Mv || c >= Nv) continue;
     double re = 0.0, im = 0.0;
     for (int q = 0; q < ri.nslot; ++q) {
-      const double2 x = partial[(int64_t)(ri.slot0 + q) * MT * MT + e];
+      const double2 x = partial[(int64_t)(ri.slot0 + q) * MTR * MTC + e];
       re += x.x;
       im += x.y;
     }
