@@ -30,7 +30,9 @@ class Schedule(C.Structure):
 class StepRecord(C.Structure):
     _fields_ = [("phase", i32), ("sys_len", i32), ("env_len", i32), ("iters", i32), ("num_state", i32),
                 ("pad", i32), ("dim", i64), ("energy", f64), ("trunc_err", f64), ("entropy", f64),
-                ("seconds", f64), ("matvec_seconds", f64), ("matvec_flops", f64)]
+                ("seconds", f64), ("matvec_seconds", f64), ("matvec_flops", f64),
+                ("bind_seconds", f64), ("wave_seconds", f64), ("lanczos_seconds", f64), ("trunc_seconds", f64),
+                ("grow_seconds", f64)]
 
 
 class Stats(C.Structure):

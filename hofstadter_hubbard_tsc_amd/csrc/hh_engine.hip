@@ -475,6 +475,175 @@ __global__ void k_start(double2* p, int64_t n, uint64_t base) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// fused Lanczos kernels.  A vector of length n is cut into blocks of
+// VB*EPT consecutive elements (thread t of block b owns elements
+// b*VB*EPT + t + VB*q, q < EPT); every reduction sums a thread's elements in
+// q order, the 64 lanes of a wave by xor-shuffle, the 4 waves in order, and
+// the blocks in order in a second kernel: deterministic for a given n.
+// ----------------------------------------------------------------------------
+static constexpr int EPT = 8;
+static constexpr int MDOT_J = 64;   // vectors per pass of the multi-dot
+
+int vec_blocks(int64_t n) { return (int)std::max<int64_t>(1, (n + VB * EPT - 1) / (VB * EPT)); }
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// partial[j * nblk + b] = sum over block b of Re(conj(V_j) w), j < nv.
+// w is read once into registers; V_j streams past it.
+__global__ __launch_bounds__(VB) void k_mdot_part(const double2* __restrict__ V, int64_t stride, int nv,
+                                                  const double2* __restrict__ w, int64_t n,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[MDOT_J][VB / 64];
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double2 x[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    x[q] = i < n ? w[i] : make_double2(0.0, 0.0);
+  }
+  for (int j0 = 0; j0 < nv; j0 += MDOT_J) {
+    const int jn = min(MDOT_J, nv - j0);
+    for (int jj = 0; jj < jn; ++jj) {
+      const double2* __restrict__ v = V + (int64_t)(j0 + jj) * stride;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int64_t i = base + (int64_t)VB * q;
+        if (i < n) {
+          const double2 a = v[i];
+          acc += a.x * x[q].x + a.y * x[q].y;
+        }
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) red[jj][wv] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < jn) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < VB / 64; ++k) t += red[threadIdx.x][k];
+      partial[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j] = sum over b of partial[j * nblk + b] (one block per j)
+__global__ __launch_bounds__(VB) void k_block_sum(const double* __restrict__ partial, int nblk,
+                                                  double* __restrict__ out) {
+  __shared__ double red[VB / 64];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += VB) acc += partial[(int64_t)j * nblk + i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < VB / 64; ++k) t += red[k];
+    out[j] = t;
+  }
+}
+
+// block sum of |y|^2 over this thread's elements, written to partial[blockIdx.x]
+__device__ __forceinline__ void block_norm(double acc, double* __restrict__ partial) {
+  __shared__ double red[VB / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < VB / 64; ++k) t += red[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// w = w - a v1 - b v2 (v2 may be null); partial |w|^2 per block
+__global__ __launch_bounds__(VB) void k_lupd_norm(double2* __restrict__ w, const double2* __restrict__ v1, double a,
+                                                  const double2* __restrict__ v2, double b, int64_t n,
+                                                  double* __restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      double2 x = w[i];
+      const double2 y = v1[i];
+      x.x -= a * y.x;
+      x.y -= a * y.y;
+      if (v2) {
+        const double2 z = v2[i];
+        x.x -= b * z.x;
+        x.y -= b * z.y;
+      }
+      w[i] = x;
+      acc += x.x * x.x + x.y * x.y;
+    }
+  }
+  block_norm(acc, partial);
+}
+
+// w = sw * w + cs * sum_j c_j V_j (c on the device); partial |w|^2 per block
+__global__ __launch_bounds__(VB) void k_combine_norm(double2* __restrict__ w, double sw,
+                                                     const double2* __restrict__ V, int64_t stride, int nv,
+                                                     const double* __restrict__ c, double cs, int64_t n,
+                                                     double* __restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  double2 x[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    x[q] = make_double2(0.0, 0.0);
+    if (i < n && sw != 0.0) {
+      const double2 y = w[i];
+      x[q] = make_double2(sw * y.x, sw * y.y);
+    }
+  }
+  for (int j = 0; j < nv; ++j) {
+    const double cj = cs * c[j];
+    const double2* __restrict__ v = V + (int64_t)j * stride;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int64_t i = base + (int64_t)VB * q;
+      if (i < n) {
+        const double2 y = v[i];
+        x[q].x += cj * y.x;
+        x[q].y += cj * y.y;
+      }
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      w[i] = x[q];
+      acc += x[q].x * x[q].x + x[q].y * x[q].y;
+    }
+  }
+  block_norm(acc, partial);
+}
+
+__global__ __launch_bounds__(VB) void k_scale_copy(double2* __restrict__ d, const double2* __restrict__ src, double a,
+                                                   int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      const double2 y = src[i];
+      d[i] = make_double2(a * y.x, a * y.y);
+    }
+  }
+}
+
 static inline int grid_for(int64_t n) { return (int)std::min<int64_t>((n + VB - 1) / VB, 8192); }
 
 void dev_zero(double2* p, int64_t n, hipStream_t s) {
@@ -504,6 +673,29 @@ void dev_combine(double2* w, double beta, const double2* V, int64_t stride, int 
                  hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(VB), 0, s, w, beta, V, stride, nv, c_dev, n);
+}
+void dev_mdots(const double2* V, int64_t stride, int nv, const double2* w, int64_t n, double* out_dev,
+               double* partial_dev, hipStream_t s) {
+  if (nv <= 0) return;
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_mdot_part, dim3(nb), dim3(VB), 0, s, V, stride, nv, w, n, partial_dev);
+  hipLaunchKernelGGL(k_block_sum, dim3(nv), dim3(VB), 0, s, partial_dev, nb, out_dev);
+}
+void dev_lanczos_update_norm(double2* w, const double2* v1, double a, const double2* v2, double b, int64_t n,
+                             double* out_dev, double* partial_dev, hipStream_t s) {
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_lupd_norm, dim3(nb), dim3(VB), 0, s, w, v1, a, v2, b, n, partial_dev);
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(VB), 0, s, partial_dev, nb, out_dev);
+}
+void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, int nv, const double* c_dev, double cs,
+                      int64_t n, double* out_dev, double* partial_dev, hipStream_t s) {
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_combine_norm, dim3(nb), dim3(VB), 0, s, w, sw, V, stride, nv, c_dev, cs, n, partial_dev);
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(VB), 0, s, partial_dev, nb, out_dev);
+}
+void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, a, n);
 }
 void dev_start_vector(double2* p, int64_t n, uint64_t seed, hipStream_t s) {
   if (n <= 0) return;

@@ -57,6 +57,20 @@ void dev_combine(double2* w, double beta, const double2* V, int64_t stride, int 
                  hipStream_t s);
 int rdot_partials();   // partial-sum slots per vector
 
+// Fused Lanczos kernels with deterministic reductions (block order fixed by n).
+int vec_blocks(int64_t n);   // partial-sum slots per vector of length n
+// out[j] = Re <V_j, w>, j < nv; w is read once
+void dev_mdots(const double2* V, int64_t stride, int nv, const double2* w, int64_t n, double* out_dev,
+               double* partial_dev, hipStream_t s);
+// w = w - a*v1 - b*v2 (v2 may be null); out[0] = |w|^2
+void dev_lanczos_update_norm(double2* w, const double2* v1, double a, const double2* v2, double b, int64_t n,
+                             double* out_dev, double* partial_dev, hipStream_t s);
+// w = sw*w + cs * sum_j c_j V_j (c on the device); out[0] = |w|^2
+void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, int nv, const double* c_dev, double cs,
+                      int64_t n, double* out_dev, double* partial_dev, hipStream_t s);
+// d = a * src
+void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s);
+
 void check_hip(hipError_t e, const char* what);
 // counter-based start vector generated on the device (same stream as the host)
 void dev_start_vector(double2* p, int64_t n, uint64_t seed, hipStream_t s);

@@ -145,6 +145,12 @@ struct hh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
+  // side streams (default priority) for independent per-sector eigensolver
+  // calls, each with its own rocBLAS handle; joined back with events
+  static constexpr int NSIDE = 3;
+  hipStream_t side[NSIDE] = {nullptr, nullptr, nullptr};
+  rocblas_handle rbs[NSIDE] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[NSIDE] = {nullptr, nullptr, nullptr};
   ModelParams model;
   bool model_set = false;
   int precision = PREC_FP64;
@@ -265,7 +271,7 @@ double rdot(hh_ctx* ctx, const double2* a, const double2* b, int64_t n) {
 void ensure_ws(hh_ctx* ctx, int64_t n, int nvec) {
   ctx->V.alloc(sizeof(double2) * (size_t)n * (size_t)(nvec + 1));
   ctx->dots.alloc(sizeof(double) * (nvec + 2));
-  ctx->partial.alloc(sizeof(double) * (size_t)rdot_partials() * (nvec + 2));
+  ctx->partial.alloc(sizeof(double) * (size_t)std::max(rdot_partials(), vec_blocks(n)) * (nvec + 2));
 }
 
 // Lanczos, lanczos_memo_new (code/lanczos.f90:4-176), levels = 1.
@@ -285,39 +291,42 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
     check_hip(hipStreamSynchronize(ctx->stream), "matvec");
     mv += wall() - t;
   };
+  double* dots = (double*)ctx->dots.p;
+  double* part = (double*)ctx->partial.p;
+  auto fetch = [&](int count) {   // first `count` device scalars -> host
+    std::vector<double> h(count);
+    check_hip(hipMemcpyAsync(h.data(), dots, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+    return h;
+  };
   // v1 = psi / |psi|
-  dev_copy(vec(0), psi, n, ctx->stream);
-  const double nrm0 = std::sqrt(rdot(ctx, vec(0), vec(0), n));
+  dev_mdots(psi, 0, 1, psi, n, dots, part, ctx->stream);
+  const double nrm0 = std::sqrt(fetch(1)[0]);
   need(nrm0 > 0.0, "zero start vector");
-  dev_scale(vec(0), 1.0 / nrm0, n, ctx->stream);
+  dev_scale_copy(vec(0), psi, 1.0 / nrm0, n, ctx->stream);
   hpsi(vec(0), w);
-  std::vector<double> alpha{rdot(ctx, vec(0), w, n)}, beta;
+  dev_mdots(vec(0), 0, 1, w, n, dots, part, ctx->stream);
+  std::vector<double> alpha{fetch(1)[0]}, beta;
   double ritz = alpha[0];
   int last = 1;
-  std::vector<double> cg(max_iter + 1);
   while (last < max_iter) {
-    dev_lanczos_update(w, vec(last - 1), alpha[last - 1], last >= 2 ? vec(last - 2) : nullptr,
-                       last >= 2 ? beta[last - 2] : 0.0, n, ctx->stream);
-    const double b = std::sqrt(rdot(ctx, w, w, n));
+    // w -= alpha v_last - beta v_{last-1}, with |w|^2 in the same pass
+    dev_lanczos_update_norm(w, vec(last - 1), alpha[last - 1], last >= 2 ? vec(last - 2) : nullptr,
+                            last >= 2 ? beta[last - 2] : 0.0, n, dots, part, ctx->stream);
+    const double b = std::sqrt(fetch(1)[0]);
     beta.push_back(b);
     if (std::fabs(b) <= lanzero) break;
-    dev_scale(w, 1.0 / b, n, ctx->stream);
-    // classical Gram-Schmidt with real coefficients (Gram_Schmidt_process,
-    // code/space_info.f90:1176-1214): c_j = Re<v_j,w>/Re<v_j,v_j>
-    dev_rdots(V, n, last, w, n, (double*)ctx->dots.p, (double*)ctx->partial.p, ctx->stream);
-    check_hip(hipMemcpyAsync(cg.data(), ctx->dots.p, sizeof(double) * last, hipMemcpyDeviceToHost, ctx->stream),
-              "d2h");
-    check_hip(hipStreamSynchronize(ctx->stream), "sync");
-    for (int j = 0; j < last; ++j) cg[j] = -cg[j];   // Lanczos vectors are unit norm
-    check_hip(hipMemcpyAsync(ctx->dots.p, cg.data(), sizeof(double) * last, hipMemcpyHostToDevice, ctx->stream),
-              "h2d");
-    dev_combine(w, 1.0, V, n, last, (const double*)ctx->dots.p, n, ctx->stream);
-    const double nw = std::sqrt(rdot(ctx, w, w, n));
-    dev_scale(w, 1.0 / nw, n, ctx->stream);
-    dev_copy(vec(last), w, n, ctx->stream);
+    // normalise, then classical Gram-Schmidt against every stored vector with
+    // real coefficients (Gram_Schmidt_process, code/space_info.f90:1176-1214;
+    // the vectors are unit norm):  w <- (w - sum_j Re<v_j,w> v_j) / b
+    dev_mdots(V, n, last, w, n, dots + 1, part, ctx->stream);
+    dev_combine_norm(w, 1.0 / b, V, n, last, dots + 1, -1.0 / b, n, dots, part, ctx->stream);
+    const double nw = std::sqrt(fetch(1)[0]);
+    dev_scale_copy(vec(last), w, 1.0 / nw, n, ctx->stream);
     ++last;
     hpsi(vec(last - 1), w);
-    alpha.push_back(rdot(ctx, vec(last - 1), w, n));
+    dev_mdots(vec(last - 1), 0, 1, w, n, dots, part, ctx->stream);
+    alpha.push_back(fetch(1)[0]);
     std::vector<double> d(alpha), e(beta);
     e.resize(d.size());
     tridiag_ql(d, e, nullptr);
@@ -331,14 +340,56 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
   std::vector<double> d(alpha), e(beta), z;
   e.resize(d.size());
   tridiag_ql(d, e, &z);
-  check_hip(hipMemcpyAsync(ctx->dots.p, z.data(), sizeof(double) * last, hipMemcpyHostToDevice, ctx->stream), "h2d");
-  dev_combine(psi, 0.0, V, n, last, (const double*)ctx->dots.p, n, ctx->stream);
-  const double np = std::sqrt(rdot(ctx, psi, psi, n));
+  check_hip(hipMemcpyAsync(dots + 1, z.data(), sizeof(double) * last, hipMemcpyHostToDevice, ctx->stream), "h2d");
+  dev_combine_norm(psi, 0.0, V, n, last, dots + 1, 1.0, n, dots, part, ctx->stream);
+  const double np = std::sqrt(fetch(1)[0]);
   dev_scale(psi, 1.0 / np, n, ctx->stream);
   check_hip(hipStreamSynchronize(ctx->stream), "lanczos");
   *e0 = d[0];
   *iters = last;
   if (mv_seconds) *mv_seconds = mv;
+}
+
+// Eigendecomposition of every sector matrix (ZHEEV, code/get_den.f90:242),
+// ascending eigenvalues and eigenvectors in place.  The sectors are
+// independent: they are spread over the context stream and its side streams
+// (largest first, each to the stream with the least n^3 so far), so that the
+// many small kernels of different sectors run concurrently.
+template <typename OnFail>
+void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, const std::vector<int>& rdim,
+                  double* evals, double* work, int64_t work_stride, rocblas_int* info, OnFail on_fail) {
+  const size_t ns = rdim.size();
+  std::vector<int64_t> eo(ns, 0);
+  for (size_t i = 1; i < ns; ++i) eo[i] = eo[i - 1] + rdim[i - 1];
+  std::vector<int> idx(ns);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return rdim[a] > rdim[b]; });
+  constexpr int NS = hh_ctx::NSIDE + 1;
+  rocblas_handle h[NS] = {ctx->rb};
+  for (int k = 1; k < NS; ++k) h[k] = ctx->rbs[k - 1];
+  // the density matrices were written on the context stream
+  check_hip(hipEventRecord(ctx->fork, ctx->stream), "event");
+  for (int k = 1; k < NS; ++k) check_hip(hipStreamWaitEvent(ctx->side[k - 1], ctx->fork, 0), "wait");
+  double load[NS] = {0.0};
+  bool ok = true;
+  for (int i : idx) {
+    int k = 0;
+    for (int q = 1; q < NS; ++q)
+      if (load[q] < load[k]) k = q;
+    load[k] += (double)rdim[i] * rdim[i] * rdim[i];
+    ok = ok && rocsolver_zheevd(h[k], rocblas_evect_original, rocblas_fill_upper, rdim[i],
+                                (rocblas_double_complex*)(evec + roff[i]), rdim[i], evals + eo[i],
+                                work + (int64_t)k * work_stride, info + i) == rocblas_status_success;
+  }
+  for (int k = 1; k < NS; ++k) {
+    check_hip(hipEventRecord(ctx->join[k - 1], ctx->side[k - 1]), "event");
+    check_hip(hipStreamWaitEvent(ctx->stream, ctx->join[k - 1], 0), "wait");
+  }
+  if (!ok) {
+    check_hip(hipStreamSynchronize(ctx->stream), "eig");
+    on_fail();
+    throw HHError(-2, "rocsolver_zheevd failed");
+  }
 }
 
 // Reduced density matrix + Get_truncation_operator (code/get_den.f90:184-428)
@@ -366,19 +417,10 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   }
   DevMem evals, work, info;
   evals.alloc(sizeof(double) * (size_t)std::max<int64_t>(nev, 1));
-  work.alloc(sizeof(double) * (size_t)dmax);
+  work.alloc(sizeof(double) * (size_t)dmax * (hh_ctx::NSIDE + 1));
   info.alloc(sizeof(rocblas_int) * rdim.size() + 16);
-  int64_t eo = 0;
-  for (size_t i = 0; i < rdim.size(); ++i) {
-    rocblas_status st = rocsolver_zheevd(ctx->rb, rocblas_evect_original, rocblas_fill_upper, rdim[i],
-                                         (rocblas_double_complex*)(t->evec.c() + roff[i]), rdim[i],
-                                         (double*)evals.p + eo, (double*)work.p, (rocblas_int*)info.p + i);
-    if (st != rocblas_status_success) {
-      delete t;
-      throw HHError(-2, "rocsolver_zheevd failed");
-    }
-    eo += rdim[i];
-  }
+  heev_sectors(ctx, t->evec.c(), roff, rdim, (double*)evals.p, (double*)work.p, dmax, (rocblas_int*)info.p,
+               [&]() { delete t; });
   std::vector<double> ev(nev);
   std::vector<rocblas_int> inf(rdim.size());
   check_hip(hipMemcpyAsync(ev.data(), evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost, ctx->stream), "d2h");
@@ -569,6 +611,20 @@ struct Driver {
 
   int kmin(int kmax) const { return std::min(3 * kmax / 4, 3600); }
 
+  // wall time per phase of the current step: bind, wave, lanczos, trunc, grow
+  double ph[5] = {0, 0, 0, 0, 0};
+  double ph_t = 0.0;
+  void lap_start() {
+    for (double& x : ph) x = 0.0;
+    ph_t = wall();
+  }
+  void lap(int i) {
+    check_hip(hipStreamSynchronize(ctx->stream), "phase sync");
+    const double t = wall();
+    ph[i] += t - ph_t;
+    ph_t = t;
+  }
+
   void record(int phase, hh_superblock* sb, double E, int it, const hh_trunc* t, double secs, double mvs, int64_t mv0,
               double fl0) {
     hh_step_record r{};
@@ -584,13 +640,22 @@ struct Driver {
     r.seconds = secs;
     r.matvec_seconds = mvs;
     r.matvec_flops = ctx->stats.flops_alg - fl0;
+    r.bind_seconds = ph[0];
+    r.wave_seconds = ph[1];
+    r.lanczos_seconds = ph[2];
+    r.trunc_seconds = ph[3];
+    r.grow_seconds = ph[4];
     (void)mv0;
     recs.push_back(r);
     // HHDMRG_VERBOSE=1: one progress line per step on stderr
     static const bool verbose = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
     if (verbose) {
-      std::fprintf(stderr, "# step phase=%d sys=%d env=%d dim=%lld iters=%d kept=%d E=%.12f %.2fs\n", r.phase,
-                   r.sys_len, r.env_len, (long long)r.dim, r.iters, r.num_state, r.energy, r.seconds);
+      std::fprintf(stderr,
+                   "# step phase=%d sys=%d env=%d dim=%lld iters=%d kept=%d E=%.12f %.2fs "
+                   "(bind %.3f wave %.3f lanczos %.3f [H.psi %.3f] trunc %.3f grow %.3f)\n",
+                   r.phase, r.sys_len, r.env_len, (long long)r.dim, r.iters, r.num_state, r.energy, r.seconds,
+                   r.bind_seconds, r.wave_seconds, r.lanczos_seconds, r.matvec_seconds, r.trunc_seconds,
+                   r.grow_seconds);
       std::fflush(stderr);
     }
   }
@@ -613,21 +678,27 @@ struct Driver {
       const int64_t mv0 = ctx->stats.matvecs;
       const double fl0 = ctx->stats.flops_alg;
       const int N = electrons_for_length(m, 2 * label + 2);
+      lap_start();
       std::unique_ptr<hh_superblock> sb(bind(ctx, sysb[label].get(), envb[label].get(), N));
+      lap(0);
       const int64_t n = sb->plan.sb.dim;
       double2* p = psi_buf(n, psi);
       dev_start_vector(p, n, S.seed + step, ctx->stream);
       ++step;
+      lap(1);
       double E, mvs;
       int it;
       lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      lap(2);
       std::unique_ptr<hh_trunc> ts, te;
       if (sb->plan.sbs.total_dim() <= kmax) ts.reset(trunc_identity(ctx, sb.get(), 0));
       else ts.reset(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
       if (sb->plan.ebs.total_dim() <= kmax) te.reset(trunc_identity(ctx, sb.get(), 1));
       else te.reset(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      lap(3);
       std::unique_ptr<hh_block> ns(grow(ctx, sb.get(), 0, ts.get()));
       std::unique_ptr<hh_block> ne(grow(ctx, sb.get(), 1, te.get()));
+      lap(4);
       record(0, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
       sysb[label + 1] = std::move(ns);
       envb[label + 1] = std::move(ne);
@@ -650,7 +721,9 @@ struct Driver {
       const double fl0 = ctx->stats.flops_alg;
       hh_block* sb_sys = sysb.at(label).get();
       hh_block* sb_env = envb.at(ns_ - label - 2).get();
+      lap_start();
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
+      lap(0);
       const int64_t n = sb->plan.sb.dim;
       double2* p;
       if (first && !prev_sb) {
@@ -669,12 +742,16 @@ struct Driver {
         p = psi.c();
       }
       ++step;
+      lap(1);
       double E, mvs;
       int it;
       lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      lap(2);
       psi_dim = n;
       std::unique_ptr<hh_trunc> ts(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
+      lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 0, ts.get()));
+      lap(4);
       record(phase, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
       // the previous superblock references blocks that stay stored
       prev_sb = std::move(sb);
@@ -696,7 +773,9 @@ struct Driver {
       const double fl0 = ctx->stats.flops_alg;
       hh_block* sb_sys = sysb.at(label - 1).get();
       hh_block* sb_env = envb.at(ns_ - label - 1).get();
+      lap_start();
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
+      lap(0);
       const int64_t n = sb->plan.sb.dim;
       double2* p;
       if (first && !prev_sb) {
@@ -715,12 +794,16 @@ struct Driver {
         p = psi.c();
       }
       ++step;
+      lap(1);
       double E, mvs;
       int it;
       lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      lap(2);
       psi_dim = n;
       std::unique_ptr<hh_trunc> te(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 1, te.get()));
+      lap(4);
       record(phase, sb.get(), E, it, te.get(), wall() - t0, mvs, mv0, fl0);
       prev_sb = std::move(sb);
       envb[ns_ - label] = std::move(nb);
@@ -788,6 +871,13 @@ int hh_ctx_create(int device, hh_ctx** out) {
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw HHError(-2, "rocblas_create_handle");
     rocblas_set_stream(c->rb, c->stream);
     for (auto& e : c->ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate");
+    for (int k = 0; k < hh_ctx::NSIDE; ++k) {
+      check_hip(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking), "hipStreamCreate");
+      if (rocblas_create_handle(&c->rbs[k]) != rocblas_status_success) throw HHError(-2, "rocblas_create_handle");
+      rocblas_set_stream(c->rbs[k], c->side[k]);
+      check_hip(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming), "hipEventCreate");
+    }
   });
   if (rc != 0) {
     create_error() = c->err;
@@ -805,6 +895,12 @@ int hh_ctx_destroy(hh_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->rb) rocblas_destroy_handle(ctx->rb);
+  for (int k = 0; k < hh_ctx::NSIDE; ++k) {
+    if (ctx->rbs[k]) rocblas_destroy_handle(ctx->rbs[k]);
+    if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
+    if (ctx->join[k]) (void)hipEventDestroy(ctx->join[k]);
+  }
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   ctx->V.reset();
   ctx->dots.reset();
   ctx->partial.reset();

@@ -170,6 +170,17 @@ class StepRecord:
     seconds: float
     matvec_seconds: float
     matvec_flops: float
+    bind_seconds: float = 0.0
+    wave_seconds: float = 0.0
+    lanczos_seconds: float = 0.0
+    trunc_seconds: float = 0.0
+    grow_seconds: float = 0.0
+
+    @classmethod
+    def from_c(cls, r):
+        return cls(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy, r.trunc_err, r.entropy,
+                   r.seconds, r.matvec_seconds, r.matvec_flops, r.bind_seconds, r.wave_seconds, r.lanczos_seconds,
+                   r.trunc_seconds, r.grow_seconds)
 
 
 class Context:
@@ -293,11 +304,7 @@ class Context:
         recs = (L.StepRecord * cap)()
         n = L.i32()
         self._rc(L.load().hh_dmrg_run(self.ptr, C.byref(s), recs, cap, C.byref(n)), "hh_dmrg_run")
-        out = []
-        for r in recs[:min(n.value, cap)]:
-            out.append(StepRecord(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy,
-                                  r.trunc_err, r.entropy, r.seconds, r.matvec_seconds, r.matvec_flops))
-        return out
+        return [StepRecord.from_c(r) for r in recs[:min(n.value, cap)]]
 
     # -- profiling -----------------------------------------------------------
     def set_profiling(self, on):
@@ -335,8 +342,7 @@ class Driver:
 
     @staticmethod
     def _recs(buf, n):
-        return [StepRecord(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy, r.trunc_err,
-                           r.entropy, r.seconds, r.matvec_seconds, r.matvec_flops) for r in buf[:n]]
+        return [StepRecord.from_c(r) for r in buf[:n]]
 
     def warmup(self):
         cap = 4096

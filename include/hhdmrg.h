@@ -72,6 +72,12 @@ typedef struct {
   double seconds;            /* wall time of the step */
   double matvec_seconds;     /* wall time inside H.psi */
   double matvec_flops;       /* reference-order algorithmic flops of H.psi */
+  /* wall time of the phases of the step (stream synchronised at each end) */
+  double bind_seconds;       /* Get_basis + Get_super_basis + hmerge */
+  double wave_seconds;       /* wave_trans_LR / wave_trans_RL (or the start vector) */
+  double lanczos_seconds;    /* lanczos_memo_new, H.psi included */
+  double trunc_seconds;      /* density matrix, eigendecomposition, kept-count rule */
+  double grow_seconds;       /* Get_Hamiltonian_Sys/Env with update_trun_* */
 } hh_step_record;
 
 /* ---- context ---------------------------------------------------------- */
