@@ -40,7 +40,7 @@ def build(verbose=False):
             print(" ".join(cmd))
         _run(cmd)
         objs.append(obj)
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", LIB + ".tmp"] + objs + [
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", "-o", LIB + ".tmp"] + objs + [
         "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     _run(cmd)
     os.replace(LIB + ".tmp", LIB)

@@ -8,6 +8,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hhdmrg.h"
@@ -352,9 +354,11 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
 
 // Eigendecomposition of every sector matrix (ZHEEV, code/get_den.f90:242),
 // ascending eigenvalues and eigenvectors in place.  The sectors are
-// independent: they are spread over the context stream and its side streams
-// (largest first, each to the stream with the least n^3 so far), so that the
-// many small kernels of different sectors run concurrently.
+// independent.  A divide-and-conquer zheevd is a long chain of small kernels
+// (about 20-30 us per column) with host synchronisations inside, so the
+// sectors are spread over the context stream and its side streams (largest
+// first, each to the stream with the fewest columns so far) and every stream
+// is driven by its own host thread.
 template <typename OnFail>
 void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, const std::vector<int>& rdim,
                   double* evals, double* work, int64_t work_stride, rocblas_int* info, OnFail on_fail) {
@@ -367,20 +371,35 @@ void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, 
   constexpr int NS = hh_ctx::NSIDE + 1;
   rocblas_handle h[NS] = {ctx->rb};
   for (int k = 1; k < NS; ++k) h[k] = ctx->rbs[k - 1];
-  // the density matrices were written on the context stream
-  check_hip(hipEventRecord(ctx->fork, ctx->stream), "event");
-  for (int k = 1; k < NS; ++k) check_hip(hipStreamWaitEvent(ctx->side[k - 1], ctx->fork, 0), "wait");
+  std::vector<int> mine[NS];
   double load[NS] = {0.0};
-  bool ok = true;
   for (int i : idx) {
     int k = 0;
     for (int q = 1; q < NS; ++q)
       if (load[q] < load[k]) k = q;
-    load[k] += (double)rdim[i] * rdim[i] * rdim[i];
-    ok = ok && rocsolver_zheevd(h[k], rocblas_evect_original, rocblas_fill_upper, rdim[i],
-                                (rocblas_double_complex*)(evec + roff[i]), rdim[i], evals + eo[i],
-                                work + (int64_t)k * work_stride, info + i) == rocblas_status_success;
+    load[k] += 64.0 + rdim[i];
+    mine[k].push_back(i);
   }
+  // the density matrices were written on the context stream
+  check_hip(hipEventRecord(ctx->fork, ctx->stream), "event");
+  for (int k = 1; k < NS; ++k) check_hip(hipStreamWaitEvent(ctx->side[k - 1], ctx->fork, 0), "wait");
+  std::atomic<bool> ok{true};
+  auto run = [&](int k) {
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+      ok = false;
+      return;
+    }
+    for (int i : mine[k])
+      if (rocsolver_zheevd(h[k], rocblas_evect_original, rocblas_fill_upper, rdim[i],
+                           (rocblas_double_complex*)(evec + roff[i]), rdim[i], evals + eo[i],
+                           work + (int64_t)k * work_stride, info + i) != rocblas_status_success)
+        ok = false;
+  };
+  std::vector<std::thread> th;
+  for (int k = 1; k < NS; ++k)
+    if (!mine[k].empty()) th.emplace_back(run, k);
+  run(0);
+  for (auto& t : th) t.join();
   for (int k = 1; k < NS; ++k) {
     check_hip(hipEventRecord(ctx->join[k - 1], ctx->side[k - 1]), "event");
     check_hip(hipStreamWaitEvent(ctx->stream, ctx->join[k - 1], 0), "wait");
