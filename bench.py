@@ -39,37 +39,10 @@ def workload(m_kept):
     return model
 
 
-def oracle_block(ctx, blk, model):
-    """Product block -> oracle Block (for the CPU baseline leg)."""
-    import numpy as np
-    from oracle import dmrg as odmrg
-    arena = blk.download()
-    N, S2, d, h = blk.sectors()
-    sites = blk.sites()
-    nk = 1 if model.kind == "hubbard" else 3
-    names = ["cdag", "S", "n"]
-    H = odmrg.Op(0, 0, False)
-    for i, dim in enumerate(d):
-        H.blk[(i, i)] = arena[h[i]:h[i] + dim * dim].reshape((dim, dim), order="F").copy()
-    ops = {}
-    for si, k in enumerate(sites):
-        ops[int(k)] = {}
-        for kind in range(nk):
-            base = ctx.op_offset(blk, si, kind)
-            bra, ket, off = ctx.layout(blk, kind)
-            op = odmrg.Op([1, 2, 0][kind], [1, 0, 0][kind], kind == 0)
-            for b_, k_, o in zip(bra, ket, off):
-                r, c = d[b_], d[k_]
-                op.blk[(int(b_), int(k_))] = arena[base + o: base + o + r * c].reshape((r, c), order="F").copy()
-            ops[int(k)][names[kind]] = op
-    _, _, _, length = blk.info()
-    return odmrg.Block(length, list(zip(N.tolist(), S2.tolist())), d.tolist(), H, ops)
-
-
 def cpu_baseline(ctx, drv, model, sys_len, budget_s):
     """numpy oracle H.psi on the superblock of the last timed step."""
-    import numpy as np
     from oracle import dmrg as odmrg
+    from tests.gpuhelp import oracle_block
     n = model.nsite
     sysb = oracle_block(ctx, drv.block(0, sys_len), model)
     envb = oracle_block(ctx, drv.block(1, n - sys_len - 2), model)
@@ -96,6 +69,92 @@ def cpu_baseline(ctx, drv, model, sys_len, budget_s):
                       % (0, sys_len, reps, t_mv, t_build)}
 
 
+def run_c3(args):
+    """Batched H.psi on 16 synthetic superblocks (one grouped launch per stage)
+    against the same 16 H.psi launched one superblock at a time."""
+    import torch
+    from hofstadter_hubbard_tsc_amd import Context, synth
+    ctx = Context(0)
+    t_prep = time.time()
+    members = synth.make_batch(ctx, args.nsb)
+    batch = ctx.batch([m[2] for m in members])
+    info = batch.info()
+    print("# c3: %d superblocks, total dim %d, %.1f GFLOP per batched H.psi, prepared in %.1f s"
+          % (args.nsb, info["dim"], info["flops_alg"] / 1e9, time.time() - t_prep), file=sys.stderr, flush=True)
+    ins = [m[3] for m in members]
+    outs = [torch.zeros_like(p) for p in ins]
+    reps = max(args.steps, 1)
+    for _ in range(max(args.warmup, 1)):
+        ctx.batch_model_wave(batch, ins, outs)
+        for m, o in zip(members, outs):
+            ctx.model_wave(m[2], m[3], o)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(reps):
+        ctx.batch_model_wave(batch, ins, outs)
+    torch.cuda.synchronize()
+    t_batch = (time.time() - t0) / reps
+    t0 = time.time()
+    for _ in range(reps):
+        for m, o in zip(members, outs):
+            ctx.model_wave(m[2], m[3], o)
+    torch.cuda.synchronize()
+    t_seq = (time.time() - t0) / reps
+    # kernel split of the batched call, HIP events on the library stream
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    for _ in range(reps):
+        ctx.batch_model_wave(batch, ins, outs)
+    ctx.set_profiling(False)
+    st = ctx.stats()
+    s2_tflops = st["stage2_flops_alg"] / max(st["stage2_ms"], 1e-30) / 1e9
+    roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "hh::k_tiles<double, 2, hh::BufTable> (batched H.psi stage 2)",
+            "avg_launch_ms": round(st["stage2_ms"] / max(st["matvecs"], 1), 4), "launches": int(st["matvecs"]),
+            "stage1_ms_per_launch": st["stage1_ms"] / max(st["matvecs"], 1),
+            "reduce_ms_per_matvec": st["reduce_ms"] / max(st["matvecs"], 1)}
+    cpu = None
+    if not args.no_cpu:
+        from tests.gpuhelp import oracle_superblock
+        from oracle import lattice as lat
+        sysb, envb, sb, psi = members[0]
+        mp = synth.model_params(1 / 64.0)
+        model = lat.Model(kind="hubbard", Lx=mp.Lx, Ly=mp.Ly, U=mp.U, t=mp.t, flux=mp.flux, n_elec=mp.n_elec)
+        t1 = time.time()
+        H = oracle_superblock(ctx, sysb, envb, model, mp.n_elec)
+        t_build = time.time() - t1
+        x = psi.cpu().numpy()
+        n_mv, t_mv = 0, 0.0
+        while n_mv == 0 or (t_mv < args.cpu_seconds and n_mv < 200):
+            t1 = time.time()
+            H.matvec(x)
+            t_mv += time.time() - t1
+            n_mv += 1
+        cpu = {"value": sb.info()["flops_alg"] * n_mv / t_mv / 1e9, "unit": "GFLOP/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+               "sample": "numpy oracle H.psi of member 0 (dim %d), %d matvecs in %.1f s (+%.1f s operator setup), "
+                         "same reference-order flop count" % (sb.dim, n_mv, t_mv, t_build)}
+    out = {
+        "metric": "batched H.psi GFLOP/s: 16 synthetic superblocks, one grouped launch per stage",
+        "value": round(info["flops_alg"] / t_batch / 1e9, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_batch, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 (FP64 MFMA)",
+        "data": "synthetic: Philox-seeded sector tables and complex Gaussian blocks (seed 0x5EED0000 + member)",
+        "config": {"workload": "16 Hubbard superblocks (Ly=4 x Lx=16 cylinder, N=64, S=0, flux (b+1)/64), "
+                               "120 sectors per side, lognormal multiplet dims (median 48, sigma 0.8, clipped to "
+                               "[1, 1024]) scaled to m in [500, 4000]",
+                   "superblocks": args.nsb, "total_dim": info["dim"],
+                   "flops_per_call": info["flops_alg"],
+                   "one_at_a_time_ms": round(1e3 * t_seq, 3),
+                   "one_at_a_time_gflops": round(info["flops_alg"] / t_seq / 1e9, 2),
+                   "batched_speedup": round(t_seq / t_batch, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +165,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+                    help="c2: DMRG steps of the flux t-J cylinder at m (default); c3: batched H.psi micro-benchmark")
+    ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -127,6 +189,9 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
+        return
+    if args.workload == "c3":
+        run_c3(args)
         return
     os.environ.setdefault("HHDMRG_VERBOSE", "1")   # progress line per DMRG step on stderr
     from hofstadter_hubbard_tsc_amd import Context, Driver
@@ -178,7 +243,7 @@ def main():
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "hh::k_tiles<double, 4> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
+            "kernel": "hh::k_tiles<double, 4, hh::BufPtrs> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
             "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(s2_launches, 1)}

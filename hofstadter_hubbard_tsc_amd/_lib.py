@@ -54,6 +54,7 @@ SIGNATURES = [
     ("hh_ctx_destroy", i32, [vp]),
     ("hh_last_error", C.c_char_p, [vp]),
     ("hh_set_model", i32, [vp, P(ModelParams)]),
+    ("hh_boundary_sites", i32, [vp, i32, P(i32), P(i32)]),
     ("hh_dev_alloc", i32, [vp, i64, P(vp)]),
     ("hh_dev_free", i32, [vp, vp]),
     ("hh_memcpy_h2d", i32, [vp, vp, vp, i64]),
@@ -74,6 +75,10 @@ SIGNATURES = [
     ("hh_superblock_pairs", i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i64)]),
     ("hh_superblock_enlarged", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32)]),
     ("hh_model_wave", i32, [vp, vp, vp, vp]),
+    ("hh_batch_create", i32, [vp, i32, P(vp), P(vp)]),
+    ("hh_batch_destroy", i32, [vp]),
+    ("hh_batch_info", i32, [vp, P(i64), P(f64)]),
+    ("hh_batch_model_wave", i32, [vp, vp, P(vp), P(vp)]),
     ("hh_lanczos", i32, [vp, vp, vp, f64, i32, P(f64), P(i32)]),
     ("hh_truncate", i32, [vp, vp, i32, vp, i32, i32, f64, P(vp), P(i32), P(f64), P(f64)]),
     ("hh_trunc_identity", i32, [vp, vp, i32, P(vp)]),

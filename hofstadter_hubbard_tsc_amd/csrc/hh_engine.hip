@@ -177,9 +177,9 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // other: no LDS, no barriers.  Two waves per SIMD (194 VGPRs): the register
 // double buffer of the operand pipeline does not fit three.
 // ----------------------------------------------------------------------------
-template <typename T, int TAG>
+template <typename T, int TAG, typename Bufs>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
-                                               const WorkItem* __restrict__ work, int nwork, BufPtrs bufs,
+                                               const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
   typedef typename Acc<T>::V V;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -327,8 +327,9 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
 }
 
 // C (+)= sum over the slots of a split macro tile, slot order fixed
+template <typename Bufs>
 __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, const ReduceItem* __restrict__ red,
-                                                BufPtrs bufs, const double2* __restrict__ partial) {
+                                                Bufs bufs, const double2* __restrict__ partial) {
   const ReduceItem ri = red[blockIdx.x];
   const Tile tile = tiles[ri.tile];
   const int Mv = min(MTR, tile.M - ri.r0), Nv = min(MTC, tile.N - ri.c0);
@@ -352,17 +353,18 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
-template <typename T>
-static void launch_t(const DevTasks& t, const BufPtrs& b, hipStream_t s, int tag) {
+template <typename T, typename Bufs>
+static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
   const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
-  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 3) hipLaunchKernelGGL((k_tiles<T, 3>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 4) hipLaunchKernelGGL((k_tiles<T, 4>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else hipLaunchKernelGGL((k_tiles<T, 0>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 3) hipLaunchKernelGGL((k_tiles<T, 3, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else if (tag == 4) hipLaunchKernelGGL((k_tiles<T, 4, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  else hipLaunchKernelGGL((k_tiles<T, 0, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
 }
 
-void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
+template <typename Bufs>
+static void launch_any(const DevTasks& t, const Bufs& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
   if (t.nwork > 0) {
     if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
     else launch_t<double>(t, b, s, tag);
@@ -371,9 +373,16 @@ void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precis
   if (mid) check_hip(hipEventRecord(mid, s), "event");
   if (t.nwork == 0) return;
   if (t.nred) {
-    hipLaunchKernelGGL(k_reduce, dim3(t.nred), dim3(256), 0, s, t.tiles, t.red, b, t.partial);
+    hipLaunchKernelGGL(k_reduce<Bufs>, dim3(t.nred), dim3(256), 0, s, t.tiles, t.red, b, t.partial);
     check_hip(hipGetLastError(), "k_reduce launch");
   }
+}
+
+void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
+  launch_any(t, b, s, precision, tag, mid);
+}
+void launch_tiles(const DevTasks& t, const BufTable& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
+  launch_any(t, b, s, precision, tag, mid);
 }
 
 // ----------------------------------------------------------------------------

@@ -12,6 +12,9 @@
 namespace hh {
 
 struct BufPtrs { double2* p[8]; };
+// Buffer pointers held in device memory (up to 256 slots): lets one launch
+// address the buffers of many superblocks.
+struct BufTable { double2* const* p; };
 // One wave's share of the work: a 32x32 macro tile of output region `tile`
 // and the term range [t0, t1); slot >= 0 sends the partial sum to the
 // partial buffer (tiles whose term list is split across several waves).
@@ -41,6 +44,8 @@ enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
 // same two inside a profiled region.  `mid`, when given, is recorded between
 // the tile kernel and the split-sum reduction.
 void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precision = PREC_FP64, int tag = 0,
+                  hipEvent_t mid = nullptr);
+void launch_tiles(const DevTasks& t, const BufTable& b, hipStream_t s, int precision = PREC_FP64, int tag = 0,
                   hipEvent_t mid = nullptr);
 
 // vector kernels (complex128 vectors of length n)

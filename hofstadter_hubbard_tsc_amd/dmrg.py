@@ -37,6 +37,27 @@ def _sync():
     torch.cuda.synchronize()
 
 
+class Batch:
+    """Independent superblocks whose H.psi runs as one grouped launch per stage."""
+
+    def __init__(self, ctx, ptr, sbs):
+        self.ctx = ctx
+        self.ptr = ptr
+        self._keep = list(sbs)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().hh_batch_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def info(self):
+        d, f = L.i64(), L.f64()
+        L.load().hh_batch_info(self.ptr, C.byref(d), C.byref(f))
+        return {"dim": d.value, "flops_alg": f.value}
+
+
 class Block:
     """A device-resident block (Total_Model, code/pubdata.f90:167-181)."""
 
@@ -82,10 +103,15 @@ class Block:
         return out[:a]
 
     def upload(self, arena):
+        """Whole arena from a complex128 numpy array or contiguous device tensor."""
         lib = L.load()
-        arena = np.ascontiguousarray(arena, dtype=np.complex128)
-        _check(self.ctx.ptr, lib.hh_block_upload(self.ctx.ptr, self.ptr, arena.ctypes.data_as(C.c_void_p)),
-               "hh_block_upload")
+        if hasattr(arena, "is_cuda") and arena.is_cuda:
+            _sync()
+            src = _dptr(arena)
+        else:
+            arena = np.ascontiguousarray(arena, dtype=np.complex128)
+            src = arena.ctypes.data_as(C.c_void_p)
+        _check(self.ctx.ptr, lib.hh_block_upload(self.ctx.ptr, self.ptr, src), "hh_block_upload")
 
 
 class Superblock:
@@ -221,6 +247,14 @@ class Context:
         self._rc(L.load().hh_block_site(self.ptr, side, C.byref(p)), "hh_block_site")
         return Block(self, p)
 
+    def boundary_sites(self, length):
+        """Sites of a block of `length` sites that keep operators (dmrg_setup0)."""
+        n = L.i32()
+        self._rc(L.load().hh_boundary_sites(self.ptr, length, C.byref(n), None), "hh_boundary_sites")
+        out = np.zeros(n.value, np.int32)
+        self._rc(L.load().hh_boundary_sites(self.ptr, length, C.byref(n), L.ip(out)), "hh_boundary_sites")
+        return out.tolist()
+
     def create_block(self, side, length, sectors, sites):
         N = np.array([q[0] for q in sectors], np.int32)
         S2 = np.array([q[1] for q in sectors], np.int32)
@@ -256,6 +290,23 @@ class Context:
         """model_wave (code/mat_vec.f90:3476): psi_out = H psi_in."""
         _sync()
         self._rc(L.load().hh_model_wave(self.ptr, sb.ptr, _dptr(psi_in), _dptr(psi_out)), "hh_model_wave")
+
+    def batch(self, sbs):
+        """Group bound superblocks for one-launch-per-stage H.psi (at most 31)."""
+        arr = (C.c_void_p * len(sbs))(*[sb.ptr for sb in sbs])
+        p = C.c_void_p()
+        self._rc(L.load().hh_batch_create(self.ptr, len(sbs), arr, C.byref(p)), "hh_batch_create")
+        return Batch(self, p, sbs)
+
+    def batch_model_wave(self, batch, psi_in, psi_out):
+        """psi_out[i] = H_i psi_in[i] for every member of the batch."""
+        _sync()
+        n = len(batch._keep)
+        if len(psi_in) != n or len(psi_out) != n:
+            raise HHError("one input and one output vector per batch member")
+        a = (C.c_void_p * n)(*[_dptr(t).value for t in psi_in])
+        b = (C.c_void_p * n)(*[_dptr(t).value for t in psi_out])
+        self._rc(L.load().hh_batch_model_wave(self.ptr, batch.ptr, a, b), "hh_batch_model_wave")
 
     def lanczos(self, sb, psi, lanzero, max_iter=100):
         """lanczos_memo_new (code/lanczos.f90:4-176), levels=1; psi updated in place."""

@@ -29,6 +29,7 @@ typedef struct hh_ctx hh_ctx;
 typedef struct hh_block hh_block;
 typedef struct hh_superblock hh_superblock;
 typedef struct hh_trunc hh_trunc;
+typedef struct hh_batch hh_batch;
 
 /* Model and lattice: replaces the hard-coded couplings and lattice setup of
  * program Quantum (code/quantum.f90:93-160), dmrg_setup
@@ -86,6 +87,11 @@ int hh_ctx_destroy(hh_ctx* ctx);
 const char* hh_last_error(const hh_ctx* ctx);
 /* couplings / lattice (code/quantum.f90:93-160, code/warm_up.f90:347-666) */
 int hh_set_model(hh_ctx* ctx, const hh_model_params* p);
+/* sites of a block of `length` sites that keep operators: those with a
+ * lattice neighbour outside the block (dmrg_setup0, code/warm_up.f90:311-341;
+ * the environment uses the same labels counted from its own end).  *n is set
+ * to the count; sites may be null to query it. */
+int hh_boundary_sites(hh_ctx* ctx, int length, int32_t* n, int32_t* sites);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
 int hh_dev_alloc(hh_ctx* ctx, int64_t bytes, void** dev);
@@ -111,8 +117,10 @@ int hh_block_sites(const hh_block* b, int32_t* sites);
  * pass NULL arrays to query nsub.  op_off[site][kind] offsets via hh_block_op_offset. */
 int hh_block_layout(const hh_block* b, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off);
 int hh_block_op_offset(const hh_block* b, int site_index, int kind, int64_t* off);
-int hh_block_upload(hh_ctx* ctx, hh_block* b, const double* host);
-int hh_block_download(hh_ctx* ctx, const hh_block* b, double* host);
+/* copy the whole arena in / out (complex128, interleaved); the other side may
+ * be host or device memory */
+int hh_block_upload(hh_ctx* ctx, hh_block* b, const double* src);
+int hh_block_download(hh_ctx* ctx, const hh_block* b, double* dst);
 
 /* ---- superblock: hmerge + task compilation (code/mat_vec.f90:3623-3852) -- */
 /* Binds sys and env (borrowed until destroy) at total N and S = 0; builds the
@@ -133,6 +141,19 @@ int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int
 /* H.psi: replaces model_wave(sys, env, sys_bs, env_bs, invec, outvec)
  * (code/mat_vec.f90:3476); psi_out is overwritten. */
 int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in_dev, double* psi_out_dev);
+
+/* Batched H.psi over independent superblocks (different flux, doping or
+ * geometry): one grouped launch per H.psi stage for all of them.  No
+ * counterpart in the reference, which solves one superblock at a time; it is
+ * model_wave (code/mat_vec.f90:3476) applied to each member.  At most 31
+ * superblocks; they must stay bound while the batch exists. */
+typedef struct hh_batch hh_batch;
+int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** out);
+int hh_batch_destroy(hh_batch* b);
+/* total psi length over the members and reference-order flops of one call */
+int hh_batch_info(const hh_batch* b, int64_t* total_dim, double* flops_alg);
+/* psi_out_dev[i] = H_i psi_in_dev[i] for every member i */
+int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in_dev, double* const* psi_out_dev);
 
 /* Lanczos: replaces lanczos_memo_new (code/lanczos.f90:4-176) with levels=1:
  * psi_dev holds the start vector on entry and the ground state on exit. */
@@ -201,7 +222,7 @@ typedef struct {
 } hh_stats;
 /* profile = 1 records HIP events around every H.psi launch (adds a sync).
  * While it is on, the H.psi tile kernels run as their own instantiations
- * (k_tiles<double, 3> stage 1, k_tiles<double, 4> stage 2, identical code),
+ * (k_tiles<double, 3, ...> stage 1, k_tiles<double, 4, ...> stage 2, identical code),
  * so a kernel-trace profile separates the timed launches from all others. */
 int hh_set_profiling(hh_ctx* ctx, int profile);
 int hh_get_stats(hh_ctx* ctx, hh_stats* out);

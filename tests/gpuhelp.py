@@ -26,6 +26,40 @@ def upload_block(ctx, block, side, nkinds):
     return b
 
 
+def oracle_block(ctx, blk, model):
+    """Product block (device arena) -> oracle Block with the same data."""
+    from oracle import dmrg as odmrg
+    arena = blk.download()
+    N, S2, d, h = blk.sectors()
+    sites = blk.sites()
+    nk = 1 if model.kind == "hubbard" else 3
+    H = odmrg.Op(0, 0, False)
+    for i, dim in enumerate(d):
+        H.blk[(i, i)] = arena[h[i]:h[i] + dim * dim].reshape((dim, dim), order="F").copy()
+    ops = {}
+    for si, k in enumerate(sites):
+        ops[int(k)] = {}
+        for kind in range(nk):
+            base = ctx.op_offset(blk, si, kind)
+            bra, ket, off = ctx.layout(blk, kind)
+            op = odmrg.Op([1, 2, 0][kind], [1, 0, 0][kind], kind == 0)
+            for b_, k_, o in zip(bra, ket, off):
+                r, c = d[b_], d[k_]
+                op.blk[(int(b_), int(k_))] = arena[base + o: base + o + r * c].reshape((r, c), order="F").copy()
+            ops[int(k)][KIND_NAMES[kind]] = op
+    _, _, _, length = blk.info()
+    return odmrg.Block(length, list(zip(N.tolist(), S2.tolist())), d.tolist(), H, ops)
+
+
+def oracle_superblock(ctx, sysb, envb, model, n_elec):
+    """Oracle H of the superblock bound from two product blocks."""
+    from oracle import dmrg as odmrg
+    so, eo = oracle_block(ctx, sysb, model), oracle_block(ctx, envb, model)
+    sg = odmrg.StepGeometry(model, so.length, eo.length)
+    return odmrg.Superblock(odmrg.enlarge_in_geometry(so, model, "sys", sg),
+                            odmrg.enlarge_in_geometry(eo, model, "env", sg), sg, model, n_elec)
+
+
 def block_h_spectra(ctx, b):
     """Per-sector eigenvalues of H (gauge invariant)."""
     arena = b.download()
