@@ -1375,6 +1375,13 @@ int hh_driver_block(const hh_driver* h, int side, int length, const hh_block** o
   return 0;
 }
 
+int hh_set_precision(hh_ctx* ctx, int precision) {
+  return guard(ctx, [&] {
+    need(precision == PREC_FP64 || precision == PREC_FP32, "precision must be 0 (FP64) or 1 (FP32)");
+    ctx->precision = precision;
+  });
+}
+
 int hh_set_profiling(hh_ctx* ctx, int profile) {
   ctx->profile = profile != 0;
   return 0;

@@ -361,6 +361,10 @@ class Context:
     def set_profiling(self, on):
         L.load().hh_set_profiling(self.ptr, 1 if on else 0)
 
+    def set_precision(self, precision):
+        """0: FP64 MFMA H.psi (plain form); 1: FP32 MFMA H.psi, FP64 vectors."""
+        self._rc(L.load().hh_set_precision(self.ptr, int(precision)), "hh_set_precision")
+
     def stats(self):
         s = L.Stats()
         L.load().hh_get_stats(self.ptr, C.byref(s))

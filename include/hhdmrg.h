@@ -220,6 +220,10 @@ typedef struct {
   double stage2_flops_alg;   /* part of flops_alg done by the stage-2 launches */
   double reduce_ms;          /* kernel time of the split-sum reductions of both stages */
 } hh_stats;
+/* Arithmetic of the H.psi tile kernels for the calls that follow on this
+ * context: 0 FP64 MFMA (plain form), 1 FP32 MFMA on FP64-stored operands
+ * with FP64 output.  Lanczos vectors and reductions stay FP64. */
+int hh_set_precision(hh_ctx* ctx, int precision);
 /* profile = 1 records HIP events around every H.psi launch (adds a sync).
  * While it is on, the H.psi tile kernels run as their own instantiations
  * (k_tiles<double, 3, ...> stage 1, k_tiles<double, 4, ...> stage 2, identical code),
