@@ -110,7 +110,7 @@ def run_c3(args):
     s2_tflops = st["stage2_flops_alg"] / max(st["stage2_ms"], 1e-30) / 1e9
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "hh::k_tiles<double, 2, hh::BufTable> (batched H.psi stage 2)",
+            "kernel": "hh::k_tiles<double, 2, hh::BufTable, *> (batched H.psi stage 2)",
             "avg_launch_ms": round(st["stage2_ms"] / max(st["matvecs"], 1), 4), "launches": int(st["matvecs"]),
             "stage1_ms_per_launch": st["stage1_ms"] / max(st["matvecs"], 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(st["matvecs"], 1)}
@@ -243,7 +243,7 @@ def main():
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "hh::k_tiles<double, 4, hh::BufPtrs> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
+            "kernel": "hh::k_tiles<double, 4, hh::BufPtrs, *> (H.psi stage 2)", "avg_launch_ms": round(s2_avg_ms, 4),
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
             "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(s2_launches, 1)}

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <stdexcept>
+#include <type_traits>
 
 #include <cmath>
 
@@ -108,6 +109,18 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
           }
         }
   }
+  // share of the MFMA work that falls on empty 16x16 sub-tiles of edge tiles
+  {
+    double all = 0.0, empty = 0.0;
+    for (auto& it : items) {
+      const Tile& t = tl.tiles[it.second.tile];
+      const int mv = std::min(MTR, t.M - it.second.r0), nv = std::min(MTC, t.N - it.second.c0);
+      const int full = (MTR / 16) * (MTC / 16), used = ((mv + 15) / 16) * ((nv + 15) / 16);
+      all += it.first * full;
+      empty += it.first * (full - used);
+    }
+    edge_skip = all > 0.0 && empty > 0.2 * all;
+  }
   // longest work items first (LPT)
   std::stable_sort(items.begin(), items.end(),
                    [](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
@@ -177,7 +190,7 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // other: no LDS, no barriers.  Two waves per SIMD (194 VGPRs): the register
 // double buffer of the operand pipeline does not fit three.
 // ----------------------------------------------------------------------------
-template <typename T, int TAG, typename Bufs>
+template <typename T, int TAG, typename Bufs, bool EDGE_SKIP>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
@@ -253,52 +266,62 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     for (int a = 0; a < WA; ++a) pa[a] = (uint32_t)(min(r0 + 16 * a + li, r0 + Mv - 1) * a_rs);
 #pragma unroll
     for (int b = 0; b < WB; ++b) pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
-    // software pipeline over two register buffers: the operands of k-step kb
-    // are converted to MFMA inputs, the loads of step kb+4 go out into the
-    // other buffer, then the MFMAs of step kb run while those loads fly
-    double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
-    auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], int k0) {
-      const uint32_t kc = (uint32_t)min(k0 + lk, kmax);
+    // With EDGE_SKIP, macro tiles that reach past the region edge skip the
+    // MFMAs of 16x16 sub-tiles holding no output (interior tiles run the
+    // branch-free copy).  Chosen per task list: it pays where small sectors
+    // leave much of each 32x32 tile empty, and costs a little elsewhere.
+    auto run_term = [&](auto edge_c) {
+      constexpr bool EDGE = decltype(edge_c)::value;
+      // software pipeline over two register buffers: the operands of k-step kb
+      // are converted to MFMA inputs, the loads of step kb+4 go out into the
+      // other buffer, then the MFMAs of step kb run while those loads fly
+      double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
+      auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], int k0) {
+        const uint32_t kc = (uint32_t)min(k0 + lk, kmax);
 #pragma unroll
-      for (int a = 0; a < WA; ++a) xa[a] = Ab[pa[a] + kc * (uint32_t)a_ks];
+        for (int a = 0; a < WA; ++a) xa[a] = Ab[pa[a] + kc * (uint32_t)a_ks];
 #pragma unroll
-      for (int b = 0; b < WB; ++b) xb[b] = Bb[pb[b] + kc * (uint32_t)b_ks];
-    };
-    auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB],
-                    int kb) {
-      // only the last k-step of a term can hold k >= K
-      const double kf = (kb + 4 <= K || kb + lk < K) ? 1.0 : 0.0;
-      T ar[WA], ai[WA], br[WB], bi[WB];
+        for (int b = 0; b < WB; ++b) xb[b] = Bb[pb[b] + kc * (uint32_t)b_ks];
+      };
+      auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB],
+                      int kb) {
+        // only the last k-step of a term can hold k >= K
+        const double kf = (kb + 4 <= K || kb + lk < K) ? 1.0 : 0.0;
+        T ar[WA], ai[WA], br[WB], bi[WB];
 #pragma unroll
-      for (int a = 0; a < WA; ++a) {
-        const double xr = kf * xa[a].x, xi = kf * sa * xa[a].y;
-        ar[a] = (T)(cr * xr - ci * xi);
-        ai[a] = (T)(cr * xi + ci * xr);
-      }
-#pragma unroll
-      for (int b = 0; b < WB; ++b) {
-        br[b] = (T)xb[b].x;
-        bi[b] = (T)(sb * xb[b].y);
-      }
-      // unconditional (k is clamped), so the wait counts are path-independent
-      load(ya, yb, kb + 4);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int a = 0; a < WA; ++a)
+        for (int a = 0; a < WA; ++a) {
+          const double xr = kf * xa[a].x, xi = kf * sa * xa[a].y;
+          ar[a] = (T)(cr * xr - ci * xi);
+          ai[a] = (T)(cr * xi + ci * xr);
+        }
 #pragma unroll
         for (int b = 0; b < WB; ++b) {
-          accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
-          accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
-          accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
-          accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
+          br[b] = (T)xb[b].x;
+          bi[b] = (T)(sb * xb[b].y);
         }
+        // unconditional (k is clamped), so the wait counts are path-independent
+        load(ya, yb, kb + 4);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int a = 0; a < WA; ++a)
+#pragma unroll
+          for (int b = 0; b < WB; ++b) {
+            if (EDGE && (16 * a >= Mv || 16 * b >= Nv)) continue;   // sub-tile past the region edge
+            accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
+            accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
+            accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
+            accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
+          }
+      };
+      load(xa0, xb0, 0);
+      for (int kb = 0; kb < K; kb += 8) {
+        step(xa0, xb0, xa1, xb1, kb);
+        if (kb + 4 >= K) break;
+        step(xa1, xb1, xa0, xb0, kb + 4);
+      }
     };
-    load(xa0, xb0, 0);
-    for (int kb = 0; kb < K; kb += 8) {
-      step(xa0, xb0, xa1, xb1, kb);
-      if (kb + 4 >= K) break;
-      step(xa1, xb1, xa0, xb0, kb + 4);
-    }
+    if (EDGE_SKIP && (Mv < MTR || Nv < MTC)) run_term(std::integral_constant<bool, EDGE_SKIP>{});
+    else run_term(std::false_type{});
   }
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
@@ -353,14 +376,22 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
+template <typename T, typename Bufs, bool SKIP>
+static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
+  const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
+#define HH_LAUNCH(TG) \
+  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, SKIP>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial)
+  if (tag == 1) HH_LAUNCH(1);
+  else if (tag == 2) HH_LAUNCH(2);
+  else if (tag == 3) HH_LAUNCH(3);
+  else if (tag == 4) HH_LAUNCH(4);
+  else HH_LAUNCH(0);
+#undef HH_LAUNCH
+}
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
-  if (tag == 1) hipLaunchKernelGGL((k_tiles<T, 1, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 2) hipLaunchKernelGGL((k_tiles<T, 2, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 3) hipLaunchKernelGGL((k_tiles<T, 3, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else if (tag == 4) hipLaunchKernelGGL((k_tiles<T, 4, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
-  else hipLaunchKernelGGL((k_tiles<T, 0, Bufs>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial);
+  if (t.edge_skip) launch_ts<T, Bufs, true>(t, b, s, tag);
+  else launch_ts<T, Bufs, false>(t, b, s, tag);
 }
 
 template <typename Bufs>

@@ -31,6 +31,7 @@ struct DevTasks {
   double2* partial = nullptr;
   int ntiles = 0, nterms = 0, nwork = 0, nred = 0, nslots = 0;
   double flops_alg = 0.0, flops_exec = 0.0;
+  bool edge_skip = false;   // > 20 % of the MFMA work would land on empty sub-tiles
   void upload(const TaskList& tl, hipStream_t s);
   void release();
   ~DevTasks() { release(); }
