@@ -223,106 +223,133 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       accR[a][b] = V{0, 0, 0, 0};
       accI[a][b] = V{0, 0, 0, 0};
     }
+  // elementwise (K = 0) terms first, straight into the accumulators
   for (int ti = w.t0; ti < w.t1; ++ti) {
     const Term tm = terms[ti];
+    if (tm.K != 0) continue;
     const double2* __restrict__ A = bufs.p[tm.a_buf];
-    if (tm.K == 0) {
-      // elementwise term, added straight into the accumulators
-      const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
+    const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
+#pragma unroll
+    for (int a = 0; a < WA; ++a)
+#pragma unroll
+      for (int b = 0; b < WB; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
+          if (r < Mv && c < Nv) {
+            double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
+            accR[a][b][v] += (T)(tm.cr * x.x - tm.ci * x.y);
+            accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
+          }
+        }
+  }
+  // GEMM terms (K > 0): one software pipeline over the whole term range.  The
+  // operands of k-step kb are converted to MFMA inputs, the loads of the next
+  // k-step go out into the other register buffer (the first k-step of the next
+  // term when kb is the last one of this term), then the MFMAs of kb run while
+  // those loads fly.  Every load is unconditional and in bounds: rows / columns
+  // past the region edge read the last valid row / column (never stored), k
+  // past K reads k = K-1 and is zeroed in the A operand at use.
+  struct OpAddr {
+    const double2* Ab;
+    const double2* Bb;
+    uint32_t pa[WA], pb[WB];   // k-independent part of each lane's element offset
+    int a_ks, b_ks, K;
+    double sa, sb, cr, ci;
+  };
+  auto make_op = [&](const Term& tm) {
+    OpAddr o;
+    const bool a_rowfast = (tm.a_op == OP_NN || tm.a_op == OP_R);
+    const bool b_kfast = (tm.b_op == OP_NN || tm.b_op == OP_R);
+    o.Ab = bufs.p[tm.a_buf] + tm.a_off;
+    o.Bb = bufs.p[tm.b_buf] + tm.b_off;
+    const int a_rs = a_rowfast ? 1 : tm.a_ld, b_cs = b_kfast ? tm.b_ld : 1;
+    o.a_ks = a_rowfast ? tm.a_ld : 1;
+    o.b_ks = b_kfast ? 1 : tm.b_ld;
+#pragma unroll
+    for (int a = 0; a < WA; ++a) o.pa[a] = (uint32_t)(min(r0 + 16 * a + li, r0 + Mv - 1) * a_rs);
+#pragma unroll
+    for (int b = 0; b < WB; ++b) o.pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
+    o.K = tm.K;
+    o.sa = (tm.a_op == OP_C || tm.a_op == OP_R) ? -1.0 : 1.0;
+    o.sb = (tm.b_op == OP_C || tm.b_op == OP_R) ? -1.0 : 1.0;
+    o.cr = tm.cr;
+    o.ci = tm.ci;
+    return o;
+  };
+  auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], const OpAddr& o, int k0) {
+    const uint32_t kc = (uint32_t)min(k0 + lk, o.K - 1);
+#pragma unroll
+    for (int a = 0; a < WA; ++a) xa[a] = o.Ab[o.pa[a] + kc * (uint32_t)o.a_ks];
+#pragma unroll
+    for (int b = 0; b < WB; ++b) xb[b] = o.Bb[o.pb[b] + kc * (uint32_t)o.b_ks];
+  };
+  auto next_gemm = [&](int ti) {
+    while (ti < w.t1 && terms[ti].K == 0) ++ti;
+    return ti;
+  };
+  auto run = [&](auto edge_c) {
+    constexpr bool EDGE = decltype(edge_c)::value;
+    int ti = next_gemm(w.t0);
+    if (ti >= w.t1) return;
+    OpAddr c = make_op(terms[ti]);
+    int tn = next_gemm(ti + 1);
+    Term dn = terms[tn < w.t1 ? tn : ti];   // descriptor of the following GEMM term
+    double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
+    load(xa0, xb0, c, 0);
+    int kb = 0;
+    bool done = false;
+    auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB]) {
+      // only the last k-step of a term can hold k >= K
+      const double kf = (kb + 4 <= c.K || kb + lk < c.K) ? 1.0 : 0.0;
+      T ar[WA], ai[WA], br[WB], bi[WB];
+#pragma unroll
+      for (int a = 0; a < WA; ++a) {
+        const double xr = kf * xa[a].x, xi = kf * c.sa * xa[a].y;
+        ar[a] = (T)(c.cr * xr - c.ci * xi);
+        ai[a] = (T)(c.cr * xi + c.ci * xr);
+      }
+#pragma unroll
+      for (int b = 0; b < WB; ++b) {
+        br[b] = (T)xb[b].x;
+        bi[b] = (T)(c.sb * xb[b].y);
+      }
+      const bool last = kb + 4 >= c.K;
+      // both arms issue the same loads, so the wait counts stay path-independent
+      if (last && tn < w.t1) load(ya, yb, make_op(dn), 0);
+      else load(ya, yb, c, kb + 4);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int a = 0; a < WA; ++a)
 #pragma unroll
-        for (int b = 0; b < WB; ++b)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
-            if (r < Mv && c < Nv) {
-              double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
-              accR[a][b][v] += (T)(tm.cr * x.x - tm.ci * x.y);
-              accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
-            }
-          }
-      continue;
-    }
-    const double2* __restrict__ B = bufs.p[tm.b_buf];
-    const bool a_rowfast = (tm.a_op == OP_NN || tm.a_op == OP_R);
-    const bool a_conj = (tm.a_op == OP_C || tm.a_op == OP_R);
-    const bool b_kfast = (tm.b_op == OP_NN || tm.b_op == OP_R);
-    const bool b_conj = (tm.b_op == OP_C || tm.b_op == OP_R);
-    // operand bases are wave-uniform; lanes keep 32-bit element offsets
-    // (one operand sub-block is far below 2^31 elements).  Every load is
-    // unconditional: rows / columns past the region edge read the last valid
-    // row / column (their results are never stored) and k past K reads k = K-1,
-    // zeroed in the A operand at use.  No branch or wait sits next to a load.
-    const double2* __restrict__ Ab = A + tm.a_off;
-    const double2* __restrict__ Bb = B + tm.b_off;
-    const int a_rs = a_rowfast ? 1 : tm.a_ld, a_ks = a_rowfast ? tm.a_ld : 1;
-    const int b_ks = b_kfast ? 1 : tm.b_ld, b_cs = b_kfast ? tm.b_ld : 1;
-    const int K = tm.K, kmax = K - 1;
-    const double sa = a_conj ? -1.0 : 1.0, sb = b_conj ? -1.0 : 1.0;
-    const double cr = tm.cr, ci = tm.ci;
-    uint32_t pa[WA], pb[WB];   // k-independent part of each lane's offset
-#pragma unroll
-    for (int a = 0; a < WA; ++a) pa[a] = (uint32_t)(min(r0 + 16 * a + li, r0 + Mv - 1) * a_rs);
-#pragma unroll
-    for (int b = 0; b < WB; ++b) pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
-    // With EDGE_SKIP, macro tiles that reach past the region edge skip the
-    // MFMAs of 16x16 sub-tiles holding no output (interior tiles run the
-    // branch-free copy).  Chosen per task list: it pays where small sectors
-    // leave much of each 32x32 tile empty, and costs a little elsewhere.
-    auto run_term = [&](auto edge_c) {
-      constexpr bool EDGE = decltype(edge_c)::value;
-      // software pipeline over two register buffers: the operands of k-step kb
-      // are converted to MFMA inputs, the loads of step kb+4 go out into the
-      // other buffer, then the MFMAs of step kb run while those loads fly
-      double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
-      auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], int k0) {
-        const uint32_t kc = (uint32_t)min(k0 + lk, kmax);
-#pragma unroll
-        for (int a = 0; a < WA; ++a) xa[a] = Ab[pa[a] + kc * (uint32_t)a_ks];
-#pragma unroll
-        for (int b = 0; b < WB; ++b) xb[b] = Bb[pb[b] + kc * (uint32_t)b_ks];
-      };
-      auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB],
-                      int kb) {
-        // only the last k-step of a term can hold k >= K
-        const double kf = (kb + 4 <= K || kb + lk < K) ? 1.0 : 0.0;
-        T ar[WA], ai[WA], br[WB], bi[WB];
-#pragma unroll
-        for (int a = 0; a < WA; ++a) {
-          const double xr = kf * xa[a].x, xi = kf * sa * xa[a].y;
-          ar[a] = (T)(cr * xr - ci * xi);
-          ai[a] = (T)(cr * xi + ci * xr);
-        }
-#pragma unroll
         for (int b = 0; b < WB; ++b) {
-          br[b] = (T)xb[b].x;
-          bi[b] = (T)(sb * xb[b].y);
+          if (EDGE && (16 * a >= Mv || 16 * b >= Nv)) continue;   // sub-tile past the region edge
+          accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
+          accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
+          accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
+          accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
         }
-        // unconditional (k is clamped), so the wait counts are path-independent
-        load(ya, yb, kb + 4);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int a = 0; a < WA; ++a)
-#pragma unroll
-          for (int b = 0; b < WB; ++b) {
-            if (EDGE && (16 * a >= Mv || 16 * b >= Nv)) continue;   // sub-tile past the region edge
-            accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
-            accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
-            accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
-            accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
-          }
-      };
-      load(xa0, xb0, 0);
-      for (int kb = 0; kb < K; kb += 8) {
-        step(xa0, xb0, xa1, xb1, kb);
-        if (kb + 4 >= K) break;
-        step(xa1, xb1, xa0, xb0, kb + 4);
+      if (!last) {
+        kb += 4;
+      } else if (tn >= w.t1) {
+        done = true;
+      } else {
+        c = make_op(dn);
+        ti = tn;
+        tn = next_gemm(ti + 1);
+        dn = terms[tn < w.t1 ? tn : ti];
+        kb = 0;
       }
     };
-    if (EDGE_SKIP && (Mv < MTR || Nv < MTC)) run_term(std::integral_constant<bool, EDGE_SKIP>{});
-    else run_term(std::false_type{});
-  }
+    while (true) {
+      step(xa0, xb0, xa1, xb1);
+      if (done) break;
+      step(xa1, xb1, xa0, xb0);
+      if (done) break;
+    }
+  };
+  if (EDGE_SKIP && (Mv < MTR || Nv < MTC)) run(std::integral_constant<bool, EDGE_SKIP>{});
+  else run(std::false_type{});
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
