@@ -406,11 +406,20 @@ void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, 
                            work + (int64_t)k * work_stride, info + i) != rocblas_status_success)
         ok = false;
   };
-  std::vector<std::thread> th;
-  for (int k = 1; k < NS; ++k)
-    if (!mine[k].empty()) th.emplace_back(run, k);
-  run(0);
-  for (auto& t : th) t.join();
+  // HHDMRG_EIG_THREADS=0 issues every stream's calls from this thread (the
+  // streams still overlap where zheevd does not synchronise); used under
+  // tools that intercept kernel launches and are not safe with concurrent
+  // launching threads
+  static const bool threads = !(std::getenv("HHDMRG_EIG_THREADS") && std::getenv("HHDMRG_EIG_THREADS")[0] == '0');
+  if (threads) {
+    std::vector<std::thread> th;
+    for (int k = 1; k < NS; ++k)
+      if (!mine[k].empty()) th.emplace_back(run, k);
+    run(0);
+    for (auto& t : th) t.join();
+  } else {
+    for (int k = 0; k < NS; ++k) run(k);
+  }
   for (int k = 1; k < NS; ++k) {
     check_hip(hipEventRecord(ctx->join[k - 1], ctx->side[k - 1]), "event");
     check_hip(hipStreamWaitEvent(ctx->stream, ctx->join[k - 1], 0), "wait");
@@ -888,6 +897,39 @@ int guard(hh_ctx* ctx, F&& f) {
 // ============================================================================
 extern "C" {
 
+// rocBLAS / rocSOLVER set up kernels and workspaces lazily on first use.
+// The eigensolver later runs on several handles from several host threads,
+// so that first use happens here, once per handle, on one thread: every
+// handle solves small diagonal problems of the sizes that select each of
+// zheevd's code paths.
+static void warm_eigensolver(hh_ctx* c) {
+  rocblas_initialize();
+  const int sizes[3] = {8, 100, 600};
+  const int nmax = 600;
+  DevMem a, w, e, inf;
+  a.alloc(sizeof(double2) * nmax * nmax);
+  w.alloc(sizeof(double) * nmax);
+  e.alloc(sizeof(double) * nmax);
+  inf.alloc(sizeof(rocblas_int));
+  std::vector<double2> h((size_t)nmax * nmax);
+  rocblas_handle hs[hh_ctx::NSIDE + 1] = {c->rb};
+  hipStream_t ss[hh_ctx::NSIDE + 1] = {c->stream};
+  for (int k = 0; k < hh_ctx::NSIDE; ++k) {
+    hs[k + 1] = c->rbs[k];
+    ss[k + 1] = c->side[k];
+  }
+  for (int k = 0; k <= hh_ctx::NSIDE; ++k)
+    for (int n : sizes) {
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) h[i + (size_t)j * n] = make_double2(i == j ? 1.0 + i : 0.0, 0.0);
+      check_hip(hipMemcpyAsync(a.p, h.data(), sizeof(double2) * n * n, hipMemcpyHostToDevice, ss[k]), "h2d");
+      if (rocsolver_zheevd(hs[k], rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex*)a.p, n,
+                           (double*)w.p, (double*)e.p, (rocblas_int*)inf.p) != rocblas_status_success)
+        throw HHError(-2, "rocsolver_zheevd warm-up failed");
+      check_hip(hipStreamSynchronize(ss[k]), "eigensolver warm-up");
+    }
+}
+
 int hh_ctx_create(int device, hh_ctx** out) {
   if (!out) return 1;
   auto* c = new hh_ctx();
@@ -908,6 +950,7 @@ int hh_ctx_create(int device, hh_ctx** out) {
       rocblas_set_stream(c->rbs[k], c->side[k]);
       check_hip(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming), "hipEventCreate");
     }
+    warm_eigensolver(c);
   });
   if (rc != 0) {
     create_error() = c->err;
