@@ -96,6 +96,7 @@ SIGNATURES = [
     ("hh_driver_block", i32, [vp, i32, i32, P(vp)]),
     ("hh_set_profiling", i32, [vp, i32]),
     ("hh_set_precision", i32, [vp, i32]),
+    ("hh_driver_density", i32, [vp, i32, vp, P(f64)]),
     ("hh_get_stats", i32, [vp, P(Stats)]),
     ("hh_reset_stats", i32, [vp]),
     ("hh_superblock_tasks", i32, [vp, i32, P(vp), P(i32), P(vp), P(i32), P(i64)]),

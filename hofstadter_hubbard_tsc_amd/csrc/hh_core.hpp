@@ -225,6 +225,33 @@ TransPlan compile_wave_trans_lr(const SuperBasis& old_sb, const EnlBasis& old_sy
                                 const std::vector<KeptSector>& sys_trun,
                                 const std::vector<KeptSector>& env_trun_of_E,
                                 const EnlBasis& new_sys, const EnlBasis& new_env, const SuperBasis& new_sb);
+// ----------------------------------------------------------------------------
+// site densities (measure_operator_dia, code/measure.f90:3-61)
+// ----------------------------------------------------------------------------
+// Rank-0 operators of a block are stored sector-diagonal: the d x d block of
+// every sector, in sector order, one operator after another.
+int64_t diag_op_size(const std::vector<Sector>& sec);
+std::vector<int64_t> diag_op_offsets(const std::vector<Sector>& sec);
+struct TrackPlan {
+  EnlBasis eb;
+  int64_t x_size = 0, old_size = 0, new_size = 0;
+  TaskList stage1, stage2;
+};
+// One growth step of the operator chain of Get_sys_oper_dia / Get_env_oper_dia:
+// the nold tracked operators of block `prev` (slot B_SYS) embedded in the
+// enlarged basis and renormalised with the truncation `kept` (U in B_AUX),
+// plus the density of the new site; nold + 1 operators of the new block to
+// B_OUT, scratch in B_X.
+TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold);
+// rho'_b = (1/sqrt(2S_b+1)) sum over psi blocks and site states of the block
+// part of psi (B_IN) for the block sectors `blk` of `side`, to B_OUT in the
+// sector-diagonal layout: <O> = Re <O, rho'> for a tracked Hermitian O.
+TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk);
+// B_OUT += n_site psi on the rows (side 0) or columns (side 1) of the centre
+// site of `side`: <n_site> = Re <psi, out> with out zeroed beforehand.
+TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb);
+
 TransPlan compile_wave_trans_rl(const SuperBasis& old_sb, const EnlBasis& old_sys, const EnlBasis& old_env,
                                 const std::vector<KeptSector>& sys_trun_of_A,
                                 const std::vector<KeptSector>& env_trun,

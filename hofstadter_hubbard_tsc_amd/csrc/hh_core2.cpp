@@ -313,6 +313,113 @@ TaskList compile_density(const SuperBasis& sb, int side, std::vector<int>& rho_s
 }
 
 // ============================================================================
+// site densities: operator chain of Get_sys_oper_dia / Get_env_oper_dia and
+// the expectation of measure_operator_dia (code/measure.f90:3-61)
+// ============================================================================
+int64_t diag_op_size(const std::vector<Sector>& sec) {
+  int64_t n = 0;
+  for (auto& s : sec) n += (int64_t)s.dim * s.dim;
+  return n;
+}
+
+std::vector<int64_t> diag_op_offsets(const std::vector<Sector>& sec) {
+  std::vector<int64_t> o;
+  int64_t n = 0;
+  for (auto& s : sec) {
+    o.push_back(n);
+    n += (int64_t)s.dim * s.dim;
+  }
+  return o;
+}
+
+TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold) {
+  TrackPlan P;
+  const auto st = site_sectors(m.kind);
+  P.eb = enlarge_basis(prev, st);
+  const EnlBasis& eb = P.eb;
+  std::vector<Sector> nsec;
+  for (auto& k : kept) nsec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
+  const std::vector<int64_t> ooff = diag_op_offsets(prev), noff = diag_op_offsets(nsec);
+  P.old_size = diag_op_size(prev);
+  P.new_size = diag_op_size(nsec);
+  Collector s1, s2;
+  for (int k = 0; k <= nold; ++k) {
+    for (int i = 0; i < (int)kept.size(); ++i) {
+      const KeptSector& ks = kept[i];
+      const int a = ks.a, J2 = eb.sec[a].S2, da = eb.sec[a].dim, kp = ks.keep;
+      const int64_t xo = P.x_size;
+      P.x_size += (int64_t)da * kp;
+      auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, da, d, kp}; };
+      for (auto& ss : eb.subs[a]) s1.ensure(rows(ss.pos, ss.dim));
+      for (auto& ss : eb.subs[a]) {
+        const int mb = ss.dim;
+        if (k < nold) {
+          // rank-0 block operator: sqrt((2J+1)/(2S_b+1)) O_b on every (b, s) (update_block_dia)
+          s1.add(rows(ss.pos, mb), mk(std::sqrt((double)(J2 + 1) / (eb.bl[ss.b].S2 + 1)), B_SYS,
+                                      (int64_t)k * P.old_size + ooff[ss.b], mb, OP_NN, B_AUX, ks.u_off + ss.pos,
+                                      ks.u_ld, OP_NN, mb));
+        } else if (st[ss.s].N != 0) {
+          // density of the new site: sqrt(2J+1) n_s (update_site_dia)
+          s1.add(rows(ss.pos, mb), mk((double)st[ss.s].N * std::sqrt((double)J2 + 1), B_AUX, ks.u_off + ss.pos,
+                                      ks.u_ld, OP_NN, 0, 0, 0, OP_NN, 0));
+        }
+      }
+      // O_new[i] = U_i^dag X
+      Region r{B_OUT, (int64_t)k * P.new_size + noff[i], kp, kp, kp};
+      s2.add(r, mk(1.0, B_AUX, ks.u_off, ks.u_ld, OP_C, B_X, xo, da, OP_NN, da));
+    }
+  }
+  s1.emit(P.stage1);
+  s2.emit(P.stage2);
+  return P;
+}
+
+TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk) {
+  const std::vector<int64_t> off = diag_op_offsets(blk);
+  Collector c;
+  for (int i = 0; i < (int)sb.pairs.size(); ++i) {
+    const int a = side == 0 ? sb.pairs[i].first : sb.pairs[i].second;
+    const int r = sb.rows[i], cc = sb.cols[i];
+    for (auto& ss : eb.subs[a]) {
+      const int d = ss.dim;
+      const Region reg{B_OUT, off[ss.b], d, d, d};
+      const double w = 1.0 / std::sqrt((double)blk[ss.b].S2 + 1);
+      if (side == 0)   // rows of the block part: psi_sub psi_sub^dag
+        c.add(reg, mk(w, B_IN, sb.off[i] + ss.pos, r, OP_NN, B_IN, sb.off[i] + ss.pos, r, OP_C, cc));
+      else   // columns: psi_sub^T conj(psi_sub)
+        c.add(reg, mk(w, B_IN, sb.off[i] + (int64_t)ss.pos * r, r, OP_T, B_IN, sb.off[i] + (int64_t)ss.pos * r, r,
+                      OP_R, r));
+    }
+  }
+  // sectors that no psi block touches still get a (zero) tile
+  for (size_t b = 0; b < blk.size(); ++b) c.ensure(Region{B_OUT, off[b], blk[b].dim, blk[b].dim, blk[b].dim});
+  TaskList tl;
+  c.emit(tl);
+  return tl;
+}
+
+TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb) {
+  TaskList tl;
+  for (int i = 0; i < (int)sb.pairs.size(); ++i) {
+    const int a = side == 0 ? sb.pairs[i].first : sb.pairs[i].second;
+    const int r = sb.rows[i], c = sb.cols[i];
+    for (auto& ss : eb.subs[a]) {
+      const int n = eb.st[ss.s].N;
+      if (n == 0) continue;
+      if (side == 0) {
+        tl.add_tile(B_OUT, sb.off[i] + ss.pos, r, ss.dim, c, 0);
+        tl.add_term(mk((double)n, B_IN, sb.off[i] + ss.pos, r, OP_NN, 0, 0, 0, OP_NN, 0));
+      } else {
+        tl.add_tile(B_OUT, sb.off[i] + (int64_t)ss.pos * r, r, r, ss.dim, 0);
+        tl.add_term(mk((double)n, B_IN, sb.off[i] + (int64_t)ss.pos * r, r, OP_NN, 0, 0, 0, OP_NN, 0));
+      }
+    }
+  }
+  return tl;
+}
+
+// ============================================================================
 // wave_trans_LR / wave_trans_RL (code/warm_up.f90:1855-2170)
 // ============================================================================
 TransPlan compile_wave_trans_lr(const SuperBasis& old_sb, const EnlBasis& old_sys, const EnlBasis& old_env,

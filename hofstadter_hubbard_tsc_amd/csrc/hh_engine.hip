@@ -60,6 +60,20 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
   nterms = (int)tl.terms.size();
   flops_alg = tl.flops_alg;
   flops_exec = tl.flops_exec;
+  // the kernel addresses an operand sub-block with 32-bit element offsets
+  // from its base: reject sub-blocks that would not fit
+  for (const Tile& t : tl.tiles)
+    for (int k = t.t0; k < t.t1; ++k) {
+      const Term& tm = tl.terms[k];
+      if (tm.K <= 0) continue;
+      const bool a_rowfast = (tm.a_op == OP_NN || tm.a_op == OP_R), b_kfast = (tm.b_op == OP_NN || tm.b_op == OP_R);
+      const int64_t a_end = a_rowfast ? (int64_t)(t.M - 1) + (int64_t)(tm.K - 1) * tm.a_ld
+                                      : (int64_t)(tm.K - 1) + (int64_t)(t.M - 1) * tm.a_ld;
+      const int64_t b_end = b_kfast ? (int64_t)(tm.K - 1) + (int64_t)(t.N - 1) * tm.b_ld
+                                    : (int64_t)(t.N - 1) + (int64_t)(tm.K - 1) * tm.b_ld;
+      if (a_end >= (int64_t(1) << 32) || b_end >= (int64_t(1) << 32))
+        throw std::runtime_error("operand sub-block exceeds 2^32 elements");
+    }
   // work per macro tile = sum of max(K, 1) over its terms
   double total = 0.0;
   std::vector<double> tw(ntiles, 0.0);

@@ -869,6 +869,94 @@ struct Driver {
   }
 };
 
+// <n_i> at every lattice position for the ground state psi of the superblock
+// (sys block sys_len, env block nsite - sys_len - 2) of the driver's stored
+// blocks: measure_operator_dia (code/measure.f90:3-61) with the operator
+// chain of Get_sys_oper_dia / Get_env_oper_dia.  out has nsite entries,
+// indexed by lattice position - 1.
+void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  const std::vector<Sector> st = site_sectors(m.kind);
+  double* dots = nullptr;
+  for (int side = 0; side < 2; ++side) {
+    const int len = side == 0 ? sys_len : env_len;
+    const auto& blocks = side == 0 ? d.sysb : d.envb;
+    const auto& truns = side == 0 ? d.syst : d.envt;
+    // length 1: the site block itself; its density has reduced elements sqrt(2S+1) N
+    std::vector<Sector> cur = blocks.at(1)->d.sec;
+    DevMem ops, nxt, X;
+    {
+      std::vector<double2> h(cur.size());
+      for (size_t i = 0; i < cur.size(); ++i) h[i] = make_double2(std::sqrt((double)cur[i].S2 + 1) * cur[i].N, 0.0);
+      ops.alloc(sizeof(double2) * h.size());
+      check_hip(hipMemcpyAsync(ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+                "h2d");
+    }
+    for (int l = 2; l <= len; ++l) {
+      need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
+      const hh_trunc* t = truns.at(l).get();
+      TrackPlan P = compile_track_step(m, cur, t->kept, l - 1);
+      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * l, 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = ops.c();
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nxt.c();
+      DevTasks s1, s2;
+      s1.upload(P.stage1, ctx->stream);
+      s2.upload(P.stage2, ctx->stream);
+      launch_tiles(s1, b, ctx->stream, PREC_FP64);
+      launch_tiles(s2, b, ctx->stream, PREC_FP64);
+      check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+      std::swap(ops.p, nxt.p);
+      std::swap(ops.bytes, nxt.bytes);
+      cur.clear();
+      for (auto& k : t->kept) cur.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+    }
+    // block sites: <O_k> = Re <O_k, rho'>
+    const EnlBasis& eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
+    const int64_t osz = diag_op_size(cur);
+    DevMem rho, Y, part, res;
+    rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(osz, 1));
+    Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+    part.alloc(sizeof(double) * (size_t)std::max(vec_blocks(osz), vec_blocks(S.dim)) * (len + 1));
+    res.alloc(sizeof(double) * (len + 1));
+    dots = (double*)res.p;
+    {
+      DevTasks rt;
+      rt.upload(compile_block_rho(S, side, eb, cur), ctx->stream);
+      BufPtrs b{};
+      b.p[B_IN] = (double2*)psi;
+      b.p[B_OUT] = rho.c();
+      launch_tiles(rt, b, ctx->stream, PREC_FP64);
+    }
+    dev_mdots(ops.c(), osz, len, rho.c(), osz, dots, (double*)part.p, ctx->stream);
+    // centre site: <n> = Re <psi, n_site psi>
+    dev_zero(Y.c(), S.dim, ctx->stream);
+    {
+      DevTasks yt;
+      yt.upload(compile_site_density(S, side, eb), ctx->stream);
+      BufPtrs b{};
+      b.p[B_IN] = (double2*)psi;
+      b.p[B_OUT] = Y.c();
+      launch_tiles(yt, b, ctx->stream, PREC_FP64);
+    }
+    dev_mdots(psi, 0, 1, Y.c(), S.dim, dots + len, (double*)part.p, ctx->stream);
+    std::vector<double> h(len + 1);
+    check_hip(hipMemcpyAsync(h.data(), dots, sizeof(double) * (len + 1), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "density");
+    for (int k = 1; k <= len; ++k) out[(side == 0 ? k : nsite + 1 - k) - 1] = h[k - 1];
+    out[(side == 0 ? sys_len + 1 : nsite - env_len) - 1] = h[len];
+  }
+}
+
 // message of a failed hh_ctx_create (the context no longer exists)
 std::string& create_error() {
   static thread_local std::string s;
@@ -1416,6 +1504,14 @@ int hh_driver_block(const hh_driver* h, int side, int length, const hh_block** o
   if (it == m.end()) return 1;
   *out = it->second.get();
   return 0;
+}
+
+int hh_driver_density(hh_driver* h, int sys_len, const double* psi, double* n_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && n_out, "null argument");
+    density_profile(h->d.ctx, h->d, sys_len, (const double2*)psi, n_out);
+  });
 }
 
 int hh_set_precision(hh_ctx* ctx, int precision) {

@@ -414,6 +414,16 @@ class Driver:
                                               1 if fresh else 0, phase, buf, cap, C.byref(n)), "hh_driver_sweep")
         return self._recs(buf, min(n.value, cap))
 
+    def density(self, sys_len, psi):
+        """<n_i> per lattice position (index i-1) in the ground state psi of the
+        superblock of the stored sys block of sys_len sites (measure_operator_dia)."""
+        _sync()
+        out = np.zeros(self.ctx.model.nsite if hasattr(self.ctx.model, "nsite") else
+                       self.ctx.model.Lx * self.ctx.model.Ly)
+        self.ctx._rc(L.load().hh_driver_density(self.ptr, sys_len, _dptr(psi),
+                                                out.ctypes.data_as(C.POINTER(C.c_double))), "hh_driver_density")
+        return out
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

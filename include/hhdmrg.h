@@ -206,6 +206,12 @@ int hh_driver_block_count(const hh_driver* d, int side, int32_t* count);
 /* borrowed handle of a stored block (side 0 sys / 1 env, by length); valid
  * until the driver replaces or frees it */
 int hh_driver_block(const hh_driver* d, int side, int length, const hh_block** out);
+/* measure_operator_dia (code/measure.f90:3-61) with the operator chain of
+ * Get_sys_oper_dia / Get_env_oper_dia (the stored truncations of every
+ * length): <n_i> for every lattice position i, n_out[i-1], in the ground
+ * state psi_dev of the superblock bound from the driver's sys block of
+ * sys_len sites and env block of Nsite - sys_len - 2 sites. */
+int hh_driver_density(hh_driver* d, int sys_len, const double* psi_dev, double* n_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

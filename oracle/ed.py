@@ -155,3 +155,23 @@ def singlet_spectrum(model, positions, bonds, n_elec, k=None):
     kk = k or 4
     w = spla.eigsh(H + 50.0 * S2, k=kk, which="SA", return_eigenvectors=False)
     return np.sort(np.real(w))
+
+
+def singlet_ground_density(model, positions, bonds, n_elec):
+    """(E0, <n> per lattice position) of the lowest total-S = 0 state;
+    the density array is indexed by position - 1."""
+    H, fs = hamiltonian(model, positions, bonds, n_elec)
+    S2 = casimir(fs)
+    dim = H.shape[0]
+    if dim <= 3000:
+        w, v = np.linalg.eigh((H + 50.0 * S2).toarray())
+        e, psi = w[0], v[:, 0]
+    else:
+        w, v = spla.eigsh(H + 50.0 * S2, k=1, which="SA")
+        e, psi = w[0], v[:, 0]
+    p = np.abs(psi) ** 2
+    out = np.zeros(max(positions))
+    for j, pos in enumerate(positions):
+        occ = ((fs.states >> (2 * j)) & 1) + ((fs.states >> (2 * j + 1)) & 1)
+        out[pos - 1] = float(np.dot(p, occ))
+    return float(np.real(e)), out

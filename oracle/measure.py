@@ -1,0 +1,95 @@
+"""Site densities <n_i> of a DMRG ground state -- TEST INFRASTRUCTURE ONLY.
+
+Restates ``measure_operator_dia`` (code/measure.f90:3-61) with the operator
+chain of ``Get_sys_oper_dia`` / ``Get_env_oper_dia``: the density of site i
+enters as the new-site operator of the block of length i and is carried to
+the measured block through every later truncation,
+
+    O_i(l) = U_l^dag  embed_block(O_i(l-1))  U_l ,   O_i(i) = U_i^dag embed_site(n) U_i,
+
+then read off the superblock ground state.  For a rank-0 block operator the
+expectation reduces to Tr(rho_b O_b) / sqrt(2 S_b + 1) summed over the block
+sectors b, with rho_b the block's reduced density matrix; the two centre
+sites read sum_(b,s) n_s |psi_(b,s)|^2.
+
+Pinned by exact diagonalisation (``ed.singlet_ground_density``): with no
+truncation the profile equals the exact one; the sum over all sites is N.
+"""
+import math
+
+import numpy as np
+
+from . import dmrg
+
+
+def _site_density_op(kind):
+    return dmrg.site_ops(kind)["n"]
+
+
+def _transform(op_enl, trun):
+    """U^dag O U for a rank-0 enlarged operator; trun = [(enlarged sector, U)]."""
+    out = dmrg.Op(0, 0, False)
+    for i, (a, U) in enumerate(trun):
+        m = op_enl.get((a, a))
+        out.blk[(i, i)] = U.conj().T @ m @ U if m is not None else np.zeros((U.shape[1], U.shape[1]), complex)
+    return out
+
+
+def tracked_ops(model, blocks, truns, length, side):
+    """{site k: O_k in the basis of blocks[length]} for k = 1 .. length."""
+    st = dmrg.site_sectors(model.kind)
+    n = _site_density_op(model.kind)
+    ops = {1: n}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        new = {}
+        for k, o in ops.items():
+            new[k] = _transform(dmrg.embed_block_op(o, prev, eb, side), truns[l])
+        new[l] = _transform(dmrg.embed_site_op(n, prev, eb, side), truns[l])
+        ops = new
+    return ops
+
+
+def _block_rho(H, psi, side):
+    """rho_b over the block sectors of `side` (sum over pairs and site states),
+    and the centre-site density sum_(b,s) n_s |psi_(b,s)|^2."""
+    sb = H.sb
+    pin = dmrg.psi_blocks(sb, psi)
+    en = H.s if side == "sys" else H.e
+    eb = en.basis
+    nsite = [st[0] for st in eb.st_sectors]   # N of each site multiplet
+    rho = {}
+    site_n = 0.0
+    for i, (x, y) in enumerate(sb.pairs):
+        a = x if side == "sys" else y
+        for (b, s, pos, d) in eb.subs[a]:
+            blk = pin[i][pos:pos + d, :] if side == "sys" else pin[i][:, pos:pos + d]
+            # sys: rho_b = psi psi^dag; env: rho_b[c, c'] = sum_r psi[r, c] conj(psi[r, c'])
+            r = blk @ blk.conj().T if side == "sys" else blk.T @ blk.conj()
+            rho[b] = rho.get(b, 0.0) + r
+            site_n += nsite[s] * float(np.real(np.vdot(blk, blk)))
+    return rho, site_n
+
+
+def density_profile(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H):
+    """<n> at every lattice position (index position-1) for the ground state
+    psi of the superblock H built from sys_blocks[sys_len], env_blocks[env_len]."""
+    nsite = model.nsite
+    out = np.zeros(nsite)
+    for side, blocks, truns, length in (("sys", sys_blocks, sys_truns, sys_len),
+                                        ("env", env_blocks, env_truns, env_len)):
+        ops = tracked_ops(model, blocks, truns, length, side)
+        rho, site_n = _block_rho(H, psi, side)
+        secs = blocks[length].sectors
+        for k, o in ops.items():
+            v = 0.0
+            for b, r in rho.items():
+                m = o.blk.get((b, b))
+                if m is not None:
+                    v += float(np.real(np.trace(r @ m))) / math.sqrt(secs[b][1] + 1)
+            pos = k if side == "sys" else nsite + 1 - k
+            out[pos - 1] = v
+        centre = length + 1 if side == "sys" else nsite - length
+        out[centre - 1] = site_n
+    return out

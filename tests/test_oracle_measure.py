@@ -1,0 +1,45 @@
+"""The oracle's density measurement against exact diagonalisation."""
+import numpy as np
+import pytest
+
+from oracle import dmrg, driver, ed, measure
+from oracle import lattice as lat
+
+
+@pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
+def test_untruncated_density_profile_equals_ed(kind, flux):
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    sg = dmrg.StepGeometry(m, m.nsite // 2 - 1, m.nsite - m.nsite // 2 - 1)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, dens = ed.singlet_ground_density(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    assert abs(dens.sum() - m.n_elec) < 1e-10
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sgm = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sgm),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sgm), sgm, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    prof = measure.density_profile(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    assert abs(prof.sum() - m.n_elec) < 1e-12
+    assert np.abs(prof - dens).max() < 1e-6
+
+
+def test_density_of_a_wrong_chain_fails():
+    """A planted mistake in the chain (truncation matrices off by a factor 2,
+    i.e. a lost normalisation) moves the profile far beyond the 1e-6 bar, so
+    the comparison above can fail."""
+    m = lat.Model(kind="hubbard", Lx=3, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=4)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    L, R = 2, m.nsite - 4
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    good = measure.density_profile(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    bad_truns = {l: [(a, 2.0 * U) for a, U in tr] for l, tr in run.sys_trun.items()}
+    bad = measure.density_profile(m, run.sys_blocks, run.env_blocks, bad_truns, run.env_trun, L, R, psi, H)
+    assert np.abs(good - bad).max() > 1e-3
