@@ -1403,6 +1403,24 @@ int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* ke
   return 0;
 }
 
+int hh_trunc_layout(const hh_trunc* t, int64_t* size, int64_t* u_off, int32_t* u_ld) {
+  if (!t) return 1;
+  if (size) *size = (int64_t)(t->evec.bytes / sizeof(double2));
+  for (size_t i = 0; i < t->kept.size(); ++i) {
+    if (u_off) u_off[i] = t->kept[i].u_off;
+    if (u_ld) u_ld[i] = t->kept[i].u_ld;
+  }
+  return 0;
+}
+
+int hh_trunc_download(hh_ctx* ctx, const hh_trunc* t, double* dst) {
+  return guard(ctx, [&] {
+    need(t && dst, "null argument");
+    check_hip(hipMemcpyAsync(dst, t->evec.p, t->evec.bytes, hipMemcpyDefault, ctx->stream), "download");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+
 int hh_grow_block(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t, hh_block** out) {
   return guard(ctx, [&] { *out = grow(ctx, sb, side, t); });
 }
@@ -1500,6 +1518,15 @@ int hh_driver_block_count(const hh_driver* h, int side, int32_t* count) {
 int hh_driver_block(const hh_driver* h, int side, int length, const hh_block** out) {
   if (!h || !out) return 1;
   const auto& m = side == 0 ? h->d.sysb : h->d.envb;
+  auto it = m.find(length);
+  if (it == m.end()) return 1;
+  *out = it->second.get();
+  return 0;
+}
+
+int hh_driver_trunc(const hh_driver* h, int side, int length, const hh_trunc** out) {
+  if (!h || !out) return 1;
+  const auto& m = side == 0 ? h->d.syst : h->d.envt;
   auto it = m.find(length);
   if (it == m.end()) return 1;
   *out = it->second.get();

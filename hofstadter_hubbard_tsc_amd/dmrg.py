@@ -168,7 +168,7 @@ class Trunc:
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and getattr(self, "ptr_owned", True):
                 L.load().hh_trunc_destroy(self.ptr)
         except Exception:
             pass
@@ -180,6 +180,22 @@ class Trunc:
         a, k = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
         lib.hh_trunc_kept(self.ptr, C.byref(n), L.ip(a), L.ip(k))
         return a, k
+
+    def matrices(self):
+        """[(enlarged sector, U)] in the new block's sector order (host copies)."""
+        lib = L.load()
+        a, k = self.kept()
+        size = L.i64()
+        off = np.zeros(len(a), np.int64)
+        ld = np.zeros(len(a), np.int32)
+        lib.hh_trunc_layout(self.ptr, C.byref(size), L.ip(off), L.ip(ld))
+        arena = np.zeros(size.value, np.complex128)
+        _check(self.ctx.ptr, lib.hh_trunc_download(self.ctx.ptr, self.ptr, arena.ctypes.data_as(C.c_void_p)),
+               "hh_trunc_download")
+        out = []
+        for ai, ki, o, d in zip(a, k, off, ld):
+            out.append((int(ai), arena[o:o + d * ki].reshape((d, ki), order="F").copy()))
+        return out
 
 
 @dataclass
@@ -413,6 +429,20 @@ class Driver:
         self.ctx._rc(L.load().hh_driver_sweep(self.ptr, direction, bg, end, lanzero, trun_err, kept_max,
                                               1 if fresh else 0, phase, buf, cap, C.byref(n)), "hh_driver_sweep")
         return self._recs(buf, min(n.value, cap))
+
+    def reset_transform(self):
+        """Forget the previous step, so the next sweep starts from its stored
+        psi or a fresh vector instead of a wave transformation."""
+        self.ctx._rc(L.load().hh_driver_reset_transform(self.ptr), "hh_driver_reset_transform")
+
+    def trunc(self, side, length):
+        """Borrowed handle of the stored truncation that made the block of `length`."""
+        p = C.c_void_p()
+        if L.load().hh_driver_trunc(self.ptr, side, length, C.byref(p)) != 0:
+            raise HHError("no stored truncation of length %d" % length)
+        t = Trunc(self.ctx, p)
+        t.ptr_owned = False
+        return t
 
     def density(self, sys_len, psi):
         """<n_i> per lattice position (index i-1) in the ground state psi of the

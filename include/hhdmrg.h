@@ -169,6 +169,11 @@ int hh_trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side, hh_trunc** out);
 int hh_trunc_destroy(hh_trunc* t);
 /* kept multiplets per enlarged sector (enlarged-basis order of that side) */
 int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* keep);
+/* kept eigenvectors U of kept sector i: columns of a (ld x keep[i]) block at
+ * element offset u_off[i] of the truncation's arena (arena length in *size;
+ * pass NULL arrays to query it); hh_trunc_download copies the arena out */
+int hh_trunc_layout(const hh_trunc* t, int64_t* size, int64_t* u_off, int32_t* u_ld);
+int hh_trunc_download(hh_ctx* ctx, const hh_trunc* t, double* dst);
 
 /* Block growth + renormalisation: replaces Get_Hamiltonian_Sys/Env
  * (code/get_ham.f90:4-659) and update_trun_dia/ndia (code/update.f90:1387-1605):
@@ -206,6 +211,8 @@ int hh_driver_block_count(const hh_driver* d, int side, int32_t* count);
 /* borrowed handle of a stored block (side 0 sys / 1 env, by length); valid
  * until the driver replaces or frees it */
 int hh_driver_block(const hh_driver* d, int side, int length, const hh_block** out);
+/* borrowed handle of the stored truncation that made the block of `length` */
+int hh_driver_trunc(const hh_driver* d, int side, int length, const hh_trunc** out);
 /* measure_operator_dia (code/measure.f90:3-61) with the operator chain of
  * Get_sys_oper_dia / Get_env_oper_dia (the stored truncations of every
  * length): <n_i> for every lattice position i, n_out[i-1], in the ground

@@ -35,6 +35,7 @@ def _gpu_profile(ctx, m, kept, tol):
                  first_tol_factor=1.0)
     drv.warmup()
     drv.sweep(0, n // 2 - 1, n - 2, tol, TRUN, kept, fresh=True, phase=1)
+    drv.reset_transform()
     drv.sweep(1, n - 2, 1, tol, TRUN, kept, phase=1)
     L = n // 2 - 1
     sb = ctx.bind(drv.block(0, L), drv.block(1, n - L - 2), m.n_elec)
@@ -56,26 +57,33 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
 
 
 def test_density_truncated_matches_oracle_chain(ctx):
-    """Hubbard 6x2 ladder with flux at m = 24: the GPU chain against the
-    oracle's chain on the oracle's own run of the same schedule.  The two runs
-    agree to the truncation level (their kept spaces differ only inside
-    degenerate multiplets at the cut), so the bar is 1e-5; the sum rule is
-    exact."""
+    """Hubbard 6x2 ladder with flux at m = 24 (truncated): the GPU chain
+    against the oracle's chain fed with the GPU run's own blocks, truncation
+    matrices and ground state (downloaded), so both sides see identical
+    inputs; the sum rule is exact."""
+    from .gpuhelp import host, oracle_block, oracle_superblock
     m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=10)
-    E, prof = _gpu_profile(ctx, m, 24, 1e-11)
-    assert abs(prof.sum() - m.n_elec) < 1e-9
-    run = driver.Run(m, driver.Schedule(kept_max=24, n_sweeps=0, tol=1e-11, seed=11, warm_tol_factor=1.0,
-                                        first_tol_factor=1.0))
-    run.warm_up()
+    from hofstadter_hubbard_tsc_amd import Driver
+    ctx.set_model(m)
     n = m.nsite
-    run.left_to_right(n // 2 - 1, n - 2, 1e-11, TRUN, 24, True, "s")
-    run.right_to_left(n - 2, 1, 1e-11, TRUN, 24, "s")
+    drv = Driver(ctx, kept_max=24, tol=1e-11, kept_max_warm=24, seed=11, warm_tol_factor=1.0, first_tol_factor=1.0)
+    drv.warmup()
+    drv.sweep(0, n // 2 - 1, n - 2, 1e-11, TRUN, 24, fresh=True, phase=1)
+    drv.reset_transform()
+    drv.sweep(1, n - 2, 1, 1e-11, TRUN, 24, phase=1)
     L = n // 2 - 1
     R = n - L - 2
-    sg = dmrg.StepGeometry(m, L, R)
-    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
-                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
-    Eo, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 5), 1e-13, 400)
-    ref = measure.density_profile(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
-    assert abs(E - Eo) < 1e-7 * abs(Eo), (E, Eo)
-    assert np.abs(prof - ref).max() < 1e-5, np.abs(prof - ref).max()
+    sb = ctx.bind(drv.block(0, L), drv.block(1, R), m.n_elec)
+    psi = dev(dmrg.start_vector(sb.dim, 5))
+    ctx.lanczos(sb, psi, 1e-13, 400)
+    prof = drv.density(L, psi)
+    assert abs(prof.sum() - m.n_elec) < 1e-9
+    sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
+    env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
+    sys_truns = {l: drv.trunc(0, l).matrices() for l in range(2, L + 1)}
+    env_truns = {l: drv.trunc(1, l).matrices() for l in range(2, R + 1)}
+    assert any(sum(u.shape[1] for _, u in t) < sum(u.shape[0] for _, u in t) for t in sys_truns.values()), \
+        "the case must truncate"
+    H = oracle_superblock(ctx, drv.block(0, L), drv.block(1, R), m, m.n_elec)
+    ref = measure.density_profile(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H)
+    assert np.abs(prof - ref).max() < 1e-10, np.abs(prof - ref).max()
