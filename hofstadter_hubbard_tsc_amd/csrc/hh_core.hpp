@@ -246,8 +246,19 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
                              const std::vector<KeptSector>& kept, int nold);
 // rho'_b = (1/sqrt(2S_b+1)) sum over psi blocks and site states of the block
 // part of psi (B_IN) for the block sectors `blk` of `side`, to B_OUT in the
-// sector-diagonal layout: <O> = Re <O, rho'> for a tracked Hermitian O.
-TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk);
+// sector-diagonal layout: <O> = Re <O, rho'> for a tracked Hermitian O.  With
+// first = B_X the first factor is another vector of the same layout (a
+// transition density matrix: Re <O, rho'> = <psi| O |x>).
+TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk,
+                           uint8_t first = B_IN, uint8_t second = B_IN);
+// B_OUT = (O_sys (x) 1) psi for a tracked sys-block operator O (slot B_AUX at
+// element offset op_off, sector-diagonal layout of `blk`), with O acting as
+// O_b / sqrt(2S_b+1) on every (b, s) row block; site = true applies the
+// centre-site density n_s instead.  The block form covers every element of
+// the output; the site form writes only the rows of site multiplets with
+// N > 0, so the caller clears B_OUT first.
+TaskList compile_apply_sys(const SuperBasis& sb, const EnlBasis& eb, const std::vector<Sector>& blk, int64_t op_off,
+                           bool site);
 // B_OUT += n_site psi on the rows (side 0) or columns (side 1) of the centre
 // site of `side`: <n_site> = Re <psi, out> with out zeroed beforehand.
 TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb);

@@ -375,7 +375,8 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
   return P;
 }
 
-TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk) {
+TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk,
+                           uint8_t first, uint8_t second) {
   const std::vector<int64_t> off = diag_op_offsets(blk);
   Collector c;
   for (int i = 0; i < (int)sb.pairs.size(); ++i) {
@@ -385,17 +386,40 @@ TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, c
       const int d = ss.dim;
       const Region reg{B_OUT, off[ss.b], d, d, d};
       const double w = 1.0 / std::sqrt((double)blk[ss.b].S2 + 1);
-      if (side == 0)   // rows of the block part: psi_sub psi_sub^dag
-        c.add(reg, mk(w, B_IN, sb.off[i] + ss.pos, r, OP_NN, B_IN, sb.off[i] + ss.pos, r, OP_C, cc));
-      else   // columns: psi_sub^T conj(psi_sub)
-        c.add(reg, mk(w, B_IN, sb.off[i] + (int64_t)ss.pos * r, r, OP_T, B_IN, sb.off[i] + (int64_t)ss.pos * r, r,
-                      OP_R, r));
+      if (side == 0)   // rows of the block part: x_sub psi_sub^dag
+        c.add(reg, mk(w, first, sb.off[i] + ss.pos, r, OP_NN, second, sb.off[i] + ss.pos, r, OP_C, cc));
+      else   // columns: x_sub^T conj(psi_sub)
+        c.add(reg, mk(w, first, sb.off[i] + (int64_t)ss.pos * r, r, OP_T, second, sb.off[i] + (int64_t)ss.pos * r,
+                      r, OP_R, r));
     }
   }
   // sectors that no psi block touches still get a (zero) tile
   for (size_t b = 0; b < blk.size(); ++b) c.ensure(Region{B_OUT, off[b], blk[b].dim, blk[b].dim, blk[b].dim});
   TaskList tl;
   c.emit(tl);
+  return tl;
+}
+
+TaskList compile_apply_sys(const SuperBasis& sb, const EnlBasis& eb, const std::vector<Sector>& blk, int64_t op_off,
+                           bool site) {
+  const std::vector<int64_t> off = diag_op_offsets(blk);
+  TaskList tl;
+  for (int i = 0; i < (int)sb.pairs.size(); ++i) {
+    const int a = sb.pairs[i].first;
+    const int r = sb.rows[i], c = sb.cols[i];
+    for (auto& ss : eb.subs[a]) {
+      if (site) {   // rows of site multiplets with N = 0 stay as the caller cleared them
+        const int n = eb.st[ss.s].N;
+        if (n == 0) continue;
+        tl.add_tile(B_OUT, sb.off[i] + ss.pos, r, ss.dim, c, 0);
+        tl.add_term(mk((double)n, B_IN, sb.off[i] + ss.pos, r, OP_NN, 0, 0, 0, OP_NN, 0));
+      } else {
+        tl.add_tile(B_OUT, sb.off[i] + ss.pos, r, ss.dim, c, 0);
+        tl.add_term(mk(1.0 / std::sqrt((double)blk[ss.b].S2 + 1), B_AUX, op_off + off[ss.b], ss.dim, OP_NN, B_IN,
+                       sb.off[i] + ss.pos, r, OP_NN, ss.dim));
+      }
+    }
+  }
   return tl;
 }
 

@@ -869,6 +869,72 @@ struct Driver {
   }
 };
 
+// Densities of every site of one side's block, carried to its current
+// length through the stored truncations (Get_sys_oper_dia / Get_env_oper_dia):
+// operator k-1 is site k; sector-diagonal layout of `sec`, `osz` elements each.
+struct Tracked {
+  DevMem ops;
+  std::vector<Sector> sec;
+  int64_t osz = 0;
+};
+
+void track_densities(hh_ctx* ctx, Driver& d, int side, int len, Tracked& T) {
+  const ModelParams& m = ctx->model;
+  const auto& blocks = side == 0 ? d.sysb : d.envb;
+  const auto& truns = side == 0 ? d.syst : d.envt;
+  // length 1: the site block itself; its density has reduced elements sqrt(2S+1) N
+  T.sec = blocks.at(1)->d.sec;
+  {
+    std::vector<double2> h(T.sec.size());
+    for (size_t i = 0; i < T.sec.size(); ++i)
+      h[i] = make_double2(std::sqrt((double)T.sec[i].S2 + 1) * T.sec[i].N, 0.0);
+    T.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  }
+  DevMem nxt, X;
+  for (int l = 2; l <= len; ++l) {
+    need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
+    const hh_trunc* t = truns.at(l).get();
+    TrackPlan P = compile_track_step(m, T.sec, t->kept, l - 1);
+    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * l, 1));
+    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+    BufPtrs b{};
+    b.p[B_SYS] = T.ops.c();
+    b.p[B_AUX] = t->evec.c();
+    b.p[B_X] = X.c();
+    b.p[B_OUT] = nxt.c();
+    DevTasks s1, s2;
+    s1.upload(P.stage1, ctx->stream);
+    s2.upload(P.stage2, ctx->stream);
+    launch_tiles(s1, b, ctx->stream, PREC_FP64);
+    launch_tiles(s2, b, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+    std::swap(T.ops.p, nxt.p);
+    std::swap(T.ops.bytes, nxt.bytes);
+    T.sec.clear();
+    for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+  }
+  T.osz = diag_op_size(T.sec);
+}
+
+void launch_on(hh_ctx* ctx, const TaskList& tl, const BufPtrs& b) {
+  DevTasks dt;
+  dt.upload(tl, ctx->stream);
+  launch_tiles(dt, b, ctx->stream, PREC_FP64);
+  check_hip(hipStreamSynchronize(ctx->stream), "measurement");
+}
+
+// Re <x_j, w> for j < nv into h (host), x_j = X + j*stride
+void mdots_to_host(hh_ctx* ctx, const double2* X, int64_t stride, int nv, const double2* w, int64_t n, double* h) {
+  DevMem part, res;
+  part.alloc(sizeof(double) * (size_t)vec_blocks(n) * nv);
+  res.alloc(sizeof(double) * nv);
+  dev_mdots(X, stride, nv, w, n, (double*)res.p, (double*)part.p, ctx->stream);
+  check_hip(hipMemcpyAsync(h, res.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+  check_hip(hipStreamSynchronize(ctx->stream), "measurement");
+}
+
 // <n_i> at every lattice position for the ground state psi of the superblock
 // (sys block sys_len, env block nsite - sys_len - 2) of the driver's stored
 // blocks: measure_operator_dia (code/measure.f90:3-61) with the operator
@@ -882,78 +948,80 @@ void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, do
   need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
   std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
   const SuperBasis& S = sb->plan.sb;
-  const std::vector<Sector> st = site_sectors(m.kind);
-  double* dots = nullptr;
   for (int side = 0; side < 2; ++side) {
     const int len = side == 0 ? sys_len : env_len;
-    const auto& blocks = side == 0 ? d.sysb : d.envb;
-    const auto& truns = side == 0 ? d.syst : d.envt;
-    // length 1: the site block itself; its density has reduced elements sqrt(2S+1) N
-    std::vector<Sector> cur = blocks.at(1)->d.sec;
-    DevMem ops, nxt, X;
-    {
-      std::vector<double2> h(cur.size());
-      for (size_t i = 0; i < cur.size(); ++i) h[i] = make_double2(std::sqrt((double)cur[i].S2 + 1) * cur[i].N, 0.0);
-      ops.alloc(sizeof(double2) * h.size());
-      check_hip(hipMemcpyAsync(ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-                "h2d");
-    }
-    for (int l = 2; l <= len; ++l) {
-      need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
-      const hh_trunc* t = truns.at(l).get();
-      TrackPlan P = compile_track_step(m, cur, t->kept, l - 1);
-      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * l, 1));
-      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
-      BufPtrs b{};
-      b.p[B_SYS] = ops.c();
-      b.p[B_AUX] = t->evec.c();
-      b.p[B_X] = X.c();
-      b.p[B_OUT] = nxt.c();
-      DevTasks s1, s2;
-      s1.upload(P.stage1, ctx->stream);
-      s2.upload(P.stage2, ctx->stream);
-      launch_tiles(s1, b, ctx->stream, PREC_FP64);
-      launch_tiles(s2, b, ctx->stream, PREC_FP64);
-      check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
-      std::swap(ops.p, nxt.p);
-      std::swap(ops.bytes, nxt.bytes);
-      cur.clear();
-      for (auto& k : t->kept) cur.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
-    }
-    // block sites: <O_k> = Re <O_k, rho'>
+    Tracked T;
+    track_densities(ctx, d, side, len, T);
     const EnlBasis& eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
-    const int64_t osz = diag_op_size(cur);
-    DevMem rho, Y, part, res;
-    rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(osz, 1));
+    // block sites: <O_k> = Re <O_k, rho'>
+    DevMem rho, Y;
+    rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
     Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
-    part.alloc(sizeof(double) * (size_t)std::max(vec_blocks(osz), vec_blocks(S.dim)) * (len + 1));
-    res.alloc(sizeof(double) * (len + 1));
-    dots = (double*)res.p;
-    {
-      DevTasks rt;
-      rt.upload(compile_block_rho(S, side, eb, cur), ctx->stream);
-      BufPtrs b{};
-      b.p[B_IN] = (double2*)psi;
-      b.p[B_OUT] = rho.c();
-      launch_tiles(rt, b, ctx->stream, PREC_FP64);
-    }
-    dev_mdots(ops.c(), osz, len, rho.c(), osz, dots, (double*)part.p, ctx->stream);
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_OUT] = rho.c();
+    launch_on(ctx, compile_block_rho(S, side, eb, T.sec), b);
+    std::vector<double> h(len + 1);
+    mdots_to_host(ctx, T.ops.c(), T.osz, len, rho.c(), T.osz, h.data());
     // centre site: <n> = Re <psi, n_site psi>
     dev_zero(Y.c(), S.dim, ctx->stream);
-    {
-      DevTasks yt;
-      yt.upload(compile_site_density(S, side, eb), ctx->stream);
-      BufPtrs b{};
-      b.p[B_IN] = (double2*)psi;
-      b.p[B_OUT] = Y.c();
-      launch_tiles(yt, b, ctx->stream, PREC_FP64);
-    }
-    dev_mdots(psi, 0, 1, Y.c(), S.dim, dots + len, (double*)part.p, ctx->stream);
-    std::vector<double> h(len + 1);
-    check_hip(hipMemcpyAsync(h.data(), dots, sizeof(double) * (len + 1), hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    check_hip(hipStreamSynchronize(ctx->stream), "density");
+    b.p[B_OUT] = Y.c();
+    launch_on(ctx, compile_site_density(S, side, eb), b);
+    mdots_to_host(ctx, psi, 0, 1, Y.c(), S.dim, &h[len]);
     for (int k = 1; k <= len; ++k) out[(side == 0 ? k : nsite + 1 - k) - 1] = h[k - 1];
     out[(side == 0 ? sys_len + 1 : nsite - env_len) - 1] = h[len];
+  }
+}
+
+// <n_i n_j> for i on the sys side (block sites and the sys centre site) and j
+// on the env side, from the same chains: with Y_i = (n_i (x) 1) psi and the
+// transition matrices R_i = sum Y_i^T conj(psi) over the env block part,
+// <n_i n_j> = Re <O_j, R_i>; the env centre site reads Re <psi, n_site Y_i>.
+// out is nsite x nsite (row-major, lattice positions - 1), symmetric; pairs on
+// the same side are NaN.
+void cross_density(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  Tracked TS, TE;
+  track_densities(ctx, d, 0, sys_len, TS);
+  track_densities(ctx, d, 1, env_len, TE);
+  for (int i = 0; i < nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  DevMem Y, R, W;
+  Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+  W.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+  R.alloc(sizeof(double2) * (size_t)std::max<int64_t>(TE.osz, 1));
+  std::vector<double> h(env_len + 1);
+  for (int k = 1; k <= sys_len + 1; ++k) {   // k = sys_len + 1: the sys centre site
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_AUX] = TS.ops.c();
+    b.p[B_OUT] = Y.c();
+    dev_zero(Y.c(), S.dim, ctx->stream);
+    launch_on(ctx, compile_apply_sys(S, sb->plan.sbs, TS.sec, (int64_t)(k - 1) * TS.osz, k == sys_len + 1), b);
+    BufPtrs r{};
+    r.p[B_IN] = (double2*)psi;
+    r.p[B_X] = Y.c();
+    r.p[B_OUT] = R.c();
+    launch_on(ctx, compile_block_rho(S, 1, sb->plan.ebs, TE.sec, B_X, B_IN), r);
+    mdots_to_host(ctx, TE.ops.c(), TE.osz, env_len, R.c(), TE.osz, h.data());
+    dev_zero(W.c(), S.dim, ctx->stream);
+    BufPtrs w{};
+    w.p[B_IN] = Y.c();
+    w.p[B_OUT] = W.c();
+    launch_on(ctx, compile_site_density(S, 1, sb->plan.ebs), w);
+    mdots_to_host(ctx, psi, 0, 1, W.c(), S.dim, &h[env_len]);
+    const int pi = k;   // sys site k sits at lattice position k
+    for (int j = 1; j <= env_len + 1; ++j) {
+      const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;
+      const double v = h[j - 1 < env_len ? j - 1 : env_len];
+      out[(int64_t)(pi - 1) * nsite + (pj - 1)] = v;
+      out[(int64_t)(pj - 1) * nsite + (pi - 1)] = v;
+    }
   }
 }
 
@@ -1538,6 +1606,14 @@ int hh_driver_density(hh_driver* h, int sys_len, const double* psi, double* n_ou
   return guard(h->d.ctx, [&] {
     need(psi && n_out, "null argument");
     density_profile(h->d.ctx, h->d, sys_len, (const double2*)psi, n_out);
+  });
+}
+
+int hh_driver_density_cross(hh_driver* h, int sys_len, const double* psi, double* nn_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && nn_out, "null argument");
+    cross_density(h->d.ctx, h->d, sys_len, (const double2*)psi, nn_out);
   });
 }
 

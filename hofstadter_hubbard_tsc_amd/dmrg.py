@@ -448,10 +448,19 @@ class Driver:
         """<n_i> per lattice position (index i-1) in the ground state psi of the
         superblock of the stored sys block of sys_len sites (measure_operator_dia)."""
         _sync()
-        out = np.zeros(self.ctx.model.nsite if hasattr(self.ctx.model, "nsite") else
-                       self.ctx.model.Lx * self.ctx.model.Ly)
+        out = np.zeros(self.ctx.model.Lx * self.ctx.model.Ly)
         self.ctx._rc(L.load().hh_driver_density(self.ptr, sys_len, _dptr(psi),
                                                 out.ctypes.data_as(C.POINTER(C.c_double))), "hh_driver_density")
+        return out
+
+    def density_cross(self, sys_len, psi):
+        """<n_i n_j> (nsite x nsite, NaN for pairs on one side) across the centre."""
+        _sync()
+        n = self.ctx.model.Lx * self.ctx.model.Ly
+        out = np.zeros((n, n))
+        self.ctx._rc(L.load().hh_driver_density_cross(self.ptr, sys_len, _dptr(psi),
+                                                      out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_density_cross")
         return out
 
     def block(self, side, length):

@@ -219,6 +219,15 @@ int hh_driver_trunc(const hh_driver* d, int side, int length, const hh_trunc** o
  * state psi_dev of the superblock bound from the driver's sys block of
  * sys_len sites and env block of Nsite - sys_len - 2 sites. */
 int hh_driver_density(hh_driver* d, int sys_len, const double* psi_dev, double* n_out);
+/* density-density correlations <n_i n_j> across the superblock's centre
+ * (i on the sys side, j on the env side, centre sites included): the pairs of
+ * Get_density_cor (code/correlation.f90:43-195) that straddle the two
+ * blocks, i.e. sys_block_env_block_cor_ndia (:1666-1929),
+ * sys_block_env_site_cor_ndia (:1932-2169), sys_site_env_site_cor_ndia
+ * (:2172-2402) and sys_site_env_block_cor_ndia (:2405-2656) for rank-0
+ * operators.  nn_out is Nsite x Nsite row-major by lattice position - 1,
+ * symmetric, NaN for pairs on one side (those straddle another cut). */
+int hh_driver_density_cross(hh_driver* d, int sys_len, const double* psi_dev, double* nn_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

@@ -157,21 +157,37 @@ def singlet_spectrum(model, positions, bonds, n_elec, k=None):
     return np.sort(np.real(w))
 
 
-def singlet_ground_density(model, positions, bonds, n_elec):
-    """(E0, <n> per lattice position) of the lowest total-S = 0 state;
-    the density array is indexed by position - 1."""
+def _singlet_ground(model, positions, bonds, n_elec):
     H, fs = hamiltonian(model, positions, bonds, n_elec)
     S2 = casimir(fs)
     dim = H.shape[0]
     if dim <= 3000:
         w, v = np.linalg.eigh((H + 50.0 * S2).toarray())
-        e, psi = w[0], v[:, 0]
     else:
         w, v = spla.eigsh(H + 50.0 * S2, k=1, which="SA")
-        e, psi = w[0], v[:, 0]
-    p = np.abs(psi) ** 2
-    out = np.zeros(max(positions))
+    return float(np.real(w[0])), v[:, 0], fs
+
+
+def _occupations(fs, positions):
+    """occ[pos - 1] = occupation of that position in every Fock state."""
+    occ = np.zeros((max(positions), len(fs.states)))
     for j, pos in enumerate(positions):
-        occ = ((fs.states >> (2 * j)) & 1) + ((fs.states >> (2 * j + 1)) & 1)
-        out[pos - 1] = float(np.dot(p, occ))
-    return float(np.real(e)), out
+        occ[pos - 1] = ((fs.states >> (2 * j)) & 1) + ((fs.states >> (2 * j + 1)) & 1)
+    return occ
+
+
+def singlet_ground_density(model, positions, bonds, n_elec):
+    """(E0, <n> per lattice position) of the lowest total-S = 0 state;
+    the density array is indexed by position - 1."""
+    e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
+    return e, _occupations(fs, positions) @ (np.abs(psi) ** 2)
+
+
+def singlet_ground_correlations(model, positions, bonds, n_elec):
+    """(E0, <n_i>, <n_i n_j>) of the lowest total-S = 0 state, indexed by
+    lattice position - 1.  The number operators are diagonal in the Fock
+    basis, so <n_i n_j> = sum_f |psi_f|^2 occ_i(f) occ_j(f)."""
+    e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
+    occ = _occupations(fs, positions)
+    p = np.abs(psi) ** 2
+    return e, occ @ p, (occ * p) @ occ.T

@@ -12,8 +12,17 @@ expectation reduces to Tr(rho_b O_b) / sqrt(2 S_b + 1) summed over the block
 sectors b, with rho_b the block's reduced density matrix; the two centre
 sites read sum_(b,s) n_s |psi_(b,s)|^2.
 
-Pinned by exact diagonalisation (``ed.singlet_ground_density``): with no
-truncation the profile equals the exact one; the sum over all sites is N.
+``cross_density`` gives the density-density correlations <n_i n_j> of the
+pairs of ``Get_density_cor`` (code/correlation.f90:43-195) that straddle the
+superblock centre (i on the system side, j on the environment side, centre
+sites included): <psi| A_i (x) B_j |psi> = <psi, Y_i B_j^T> per superblock
+pair block, Y_i = A_i psi, with the physical action O_b / sqrt(2 S_b + 1) of
+a rank-0 reduced block operator on block sector b.
+
+Pinned by exact diagonalisation (``ed.singlet_ground_density``,
+``ed.singlet_ground_correlations``): with no truncation the profile and the
+correlations equal the exact ones; the densities sum to N and, for each i,
+sum_j <n_i n_j> over the other side equals <n_i N_other>.
 """
 import math
 
@@ -92,4 +101,53 @@ def density_profile(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len
             out[pos - 1] = v
         centre = length + 1 if side == "sys" else nsite - length
         out[centre - 1] = site_n
+    return out
+
+
+def _env_column_blocks(H, side):
+    en = H.s if side == "sys" else H.e
+    return en.basis, [st[0] for st in en.basis.st_sectors]
+
+
+def cross_density(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H):
+    """<n_i n_j> as an nsite x nsite array (index position - 1), symmetric,
+    for i on the system side and j on the environment side; NaN elsewhere."""
+    nsite = model.nsite
+    out = np.full((nsite, nsite), np.nan)
+    sb = H.sb
+    pin = dmrg.psi_blocks(sb, psi)
+    seb, s_n = _env_column_blocks(H, "sys")
+    eeb, e_n = _env_column_blocks(H, "env")
+    s_ops = tracked_ops(model, sys_blocks, sys_truns, sys_len, "sys")
+    e_ops = tracked_ops(model, env_blocks, env_truns, env_len, "env")
+    s_secs, e_secs = sys_blocks[sys_len].sectors, env_blocks[env_len].sectors
+    s_list = [(k, s_ops[k]) for k in range(1, sys_len + 1)] + [(sys_len + 1, None)]
+    e_list = [(k, e_ops[k]) for k in range(1, env_len + 1)] + [(env_len + 1, None)]
+    for ki, A in s_list:
+        # Y = (A (x) 1) psi, pair block by pair block
+        Y = []
+        for i, (x, _) in enumerate(sb.pairs):
+            y = np.zeros_like(pin[i])
+            for (b, s, pos, d) in seb.subs[x]:
+                if A is None:
+                    y[pos:pos + d, :] = s_n[s] * pin[i][pos:pos + d, :]
+                else:
+                    m = A.blk.get((b, b))
+                    if m is not None:
+                        y[pos:pos + d, :] = (m / math.sqrt(s_secs[b][1] + 1)) @ pin[i][pos:pos + d, :]
+            Y.append(y)
+        pi = ki
+        for kj, B in e_list:
+            v = 0.0
+            for i, (_, yy) in enumerate(sb.pairs):
+                for (b, s, pos, d) in eeb.subs[yy]:
+                    ps, ys = pin[i][:, pos:pos + d], Y[i][:, pos:pos + d]
+                    if B is None:
+                        v += e_n[s] * float(np.real(np.vdot(ps, ys)))
+                    else:
+                        m = B.blk.get((b, b))
+                        if m is not None:
+                            v += float(np.real(np.vdot(ps, ys @ (m / math.sqrt(e_secs[b][1] + 1)).T)))
+            pj = nsite + 1 - kj if B is not None else nsite - env_len
+            out[pi - 1, pj - 1] = out[pj - 1, pi - 1] = v
     return out

@@ -1,7 +1,8 @@
 """GPU parity of the site-density measurement (measure_operator_dia with the
-operator chain of Get_sys_oper_dia / Get_env_oper_dia): untruncated runs
-against exact diagonalisation, truncated runs against the oracle's chain, and
-the sum rule sum_i <n_i> = N."""
+operator chain of Get_sys_oper_dia / Get_env_oper_dia) and of the density
+correlations <n_i n_j> across the superblock centre (the straddling pairs of
+Get_density_cor): untruncated runs against exact diagonalisation, truncated
+runs against the oracle's chain, and the sum rule sum_i <n_i> = N."""
 import numpy as np
 import pytest
 
@@ -26,8 +27,8 @@ def ctx():
 
 
 def _gpu_profile(ctx, m, kept, tol):
-    """warm-up, LR half sweep, RL sweep; then the centre ground state and its
-    density profile"""
+    """warm-up, LR half sweep, RL sweep; then the centre ground state, its
+    density profile and its cross-centre correlations"""
     from hofstadter_hubbard_tsc_amd import Driver
     ctx.set_model(m)
     n = m.nsite
@@ -41,7 +42,7 @@ def _gpu_profile(ctx, m, kept, tol):
     sb = ctx.bind(drv.block(0, L), drv.block(1, n - L - 2), m.n_elec)
     psi = dev(dmrg.start_vector(sb.dim, 5))
     E, it = ctx.lanczos(sb, psi, 1e-13, 400)
-    return E, drv.density(L, psi)
+    return E, drv.density(L, psi), drv.density_cross(L, psi)
 
 
 @pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
@@ -49,11 +50,16 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
     sg = dmrg.StepGeometry(m, m.nsite // 2 - 1, m.nsite - m.nsite // 2 - 1)
     bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
-    e0, dens = ed.singlet_ground_density(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
-    E, prof = _gpu_profile(ctx, m, 2000, 1e-12)
+    e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    E, prof, cc = _gpu_profile(ctx, m, 2000, 1e-12)
     assert abs(E - e0) < 1e-10 * abs(e0)
     assert abs(prof.sum() - m.n_elec) < 1e-10
     assert np.abs(prof - dens).max() < 1e-6, (prof, dens)
+    left, right = np.arange(m.nsite // 2), np.arange(m.nsite // 2, m.nsite)
+    x = np.ix_(left, right)
+    assert np.abs(cc[x] - nn[x]).max() < 1e-6, (cc[x], nn[x])
+    assert np.array_equal(cc[x], cc.T[x])
+    assert np.isnan(cc[np.ix_(left, left)]).all() and np.isnan(cc[np.ix_(right, right)]).all()
 
 
 def test_density_truncated_matches_oracle_chain(ctx):
@@ -77,6 +83,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     psi = dev(dmrg.start_vector(sb.dim, 5))
     ctx.lanczos(sb, psi, 1e-13, 400)
     prof = drv.density(L, psi)
+    cc = drv.density_cross(L, psi)
     assert abs(prof.sum() - m.n_elec) < 1e-9
     sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
     env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
@@ -87,3 +94,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     H = oracle_superblock(ctx, drv.block(0, L), drv.block(1, R), m, m.n_elec)
     ref = measure.density_profile(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H)
     assert np.abs(prof - ref).max() < 1e-10, np.abs(prof - ref).max()
+    cref = measure.cross_density(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H)
+    fin = ~np.isnan(cref)
+    assert np.array_equal(fin, ~np.isnan(cc))
+    assert np.abs(cc[fin] - cref[fin]).max() < 1e-10, np.abs(cc[fin] - cref[fin]).max()

@@ -43,3 +43,48 @@ def test_density_of_a_wrong_chain_fails():
     bad_truns = {l: [(a, 2.0 * U) for a, U in tr] for l, tr in run.sys_trun.items()}
     bad = measure.density_profile(m, run.sys_blocks, run.env_blocks, bad_truns, run.env_trun, L, R, psi, H)
     assert np.abs(good - bad).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
+def test_untruncated_cross_density_equals_ed(kind, flux):
+    """<n_i n_j> across the centre from the tracked operators equals the exact
+    two-point function of the ED ground state."""
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    assert np.abs(nn.sum(axis=1) - m.n_elec * dens).max() < 1e-10   # sum_j n_i n_j = n_i N
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    cc = measure.cross_density(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    left = np.arange(L + 1)            # positions 1 .. L+1 (index - 1)
+    right = np.arange(L + 1, m.nsite)
+    assert not np.isnan(cc[np.ix_(left, right)]).any()
+    assert np.isnan(cc[np.ix_(left, left)]).all() and np.isnan(cc[np.ix_(right, right)]).all()
+    assert np.abs(cc[np.ix_(left, right)] - nn[np.ix_(left, right)]).max() < 1e-6
+    assert np.abs(cc - cc.T)[np.ix_(left, right)].max() == 0.0
+
+
+def test_cross_density_with_a_wrong_scale_fails():
+    """A planted mistake in the environment chain (its truncation matrices
+    scaled by 1.5, a lost normalisation) moves the correlations beyond the
+    1e-6 bar of the comparison above."""
+    m = lat.Model(kind="hubbard", Lx=3, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=4)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    L, R = 2, m.nsite - 4
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    good = measure.cross_density(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    bad_truns = {l: [(a, 1.5 * U) for a, U in tr] for l, tr in run.env_trun.items()}
+    bad = measure.cross_density(m, run.sys_blocks, run.env_blocks, run.sys_trun, bad_truns, L, R, psi, H)
+    fin = ~np.isnan(good)
+    assert np.abs(good[fin] - bad[fin]).max() > 1e-3
