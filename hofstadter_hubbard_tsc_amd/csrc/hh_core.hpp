@@ -244,6 +244,17 @@ struct TrackPlan {
 // B_OUT, scratch in B_X.
 TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold);
+// The same step for an explicit list of new operators: CARRY k embeds old
+// operator k, SITE is the density of the new site, PRODUCT k is old operator
+// k times the new site's density (the two-site operator n_k n_new of
+// Get_sys2b, created when its second site joins the block).  New operator q
+// goes to B_OUT at q * new_size.
+struct TrackSrc {
+  enum What { CARRY, SITE, PRODUCT } what;
+  int k;
+};
+TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,
+                            const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs);
 // rho'_b = (1/sqrt(2S_b+1)) sum over psi blocks and site states of the block
 // part of psi (B_IN) for the block sectors `blk` of `side`, to B_OUT in the
 // sector-diagonal layout: <O> = Re <O, rho'> for a tracked Hermitian O.  With

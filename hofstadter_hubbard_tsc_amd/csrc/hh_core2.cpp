@@ -332,8 +332,8 @@ std::vector<int64_t> diag_op_offsets(const std::vector<Sector>& sec) {
   return o;
 }
 
-TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
-                             const std::vector<KeptSector>& kept, int nold) {
+TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs) {
   TrackPlan P;
   const auto st = site_sectors(m.kind);
   P.eb = enlarge_basis(prev, st);
@@ -344,7 +344,8 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
   P.old_size = diag_op_size(prev);
   P.new_size = diag_op_size(nsec);
   Collector s1, s2;
-  for (int k = 0; k <= nold; ++k) {
+  for (size_t q = 0; q < srcs.size(); ++q) {
+    const TrackSrc& src = srcs[q];
     for (int i = 0; i < (int)kept.size(); ++i) {
       const KeptSector& ks = kept[i];
       const int a = ks.a, J2 = eb.sec[a].S2, da = eb.sec[a].dim, kp = ks.keep;
@@ -354,25 +355,39 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
       for (auto& ss : eb.subs[a]) s1.ensure(rows(ss.pos, ss.dim));
       for (auto& ss : eb.subs[a]) {
         const int mb = ss.dim;
-        if (k < nold) {
-          // rank-0 block operator: sqrt((2J+1)/(2S_b+1)) O_b on every (b, s) (update_block_dia)
-          s1.add(rows(ss.pos, mb), mk(std::sqrt((double)(J2 + 1) / (eb.bl[ss.b].S2 + 1)), B_SYS,
-                                      (int64_t)k * P.old_size + ooff[ss.b], mb, OP_NN, B_AUX, ks.u_off + ss.pos,
-                                      ks.u_ld, OP_NN, mb));
-        } else if (st[ss.s].N != 0) {
+        const double ns = (double)st[ss.s].N;
+        if (src.what == TrackSrc::SITE) {
           // density of the new site: sqrt(2J+1) n_s (update_site_dia)
-          s1.add(rows(ss.pos, mb), mk((double)st[ss.s].N * std::sqrt((double)J2 + 1), B_AUX, ks.u_off + ss.pos,
-                                      ks.u_ld, OP_NN, 0, 0, 0, OP_NN, 0));
+          if (ns != 0.0)
+            s1.add(rows(ss.pos, mb), mk(ns * std::sqrt((double)J2 + 1), B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, 0,
+                                        0, 0, OP_NN, 0));
+          continue;
         }
+        // rank-0 block operator: sqrt((2J+1)/(2S_b+1)) O_b on every (b, s)
+        // (update_block_dia); a product with the new site's density scales
+        // the (b, s) block by n_s
+        double c = std::sqrt((double)(J2 + 1) / (eb.bl[ss.b].S2 + 1));
+        if (src.what == TrackSrc::PRODUCT) c *= ns;
+        if (c == 0.0) continue;
+        s1.add(rows(ss.pos, mb), mk(c, B_SYS, (int64_t)src.k * P.old_size + ooff[ss.b], mb, OP_NN, B_AUX,
+                                    ks.u_off + ss.pos, ks.u_ld, OP_NN, mb));
       }
       // O_new[i] = U_i^dag X
-      Region r{B_OUT, (int64_t)k * P.new_size + noff[i], kp, kp, kp};
+      Region r{B_OUT, (int64_t)q * P.new_size + noff[i], kp, kp, kp};
       s2.add(r, mk(1.0, B_AUX, ks.u_off, ks.u_ld, OP_C, B_X, xo, da, OP_NN, da));
     }
   }
   s1.emit(P.stage1);
   s2.emit(P.stage2);
   return P;
+}
+
+TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold) {
+  std::vector<TrackSrc> srcs;
+  for (int k = 0; k < nold; ++k) srcs.push_back({TrackSrc::CARRY, k});
+  srcs.push_back({TrackSrc::SITE, 0});
+  return compile_track_ops(m, prev, kept, srcs);
 }
 
 TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, const std::vector<Sector>& blk,

@@ -869,21 +869,33 @@ struct Driver {
   }
 };
 
-// Densities of every site of one side's block, carried to its current
-// length through the stored truncations (Get_sys_oper_dia / Get_env_oper_dia):
-// operator k-1 is site k; sector-diagonal layout of `sec`, `osz` elements each.
+// Tracked rank-0 operators of one side's block, carried to its current
+// length through the stored truncations (Get_sys_oper_dia / Get_env_oper_dia,
+// and Get_sys2b for two-site products): operator q in sector-diagonal layout
+// of `sec` at q * osz; lbl[q] = {a, 0} for the density of block site a, {a, b}
+// for the product n_a n_b.
 struct Tracked {
   DevMem ops;
   std::vector<Sector> sec;
+  std::vector<std::pair<int, int>> lbl;
   int64_t osz = 0;
+  int find(int a, int b) const {
+    for (size_t q = 0; q < lbl.size(); ++q)
+      if (lbl[q].first == a && lbl[q].second == b) return (int)q;
+    return -1;
+  }
 };
 
-void track_densities(hh_ctx* ctx, Driver& d, int side, int len, Tracked& T) {
+// row = 0: the density of every site.  row = r >= 1: the density of site r
+// and every product n_r n_j (j != r) of block sites; the densities n_j with
+// j < r are carried only until site r joins.
+void track_chain(hh_ctx* ctx, Driver& d, int side, int len, int row, Tracked& T) {
   const ModelParams& m = ctx->model;
   const auto& blocks = side == 0 ? d.sysb : d.envb;
   const auto& truns = side == 0 ? d.syst : d.envt;
   // length 1: the site block itself; its density has reduced elements sqrt(2S+1) N
   T.sec = blocks.at(1)->d.sec;
+  T.lbl = {{1, 0}};
   {
     std::vector<double2> h(T.sec.size());
     for (size_t i = 0; i < T.sec.size(); ++i)
@@ -896,8 +908,32 @@ void track_densities(hh_ctx* ctx, Driver& d, int side, int len, Tracked& T) {
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
     const hh_trunc* t = truns.at(l).get();
-    TrackPlan P = compile_track_step(m, T.sec, t->kept, l - 1);
-    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * l, 1));
+    std::vector<TrackSrc> srcs;
+    std::vector<std::pair<int, int>> nl;
+    if (row == 0 || l < row) {
+      for (size_t q = 0; q < T.lbl.size(); ++q) {
+        srcs.push_back({TrackSrc::CARRY, (int)q});
+        nl.push_back(T.lbl[q]);
+      }
+      srcs.push_back({TrackSrc::SITE, 0});
+      nl.push_back({l, 0});
+    } else if (l == row) {
+      srcs.push_back({TrackSrc::SITE, 0});
+      nl.push_back({row, 0});
+      for (size_t q = 0; q < T.lbl.size(); ++q) {
+        srcs.push_back({TrackSrc::PRODUCT, (int)q});
+        nl.push_back({T.lbl[q].first, row});
+      }
+    } else {
+      for (size_t q = 0; q < T.lbl.size(); ++q) {
+        srcs.push_back({TrackSrc::CARRY, (int)q});
+        nl.push_back(T.lbl[q]);
+      }
+      srcs.push_back({TrackSrc::PRODUCT, T.find(row, 0)});
+      nl.push_back({row, l});
+    }
+    TrackPlan P = compile_track_ops(m, T.sec, t->kept, srcs);
+    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (int64_t)srcs.size(), 1));
     X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
     BufPtrs b{};
     b.p[B_SYS] = T.ops.c();
@@ -914,6 +950,7 @@ void track_densities(hh_ctx* ctx, Driver& d, int side, int len, Tracked& T) {
     std::swap(T.ops.bytes, nxt.bytes);
     T.sec.clear();
     for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+    T.lbl = nl;
   }
   T.osz = diag_op_size(T.sec);
 }
@@ -951,7 +988,7 @@ void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, do
   for (int side = 0; side < 2; ++side) {
     const int len = side == 0 ? sys_len : env_len;
     Tracked T;
-    track_densities(ctx, d, side, len, T);
+    track_chain(ctx, d, side, len, 0, T);
     const EnlBasis& eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
     // block sites: <O_k> = Re <O_k, rho'>
     DevMem rho, Y;
@@ -973,12 +1010,49 @@ void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, do
   }
 }
 
+// Env-side entries of a correlation row: with Y = (A (x) 1) psi for a sys-side
+// operator A, <A n_j> = Re <O_j, R> for the env block sites, R = sum Y^T
+// conj(psi) over the env block part, and Re <psi, n_site Y> for the env centre
+// site.  row points at nsite values indexed by lattice position - 1.
+void env_entries(hh_ctx* ctx, hh_superblock* sb, const Tracked& TE, int env_len, const double2* psi,
+                 const double2* Y, double* row) {
+  const int nsite = ctx->model.nsite();
+  const SuperBasis& S = sb->plan.sb;
+  DevMem R, W;
+  R.alloc(sizeof(double2) * (size_t)std::max<int64_t>(TE.osz, 1));
+  W.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+  std::vector<double> h(env_len + 1);
+  BufPtrs r{};
+  r.p[B_IN] = (double2*)psi;
+  r.p[B_X] = (double2*)Y;
+  r.p[B_OUT] = R.c();
+  launch_on(ctx, compile_block_rho(S, 1, sb->plan.ebs, TE.sec, B_X, B_IN), r);
+  mdots_to_host(ctx, TE.ops.c(), TE.osz, env_len, R.c(), TE.osz, h.data());
+  dev_zero(W.c(), S.dim, ctx->stream);
+  BufPtrs w{};
+  w.p[B_IN] = (double2*)Y;
+  w.p[B_OUT] = W.c();
+  launch_on(ctx, compile_site_density(S, 1, sb->plan.ebs), w);
+  mdots_to_host(ctx, psi, 0, 1, W.c(), S.dim, &h[env_len]);
+  for (int j = 1; j <= env_len; ++j) row[nsite - j] = h[j - 1];   // env block site j: position nsite + 1 - j
+  row[nsite - env_len - 1] = h[env_len];                          // env centre: position nsite - env_len
+}
+
+// Y = (A (x) 1) psi for tracked sys operator q of T, or for the sys centre
+// site's density when q < 0
+void apply_sys(hh_ctx* ctx, hh_superblock* sb, const Tracked& T, int q, const double2* psi, double2* Y) {
+  const SuperBasis& S = sb->plan.sb;
+  BufPtrs b{};
+  b.p[B_IN] = (double2*)psi;
+  b.p[B_AUX] = T.ops.c();
+  b.p[B_OUT] = Y;
+  dev_zero(Y, S.dim, ctx->stream);
+  launch_on(ctx, compile_apply_sys(S, sb->plan.sbs, T.sec, q < 0 ? 0 : (int64_t)q * T.osz, q < 0), b);
+}
+
 // <n_i n_j> for i on the sys side (block sites and the sys centre site) and j
-// on the env side, from the same chains: with Y_i = (n_i (x) 1) psi and the
-// transition matrices R_i = sum Y_i^T conj(psi) over the env block part,
-// <n_i n_j> = Re <O_j, R_i>; the env centre site reads Re <psi, n_site Y_i>.
-// out is nsite x nsite (row-major, lattice positions - 1), symmetric; pairs on
-// the same side are NaN.
+// on the env side, from the same chains.  out is nsite x nsite (row-major,
+// lattice positions - 1), symmetric; pairs on the same side are NaN.
 void cross_density(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
   const ModelParams& m = ctx->model;
   const int nsite = m.nsite();
@@ -986,43 +1060,69 @@ void cross_density(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, doub
   need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
   need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
   std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
-  const SuperBasis& S = sb->plan.sb;
   Tracked TS, TE;
-  track_densities(ctx, d, 0, sys_len, TS);
-  track_densities(ctx, d, 1, env_len, TE);
+  track_chain(ctx, d, 0, sys_len, 0, TS);
+  track_chain(ctx, d, 1, env_len, 0, TE);
   for (int i = 0; i < nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
-  DevMem Y, R, W;
+  DevMem Y;
+  Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(sb->plan.sb.dim, 1));
+  for (int k = 1; k <= sys_len + 1; ++k) {   // k = sys_len + 1: the sys centre site (position k)
+    apply_sys(ctx, sb.get(), TS, k <= sys_len ? k - 1 : -1, psi, Y.c());
+    double* row = out + (int64_t)(k - 1) * nsite;
+    env_entries(ctx, sb.get(), TE, env_len, psi, Y.c(), row);
+    for (int j = sys_len + 2; j <= nsite; ++j) out[(int64_t)(j - 1) * nsite + (k - 1)] = row[j - 1];
+  }
+}
+
+// One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
+// site r on the sys side (block or centre) and every other site j.  Pairs of
+// block sites use the tracked products n_r n_j (Get_sys2b / measure_sys_block);
+// a block site with the sys centre site uses the transition matrix of
+// (n_centre psi, psi) (sys_block_site_cor_ndia); the env side uses
+// env_entries.  out has nsite values, out[r-1] = NaN.
+void density_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(r >= 1 && r <= sys_len + 1, "the row site must lie on the sys side of the cut (1 .. sys_len + 1)");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  for (int j = 0; j < nsite; ++j) out[j] = std::numeric_limits<double>::quiet_NaN();
+  Tracked TS, TE;
+  const bool centre = r == sys_len + 1;
+  track_chain(ctx, d, 0, sys_len, centre ? 0 : r, TS);
+  track_chain(ctx, d, 1, env_len, 0, TE);
+  DevMem Y, rho;
   Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
-  W.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
-  R.alloc(sizeof(double2) * (size_t)std::max<int64_t>(TE.osz, 1));
-  std::vector<double> h(env_len + 1);
-  for (int k = 1; k <= sys_len + 1; ++k) {   // k = sys_len + 1: the sys centre site
-    BufPtrs b{};
-    b.p[B_IN] = (double2*)psi;
-    b.p[B_AUX] = TS.ops.c();
-    b.p[B_OUT] = Y.c();
-    dev_zero(Y.c(), S.dim, ctx->stream);
-    launch_on(ctx, compile_apply_sys(S, sb->plan.sbs, TS.sec, (int64_t)(k - 1) * TS.osz, k == sys_len + 1), b);
-    BufPtrs r{};
-    r.p[B_IN] = (double2*)psi;
-    r.p[B_X] = Y.c();
-    r.p[B_OUT] = R.c();
-    launch_on(ctx, compile_block_rho(S, 1, sb->plan.ebs, TE.sec, B_X, B_IN), r);
-    mdots_to_host(ctx, TE.ops.c(), TE.osz, env_len, R.c(), TE.osz, h.data());
-    dev_zero(W.c(), S.dim, ctx->stream);
-    BufPtrs w{};
-    w.p[B_IN] = Y.c();
-    w.p[B_OUT] = W.c();
-    launch_on(ctx, compile_site_density(S, 1, sb->plan.ebs), w);
-    mdots_to_host(ctx, psi, 0, 1, W.c(), S.dim, &h[env_len]);
-    const int pi = k;   // sys site k sits at lattice position k
-    for (int j = 1; j <= env_len + 1; ++j) {
-      const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;
-      const double v = h[j - 1 < env_len ? j - 1 : env_len];
-      out[(int64_t)(pi - 1) * nsite + (pj - 1)] = v;
-      out[(int64_t)(pj - 1) * nsite + (pi - 1)] = v;
+  rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(TS.osz, 1));
+  const int nops = (int)TS.lbl.size();
+  std::vector<double> h(nops);
+  BufPtrs b{};
+  b.p[B_IN] = (double2*)psi;
+  b.p[B_OUT] = rho.c();
+  if (!centre) {
+    // pairs inside the sys block: <n_r n_j> = Re <P_rj, rho'(psi, psi)>
+    launch_on(ctx, compile_block_rho(S, 0, sb->plan.sbs, TS.sec), b);
+    mdots_to_host(ctx, TS.ops.c(), TS.osz, nops, rho.c(), TS.osz, h.data());
+    for (int q = 0; q < nops; ++q) {
+      const auto& l = TS.lbl[q];
+      if (l.second != 0) out[(l.first == r ? l.second : l.first) - 1] = h[q];
     }
   }
+  // the sys centre site: Y = n_centre psi; <n_c n_j> = Re <O_j, rho'(Y, psi)>
+  apply_sys(ctx, sb.get(), TS, -1, psi, Y.c());
+  b.p[B_X] = Y.c();
+  launch_on(ctx, compile_block_rho(S, 0, sb->plan.sbs, TS.sec, B_X, B_IN), b);
+  mdots_to_host(ctx, TS.ops.c(), TS.osz, nops, rho.c(), TS.osz, h.data());
+  if (centre) {
+    for (int q = 0; q < nops; ++q) out[TS.lbl[q].first - 1] = h[q];
+  } else {
+    out[sys_len] = h[TS.find(r, 0)];
+    apply_sys(ctx, sb.get(), TS, TS.find(r, 0), psi, Y.c());
+  }
+  env_entries(ctx, sb.get(), TE, env_len, psi, Y.c(), out);
 }
 
 // message of a failed hh_ctx_create (the context no longer exists)
@@ -1614,6 +1714,14 @@ int hh_driver_density_cross(hh_driver* h, int sys_len, const double* psi, double
   return guard(h->d.ctx, [&] {
     need(psi && nn_out, "null argument");
     cross_density(h->d.ctx, h->d, sys_len, (const double2*)psi, nn_out);
+  });
+}
+
+int hh_driver_density_row(hh_driver* h, int sys_len, int site, const double* psi, double* nn_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && nn_out, "null argument");
+    density_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, nn_out);
   });
 }
 

@@ -228,6 +228,16 @@ int hh_driver_density(hh_driver* d, int sys_len, const double* psi_dev, double* 
  * operators.  nn_out is Nsite x Nsite row-major by lattice position - 1,
  * symmetric, NaN for pairs on one side (those straddle another cut). */
 int hh_driver_density_cross(hh_driver* d, int sys_len, const double* psi_dev, double* nn_out);
+/* one row of Get_density_cor (code/correlation.f90:43-195) with
+ * oper1 = oper2 = the density: <n_site n_j> for every lattice position j
+ * (nn_out[j-1]; NaN at j = site), for a row site on the sys side of the cut
+ * (1 .. sys_len + 1).  Pairs inside the sys block use the two-site operator
+ * carried through the truncations (Get_sys2b, :1-40, measured as
+ * measure_sys_block); the block site with the sys centre site is
+ * sys_block_site_cor_ndia (:1335-1489); the env side as in
+ * hh_driver_density_cross.  The reference evaluates this row for j on the
+ * same leg as the row site; every j is returned here. */
+int hh_driver_density_row(hh_driver* d, int sys_len, int site, const double* psi_dev, double* nn_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

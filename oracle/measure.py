@@ -19,10 +19,15 @@ sites included): <psi| A_i (x) B_j |psi> = <psi, Y_i B_j^T> per superblock
 pair block, Y_i = A_i psi, with the physical action O_b / sqrt(2 S_b + 1) of
 a rank-0 reduced block operator on block sector b.
 
+``density_row`` gives one row <n_r n_j> of the same correlation for a row
+site r on the system side and every j, with the two-site products n_r n_j of
+block sites carried through the truncations (``tracked_pairs``).
+
 Pinned by exact diagonalisation (``ed.singlet_ground_density``,
 ``ed.singlet_ground_correlations``): with no truncation the profile and the
 correlations equal the exact ones; the densities sum to N and, for each i,
-sum_j <n_i n_j> over the other side equals <n_i N_other>.
+sum_j <n_i n_j> over the other side equals <n_i N_other>; a full row plus
+<n_r^2> sums to N <n_r>.
 """
 import math
 
@@ -150,4 +155,91 @@ def cross_density(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, 
                             v += float(np.real(np.vdot(ps, ys @ (m / math.sqrt(e_secs[b][1] + 1)).T)))
             pj = nsite + 1 - kj if B is not None else nsite - env_len
             out[pi - 1, pj - 1] = out[pj - 1, pi - 1] = v
+    return out
+
+
+def _product(A, B, eb):
+    """Reduced matrices of the product of two rank-0 operators of the
+    enlarged basis: (AB)_a = A_a B_a / sqrt(2 J_a + 1)."""
+    out = {}
+    for key, m in A.items():
+        o = B.get(key)
+        if o is not None:
+            out[key] = m @ o / math.sqrt(eb.sectors[key[0]][1] + 1)
+    return out
+
+
+def tracked_pairs(model, blocks, truns, length, r, side="sys"):
+    """{j: P_rj} with P_rj = n_r n_j (j != r, both block sites) in the basis of
+    blocks[length] (Get_sys2b, code/correlation.f90:1-40): the product is
+    formed in the enlarged basis of the step where its later site joins and is
+    carried through every later truncation."""
+    st = dmrg.site_sectors(model.kind)
+    n = _site_density_op(model.kind)
+    dens = {1: n}
+    pairs = {}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        site = dmrg.embed_site_op(n, prev, eb, side)
+        emb = lambda o: dmrg.embed_block_op(o, prev, eb, side)
+        newp = {j: _transform(emb(P), truns[l]) for j, P in pairs.items()}
+        if l < r:
+            newd = {k: _transform(emb(o), truns[l]) for k, o in dens.items()}
+            newd[l] = _transform(site, truns[l])
+        elif l == r:
+            for k, o in dens.items():
+                newp[k] = _transform(_product(emb(o), site, eb), truns[l])
+            newd = {r: _transform(site, truns[l])}
+        else:
+            e = emb(dens[r])
+            newp[l] = _transform(_product(e, site, eb), truns[l])
+            newd = {r: _transform(e, truns[l])}
+        dens, pairs = newd, newp
+    return pairs
+
+
+def density_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H, r):
+    """<n_r n_j> for every lattice position j (index j-1, NaN at j = r), for a
+    row site r on the system side (1 .. sys_len + 1): one row of
+    ``Get_density_cor`` (code/correlation.f90:43-195) over every j.
+
+    Block-block pairs: Tr(rho_b P_b) / sqrt(2 S_b + 1) with the carried
+    products P of ``tracked_pairs``.  Block site with the system centre site:
+    sum over pair blocks and (b, s) of n_s <psi_sub, A_b psi_sub> /
+    sqrt(2 S_b + 1).  Environment side: ``cross_density``."""
+    nsite = model.nsite
+    out = np.full(nsite, np.nan)
+    cc = cross_density(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H)
+    out[sys_len + 1:] = cc[r - 1, sys_len + 1:]
+    ops = tracked_ops(model, sys_blocks, sys_truns, sys_len, "sys")
+    secs = sys_blocks[sys_len].sectors
+    pin = dmrg.psi_blocks(H.sb, psi)
+    eb = H.s.basis
+    ns = [st[0] for st in eb.st_sectors]
+
+    def with_centre(A):
+        v = 0.0
+        for i, (x, _) in enumerate(H.sb.pairs):
+            for (b, s, pos, d) in eb.subs[x]:
+                m = A.blk.get((b, b))
+                if m is None or ns[s] == 0:
+                    continue
+                blk = pin[i][pos:pos + d, :]
+                v += ns[s] * float(np.real(np.vdot(blk, (m / math.sqrt(secs[b][1] + 1)) @ blk)))
+        return v
+
+    if r == sys_len + 1:
+        for k in range(1, sys_len + 1):
+            out[k - 1] = with_centre(ops[k])
+        return out
+    out[sys_len] = with_centre(ops[r])
+    rho, _ = _block_rho(H, psi, "sys")
+    for j, P in tracked_pairs(model, sys_blocks, sys_truns, sys_len, r).items():
+        v = 0.0
+        for b, rb in rho.items():
+            m = P.blk.get((b, b))
+            if m is not None:
+                v += float(np.real(np.trace(rb @ m))) / math.sqrt(secs[b][1] + 1)
+        out[j - 1] = v
     return out

@@ -88,3 +88,47 @@ def test_cross_density_with_a_wrong_scale_fails():
     bad = measure.cross_density(m, run.sys_blocks, run.env_blocks, run.sys_trun, bad_truns, L, R, psi, H)
     fin = ~np.isnan(good)
     assert np.abs(good[fin] - bad[fin]).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind,flux,r", [("hubbard", 0.15, 1), ("hubbard", 0.15, 2), ("tj", 0.2, 3),
+                                         ("tj", 0.2, 4)])
+def test_untruncated_density_row_equals_ed(kind, flux, r):
+    """Every pair class of a correlation row (block-block through the carried
+    two-site products, block-centre, centre-env, block-env) equals the exact
+    two-point function; r = 4 is the system centre site of the 4 x 2 cut."""
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    row = measure.density_row(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H, r)
+    others = [j for j in range(m.nsite) if j != r - 1]
+    assert np.isnan(row[r - 1]) and not np.isnan(row[others]).any()
+    assert np.abs(row[others] - nn[r - 1, others]).max() < 1e-6
+
+
+def test_density_row_with_a_wrong_product_fails():
+    """A planted mistake in the two-site product (the 1/sqrt(2J+1) of a
+    product of reduced rank-0 operators dropped) moves the block-block pairs."""
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    L, R = 3, 3
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    good = measure.density_row(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H, 1)
+    saved = measure._product
+    try:
+        measure._product = lambda A, B, eb: {k: A[k] @ B[k] for k in A if k in B}
+        bad = measure.density_row(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H, 1)
+    finally:
+        measure._product = saved
+    assert np.abs(good[1:L] - bad[1:L]).max() > 1e-3
