@@ -442,7 +442,7 @@ std::vector<Trans> transitions(const Piece& p, int from, const std::vector<Secto
 }  // namespace
 
 MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDesc& sys, const BlockDesc& env,
-                          int N) {
+                          int N, const MeasureTerm* only) {
   MatvecPlan P;
   const auto st = site_sectors(m.kind);
   const int nst = (int)st.size();
@@ -669,32 +669,6 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
       for (auto& cs : EB.subs[sb.pairs[i].second])
         out.ensure(TileKey{B_OUT, in_tile_off(i, rs.pos, cs.pos), sb.rows[i], SB.bl[rs.b].dim, EB.bl[cs.b].dim});
 
-  // ---------- block Hamiltonians (block_vec_dia / vec_block_dia) -----------------
-  for (int i = 0; i < (int)sb.pairs.size(); ++i) {
-    const int al = sb.pairs[i].first, be = sb.pairs[i].second;
-    for (auto& rs : SB.subs[al])
-      for (auto& cs : EB.subs[be]) {
-        const int mb = SB.bl[rs.b].dim, me = EB.bl[cs.b].dim;
-        TileKey k{B_OUT, in_tile_off(i, rs.pos, cs.pos), sb.rows[i], mb, me};
-        const int64_t tin = in_tile_off(i, rs.pos, cs.pos);
-        out.add(k, mk_term(1.0 / std::sqrt((double)SB.bl[rs.b].S2 + 1), B_SYS, sys.h_off[rs.b], mb, OP_NN, B_IN, tin,
-                           sb.rows[i], OP_NN, mb));
-        out.add(k, mk_term(1.0 / std::sqrt((double)EB.bl[cs.b].S2 + 1), B_IN, tin, sb.rows[i], OP_NN, B_ENV,
-                           env.h_off[cs.b], me, OP_T, me));
-        P.stage2.flops_alg += 8.0 * mb * me * (mb + me);
-        // on-site U of the two centre sites (site_vec_site_dia with st_double)
-        if (m.kind == HUBBARD && m.U != 0.0) {
-          double u = 0.0;
-          if (rs.s == 2) u += m.U;
-          if (cs.s == 2) u += m.U;
-          if (u != 0.0) {
-            out.add(k, mk_term(u, B_IN, tin, sb.rows[i], OP_NN, 0, 0, 0, OP_NN, 0));
-            P.stage2.flops_alg += 8.0 * mb * me;
-          }
-        }
-      }
-  }
-
   // ---------- couplings ----------------------------------------------------------
   const int psc = g.sys_centre(), pec = g.env_centre();
   auto sys_block_piece = [&](int kind, int64_t base, uint8_t buf) {
@@ -716,131 +690,167 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
     else p.site = site_fn(kind);
     return p;
   };
-  // bonds of every stored boundary site against the centre sites / env sites
-  struct Bond { int si; int ej; Coupling c; };
-  std::vector<std::pair<int, Coupling>> sb_sc, sb_ec, eb_sc, eb_ec;   // (site idx, coupling)
-  std::vector<Bond> bb;
-  for (size_t a = 0; a < sys.sites.size(); ++a) {
-    Coupling c;
-    const int p = g.sys_pos(sys.sites[a]);
-    if (g.coupling(p, psc, c)) sb_sc.push_back({(int)a, c});
-    if (g.coupling(p, pec, c)) sb_ec.push_back({(int)a, c});
-    for (size_t e = 0; e < env.sites.size(); ++e)
-      if (g.coupling(p, g.env_pos(env.sites[e]), c)) bb.push_back({(int)a, (int)e, c});
-  }
-  for (size_t e = 0; e < env.sites.size(); ++e) {
-    Coupling c;
-    const int p = g.env_pos(env.sites[e]);
-    if (g.coupling(psc, p, c)) eb_sc.push_back({(int)e, c});
-    if (g.coupling(pec, p, c)) eb_ec.push_back({(int)e, c});
-  }
-  Coupling cc;
-  const bool has_ss = g.coupling(psc, pec, cc);
-
-  const double r2 = std::sqrt(2.0), r3 = std::sqrt(3.0);
-  const int nk = n_op_kinds(m.kind);
-  // helper to build a merge and return its piece
-  auto merge_sys = [&](int kind, const std::vector<std::pair<int, cplx>>& parts) {
-    int id = mb.add(0, kind, parts, sys.lay[kind]);
-    return sys_block_piece(kind, P.merges[id].off, B_MRG);
-  };
-  auto merge_env = [&](int kind, const std::vector<std::pair<int, cplx>>& parts, bool tilde) {
-    int id = mb.add(1, kind, parts, env.lay[kind]);
-    return env_block_piece(kind, P.merges[id].off, B_MRG, tilde);
-  };
-
-  // sys block <-> sys centre (block_site_vec_ndia / _dia)
-  if (!sb_sc.empty()) {
-    std::vector<std::pair<int, cplx>> ph, pj, pv;
-    for (auto& x : sb_sc) {
-      if (x.second.hop != 0.0) ph.push_back({x.first, x.second.hop});
-      if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
-      if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
-    }
-    if (!ph.empty()) within(0, merge_sys(OP_CDAG, ph), site_piece(0, OP_CDAG, true), r2, true);
-    if (nk > 1 && !pj.empty()) within(0, merge_sys(OP_S, pj), site_piece(0, OP_S, false), -r3, false);
-    if (nk > 1 && !pv.empty()) within(0, merge_sys(OP_N, pv), site_piece(0, OP_N, false), 1.0, false);
-  }
-  // env centre <-> env block (vec_block_site_ndia / _dia); fermion order centre first
-  if (!eb_ec.empty()) {
-    std::vector<std::pair<int, cplx>> ph, pj, pv;
-    for (auto& x : eb_ec) {
-      if (x.second.hop != 0.0) ph.push_back({x.first, std::conj(x.second.hop)});
-      if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
-      if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
-    }
-    if (!ph.empty()) within(1, merge_env(OP_CDAG, ph, true), site_piece(1, OP_CDAG, false), r2, true);
-    if (nk > 1 && !pj.empty()) within(1, merge_env(OP_S, pj, false), site_piece(1, OP_S, false), -r3, false);
-    if (nk > 1 && !pv.empty()) within(1, merge_env(OP_N, pv, false), site_piece(1, OP_N, false), 1.0, false);
-  }
-  // sys centre <-> env centre (site_vec_site_ndia / _dia)
-  if (has_ss) {
-    if (cc.hop != 0.0) cross(site_piece(0, OP_CDAG, false), site_piece(1, OP_CDAG, true), r2 * cc.hop, true);
-    if (nk > 1 && cc.J != 0.0) cross(site_piece(0, OP_S, false), site_piece(1, OP_S, false), -r3 * cc.J, false);
-    if (nk > 1 && cc.V != 0.0) cross(site_piece(0, OP_N, false), site_piece(1, OP_N, false), cc.V, false);
-  }
-  // sys block <-> env centre (block_vec_site_ndia / _dia)
-  if (!sb_ec.empty()) {
-    std::vector<std::pair<int, cplx>> ph, pj, pv;
-    for (auto& x : sb_ec) {
-      if (x.second.hop != 0.0) ph.push_back({x.first, x.second.hop});
-      if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
-      if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
-    }
-    if (!ph.empty()) cross(merge_sys(OP_CDAG, ph), site_piece(1, OP_CDAG, true), r2, true);
-    if (nk > 1 && !pj.empty()) cross(merge_sys(OP_S, pj), site_piece(1, OP_S, false), -r3, false);
-    if (nk > 1 && !pv.empty()) cross(merge_sys(OP_N, pv), site_piece(1, OP_N, false), 1.0, false);
-  }
-  // sys centre <-> env block (site_vec_block_ndia / _dia)
-  if (!eb_sc.empty()) {
-    std::vector<std::pair<int, cplx>> ph, pj, pv;
-    for (auto& x : eb_sc) {
-      if (x.second.hop != 0.0) ph.push_back({x.first, std::conj(x.second.hop)});
-      if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
-      if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
-    }
-    if (!ph.empty()) cross(site_piece(0, OP_CDAG, false), merge_env(OP_CDAG, ph, true), r2, true);
-    if (nk > 1 && !pj.empty()) cross(site_piece(0, OP_S, false), merge_env(OP_S, pj, false), -r3, false);
-    if (nk > 1 && !pv.empty()) cross(site_piece(0, OP_N, false), merge_env(OP_N, pv, false), 1.0, false);
-  }
-  // block <-> block (block_vec_block_ndia / _dia): one term per boundary site
-  // on the side with more cross-bonded sites, partners merged on the other
-  // side (hmerge's inb / coup_bb rule, code/mat_vec.f90:3739-3771)
-  if (!bb.empty()) {
-    std::set<int> ssites, esites;
-    for (auto& x : bb) { ssites.insert(x.si); esites.insert(x.ej); }
-    const bool per_env = esites.size() < ssites.size();
-    if (!per_env) {
-      for (int si : ssites) {
-        std::vector<std::pair<int, cplx>> ph, pj, pv;
-        for (auto& x : bb)
-          if (x.si == si) {
-            if (x.c.hop != 0.0) ph.push_back({x.ej, std::conj(x.c.hop)});
-            if (x.c.J != 0.0) pj.push_back({x.ej, x.c.J});
-            if (x.c.V != 0.0) pv.push_back({x.ej, x.c.V});
+  if (only) {
+    auto piece = [&](const MeasurePiece& q, int side) {
+      if (!q.block) return site_piece(side, q.kind, false);
+      Piece p;
+      p.block = true; p.side = side; p.kind = q.kind; p.tilde = false; p.buf = q.buf; p.base = q.base;
+      p.lay = q.lay; p.k2 = op_k2(q.kind); p.fermion = op_fermion(q.kind);
+      return p;
+    };
+    cross(piece(only->s, 0), piece(only->e, 1), only->amp, false);
+  } else {
+    // ---------- block Hamiltonians (block_vec_dia / vec_block_dia) -----------------
+    for (int i = 0; i < (int)sb.pairs.size(); ++i) {
+      const int al = sb.pairs[i].first, be = sb.pairs[i].second;
+      for (auto& rs : SB.subs[al])
+        for (auto& cs : EB.subs[be]) {
+          const int mb = SB.bl[rs.b].dim, me = EB.bl[cs.b].dim;
+          TileKey k{B_OUT, in_tile_off(i, rs.pos, cs.pos), sb.rows[i], mb, me};
+          const int64_t tin = in_tile_off(i, rs.pos, cs.pos);
+          out.add(k, mk_term(1.0 / std::sqrt((double)SB.bl[rs.b].S2 + 1), B_SYS, sys.h_off[rs.b], mb, OP_NN, B_IN, tin,
+                             sb.rows[i], OP_NN, mb));
+          out.add(k, mk_term(1.0 / std::sqrt((double)EB.bl[cs.b].S2 + 1), B_IN, tin, sb.rows[i], OP_NN, B_ENV,
+                             env.h_off[cs.b], me, OP_T, me));
+          P.stage2.flops_alg += 8.0 * mb * me * (mb + me);
+          // on-site U of the two centre sites (site_vec_site_dia with st_double)
+          if (m.kind == HUBBARD && m.U != 0.0) {
+            double u = 0.0;
+            if (rs.s == 2) u += m.U;
+            if (cs.s == 2) u += m.U;
+            if (u != 0.0) {
+              out.add(k, mk_term(u, B_IN, tin, sb.rows[i], OP_NN, 0, 0, 0, OP_NN, 0));
+              P.stage2.flops_alg += 8.0 * mb * me;
+            }
           }
-        if (!ph.empty())
-          cross(sys_block_piece(OP_CDAG, sys.op_off[si][OP_CDAG], B_SYS), merge_env(OP_CDAG, ph, true), r2, true);
-        if (nk > 1 && !pj.empty())
-          cross(sys_block_piece(OP_S, sys.op_off[si][OP_S], B_SYS), merge_env(OP_S, pj, false), -r3, false);
-        if (nk > 1 && !pv.empty())
-          cross(sys_block_piece(OP_N, sys.op_off[si][OP_N], B_SYS), merge_env(OP_N, pv, false), 1.0, false);
+        }
+    }
+    // bonds of every stored boundary site against the centre sites / env sites
+    struct Bond { int si; int ej; Coupling c; };
+    std::vector<std::pair<int, Coupling>> sb_sc, sb_ec, eb_sc, eb_ec;   // (site idx, coupling)
+    std::vector<Bond> bb;
+    for (size_t a = 0; a < sys.sites.size(); ++a) {
+      Coupling c;
+      const int p = g.sys_pos(sys.sites[a]);
+      if (g.coupling(p, psc, c)) sb_sc.push_back({(int)a, c});
+      if (g.coupling(p, pec, c)) sb_ec.push_back({(int)a, c});
+      for (size_t e = 0; e < env.sites.size(); ++e)
+        if (g.coupling(p, g.env_pos(env.sites[e]), c)) bb.push_back({(int)a, (int)e, c});
+    }
+    for (size_t e = 0; e < env.sites.size(); ++e) {
+      Coupling c;
+      const int p = g.env_pos(env.sites[e]);
+      if (g.coupling(psc, p, c)) eb_sc.push_back({(int)e, c});
+      if (g.coupling(pec, p, c)) eb_ec.push_back({(int)e, c});
+    }
+    Coupling cc;
+    const bool has_ss = g.coupling(psc, pec, cc);
+
+    const double r2 = std::sqrt(2.0), r3 = std::sqrt(3.0);
+    const int nk = n_op_kinds(m.kind);
+    // helper to build a merge and return its piece
+    auto merge_sys = [&](int kind, const std::vector<std::pair<int, cplx>>& parts) {
+      int id = mb.add(0, kind, parts, sys.lay[kind]);
+      return sys_block_piece(kind, P.merges[id].off, B_MRG);
+    };
+    auto merge_env = [&](int kind, const std::vector<std::pair<int, cplx>>& parts, bool tilde) {
+      int id = mb.add(1, kind, parts, env.lay[kind]);
+      return env_block_piece(kind, P.merges[id].off, B_MRG, tilde);
+    };
+
+    // sys block <-> sys centre (block_site_vec_ndia / _dia)
+    if (!sb_sc.empty()) {
+      std::vector<std::pair<int, cplx>> ph, pj, pv;
+      for (auto& x : sb_sc) {
+        if (x.second.hop != 0.0) ph.push_back({x.first, x.second.hop});
+        if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
+        if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
       }
-    } else {
-      for (int ej : esites) {
-        std::vector<std::pair<int, cplx>> ph, pj, pv;
-        for (auto& x : bb)
-          if (x.ej == ej) {
-            if (x.c.hop != 0.0) ph.push_back({x.si, x.c.hop});
-            if (x.c.J != 0.0) pj.push_back({x.si, x.c.J});
-            if (x.c.V != 0.0) pv.push_back({x.si, x.c.V});
-          }
-        if (!ph.empty())
-          cross(merge_sys(OP_CDAG, ph), env_block_piece(OP_CDAG, env.op_off[ej][OP_CDAG], B_ENV, true), r2, true);
-        if (nk > 1 && !pj.empty())
-          cross(merge_sys(OP_S, pj), env_block_piece(OP_S, env.op_off[ej][OP_S], B_ENV, false), -r3, false);
-        if (nk > 1 && !pv.empty())
-          cross(merge_sys(OP_N, pv), env_block_piece(OP_N, env.op_off[ej][OP_N], B_ENV, false), 1.0, false);
+      if (!ph.empty()) within(0, merge_sys(OP_CDAG, ph), site_piece(0, OP_CDAG, true), r2, true);
+      if (nk > 1 && !pj.empty()) within(0, merge_sys(OP_S, pj), site_piece(0, OP_S, false), -r3, false);
+      if (nk > 1 && !pv.empty()) within(0, merge_sys(OP_N, pv), site_piece(0, OP_N, false), 1.0, false);
+    }
+    // env centre <-> env block (vec_block_site_ndia / _dia); fermion order centre first
+    if (!eb_ec.empty()) {
+      std::vector<std::pair<int, cplx>> ph, pj, pv;
+      for (auto& x : eb_ec) {
+        if (x.second.hop != 0.0) ph.push_back({x.first, std::conj(x.second.hop)});
+        if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
+        if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
+      }
+      if (!ph.empty()) within(1, merge_env(OP_CDAG, ph, true), site_piece(1, OP_CDAG, false), r2, true);
+      if (nk > 1 && !pj.empty()) within(1, merge_env(OP_S, pj, false), site_piece(1, OP_S, false), -r3, false);
+      if (nk > 1 && !pv.empty()) within(1, merge_env(OP_N, pv, false), site_piece(1, OP_N, false), 1.0, false);
+    }
+    // sys centre <-> env centre (site_vec_site_ndia / _dia)
+    if (has_ss) {
+      if (cc.hop != 0.0) cross(site_piece(0, OP_CDAG, false), site_piece(1, OP_CDAG, true), r2 * cc.hop, true);
+      if (nk > 1 && cc.J != 0.0) cross(site_piece(0, OP_S, false), site_piece(1, OP_S, false), -r3 * cc.J, false);
+      if (nk > 1 && cc.V != 0.0) cross(site_piece(0, OP_N, false), site_piece(1, OP_N, false), cc.V, false);
+    }
+    // sys block <-> env centre (block_vec_site_ndia / _dia)
+    if (!sb_ec.empty()) {
+      std::vector<std::pair<int, cplx>> ph, pj, pv;
+      for (auto& x : sb_ec) {
+        if (x.second.hop != 0.0) ph.push_back({x.first, x.second.hop});
+        if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
+        if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
+      }
+      if (!ph.empty()) cross(merge_sys(OP_CDAG, ph), site_piece(1, OP_CDAG, true), r2, true);
+      if (nk > 1 && !pj.empty()) cross(merge_sys(OP_S, pj), site_piece(1, OP_S, false), -r3, false);
+      if (nk > 1 && !pv.empty()) cross(merge_sys(OP_N, pv), site_piece(1, OP_N, false), 1.0, false);
+    }
+    // sys centre <-> env block (site_vec_block_ndia / _dia)
+    if (!eb_sc.empty()) {
+      std::vector<std::pair<int, cplx>> ph, pj, pv;
+      for (auto& x : eb_sc) {
+        if (x.second.hop != 0.0) ph.push_back({x.first, std::conj(x.second.hop)});
+        if (x.second.J != 0.0) pj.push_back({x.first, x.second.J});
+        if (x.second.V != 0.0) pv.push_back({x.first, x.second.V});
+      }
+      if (!ph.empty()) cross(site_piece(0, OP_CDAG, false), merge_env(OP_CDAG, ph, true), r2, true);
+      if (nk > 1 && !pj.empty()) cross(site_piece(0, OP_S, false), merge_env(OP_S, pj, false), -r3, false);
+      if (nk > 1 && !pv.empty()) cross(site_piece(0, OP_N, false), merge_env(OP_N, pv, false), 1.0, false);
+    }
+    // block <-> block (block_vec_block_ndia / _dia): one term per boundary site
+    // on the side with more cross-bonded sites, partners merged on the other
+    // side (hmerge's inb / coup_bb rule, code/mat_vec.f90:3739-3771)
+    if (!bb.empty()) {
+      std::set<int> ssites, esites;
+      for (auto& x : bb) { ssites.insert(x.si); esites.insert(x.ej); }
+      const bool per_env = esites.size() < ssites.size();
+      if (!per_env) {
+        for (int si : ssites) {
+          std::vector<std::pair<int, cplx>> ph, pj, pv;
+          for (auto& x : bb)
+            if (x.si == si) {
+              if (x.c.hop != 0.0) ph.push_back({x.ej, std::conj(x.c.hop)});
+              if (x.c.J != 0.0) pj.push_back({x.ej, x.c.J});
+              if (x.c.V != 0.0) pv.push_back({x.ej, x.c.V});
+            }
+          if (!ph.empty())
+            cross(sys_block_piece(OP_CDAG, sys.op_off[si][OP_CDAG], B_SYS), merge_env(OP_CDAG, ph, true), r2, true);
+          if (nk > 1 && !pj.empty())
+            cross(sys_block_piece(OP_S, sys.op_off[si][OP_S], B_SYS), merge_env(OP_S, pj, false), -r3, false);
+          if (nk > 1 && !pv.empty())
+            cross(sys_block_piece(OP_N, sys.op_off[si][OP_N], B_SYS), merge_env(OP_N, pv, false), 1.0, false);
+        }
+      } else {
+        for (int ej : esites) {
+          std::vector<std::pair<int, cplx>> ph, pj, pv;
+          for (auto& x : bb)
+            if (x.ej == ej) {
+              if (x.c.hop != 0.0) ph.push_back({x.si, x.c.hop});
+              if (x.c.J != 0.0) pj.push_back({x.si, x.c.J});
+              if (x.c.V != 0.0) pv.push_back({x.si, x.c.V});
+            }
+          if (!ph.empty())
+            cross(merge_sys(OP_CDAG, ph), env_block_piece(OP_CDAG, env.op_off[ej][OP_CDAG], B_ENV, true), r2, true);
+          if (nk > 1 && !pj.empty())
+            cross(merge_sys(OP_S, pj), env_block_piece(OP_S, env.op_off[ej][OP_S], B_ENV, false), -r3, false);
+          if (nk > 1 && !pv.empty())
+            cross(merge_sys(OP_N, pv), env_block_piece(OP_N, env.op_off[ej][OP_N], B_ENV, false), 1.0, false);
+        }
       }
     }
   }

@@ -193,8 +193,26 @@ struct MatvecPlan {
   double bytes_alg = 0.0;        // 16*(2 D) + 16*(unique operator sub-block elements)
 };
 
+// One operator of a two-point measurement term: a block operator (layout
+// `lay` over the block's sectors, data at `base` of slot `buf`) or the centre
+// site's operator of that kind.
+struct MeasurePiece {
+  bool block = false;
+  int kind = OP_S;
+  uint8_t buf = B_SYS;
+  int64_t base = 0;
+  const OpLayout* lay = nullptr;
+};
+// [A_sys (x) B_env]^0 scaled by amp, the coupling form of the H.psi cross terms
+struct MeasureTerm {
+  MeasurePiece s, e;
+  double amp = 1.0;
+};
+// With only != nullptr the plan holds that one term instead of the model's H
+// (block Hamiltonians, U and every bond left out); sys.sec / env.sec give the
+// bases and the blocks' own operators are not read.
 MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDesc& sys,
-                          const BlockDesc& env, int N);
+                          const BlockDesc& env, int N, const MeasureTerm* only = nullptr);
 
 // ----------------------------------------------------------------------------
 // growth + renormalisation compiler (Get_Hamiltonian_Sys/Env + update_trun_*)
@@ -249,6 +267,14 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
 // k times the new site's density (the two-site operator n_k n_new of
 // Get_sys2b, created when its second site joins the block).  New operator q
 // goes to B_OUT at q * new_size.
+// One step of the chain of a rank-1 operator (the spin S of every block
+// site): the nold operators of block `prev` (layout make_layout(prev, 2, 0),
+// slot B_SYS, old_size apart) embedded in the enlarged basis
+// (update_block_ndia_*) plus the new site's S (update_site_ndia_*), each
+// renormalised as U_bra^dag O U_ket; nold + 1 operators in
+// make_layout(new sectors, 2, 0) to B_OUT, new_size apart.
+TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold);
 struct TrackSrc {
   enum What { CARRY, SITE, PRODUCT } what;
   int k;

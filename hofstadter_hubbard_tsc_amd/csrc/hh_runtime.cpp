@@ -1074,6 +1074,125 @@ void cross_density(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, doub
   }
 }
 
+// Spin operators S_k (rank 1) of every site of one side's block, carried to
+// its current length through the stored truncations: operator k-1 is site k,
+// layout `lay` over the sectors `sec`, lay.size elements apart.
+struct TrackedSpin {
+  DevMem ops;
+  std::vector<Sector> sec;
+  OpLayout lay;
+  int n = 0;
+};
+
+void track_spin_chain(hh_ctx* ctx, Driver& d, int side, int len, TrackedSpin& T) {
+  const ModelParams& m = ctx->model;
+  const auto& blocks = side == 0 ? d.sysb : d.envb;
+  const auto& truns = side == 0 ? d.syst : d.envt;
+  T.sec = blocks.at(1)->d.sec;
+  T.lay = make_layout(T.sec, 2, 0);
+  T.n = 1;
+  {
+    std::vector<double2> h((size_t)std::max<int64_t>(T.lay.size, 1), make_double2(0.0, 0.0));
+    for (auto& sub : T.lay.subs) {   // site multiplets have dimension 1
+      const cplx v = site_val(m.kind, OP_S, sub.bra, sub.ket);
+      h[sub.off] = make_double2(v.real(), v.imag());
+    }
+    T.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  }
+  DevMem nxt, X;
+  for (int l = 2; l <= len; ++l) {
+    need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
+    const hh_trunc* t = truns.at(l).get();
+    TrackPlan P = compile_track_spin(m, T.sec, t->kept, T.n);
+    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (T.n + 1), 1));
+    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+    BufPtrs b{};
+    b.p[B_SYS] = T.ops.c();
+    b.p[B_AUX] = t->evec.c();
+    b.p[B_X] = X.c();
+    b.p[B_OUT] = nxt.c();
+    DevTasks s1, s2;
+    s1.upload(P.stage1, ctx->stream);
+    s2.upload(P.stage2, ctx->stream);
+    launch_tiles(s1, b, ctx->stream, PREC_FP64);
+    launch_tiles(s2, b, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "spin chain");
+    std::swap(T.ops.p, nxt.p);
+    std::swap(T.ops.bytes, nxt.bytes);
+    T.sec.clear();
+    for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+    T.lay = make_layout(T.sec, 2, 0);
+    T.n += 1;
+  }
+}
+
+// <S_i . S_j> for i on the sys side (block sites and the sys centre site) and
+// j on the env side: the pairs of Get_density_cor with oper1 = oper2 = the
+// spin (code/correlation.f90:43-195; sys_block_env_block_cor_ndia and the
+// block/site variants, :1666-2656).  Each pair is one two-point term
+// -sqrt(3) [S_i (x) S_j]^0 compiled like the exchange bond of H.psi and
+// applied to psi; <S_i . S_j> = Re <psi, term psi>.  out as cross_density.
+void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  TrackedSpin TS, TE;
+  track_spin_chain(ctx, d, 0, sys_len, TS);
+  track_spin_chain(ctx, d, 1, env_len, TE);
+  BlockDesc sd, ed;
+  sd.side = 0;
+  sd.length = sys_len;
+  sd.sec = TS.sec;
+  ed.side = 1;
+  ed.length = env_len;
+  ed.sec = TE.sec;
+  const Geometry g = make_geometry(m, sys_len, env_len);
+  for (int i = 0; i < nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  DevMem Y, X;
+  for (int i = 1; i <= sys_len + 1; ++i)
+    for (int j = 1; j <= env_len + 1; ++j) {
+      MeasureTerm t;
+      t.amp = -std::sqrt(3.0);
+      t.s.kind = t.e.kind = OP_S;
+      if (i <= sys_len) {
+        t.s.block = true;
+        t.s.buf = B_SYS;
+        t.s.base = (int64_t)(i - 1) * TS.lay.size;
+        t.s.lay = &TS.lay;
+      }
+      if (j <= env_len) {
+        t.e.block = true;
+        t.e.buf = B_ENV;
+        t.e.base = (int64_t)(j - 1) * TE.lay.size;
+        t.e.lay = &TE.lay;
+      }
+      MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
+      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.sb.dim, 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      BufPtrs b{};
+      b.p[B_IN] = (double2*)psi;
+      b.p[B_OUT] = Y.c();
+      b.p[B_SYS] = TS.ops.c();
+      b.p[B_ENV] = TE.ops.c();
+      b.p[B_X] = X.c();
+      DevTasks s1, s2;
+      s1.upload(P.stage1, ctx->stream);
+      s2.upload(P.stage2, ctx->stream);
+      launch_tiles(s1, b, ctx->stream, PREC_FP64);
+      launch_tiles(s2, b, ctx->stream, PREC_FP64);
+      double v = 0.0;
+      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &v);
+      const int pi = i;   // sys site i (or the sys centre, i = sys_len + 1) sits at position i
+      const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;
+      out[(int64_t)(pi - 1) * nsite + (pj - 1)] = v;
+      out[(int64_t)(pj - 1) * nsite + (pi - 1)] = v;
+    }
+}
+
 // One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
 // site r on the sys side (block or centre) and every other site j.  Pairs of
 // block sites use the tracked products n_r n_j (Get_sys2b / measure_sys_block);
@@ -1722,6 +1841,14 @@ int hh_driver_density_row(hh_driver* h, int sys_len, int site, const double* psi
   return guard(h->d.ctx, [&] {
     need(psi && nn_out, "null argument");
     density_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, nn_out);
+  });
+}
+
+int hh_driver_spin_cross(hh_driver* h, int sys_len, const double* psi, double* ss_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && ss_out, "null argument");
+    spin_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, ss_out);
   });
 }
 

@@ -473,6 +473,16 @@ class Driver:
                      "hh_driver_density_row")
         return out
 
+    def spin_cross(self, sys_len, psi):
+        """<S_i . S_j> (nsite x nsite, NaN for pairs on one side) across the centre."""
+        _sync()
+        n = self.ctx.model.Lx * self.ctx.model.Ly
+        out = np.zeros((n, n))
+        self.ctx._rc(L.load().hh_driver_spin_cross(self.ptr, sys_len, _dptr(psi),
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_spin_cross")
+        return out
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

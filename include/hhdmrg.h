@@ -238,6 +238,16 @@ int hh_driver_density_cross(hh_driver* d, int sys_len, const double* psi_dev, do
  * hh_driver_density_cross.  The reference evaluates this row for j on the
  * same leg as the row site; every j is returned here. */
 int hh_driver_density_row(hh_driver* d, int sys_len, int site, const double* psi_dev, double* nn_out);
+/* spin correlations <S_i . S_j> across the superblock's centre (i on the sys
+ * side, j on the env side, centre sites included): Get_density_cor
+ * (code/correlation.f90:43-195) with oper1 = oper2 = st_sd, its pairs
+ * sys_block_env_block_cor_ndia (:1666-1929), sys_block_env_site_cor_ndia
+ * (:1932-2169), sys_site_env_site_cor_ndia (:2172-2402) and
+ * sys_site_env_block_cor_ndia (:2405-2656), with the spin of every block site
+ * carried through the stored truncations (Get_sys_oper_ndia /
+ * update_trun_ndia).  Returns the physical <S_i . S_j>.  ss_out as in
+ * hh_driver_density_cross. */
+int hh_driver_spin_cross(hh_driver* d, int sys_len, const double* psi_dev, double* ss_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

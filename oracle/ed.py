@@ -191,3 +191,35 @@ def singlet_ground_correlations(model, positions, bonds, n_elec):
     occ = _occupations(fs, positions)
     p = np.abs(psi) ** 2
     return e, occ @ p, (occ * p) @ occ.T
+
+
+def singlet_ground_spin_correlations(model, positions, bonds, n_elec):
+    """(E0, <S_i . S_j>) of the lowest total-S = 0 state, indexed by lattice
+    position - 1 (i != j; the diagonal is 3/4 <n_i - 2 n_i,up n_i,down>, left
+    as computed by the same formula)."""
+    e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
+    n = max(positions)
+    out = np.zeros((n, n))
+    order = {p: i for i, p in enumerate(positions)}
+    dim = len(fs.states)
+    for pa in positions:
+        for pb in positions:
+            i, j = order[pa], order[pb]
+            rows, cols, vals = [], [], []
+
+            def add(ops, c):
+                src, dst, sg = fs.apply(ops)
+                rows.extend(dst.tolist())
+                cols.extend(src.tolist())
+                vals.extend((c * sg).tolist())
+
+            for a in (0, 1):
+                for b in (0, 1):
+                    sa = 0.5 if a == 0 else -0.5
+                    sb = 0.5 if b == 0 else -0.5
+                    add([(2 * i + a, True), (2 * i + a, False), (2 * j + b, True), (2 * j + b, False)], sa * sb)
+            add([(2 * i, True), (2 * i + 1, False), (2 * j + 1, True), (2 * j, False)], 0.5)
+            add([(2 * i + 1, True), (2 * i, False), (2 * j, True), (2 * j + 1, False)], 0.5)
+            O = sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim), dtype=np.complex128)
+            out[pa - 1, pb - 1] = float(np.real(np.vdot(psi, O @ psi)))
+    return e, out

@@ -19,13 +19,18 @@ sites included): <psi| A_i (x) B_j |psi> = <psi, Y_i B_j^T> per superblock
 pair block, Y_i = A_i psi, with the physical action O_b / sqrt(2 S_b + 1) of
 a rank-0 reduced block operator on block sector b.
 
+``spin_cross`` gives <S_i . S_j> for the same pairs across the centre: the
+spin of every block site carried as a rank-1 operator (U_bra^dag O U_ket),
+embedded in the superblock's enlarged bases and contracted as the exchange
+bond, -sqrt(3) [S_i (x) S_j]^0.
+
 ``density_row`` gives one row <n_r n_j> of the same correlation for a row
 site r on the system side and every j, with the two-site products n_r n_j of
 block sites carried through the truncations (``tracked_pairs``).
 
 Pinned by exact diagonalisation (``ed.singlet_ground_density``,
-``ed.singlet_ground_correlations``): with no truncation the profile and the
-correlations equal the exact ones; the densities sum to N and, for each i,
+``ed.singlet_ground_correlations``, ``ed.singlet_ground_spin_correlations``):
+with no truncation the profile and the correlations equal the exact ones; the densities sum to N and, for each i,
 sum_j <n_i n_j> over the other side equals <n_i N_other>; a full row plus
 <n_r^2> sums to N <n_r>.
 """
@@ -242,4 +247,77 @@ def density_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, en
             if m is not None:
                 v += float(np.real(np.trace(rb @ m))) / math.sqrt(secs[b][1] + 1)
         out[j - 1] = v
+    return out
+
+
+def _transform_any(op_enl, trun, k2):
+    """U_bra^dag O U_ket for an enlarged operator of rank k2/2; trun =
+    [(enlarged sector, U)] in new-block order."""
+    pos = {a: i for i, (a, _) in enumerate(trun)}
+    out = dmrg.Op(k2, 0, False)
+    for (ap, a), m in op_enl.items():
+        if ap in pos and a in pos:
+            Ub, Uk = trun[pos[ap]][1], trun[pos[a]][1]
+            out.blk[(pos[ap], pos[a])] = Ub.conj().T @ m @ Uk
+    return out
+
+
+def tracked_spin_ops(model, blocks, truns, length, side):
+    """{site k: S_k in the basis of blocks[length]} (Get_sys_oper_ndia /
+    update_trun_ndia for the rank-1 spin), carried like ``tracked_ops``."""
+    st = dmrg.site_sectors(model.kind)
+    S = dmrg.site_ops(model.kind)["S"]
+    ops = {1: S}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        new = {k: _transform_any(dmrg.embed_block_op(o, prev, eb, side), truns[l], 2) for k, o in ops.items()}
+        new[l] = _transform_any(dmrg.embed_site_op(S, prev, eb, side), truns[l], 2)
+        ops = new
+    return ops
+
+
+def _cross_apply(H, T, U, k2, amp, psi):
+    """[T (x) U]^0 psi for enlarged-basis operators T (system) and U
+    (environment), the cross-term contraction of the superblock
+    (code/mat_vec.f90:1505-1928 without the h.c.)."""
+    sb = H.sb
+    out = np.zeros_like(psi)
+    pin = dmrg.psi_blocks(sb, psi)
+    pout = dmrg.psi_blocks(sb, out)
+    ssec = H.s.basis.sectors
+    for (xp, x), tm in T.items():
+        if x not in H.sidx or xp not in H.sidx:
+            continue
+        i, ip = H.sidx[x], H.sidx[xp]
+        y, yp = sb.pairs[i][1], sb.pairs[ip][1]
+        if (yp, y) not in U:
+            continue
+        J2, J2p = ssec[x][1], ssec[xp][1]
+        c = amp / math.sqrt((k2 + 1) * (J2 + 1) * (J2p + 1))
+        pout[ip] += c * (tm @ pin[i] @ U[(yp, y)].T)
+    return out
+
+
+def spin_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H):
+    """<S_i . S_j> = -sqrt(3) <[S_i (x) S_j]^0> for i on the system side and j
+    on the environment side (nsite x nsite, symmetric, NaN elsewhere): the
+    operators are embedded in the enlarged bases of the superblock and
+    contracted as the superblock's exchange bond."""
+    nsite = model.nsite
+    out = np.full((nsite, nsite), np.nan)
+    S = dmrg.site_ops(model.kind)["S"]
+    s_ops = tracked_spin_ops(model, sys_blocks, sys_truns, sys_len, "sys")
+    e_ops = tracked_spin_ops(model, env_blocks, env_truns, env_len, "env")
+    sbl, ebl = sys_blocks[sys_len], env_blocks[env_len]
+    seb, eeb = H.s.basis, H.e.basis
+    T = {k: dmrg.embed_block_op(o, sbl, seb, "sys") for k, o in s_ops.items()}
+    T[sys_len + 1] = dmrg.embed_site_op(S, sbl, seb, "sys")
+    U = {k: dmrg.embed_block_op(o, ebl, eeb, "env") for k, o in e_ops.items()}
+    U[env_len + 1] = dmrg.embed_site_op(S, ebl, eeb, "env")
+    for ki, t in T.items():
+        for kj, u in U.items():
+            v = float(np.real(np.vdot(psi, _cross_apply(H, t, u, 2, -math.sqrt(3.0), psi))))
+            pj = nsite + 1 - kj if kj <= env_len else nsite - env_len
+            out[ki - 1, pj - 1] = out[pj - 1, ki - 1] = v
     return out

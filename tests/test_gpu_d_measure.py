@@ -2,7 +2,8 @@
 operator chain of Get_sys_oper_dia / Get_env_oper_dia) and of the density
 correlations <n_i n_j> (Get_density_cor: the pairs across the superblock
 centre, and whole rows with the two-site products carried in the system
-block): untruncated runs against exact diagonalisation, truncated
+block), and of the spin correlations <S_i . S_j> across the centre:
+untruncated runs against exact diagonalisation, truncated
 runs against the oracle's chain, and the sum rule sum_i <n_i> = N."""
 import numpy as np
 import pytest
@@ -44,7 +45,7 @@ def _gpu_profile(ctx, m, kept, tol):
     psi = dev(dmrg.start_vector(sb.dim, 5))
     E, it = ctx.lanczos(sb, psi, 1e-13, 400)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 2, L + 1)}
-    return E, drv.density(L, psi), drv.density_cross(L, psi), rows
+    return E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi)
 
 
 @pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
@@ -53,7 +54,8 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     sg = dmrg.StepGeometry(m, m.nsite // 2 - 1, m.nsite - m.nsite // 2 - 1)
     bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
     e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
-    E, prof, cc, rows = _gpu_profile(ctx, m, 2000, 1e-12)
+    _, ss = ed.singlet_ground_spin_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    E, prof, cc, rows, sc = _gpu_profile(ctx, m, 2000, 1e-12)
     assert abs(E - e0) < 1e-10 * abs(e0)
     assert abs(prof.sum() - m.n_elec) < 1e-10
     assert np.abs(prof - dens).max() < 1e-6, (prof, dens)
@@ -62,6 +64,8 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     assert np.abs(cc[x] - nn[x]).max() < 1e-6, (cc[x], nn[x])
     assert np.array_equal(cc[x], cc.T[x])
     assert np.isnan(cc[np.ix_(left, left)]).all() and np.isnan(cc[np.ix_(right, right)]).all()
+    assert np.abs(sc[x] - ss[x]).max() < 1e-6, (sc[x], ss[x])
+    assert np.array_equal(sc[x], sc.T[x]) and np.isnan(sc[np.ix_(left, left)]).all()
     for r, row in rows.items():   # block-block (carried products), block-centre, env side
         others = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
@@ -90,6 +94,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     ctx.lanczos(sb, psi, 1e-13, 400)
     prof = drv.density(L, psi)
     cc = drv.density_cross(L, psi)
+    sc = drv.spin_cross(L, psi)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 3, L + 1)}
     assert abs(prof.sum() - m.n_elec) < 1e-9
     sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
@@ -105,6 +110,10 @@ def test_density_truncated_matches_oracle_chain(ctx):
     fin = ~np.isnan(cref)
     assert np.array_equal(fin, ~np.isnan(cc))
     assert np.abs(cc[fin] - cref[fin]).max() < 1e-10, np.abs(cc[fin] - cref[fin]).max()
+    sref = measure.spin_cross(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H)
+    fin = ~np.isnan(sref)
+    assert np.array_equal(fin, ~np.isnan(sc))
+    assert np.abs(sc[fin] - sref[fin]).max() < 1e-10, np.abs(sc[fin] - sref[fin]).max()
     for r, row in rows.items():
         rref = measure.density_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
         assert np.array_equal(np.isnan(row), np.isnan(rref))

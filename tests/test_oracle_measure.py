@@ -132,3 +132,47 @@ def test_density_row_with_a_wrong_product_fails():
     finally:
         measure._product = saved
     assert np.abs(good[1:L] - bad[1:L]).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
+def test_untruncated_spin_cross_equals_ed(kind, flux):
+    """<S_i . S_j> across the centre from the carried rank-1 spins, embedded in
+    the enlarged bases and contracted as the exchange bond, equals the exact
+    spin correlation (S^z S^z + (S^+ S^- + S^- S^+)/2 on Fock states)."""
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, ss = ed.singlet_ground_spin_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    cc = measure.spin_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    x = np.ix_(np.arange(L + 1), np.arange(L + 1, m.nsite))
+    assert np.abs(cc[x] - ss[x]).max() < 1e-6
+    # total S = 0: sum_j S_i . S_j = 0, so the cross row sums to minus the same-side sum
+    assert np.abs(ss.sum(axis=1)).max() < 1e-10
+
+
+def test_spin_chain_with_a_wrong_rank_fails():
+    """A planted mistake (the carried spins recoupled as rank 0) moves every
+    correlation of a site carried through a truncation."""
+    m = lat.Model(kind="tj", Lx=4, Ly=2, J=0.4, t=1.0, flux=0.2, n_elec=6)
+    L, R = 3, 3
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    good = measure.spin_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    saved = measure._transform_any
+    try:
+        measure._transform_any = lambda op, trun, k2: saved(op, trun, 0)
+        bad = measure.spin_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    finally:
+        measure._transform_any = saved
+    assert np.abs(good[0, L + 1:] - bad[0, L + 1:]).max() > 1e-3
