@@ -692,9 +692,9 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
   };
   if (only) {
     auto piece = [&](const MeasurePiece& q, int side) {
-      if (!q.block) return site_piece(side, q.kind, false);
+      if (!q.block) return site_piece(side, q.kind, q.tilde);
       Piece p;
-      p.block = true; p.side = side; p.kind = q.kind; p.tilde = false; p.buf = q.buf; p.base = q.base;
+      p.block = true; p.side = side; p.kind = q.kind; p.tilde = q.tilde; p.buf = q.buf; p.base = q.base;
       p.lay = q.lay; p.k2 = op_k2(q.kind); p.fermion = op_fermion(q.kind);
       return p;
     };

@@ -198,6 +198,7 @@ struct MatvecPlan {
 // site's operator of that kind.
 struct MeasurePiece {
   bool block = false;
+  bool tilde = false;   // env c^dag layout read as the annihilator partner (as in the hopping bond)
   int kind = OP_S;
   uint8_t buf = B_SYS;
   int64_t base = 0;
@@ -206,7 +207,7 @@ struct MeasurePiece {
 // [A_sys (x) B_env]^0 scaled by amp, the coupling form of the H.psi cross terms
 struct MeasureTerm {
   MeasurePiece s, e;
-  double amp = 1.0;
+  cplx amp = 1.0;
 };
 // With only != nullptr the plan holds that one term instead of the model's H
 // (block Hamiltonians, U and every bond left out); sys.sec / env.sec give the
@@ -275,6 +276,11 @@ TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& pr
 // make_layout(new sectors, 2, 0) to B_OUT, new_size apart.
 TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold);
+// The same chain for an operator kind of any rank (OP_CDAG: c^dag, rank 1/2,
+// N + 1, fermionic, with the fermion signs of update_block_ndia_* /
+// update_site_ndia_* for `side`), layout make_layout(sectors, k2, dN).
+TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold, int kind, int side);
 struct TrackSrc {
   enum What { CARRY, SITE, PRODUCT } what;
   int k;

@@ -382,17 +382,18 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
   return P;
 }
 
-TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& prev,
-                             const std::vector<KeptSector>& kept, int nold) {
+TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold, int kind, int side) {
   TrackPlan P;
   const auto st = site_sectors(m.kind);
   const int nst = (int)st.size();
-  const int k2 = 2;
+  const int k2 = op_k2(kind), dN = op_dN(kind);
+  const bool ferm = op_fermion(kind);
   P.eb = enlarge_basis(prev, st);
   const EnlBasis& eb = P.eb;
   std::vector<Sector> nsec;
   for (auto& k : kept) nsec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
-  const OpLayout ol = make_layout(prev, k2, 0), nl = make_layout(nsec, k2, 0);
+  const OpLayout ol = make_layout(prev, k2, dN), nl = make_layout(nsec, k2, dN);
   P.old_size = ol.size;
   P.new_size = nl.size;
   Collector s1, s2;
@@ -407,7 +408,8 @@ TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& pr
       auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, dap, d, kp}; };
       for (auto& ss : eb.subs[ap]) s1.ensure(rows(ss.pos, ss.dim));
       if (q < nold) {
-        // carried block operator: e1 recoupling on every (b', s) <- (b, s)
+        // carried block operator: e1 recoupling on every (b', s) <- (b, s);
+        // an env block follows its site in the fermion order
         for (auto& osb : ol.subs)
           for (auto& ss : eb.subs[a]) {
             if (ss.b != osb.ket) continue;
@@ -415,23 +417,26 @@ TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& pr
             for (auto& t : eb.subs[ap])
               if (t.b == osb.bra && t.s == ss.s) { pos_out = t.pos; d_out = t.dim; }
             if (pos_out < 0) continue;
-            const double c = e1(eb.bl[osb.bra].S2, eb.bl[osb.ket].S2, st[ss.s].S2, J2p, J2, k2);
+            double c = e1(eb.bl[osb.bra].S2, eb.bl[osb.ket].S2, st[ss.s].S2, J2p, J2, k2);
             if (c == 0.0) continue;
+            if (side == 1 && ferm) c *= nsign(st[ss.s].N);
             s1.add(rows(pos_out, d_out), mk(c, B_SYS, (int64_t)q * P.old_size + osb.off, osb.rows, OP_NN, B_AUX,
                                             kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
           }
       } else {
-        // spin of the new site: e2 recoupling on every (b, s') <- (b, s)
+        // operator of the new site: e2 recoupling on every (b, s') <- (b, s);
+        // a sys site follows its block in the fermion order
         for (auto& ss : eb.subs[a])
           for (int sp = 0; sp < nst; ++sp) {
-            const cplx v = site_val(m.kind, OP_S, sp, ss.s);
+            const cplx v = site_val(m.kind, kind, sp, ss.s);
             if (v == 0.0) continue;
             int pos_out = -1, d_out = 0;
             for (auto& t : eb.subs[ap])
               if (t.b == ss.b && t.s == sp) { pos_out = t.pos; d_out = t.dim; }
             if (pos_out < 0) continue;
-            const double c = e2(st[sp].S2, st[ss.s].S2, eb.bl[ss.b].S2, J2p, J2, k2);
+            double c = e2(st[sp].S2, st[ss.s].S2, eb.bl[ss.b].S2, J2p, J2, k2);
             if (c == 0.0) continue;
+            if (side == 0 && ferm) c *= nsign(eb.bl[ss.b].N);
             s1.add(rows(pos_out, d_out), mk(c * v, B_AUX, kk.u_off + ss.pos, kk.u_ld, OP_NN, 0, 0, 0, OP_NN, 0));
           }
       }
@@ -443,6 +448,11 @@ TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& pr
   s1.emit(P.stage1);
   s2.emit(P.stage2);
   return P;
+}
+
+TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, int nold) {
+  return compile_track_rank(m, prev, kept, nold, OP_S, 0);
 }
 
 TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,

@@ -778,6 +778,23 @@ void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStre
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, a, n);
 }
+// d = (ar + i ai) src
+__global__ __launch_bounds__(VB) void k_cscale_copy(double2* __restrict__ d, const double2* __restrict__ src,
+                                                    double ar, double ai, int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      const double2 y = src[i];
+      d[i] = make_double2(ar * y.x - ai * y.y, ar * y.y + ai * y.x);
+    }
+  }
+}
+void dev_cscale_copy(double2* d, const double2* src, double ar, double ai, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cscale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, ar, ai, n);
+}
 void dev_start_vector(double2* p, int64_t n, uint64_t seed, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_start, dim3(grid_for(n)), dim3(VB), 0, s, p, n, (seed & 0xFFFFFFFFull) << 32);

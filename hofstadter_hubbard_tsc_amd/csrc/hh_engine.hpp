@@ -76,6 +76,8 @@ void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, i
                       int64_t n, double* out_dev, double* partial_dev, hipStream_t s);
 // d = a * src
 void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s);
+// d = (ar + i ai) src
+void dev_cscale_copy(double2* d, const double2* src, double ar, double ai, int64_t n, hipStream_t s);
 
 void check_hip(hipError_t e, const char* what);
 // counter-based start vector generated on the device (same stream as the host)

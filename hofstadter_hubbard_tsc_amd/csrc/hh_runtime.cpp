@@ -1074,27 +1074,29 @@ void cross_density(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, doub
   }
 }
 
-// Spin operators S_k (rank 1) of every site of one side's block, carried to
-// its current length through the stored truncations: operator k-1 is site k,
+// Operators of one kind (S: rank 1; c^dag: rank 1/2, fermionic) of every
+// site of one side's block, carried to its current length through the stored
+// truncations (Get_sys_oper_ndia / update_trun_ndia): operator k-1 is site k,
 // layout `lay` over the sectors `sec`, lay.size elements apart.
-struct TrackedSpin {
+struct TrackedRank {
   DevMem ops;
   std::vector<Sector> sec;
   OpLayout lay;
   int n = 0;
 };
 
-void track_spin_chain(hh_ctx* ctx, Driver& d, int side, int len, TrackedSpin& T) {
+void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, TrackedRank& T) {
   const ModelParams& m = ctx->model;
   const auto& blocks = side == 0 ? d.sysb : d.envb;
   const auto& truns = side == 0 ? d.syst : d.envt;
+  const int k2 = op_k2(kind), dN = op_dN(kind);
   T.sec = blocks.at(1)->d.sec;
-  T.lay = make_layout(T.sec, 2, 0);
+  T.lay = make_layout(T.sec, k2, dN);
   T.n = 1;
   {
     std::vector<double2> h((size_t)std::max<int64_t>(T.lay.size, 1), make_double2(0.0, 0.0));
     for (auto& sub : T.lay.subs) {   // site multiplets have dimension 1
-      const cplx v = site_val(m.kind, OP_S, sub.bra, sub.ket);
+      const cplx v = site_val(m.kind, kind, sub.bra, sub.ket);
       h[sub.off] = make_double2(v.real(), v.imag());
     }
     T.ops.alloc(sizeof(double2) * h.size());
@@ -1105,7 +1107,7 @@ void track_spin_chain(hh_ctx* ctx, Driver& d, int side, int len, TrackedSpin& T)
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
     const hh_trunc* t = truns.at(l).get();
-    TrackPlan P = compile_track_spin(m, T.sec, t->kept, T.n);
+    TrackPlan P = compile_track_rank(m, T.sec, t->kept, T.n, kind, side);
     nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (T.n + 1), 1));
     X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
     BufPtrs b{};
@@ -1118,31 +1120,33 @@ void track_spin_chain(hh_ctx* ctx, Driver& d, int side, int len, TrackedSpin& T)
     s2.upload(P.stage2, ctx->stream);
     launch_tiles(s1, b, ctx->stream, PREC_FP64);
     launch_tiles(s2, b, ctx->stream, PREC_FP64);
-    check_hip(hipStreamSynchronize(ctx->stream), "spin chain");
+    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
     std::swap(T.ops.p, nxt.p);
     std::swap(T.ops.bytes, nxt.bytes);
     T.sec.clear();
     for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
-    T.lay = make_layout(T.sec, 2, 0);
+    T.lay = make_layout(T.sec, k2, dN);
     T.n += 1;
   }
 }
 
-// <S_i . S_j> for i on the sys side (block sites and the sys centre site) and
-// j on the env side: the pairs of Get_density_cor with oper1 = oper2 = the
-// spin (code/correlation.f90:43-195; sys_block_env_block_cor_ndia and the
-// block/site variants, :1666-2656).  Each pair is one two-point term
-// -sqrt(3) [S_i (x) S_j]^0 compiled like the exchange bond of H.psi and
-// applied to psi; <S_i . S_j> = Re <psi, term psi>.  out as cross_density.
-void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+// <psi| term |psi> for every sys-side i and env-side j (centre sites
+// included): the two-point term amp [A_i (x) B_j]^0 of operator kind `kind`
+// (env operator read as the annihilator partner when `tilde`), compiled by
+// compile_matvec's single-term mode and applied to psi.  val(i, j) receives
+// the complex expectation; i = sys_len + 1 and j = env_len + 1 are the centre
+// sites.
+template <typename Sink>
+void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int kind, bool tilde, cplx amp,
+                 Sink val) {
   const ModelParams& m = ctx->model;
   const int nsite = m.nsite();
   const int env_len = nsite - sys_len - 2;
   need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
   need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
-  TrackedSpin TS, TE;
-  track_spin_chain(ctx, d, 0, sys_len, TS);
-  track_spin_chain(ctx, d, 1, env_len, TE);
+  TrackedRank TS, TE;
+  track_rank_chain(ctx, d, 0, sys_len, kind, TS);
+  track_rank_chain(ctx, d, 1, env_len, kind, TE);
   BlockDesc sd, ed;
   sd.side = 0;
   sd.length = sys_len;
@@ -1151,13 +1155,13 @@ void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double*
   ed.length = env_len;
   ed.sec = TE.sec;
   const Geometry g = make_geometry(m, sys_len, env_len);
-  for (int i = 0; i < nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
-  DevMem Y, X;
+  DevMem Y, X, ipsi;
   for (int i = 1; i <= sys_len + 1; ++i)
     for (int j = 1; j <= env_len + 1; ++j) {
       MeasureTerm t;
-      t.amp = -std::sqrt(3.0);
-      t.s.kind = t.e.kind = OP_S;
+      t.amp = amp;
+      t.s.kind = t.e.kind = kind;
+      t.e.tilde = tilde;
       if (i <= sys_len) {
         t.s.block = true;
         t.s.buf = B_SYS;
@@ -1171,8 +1175,13 @@ void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double*
         t.e.lay = &TE.lay;
       }
       MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
-      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.sb.dim, 1));
+      const int64_t n = P.sb.dim;
+      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
       X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      if (!ipsi.p) {   // i psi: Re <i psi, y> = Im <psi, y>
+        ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
+        dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, n, ctx->stream);
+      }
       BufPtrs b{};
       b.p[B_IN] = (double2*)psi;
       b.p[B_OUT] = Y.c();
@@ -1184,13 +1193,46 @@ void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double*
       s2.upload(P.stage2, ctx->stream);
       launch_tiles(s1, b, ctx->stream, PREC_FP64);
       launch_tiles(s2, b, ctx->stream, PREC_FP64);
-      double v = 0.0;
-      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &v);
-      const int pi = i;   // sys site i (or the sys centre, i = sys_len + 1) sits at position i
-      const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;
-      out[(int64_t)(pi - 1) * nsite + (pj - 1)] = v;
-      out[(int64_t)(pj - 1) * nsite + (pi - 1)] = v;
+      double re = 0.0, im = 0.0;
+      mdots_to_host(ctx, psi, 0, 1, Y.c(), n, &re);
+      mdots_to_host(ctx, ipsi.c(), 0, 1, Y.c(), n, &im);
+      val(i, j, cplx(re, im));
     }
+}
+
+// <S_i . S_j> for i on the sys side (block sites and the sys centre site) and
+// j on the env side: the pairs of Get_density_cor with oper1 = oper2 = the
+// spin (code/correlation.f90:43-195; sys_block_env_block_cor_ndia and the
+// block/site variants, :1666-2656).  Each pair is the exchange-bond term
+// -sqrt(3) [S_i (x) S_j]^0.  out as cross_density.
+void spin_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+  const int nsite = ctx->model.nsite();
+  const int env_len = nsite - sys_len - 2;
+  for (int i = 0; i < nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  cross_terms(ctx, d, sys_len, psi, OP_S, false, -std::sqrt(3.0), [&](int i, int j, cplx v) {
+    const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;   // sys site i sits at position i
+    out[(int64_t)(i - 1) * nsite + (pj - 1)] = v.real();
+    out[(int64_t)(pj - 1) * nsite + (i - 1)] = v.real();
+  });
+}
+
+// sum_sigma <c^dag_i,sigma c_j,sigma> for i on the sys side and j on the env
+// side (Get_density_cor with st_elec_up / st_elec_down, flag 'F'): the hopping
+// bond's term sqrt(2) [c^dag_i (x) c~_j]^0 without its h.c.  out is nsite x
+// nsite complex (re, im interleaved, row-major by lattice position - 1); the
+// mirrored entry holds the conjugate, <c^dag_j c_i>; pairs on one side NaN.
+void hopping_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+  const int nsite = ctx->model.nsite();
+  const int env_len = nsite - sys_len - 2;
+  for (int i = 0; i < 2 * nsite * nsite; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  cross_terms(ctx, d, sys_len, psi, OP_CDAG, true, std::sqrt(2.0), [&](int i, int j, cplx v) {
+    const int pj = j <= env_len ? nsite + 1 - j : nsite - env_len;
+    const int64_t a = (int64_t)(i - 1) * nsite + (pj - 1), b = (int64_t)(pj - 1) * nsite + (i - 1);
+    out[2 * a] = v.real();
+    out[2 * a + 1] = v.imag();
+    out[2 * b] = v.real();
+    out[2 * b + 1] = -v.imag();
+  });
 }
 
 // One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
@@ -1849,6 +1891,14 @@ int hh_driver_spin_cross(hh_driver* h, int sys_len, const double* psi, double* s
   return guard(h->d.ctx, [&] {
     need(psi && ss_out, "null argument");
     spin_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, ss_out);
+  });
+}
+
+int hh_driver_hopping_cross(hh_driver* h, int sys_len, const double* psi, double* g_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && g_out, "null argument");
+    hopping_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, g_out);
   });
 }
 

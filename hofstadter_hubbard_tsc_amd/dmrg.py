@@ -483,6 +483,17 @@ class Driver:
                      "hh_driver_spin_cross")
         return out
 
+    def hopping_cross(self, sys_len, psi):
+        """sum_sigma <c^dag_i c_j> (nsite x nsite complex, NaN for pairs on one side)
+        across the centre."""
+        _sync()
+        n = self.ctx.model.Lx * self.ctx.model.Ly
+        out = np.zeros((n, n, 2))
+        self.ctx._rc(L.load().hh_driver_hopping_cross(self.ptr, sys_len, _dptr(psi),
+                                                      out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_hopping_cross")
+        return out[..., 0] + 1j * out[..., 1]
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

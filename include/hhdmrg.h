@@ -248,6 +248,14 @@ int hh_driver_density_row(hh_driver* d, int sys_len, int site, const double* psi
  * update_trun_ndia).  Returns the physical <S_i . S_j>.  ss_out as in
  * hh_driver_density_cross. */
 int hh_driver_spin_cross(hh_driver* d, int sys_len, const double* psi_dev, double* ss_out);
+/* single-particle correlations sum_sigma <c^dag_i,sigma c_j,sigma> across the
+ * superblock's centre: Get_density_cor (code/correlation.f90:43-195) with
+ * st_elec_up / st_elec_down and flag 'F', its pairs across the centre, with
+ * c^dag of every block site carried through the stored truncations (fermion
+ * signs of update_block_ndia_* / update_site_ndia_*).  g_out is Nsite x Nsite
+ * complex (re, im interleaved), row-major by lattice position - 1; the entry
+ * (j, i) holds the conjugate; pairs on one side are NaN. */
+int hh_driver_hopping_cross(hh_driver* d, int sys_len, const double* psi_dev, double* g_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

@@ -223,3 +223,23 @@ def singlet_ground_spin_correlations(model, positions, bonds, n_elec):
             O = sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim), dtype=np.complex128)
             out[pa - 1, pb - 1] = float(np.real(np.vdot(psi, O @ psi)))
     return e, out
+
+
+def singlet_ground_hopping(model, positions, bonds, n_elec):
+    """(E0, G) with G[i-1, j-1] = sum_sigma <c^dag_i,sigma c_j,sigma> in the
+    lowest total-S = 0 state (i != j; lattice positions)."""
+    e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
+    n = max(positions)
+    out = np.zeros((n, n), complex)
+    order = {p: i for i, p in enumerate(positions)}
+    for pa in positions:
+        for pb in positions:
+            if pa == pb:
+                continue
+            i, j = order[pa], order[pb]
+            v = 0.0
+            for sgm in (0, 1):
+                src, dst, sg = fs.apply([(2 * i + sgm, True), (2 * j + sgm, False)])
+                v += np.sum(np.conj(psi[dst]) * sg * psi[src])
+            out[pa - 1, pb - 1] = v
+    return e, out

@@ -22,14 +22,18 @@ a rank-0 reduced block operator on block sector b.
 ``spin_cross`` gives <S_i . S_j> for the same pairs across the centre: the
 spin of every block site carried as a rank-1 operator (U_bra^dag O U_ket),
 embedded in the superblock's enlarged bases and contracted as the exchange
-bond, -sqrt(3) [S_i (x) S_j]^0.
+bond, -sqrt(3) [S_i (x) S_j]^0.  ``hopping_cross`` gives sum_sigma
+<c^dag_i c_j> the same way, with c^dag carried as a rank-1/2 fermionic
+operator and the environment one read as its annihilator partner (the
+hopping bond without its h.c.).
 
 ``density_row`` gives one row <n_r n_j> of the same correlation for a row
 site r on the system side and every j, with the two-site products n_r n_j of
 block sites carried through the truncations (``tracked_pairs``).
 
 Pinned by exact diagonalisation (``ed.singlet_ground_density``,
-``ed.singlet_ground_correlations``, ``ed.singlet_ground_spin_correlations``):
+``ed.singlet_ground_correlations``, ``ed.singlet_ground_spin_correlations``,
+``ed.singlet_ground_hopping``):
 with no truncation the profile and the correlations equal the exact ones; the densities sum to N and, for each i,
 sum_j <n_i n_j> over the other side equals <n_i N_other>; a full row plus
 <n_r^2> sums to N <n_r>.
@@ -250,11 +254,11 @@ def density_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, en
     return out
 
 
-def _transform_any(op_enl, trun, k2):
+def _transform_any(op_enl, trun, k2, dN=0, fermion=False):
     """U_bra^dag O U_ket for an enlarged operator of rank k2/2; trun =
     [(enlarged sector, U)] in new-block order."""
     pos = {a: i for i, (a, _) in enumerate(trun)}
-    out = dmrg.Op(k2, 0, False)
+    out = dmrg.Op(k2, dN, fermion)
     for (ap, a), m in op_enl.items():
         if ap in pos and a in pos:
             Ub, Uk = trun[pos[ap]][1], trun[pos[a]][1]
@@ -262,25 +266,33 @@ def _transform_any(op_enl, trun, k2):
     return out
 
 
-def tracked_spin_ops(model, blocks, truns, length, side):
-    """{site k: S_k in the basis of blocks[length]} (Get_sys_oper_ndia /
-    update_trun_ndia for the rank-1 spin), carried like ``tracked_ops``."""
+def tracked_rank_ops(model, blocks, truns, length, side, name):
+    """{site k: O_k in the basis of blocks[length]} for the site operator
+    ``name`` ("S": rank 1; "cdag": rank 1/2, fermionic), carried through every
+    truncation (Get_sys_oper_ndia / update_trun_ndia)."""
     st = dmrg.site_sectors(model.kind)
-    S = dmrg.site_ops(model.kind)["S"]
-    ops = {1: S}
+    O = dmrg.site_ops(model.kind)[name]
+    ops = {1: O}
     for l in range(2, length + 1):
         prev = blocks[l - 1]
         eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
-        new = {k: _transform_any(dmrg.embed_block_op(o, prev, eb, side), truns[l], 2) for k, o in ops.items()}
-        new[l] = _transform_any(dmrg.embed_site_op(S, prev, eb, side), truns[l], 2)
+        tr = lambda e: _transform_any(e, truns[l], O.k2, O.dN, O.fermion)
+        new = {k: tr(dmrg.embed_block_op(o, prev, eb, side)) for k, o in ops.items()}
+        new[l] = tr(dmrg.embed_site_op(O, prev, eb, side))
         ops = new
     return ops
 
 
-def _cross_apply(H, T, U, k2, amp, psi):
+def tracked_spin_ops(model, blocks, truns, length, side):
+    """{site k: S_k in the basis of blocks[length]}."""
+    return tracked_rank_ops(model, blocks, truns, length, side, "S")
+
+
+def _cross_apply(H, T, U, k2, amp, psi, fermion=False):
     """[T (x) U]^0 psi for enlarged-basis operators T (system) and U
     (environment), the cross-term contraction of the superblock
-    (code/mat_vec.f90:1505-1928 without the h.c.)."""
+    (code/mat_vec.f90:1505-1928 without the h.c.); a fermionic U passes the
+    system part, (-1)^{N_sys}."""
     sb = H.sb
     out = np.zeros_like(psi)
     pin = dmrg.psi_blocks(sb, psi)
@@ -295,6 +307,8 @@ def _cross_apply(H, T, U, k2, amp, psi):
             continue
         J2, J2p = ssec[x][1], ssec[xp][1]
         c = amp / math.sqrt((k2 + 1) * (J2 + 1) * (J2p + 1))
+        if fermion:
+            c *= (-1.0) ** ssec[x][0]
         pout[ip] += c * (tm @ pin[i] @ U[(yp, y)].T)
     return out
 
@@ -320,4 +334,31 @@ def spin_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env
             v = float(np.real(np.vdot(psi, _cross_apply(H, t, u, 2, -math.sqrt(3.0), psi))))
             pj = nsite + 1 - kj if kj <= env_len else nsite - env_len
             out[ki - 1, pj - 1] = out[pj - 1, ki - 1] = v
+    return out
+
+
+def hopping_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H):
+    """sum_sigma <c^dag_i,sigma c_j,sigma> = sqrt(2) <[c^dag_i (x) c~_j]^0> for i
+    on the system side and j on the environment side (complex nsite x nsite;
+    entry (j, i) the conjugate; NaN elsewhere): c^dag carried as a rank-1/2
+    fermionic operator, the environment one read as its annihilator partner
+    c~ (the superblock's hopping bond without its h.c.)."""
+    nsite = model.nsite
+    out = np.full((nsite, nsite), np.nan, dtype=complex)
+    C = dmrg.site_ops(model.kind)["cdag"]
+    s_ops = tracked_rank_ops(model, sys_blocks, sys_truns, sys_len, "sys", "cdag")
+    e_ops = tracked_rank_ops(model, env_blocks, env_truns, env_len, "env", "cdag")
+    sbl, ebl = sys_blocks[sys_len], env_blocks[env_len]
+    seb, eeb = H.s.basis, H.e.basis
+    T = {k: dmrg.embed_block_op(o, sbl, seb, "sys") for k, o in s_ops.items()}
+    T[sys_len + 1] = dmrg.embed_site_op(C, sbl, seb, "sys")
+    U = {k: dmrg.embed_block_op(o, ebl, eeb, "env") for k, o in e_ops.items()}
+    U[env_len + 1] = dmrg.embed_site_op(C, ebl, eeb, "env")
+    U = {k: dmrg._tilde_dict(u, eeb.sectors) for k, u in U.items()}
+    for ki, t in T.items():
+        for kj, u in U.items():
+            v = complex(np.vdot(psi, _cross_apply(H, t, u, 1, math.sqrt(2.0), psi, fermion=True)))
+            pj = nsite + 1 - kj if kj <= env_len else nsite - env_len
+            out[ki - 1, pj - 1] = v
+            out[pj - 1, ki - 1] = np.conj(v)
     return out

@@ -2,7 +2,8 @@
 operator chain of Get_sys_oper_dia / Get_env_oper_dia) and of the density
 correlations <n_i n_j> (Get_density_cor: the pairs across the superblock
 centre, and whole rows with the two-site products carried in the system
-block), and of the spin correlations <S_i . S_j> across the centre:
+block), and of the spin correlations <S_i . S_j> and one-body correlations
+sum_sigma <c^dag_i c_j> across the centre:
 untruncated runs against exact diagonalisation, truncated
 runs against the oracle's chain, and the sum rule sum_i <n_i> = N."""
 import numpy as np
@@ -45,7 +46,7 @@ def _gpu_profile(ctx, m, kept, tol):
     psi = dev(dmrg.start_vector(sb.dim, 5))
     E, it = ctx.lanczos(sb, psi, 1e-13, 400)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 2, L + 1)}
-    return E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi)
+    return E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi), drv.hopping_cross(L, psi)
 
 
 @pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
@@ -55,7 +56,8 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
     e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
     _, ss = ed.singlet_ground_spin_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
-    E, prof, cc, rows, sc = _gpu_profile(ctx, m, 2000, 1e-12)
+    _, G = ed.singlet_ground_hopping(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    E, prof, cc, rows, sc, hc = _gpu_profile(ctx, m, 2000, 1e-12)
     assert abs(E - e0) < 1e-10 * abs(e0)
     assert abs(prof.sum() - m.n_elec) < 1e-10
     assert np.abs(prof - dens).max() < 1e-6, (prof, dens)
@@ -66,6 +68,8 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     assert np.isnan(cc[np.ix_(left, left)]).all() and np.isnan(cc[np.ix_(right, right)]).all()
     assert np.abs(sc[x] - ss[x]).max() < 1e-6, (sc[x], ss[x])
     assert np.array_equal(sc[x], sc.T[x]) and np.isnan(sc[np.ix_(left, left)]).all()
+    assert np.abs(hc[x] - G[x]).max() < 1e-6, (hc[x], G[x])
+    assert np.array_equal(hc.T[x], np.conj(hc[x]))
     for r, row in rows.items():   # block-block (carried products), block-centre, env side
         others = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
@@ -95,6 +99,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     prof = drv.density(L, psi)
     cc = drv.density_cross(L, psi)
     sc = drv.spin_cross(L, psi)
+    hc = drv.hopping_cross(L, psi)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 3, L + 1)}
     assert abs(prof.sum() - m.n_elec) < 1e-9
     sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
@@ -114,6 +119,10 @@ def test_density_truncated_matches_oracle_chain(ctx):
     fin = ~np.isnan(sref)
     assert np.array_equal(fin, ~np.isnan(sc))
     assert np.abs(sc[fin] - sref[fin]).max() < 1e-10, np.abs(sc[fin] - sref[fin]).max()
+    href = measure.hopping_cross(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H)
+    fin = ~np.isnan(href)
+    assert np.array_equal(fin, ~np.isnan(hc))
+    assert np.abs(hc[fin] - href[fin]).max() < 1e-10, np.abs(hc[fin] - href[fin]).max()
     for r, row in rows.items():
         rref = measure.density_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
         assert np.array_equal(np.isnan(row), np.isnan(rref))

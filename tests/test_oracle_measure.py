@@ -171,8 +171,52 @@ def test_spin_chain_with_a_wrong_rank_fails():
     good = measure.spin_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
     saved = measure._transform_any
     try:
-        measure._transform_any = lambda op, trun, k2: saved(op, trun, 0)
+        measure._transform_any = lambda op, trun, k2, dN=0, fermion=False: saved(op, trun, 0, dN, fermion)
         bad = measure.spin_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
     finally:
         measure._transform_any = saved
     assert np.abs(good[0, L + 1:] - bad[0, L + 1:]).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
+def test_untruncated_hopping_cross_equals_ed(kind, flux):
+    """sum_sigma <c^dag_i c_j> across the centre (complex with flux) from the
+    carried fermionic c^dag equals the exact one-body correlation."""
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, G = ed.singlet_ground_hopping(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    cc = measure.hopping_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    x = np.ix_(np.arange(L + 1), np.arange(L + 1, m.nsite))
+    assert np.abs(G[x].imag).max() > 1e-3            # the flux makes them complex
+    assert np.abs(cc[x] - G[x]).max() < 1e-6
+    assert np.abs(cc.T[x] - np.conj(G[x])).max() < 1e-6
+
+
+def test_hopping_chain_without_fermion_signs_fails():
+    """A planted mistake (the carried c^dag treated as a boson, so the string
+    signs of the embedding are lost) changes the correlations of carried sites."""
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    L, R = 3, 3
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    good = measure.hopping_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    saved = measure._transform_any
+    try:
+        measure._transform_any = lambda op, trun, k2, dN=0, fermion=False: saved(op, trun, k2, dN, False)
+        bad = measure.hopping_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H)
+    finally:
+        measure._transform_any = saved
+    x = np.ix_(np.arange(L + 1), np.arange(L + 1, m.nsite))
+    assert np.abs(good[x] - bad[x]).max() > 1e-3
