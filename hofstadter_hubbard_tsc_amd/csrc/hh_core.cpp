@@ -427,7 +427,9 @@ std::vector<Trans> transitions(const Piece& p, int from, const std::vector<Secto
       for (int i = 0; i < (int)p.lay->subs.size(); ++i)
         if (p.lay->subs[i].bra == from) {
           int ep = p.lay->subs[i].ket;
-          double s = -phase2(1 + sec[from].S2 - sec[ep].S2);
+          // fermionic c^dag: the annihilator partner's sign; a bosonic
+          // rank-0 operator: its plain adjoint
+          double s = p.fermion ? -phase2(1 + sec[from].S2 - sec[ep].S2) : 1.0;
           out.push_back({ep, i, 0.0, s});
         }
     }

@@ -106,9 +106,11 @@ struct OpLayout {
 OpLayout make_layout(const std::vector<Sector>& sec, int k2, int dN);
 
 // Operator kinds stored per boundary site: 0 = c^dag, 1 = S, 2 = n.
-enum OpKind { OP_CDAG = 0, OP_S = 1, OP_N = 2 };
+// OP_PAIR (never stored in a block) is the singlet pair creator [c^dag x
+// c^dag]^0 of two block sites: rank 0, N + 2, bosonic.
+enum OpKind { OP_CDAG = 0, OP_S = 1, OP_N = 2, OP_PAIR = 3 };
 inline int op_k2(int k) { return k == OP_CDAG ? 1 : (k == OP_S ? 2 : 0); }
-inline int op_dN(int k) { return k == OP_CDAG ? 1 : 0; }
+inline int op_dN(int k) { return k == OP_CDAG ? 1 : (k == OP_PAIR ? 2 : 0); }
 inline bool op_fermion(int k) { return k == OP_CDAG; }
 int n_op_kinds(int model_kind);    // Hubbard: c^dag; t-J: c^dag, S, n
 
@@ -281,6 +283,19 @@ TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& pr
 // update_site_ndia_* for `side`), layout make_layout(sectors, k2, dN).
 TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold, int kind, int side);
+// One step of the chain of singlet pair creators P_12 = [c^dag_1 x
+// c^dag_2]^0 of block sites (Get_sys_two, code/correlation.f90:331-367):
+// CARRY q embeds old pair operator q (slot B_SYS, layout make_layout(prev, 0,
+// 2)); CREATE k forms the pair of old carried c^dag number k (slot B_ENV,
+// layout make_layout(prev, 1, 1), cd_stride apart) with the new site's
+// c^dag.  Each is renormalised as U_bra^dag P U_ket into make_layout(new, 0,
+// 2) at q * new_size of B_OUT.
+struct PairSrc {
+  enum What { CARRY, CREATE } what;
+  int k;
+};
+TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& prev,
+                              const std::vector<KeptSector>& kept, const std::vector<PairSrc>& srcs, int side);
 struct TrackSrc {
   enum What { CARRY, SITE, PRODUCT } what;
   int k;

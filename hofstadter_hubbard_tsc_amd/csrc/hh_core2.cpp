@@ -450,6 +450,78 @@ TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& pr
   return P;
 }
 
+TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& prev,
+                              const std::vector<KeptSector>& kept, const std::vector<PairSrc>& srcs, int side) {
+  TrackPlan P;
+  const auto st = site_sectors(m.kind);
+  const int nst = (int)st.size();
+  P.eb = enlarge_basis(prev, st);
+  const EnlBasis& eb = P.eb;
+  std::vector<Sector> nsec;
+  for (auto& k : kept) nsec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
+  const OpLayout ol = make_layout(prev, 0, 2), cl = make_layout(prev, 1, 1), nl = make_layout(nsec, 0, 2);
+  P.old_size = ol.size;
+  P.new_size = nl.size;
+  Collector s1, s2;
+  for (size_t q = 0; q < srcs.size(); ++q) {
+    const PairSrc& src = srcs[q];
+    for (auto& nsb : nl.subs) {
+      const KeptSector& kb = kept[nsb.bra];
+      const KeptSector& kk = kept[nsb.ket];
+      const int ap = kb.a, a = kk.a;
+      const int J2p = eb.sec[ap].S2, J2 = eb.sec[a].S2, dap = eb.sec[ap].dim, kp = kk.keep;
+      const int64_t xo = P.x_size;
+      P.x_size += (int64_t)dap * kp;
+      auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, dap, d, kp}; };
+      for (auto& ss : eb.subs[ap]) s1.ensure(rows(ss.pos, ss.dim));
+      if (src.what == PairSrc::CARRY) {
+        // carried rank-0 operator: e1 recoupling on every (b', s) <- (b, s)
+        for (auto& osb : ol.subs)
+          for (auto& ss : eb.subs[a]) {
+            if (ss.b != osb.ket) continue;
+            int pos_out = -1, d_out = 0;
+            for (auto& t : eb.subs[ap])
+              if (t.b == osb.bra && t.s == ss.s) { pos_out = t.pos; d_out = t.dim; }
+            if (pos_out < 0) continue;
+            const double c = e1(eb.bl[osb.bra].S2, eb.bl[osb.ket].S2, st[ss.s].S2, J2p, J2, 0);
+            if (c == 0.0) continue;
+            s1.add(rows(pos_out, d_out), mk(c, B_SYS, (int64_t)src.k * P.old_size + osb.off, osb.rows, OP_NN, B_AUX,
+                                            kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
+          }
+      } else if (J2p == J2) {
+        // [c^dag(block) x c^dag(new site)]^0 on (b', s') <- (b, s), the
+        // scalar product of Get_hamiltonian_ndia's bond terms with the
+        // site's creator in place of its annihilator
+        for (auto& csb : cl.subs)
+          for (auto& ss : eb.subs[a]) {
+            if (ss.b != csb.ket) continue;
+            for (int sp = 0; sp < nst; ++sp) {
+              const cplx sv = site_val(m.kind, OP_CDAG, sp, ss.s);
+              if (sv == 0.0) continue;
+              int pos_out = -1, d_out = 0;
+              for (auto& t : eb.subs[ap])
+                if (t.b == csb.bra && t.s == sp) { pos_out = t.pos; d_out = t.dim; }
+              if (pos_out < 0) continue;
+              double c = x12(eb.bl[csb.bra].S2, eb.bl[csb.ket].S2, st[sp].S2, st[ss.s].S2, J2, 1);
+              if (c == 0.0) continue;
+              c *= side == 0 ? nsign(eb.bl[ss.b].N) : nsign(st[ss.s].N);
+              if (side == 1) c = -c;
+              const cplx coef = c * sv * std::sqrt((double)J2 + 1);
+              s1.add(rows(pos_out, d_out), mk(coef, B_ENV, (int64_t)src.k * cl.size + csb.off, csb.rows, OP_NN,
+                                              B_AUX, kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
+            }
+          }
+      }
+      // P_new[bra <- ket] = U_bra^dag X
+      Region r{B_OUT, (int64_t)q * P.new_size + nsb.off, nsb.rows, nsb.rows, nsb.cols};
+      s2.add(r, mk(1.0, B_AUX, kb.u_off, kb.u_ld, OP_C, B_X, xo, dap, OP_NN, dap));
+    }
+  }
+  s1.emit(P.stage1);
+  s2.emit(P.stage2);
+  return P;
+}
+
 TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold) {
   return compile_track_rank(m, prev, kept, nold, OP_S, 0);

@@ -1235,6 +1235,198 @@ void hopping_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, doub
   });
 }
 
+// Singlet pair creators P_12 = [c^dag_1 x c^dag_2]^0 of bonds (s1 < s2,
+// block-site labels) of one side's block, carried to its current length
+// (Get_sys_two / Get_SCOP_Operator_Env): operator q is bonds[q], layout `lay`
+// = make_layout(sec, 0, 2), lay.size elements apart.
+struct TrackedPairs {
+  DevMem ops;
+  std::vector<Sector> sec;
+  OpLayout lay;
+  std::vector<std::pair<int, int>> bonds;
+};
+
+void track_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, const std::vector<std::pair<int, int>>& want,
+                      TrackedPairs& T) {
+  const ModelParams& m = ctx->model;
+  const auto& blocks = side == 0 ? d.sysb : d.envb;
+  const auto& truns = side == 0 ? d.syst : d.envt;
+  int last = 0;
+  for (auto& b : want) {
+    need(b.first >= 1 && b.first < b.second && b.second <= len, "pair bond outside the block");
+    last = std::max(last, b.second);
+  }
+  // c^dag of every site, one step behind the pairs
+  TrackedRank C;
+  C.sec = blocks.at(1)->d.sec;
+  C.lay = make_layout(C.sec, 1, 1);
+  C.n = 1;
+  {
+    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
+    for (auto& sub : C.lay.subs) {
+      const cplx v = site_val(m.kind, OP_CDAG, sub.bra, sub.ket);
+      h[sub.off] = make_double2(v.real(), v.imag());
+    }
+    C.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  }
+  T.sec = C.sec;
+  T.lay = make_layout(T.sec, 0, 2);
+  T.bonds.clear();
+  DevMem nxt, X;
+  for (int l = 2; l <= len; ++l) {
+    need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
+    const hh_trunc* t = truns.at(l).get();
+    std::vector<PairSrc> srcs;
+    std::vector<std::pair<int, int>> nb;
+    for (size_t q = 0; q < T.bonds.size(); ++q) {
+      srcs.push_back({PairSrc::CARRY, (int)q});
+      nb.push_back(T.bonds[q]);
+    }
+    for (auto& b : want)
+      if (b.second == l) {
+        srcs.push_back({PairSrc::CREATE, b.first - 1});
+        nb.push_back(b);
+      }
+    TrackPlan P = compile_track_pairs(m, T.sec, t->kept, srcs, side);
+    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (int64_t)srcs.size(), 1));
+    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+    BufPtrs b{};
+    b.p[B_SYS] = T.ops.p ? T.ops.c() : nullptr;
+    b.p[B_ENV] = C.ops.c();
+    b.p[B_AUX] = t->evec.c();
+    b.p[B_X] = X.c();
+    b.p[B_OUT] = nxt.c();
+    DevTasks s1, s2;
+    s1.upload(P.stage1, ctx->stream);
+    s2.upload(P.stage2, ctx->stream);
+    launch_tiles(s1, b, ctx->stream, PREC_FP64);
+    launch_tiles(s2, b, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "pair chain");
+    std::swap(T.ops.p, nxt.p);
+    std::swap(T.ops.bytes, nxt.bytes);
+    T.sec.clear();
+    for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+    T.lay = make_layout(T.sec, 0, 2);
+    T.bonds = nb;
+    if (l < last) {   // advance c^dag for the bonds still to be formed
+      TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_CDAG, side);
+      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
+      BufPtrs c{};
+      c.p[B_SYS] = C.ops.c();
+      c.p[B_AUX] = t->evec.c();
+      c.p[B_X] = X.c();
+      c.p[B_OUT] = nxt.c();
+      DevTasks q1, q2;
+      q1.upload(Q.stage1, ctx->stream);
+      q2.upload(Q.stage2, ctx->stream);
+      launch_tiles(q1, c, ctx->stream, PREC_FP64);
+      launch_tiles(q2, c, ctx->stream, PREC_FP64);
+      check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+      std::swap(C.ops.p, nxt.p);
+      std::swap(C.ops.bytes, nxt.bytes);
+      C.sec = T.sec;
+      C.lay = make_layout(C.sec, 1, 1);
+      C.n += 1;
+    }
+  }
+}
+
+// <Delta^dag_a Delta_b> for bonds a in the sys block and b in the env block,
+// Delta^dag_ij = (c^dag_i,up c^dag_j,down - c^dag_i,down c^dag_j,up)/sqrt(2):
+// the block-block pairs of Get_mSCOP_Operator_cor0 (code/correlation.f90:
+// 198-327, sys_block_env_block_cor_ndia).  Each pair is one rank-0 two-point
+// term [P_a (x) P_b^dag]^0, the env annihilator read as the adjoint of its
+// carried creator.  bonds are nb pairs of lattice positions; out is nb x nb
+// complex (re, im), entry (b, a) the conjugate; other entries NaN.
+void pair_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int nb, const int32_t* bonds,
+                double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  need(nb >= 0, "negative bond count");
+  for (int64_t i = 0; i < 2 * (int64_t)nb * nb; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+  std::vector<std::pair<int, int>> sw, ew;   // block-site labels
+  std::vector<int> sidx(nb, -1), eidx(nb, -1);
+  for (int q = 0; q < nb; ++q) {
+    int u = std::min(bonds[2 * q], bonds[2 * q + 1]), v = std::max(bonds[2 * q], bonds[2 * q + 1]);
+    need(u >= 1 && v <= nsite && u != v, "bond sites out of range");
+    if (v <= sys_len) {
+      sidx[q] = (int)sw.size();
+      sw.push_back({u, v});
+    } else if (u >= nsite + 1 - env_len) {   // env block: label nsite + 1 - position
+      eidx[q] = (int)ew.size();
+      ew.push_back({nsite + 1 - v, nsite + 1 - u});
+    }
+  }
+  if (sw.empty() || ew.empty()) return;
+  TrackedPairs TS, TE;
+  track_pair_chain(ctx, d, 0, sys_len, sw, TS);
+  track_pair_chain(ctx, d, 1, env_len, ew, TE);
+  auto slot = [](const TrackedPairs& T, std::pair<int, int> b) {
+    for (size_t q = 0; q < T.bonds.size(); ++q)
+      if (T.bonds[q] == b) return (int64_t)q;
+    throw HHError(1, "pair operator missing");
+  };
+  BlockDesc sd, ed;
+  sd.side = 0;
+  sd.length = sys_len;
+  sd.sec = TS.sec;
+  ed.side = 1;
+  ed.length = env_len;
+  ed.sec = TE.sec;
+  const Geometry g = make_geometry(m, sys_len, env_len);
+  DevMem Y, X, ipsi;
+  for (int a = 0; a < nb; ++a) {
+    if (sidx[a] < 0) continue;
+    for (int b = 0; b < nb; ++b) {
+      if (eidx[b] < 0) continue;
+      MeasureTerm t;
+      t.amp = 1.0;
+      t.s.block = t.e.block = true;
+      t.s.kind = t.e.kind = OP_PAIR;
+      t.s.buf = B_SYS;
+      t.s.base = slot(TS, sw[sidx[a]]) * TS.lay.size;
+      t.s.lay = &TS.lay;
+      t.e.buf = B_ENV;
+      t.e.tilde = true;
+      t.e.base = slot(TE, ew[eidx[b]]) * TE.lay.size;
+      t.e.lay = &TE.lay;
+      MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
+      const int64_t n = P.sb.dim;
+      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      if (!ipsi.p) {
+        ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
+        dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, n, ctx->stream);
+      }
+      BufPtrs bp{};
+      bp.p[B_IN] = (double2*)psi;
+      bp.p[B_OUT] = Y.c();
+      bp.p[B_SYS] = TS.ops.c();
+      bp.p[B_ENV] = TE.ops.c();
+      bp.p[B_X] = X.c();
+      DevTasks s1, s2;
+      s1.upload(P.stage1, ctx->stream);
+      s2.upload(P.stage2, ctx->stream);
+      launch_tiles(s1, bp, ctx->stream, PREC_FP64);
+      launch_tiles(s2, bp, ctx->stream, PREC_FP64);
+      double re = 0.0, im = 0.0;
+      mdots_to_host(ctx, psi, 0, 1, Y.c(), n, &re);
+      mdots_to_host(ctx, ipsi.c(), 0, 1, Y.c(), n, &im);
+      const int64_t ab = (int64_t)a * nb + b, ba = (int64_t)b * nb + a;
+      out[2 * ab] = re;
+      out[2 * ab + 1] = im;
+      out[2 * ba] = re;
+      out[2 * ba + 1] = -im;
+    }
+  }
+}
+
 // One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
 // site r on the sys side (block or centre) and every other site j.  Pairs of
 // block sites use the tracked products n_r n_j (Get_sys2b / measure_sys_block);
@@ -1899,6 +2091,15 @@ int hh_driver_hopping_cross(hh_driver* h, int sys_len, const double* psi, double
   return guard(h->d.ctx, [&] {
     need(psi && g_out, "null argument");
     hopping_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, g_out);
+  });
+}
+
+int hh_driver_pair_cross(hh_driver* h, int sys_len, const double* psi, int nbond, const int32_t* bonds,
+                         double* p_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && p_out && (nbond == 0 || bonds), "null argument");
+    pair_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, nbond, bonds, p_out);
   });
 }
 

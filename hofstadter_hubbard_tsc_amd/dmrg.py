@@ -494,6 +494,19 @@ class Driver:
                      "hh_driver_hopping_cross")
         return out[..., 0] + 1j * out[..., 1]
 
+    def pair_cross(self, sys_len, psi, bonds):
+        """<Delta^dag_a Delta_b> (len(bonds) x len(bonds) complex) for bonds a in the
+        sys block and b in the env block (lattice positions); NaN elsewhere."""
+        _sync()
+        bonds = np.ascontiguousarray(np.asarray(bonds, dtype=np.int32).reshape(-1, 2))
+        nb = len(bonds)
+        out = np.zeros((nb, nb, 2))
+        self.ctx._rc(L.load().hh_driver_pair_cross(self.ptr, sys_len, _dptr(psi), nb,
+                                                   bonds.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_pair_cross")
+        return out[..., 0] + 1j * out[..., 1]
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

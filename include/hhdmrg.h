@@ -256,6 +256,18 @@ int hh_driver_spin_cross(hh_driver* d, int sys_len, const double* psi_dev, doubl
  * complex (re, im interleaved), row-major by lattice position - 1; the entry
  * (j, i) holds the conjugate; pairs on one side are NaN. */
 int hh_driver_hopping_cross(hh_driver* d, int sys_len, const double* psi_dev, double* g_out);
+/* singlet pair correlations <Delta^dag_a Delta_b>, Delta^dag_ij =
+ * (c^dag_i,up c^dag_j,down - c^dag_i,down c^dag_j,up)/sqrt(2), for bonds a
+ * inside the sys block and b inside the env block: the block-block part of
+ * Get_mSCOP_Operator_cor0 (code/correlation.f90:198-327), with the pair
+ * creators formed when their later site joins (Get_sys_two, :331-367) and
+ * carried through the stored truncations, measured as
+ * sys_block_env_block_cor_ndia (:1666-1929).  bonds holds nbond pairs of
+ * lattice positions; p_out is nbond x nbond complex (re, im interleaved),
+ * row-major; entry (b, a) holds the conjugate; pairs that are not
+ * (sys block, env block) are NaN. */
+int hh_driver_pair_cross(hh_driver* d, int sys_len, const double* psi_dev, int nbond, const int32_t* bonds,
+                         double* p_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

@@ -243,3 +243,24 @@ def singlet_ground_hopping(model, positions, bonds, n_elec):
                 v += np.sum(np.conj(psi[dst]) * sg * psi[src])
             out[pa - 1, pb - 1] = v
     return e, out
+
+
+def singlet_ground_pairing(model, positions, bonds, n_elec, left, right):
+    """(E0, <Delta^dag_b Delta_b'>) for bonds b in ``left`` and b' in
+    ``right`` (pairs of lattice positions), Delta^dag_ij = (c^dag_i,up
+    c^dag_j,down - c^dag_i,down c^dag_j,up) / sqrt(2), in the lowest total-S
+    = 0 state."""
+    e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
+    order = {p: i for i, p in enumerate(positions)}
+    out = np.zeros((len(left), len(right)), complex)
+    for a, (pi, pj) in enumerate(left):
+        for b, (pk, pl) in enumerate(right):
+            i, j, k, l = order[pi], order[pj], order[pk], order[pl]
+            v = 0.0
+            # (1/2) sum of the four singlet products
+            for (s1, s2, s3, s4, c) in ((0, 1, 1, 0, 1.0), (0, 1, 0, 1, -1.0), (1, 0, 1, 0, -1.0), (1, 0, 0, 1, 1.0)):
+                ops = [(2 * i + s1, True), (2 * j + s2, True), (2 * l + s3, False), (2 * k + s4, False)]
+                src, dst, sg = fs.apply(ops)
+                v += 0.5 * c * np.sum(np.conj(psi[dst]) * sg * psi[src])
+            out[a, b] = v
+    return e, out

@@ -25,7 +25,10 @@ embedded in the superblock's enlarged bases and contracted as the exchange
 bond, -sqrt(3) [S_i (x) S_j]^0.  ``hopping_cross`` gives sum_sigma
 <c^dag_i c_j> the same way, with c^dag carried as a rank-1/2 fermionic
 operator and the environment one read as its annihilator partner (the
-hopping bond without its h.c.).
+hopping bond without its h.c.).  ``pair_cross`` gives the singlet pair
+correlations <Delta^dag_a Delta_b> of bonds in the two blocks
+(Get_mSCOP_Operator_cor0's block-block part), with each pair creator formed
+as the Edmonds rank-0 product of the carried c^dag and the new site's c^dag.
 
 ``density_row`` gives one row <n_r n_j> of the same correlation for a row
 site r on the system side and every j, with the two-site products n_r n_j of
@@ -33,7 +36,7 @@ block sites carried through the truncations (``tracked_pairs``).
 
 Pinned by exact diagonalisation (``ed.singlet_ground_density``,
 ``ed.singlet_ground_correlations``, ``ed.singlet_ground_spin_correlations``,
-``ed.singlet_ground_hopping``):
+``ed.singlet_ground_hopping``, ``ed.singlet_ground_pairing``):
 with no truncation the profile and the correlations equal the exact ones; the densities sum to N and, for each i,
 sum_j <n_i n_j> over the other side equals <n_i N_other>; a full row plus
 <n_r^2> sums to N <n_r>.
@@ -43,6 +46,7 @@ import math
 import numpy as np
 
 from . import dmrg
+from .su2 import phase
 
 
 def _site_density_op(kind):
@@ -361,4 +365,69 @@ def hopping_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, 
             pj = nsite + 1 - kj if kj <= env_len else nsite - env_len
             out[ki - 1, pj - 1] = v
             out[pj - 1, ki - 1] = np.conj(v)
+    return out
+
+
+def _couple0(T, U, eb, k2):
+    """[T x U]^0 of two rank-k2/2 operators on one enlarged basis (the
+    operator product of block_mul_block_ndia): Edmonds (7.1.1) with k = 0,
+    <a'||[T x U]^0||a> = (-1)^{2J} sum_{a''} (-1)^{k + J + J''} /
+    sqrt((2k+1)(2J+1)) <a'||T||a''><a''||U||a>, J' = J."""
+    out = {}
+    for (ap, am), t in T.items():
+        for (am2, a), u in U.items():
+            if am2 != am or eb.sectors[ap][1] != eb.sectors[a][1]:
+                continue
+            J2, Jm2 = eb.sectors[a][1], eb.sectors[am][1]
+            c = (-1.0) ** J2 * phase(k2 + J2 + Jm2) / math.sqrt((k2 + 1) * (J2 + 1))
+            out[(ap, a)] = out.get((ap, a), 0.0) + c * (t @ u)
+    return out
+
+
+def tracked_pairs_cdag(model, blocks, truns, length, side, bonds):
+    """{(s1, s2): P_12 in the basis of blocks[length]} for every bond (s1 <
+    s2, block-site labels) with P_12 = [c^dag_s1 x c^dag_s2]^0, the singlet
+    pair creator of Get_sys_two (code/correlation.f90:331-367): formed in the
+    enlarged basis of the step where s2 joins, from the carried c^dag_s1 and
+    the new site's c^dag, then carried through every later truncation as a
+    rank-0, N + 2 operator."""
+    st = dmrg.site_sectors(model.kind)
+    C = dmrg.site_ops(model.kind)["cdag"]
+    cd = {1: C}
+    pairs = {}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        newp = {b: _transform_any(dmrg.embed_block_op(P, prev, eb, side), truns[l], 0, 2, False)
+                for b, P in pairs.items()}
+        site = dmrg.embed_site_op(C, prev, eb, side)
+        for (s1, s2) in bonds:
+            if s2 == l:
+                P = _couple0(dmrg.embed_block_op(cd[s1], prev, eb, side), site, eb, 1)
+                newp[(s1, s2)] = _transform_any(P, truns[l], 0, 2, False)
+        cd = {k: _transform_any(dmrg.embed_block_op(o, prev, eb, side), truns[l], 1, 1, True) for k, o in cd.items()}
+        cd[l] = _transform_any(site, truns[l], 1, 1, True)
+        pairs = newp
+    return pairs
+
+
+def pair_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H, sys_bonds,
+               env_bonds):
+    """<Delta^dag_b Delta_b'> for bonds b of the system block and b' of the
+    environment block (block-site labels), Delta^dag_ij = (c^dag_i,up
+    c^dag_j,down - c^dag_i,down c^dag_j,up) / sqrt(2): the block-block part of
+    Get_mSCOP_Operator_cor0 (code/correlation.f90:198-327).  The environment
+    annihilator is the adjoint of its carried creator; both are embedded in
+    the superblock's enlarged bases and contracted as a rank-0 cross term.
+    Returns a complex array (len(sys_bonds), len(env_bonds))."""
+    Ps = tracked_pairs_cdag(model, sys_blocks, sys_truns, sys_len, "sys", sys_bonds)
+    Pe = tracked_pairs_cdag(model, env_blocks, env_truns, env_len, "env", env_bonds)
+    sbl, ebl = sys_blocks[sys_len], env_blocks[env_len]
+    out = np.zeros((len(sys_bonds), len(env_bonds)), complex)
+    for a, bs in enumerate(sys_bonds):
+        T = dmrg.embed_block_op(Ps[bs], sbl, H.s.basis, "sys")
+        for b, be in enumerate(env_bonds):
+            Ue = dmrg.embed_block_op(Pe[be], ebl, H.e.basis, "env")
+            U = {(y, yp): np.conj(m).T for (yp, y), m in Ue.items()}
+            out[a, b] = complex(np.vdot(psi, _cross_apply(H, T, U, 0, 1.0, psi)))
     return out

@@ -3,7 +3,8 @@ operator chain of Get_sys_oper_dia / Get_env_oper_dia) and of the density
 correlations <n_i n_j> (Get_density_cor: the pairs across the superblock
 centre, and whole rows with the two-site products carried in the system
 block), and of the spin correlations <S_i . S_j> and one-body correlations
-sum_sigma <c^dag_i c_j> across the centre:
+sum_sigma <c^dag_i c_j> and singlet pair correlations <Delta^dag_a Delta_b>
+across the centre:
 untruncated runs against exact diagonalisation, truncated
 runs against the oracle's chain, and the sum rule sum_i <n_i> = N."""
 import numpy as np
@@ -128,3 +129,67 @@ def test_density_truncated_matches_oracle_chain(ctx):
         assert np.array_equal(np.isnan(row), np.isnan(rref))
         ok = ~np.isnan(rref)
         assert np.abs(row[ok] - rref[ok]).max() < 1e-10, (r, np.abs(row[ok] - rref[ok]).max())
+
+
+def _pair_bonds(m, L, R):
+    sys_b = [(a, b) for a in range(1, L + 1) for b in range(a + 1, L + 1)]
+    env_b = [(a, b) for a in range(1, R + 1) for b in range(a + 1, R + 1)]
+    return sys_b, env_b, [(m.nsite + 1 - b, m.nsite + 1 - a) for (a, b) in env_b]
+
+
+def _sweep(ctx, m, kept, tol):
+    from hofstadter_hubbard_tsc_amd import Driver
+    ctx.set_model(m)
+    n = m.nsite
+    drv = Driver(ctx, kept_max=kept, tol=tol, kept_max_warm=kept, seed=11, warm_tol_factor=1.0, first_tol_factor=1.0)
+    drv.warmup()
+    drv.sweep(0, n // 2 - 1, n - 2, tol, TRUN, kept, fresh=True, phase=1)
+    drv.reset_transform()
+    drv.sweep(1, n - 2, 1, tol, TRUN, kept, phase=1)
+    L = n // 2 - 1
+    R = n - L - 2
+    sb = ctx.bind(drv.block(0, L), drv.block(1, R), m.n_elec)
+    psi = dev(dmrg.start_vector(sb.dim, 5))
+    E, _ = ctx.lanczos(sb, psi, 1e-13, 400)
+    return drv, L, R, E, psi
+
+
+@pytest.mark.parametrize("kind,flux,Lx,N", [("hubbard", 0.15, 4, 6), ("tj", 0.2, 4, 6)])
+def test_pair_cross_untruncated_equals_ed(ctx, kind, flux, Lx, N):
+    """<Delta^dag_a Delta_b> (Get_mSCOP_Operator_cor0's block-block pairs) for
+    every site pair a of the sys block and b of the env block against exact
+    diagonalisation; bonds on one side only come back NaN."""
+    m = lat.Model(kind=kind, Lx=Lx, Ly=2, U=4.0, J=0.4, t=1.0, flux=flux, n_elec=N)
+    drv, L, R, E, psi = _sweep(ctx, m, 2000, 1e-12)
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    sys_b, env_b, env_pos = _pair_bonds(m, L, R)
+    e0, D = ed.singlet_ground_pairing(m, list(range(1, m.nsite + 1)), bonds, m.n_elec, sys_b, env_pos)
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    allb = sys_b + env_pos
+    P = drv.pair_cross(L, psi, allb)
+    ns = len(sys_b)
+    got = P[:ns, ns:]
+    assert np.abs(got - D).max() < 1e-6, (got, D)
+    assert np.array_equal(P[ns:, :ns], np.conj(got).T)
+    assert np.isnan(P[:ns, :ns]).all() and np.isnan(P[ns:, ns:]).all()
+
+
+def test_pair_cross_truncated_matches_oracle(ctx):
+    """Hubbard 6 x 2 ladder with flux at m = 24: the GPU pair chains and
+    correlations against the oracle's on the GPU run's own blocks,
+    truncations and ground state."""
+    from .gpuhelp import host, oracle_block, oracle_superblock
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=4.0, t=1.0, flux=0.1, n_elec=10)
+    drv, L, R, E, psi = _sweep(ctx, m, 24, 1e-11)
+    sys_b, env_b, env_pos = _pair_bonds(m, L, R)
+    sys_b, env_b, env_pos = sys_b[::3], env_b[::3], env_pos[::3]
+    P = drv.pair_cross(L, psi, sys_b + env_pos)[:len(sys_b), len(sys_b):]
+    sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
+    env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
+    sys_truns = {l: drv.trunc(0, l).matrices() for l in range(2, L + 1)}
+    env_truns = {l: drv.trunc(1, l).matrices() for l in range(2, R + 1)}
+    H = oracle_superblock(ctx, drv.block(0, L), drv.block(1, R), m, m.n_elec)
+    ref = measure.pair_cross(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, sys_b, env_b)
+    assert np.abs(ref).max() > 1e-4
+    assert np.abs(P - ref).max() < 1e-10, np.abs(P - ref).max()

@@ -220,3 +220,55 @@ def test_hopping_chain_without_fermion_signs_fails():
         measure._transform_any = saved
     x = np.ix_(np.arange(L + 1), np.arange(L + 1, m.nsite))
     assert np.abs(good[x] - bad[x]).max() > 1e-3
+
+
+def _pair_bonds(m, L, R):
+    sys_b = [(a, b) for a in range(1, L + 1) for b in range(a + 1, L + 1)]
+    env_b = [(a, b) for a in range(1, R + 1) for b in range(a + 1, R + 1)]
+    return sys_b, env_b, [(m.nsite + 1 - b, m.nsite + 1 - a) for (a, b) in env_b]
+
+
+@pytest.mark.parametrize("kind,flux,Lx,N", [("hubbard", 0.15, 4, 6), ("tj", 0.2, 4, 6), ("hubbard", 0.1, 5, 8)])
+def test_untruncated_pair_cross_equals_ed(kind, flux, Lx, N):
+    """<Delta^dag_a Delta_b> for every pair of sites a in the system block and
+    b in the environment block, from the carried singlet pair creators, equals
+    the exact pair correlation."""
+    m = lat.Model(kind=kind, Lx=Lx, Ly=2, U=4.0, J=0.4, t=1.0, flux=flux, n_elec=N)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    sys_b, env_b, env_pos = _pair_bonds(m, L, R)
+    e0, D = ed.singlet_ground_pairing(m, list(range(1, m.nsite + 1)), bonds, m.n_elec, sys_b, env_pos)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    P = measure.pair_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H, sys_b, env_b)
+    assert np.abs(D).max() > 1e-3
+    assert np.abs(P - D).max() < 1e-6
+
+
+def test_pair_product_with_a_wrong_phase_fails():
+    """A planted mistake in the pair creator's coupling (the (-1)^{k+J+J''}
+    of the rank-0 product dropped) changes the pair correlations."""
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=4.0, t=1.0, flux=0.15, n_elec=6)
+    L, R = 3, 3
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    sg = dmrg.StepGeometry(m, L, R)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    sys_b, env_b, _ = _pair_bonds(m, L, R)
+    good = measure.pair_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H,
+                              sys_b, env_b)
+    saved = measure.phase
+    try:
+        measure.phase = lambda two_x: 1.0
+        bad = measure.pair_cross(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H,
+                                 sys_b, env_b)
+    finally:
+        measure.phase = saved
+    assert np.abs(good - bad).max() > 1e-3
