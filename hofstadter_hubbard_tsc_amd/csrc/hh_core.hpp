@@ -210,6 +210,10 @@ struct MeasurePiece {
 struct MeasureTerm {
   MeasurePiece s, e;
   cplx amp = 1.0;
+  // within = true: [A_block (x) B_site]^0 inside the enlarged sys block (s is
+  // the block operator, e the sys centre site's operator), the form of the
+  // block-site bond of H.psi
+  bool within = false;
 };
 // With only != nullptr the plan holds that one term instead of the model's H
 // (block Hamiltonians, U and every bond left out); sys.sec / env.sec give the
@@ -296,8 +300,11 @@ struct PairSrc {
 };
 TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& prev,
                               const std::vector<KeptSector>& kept, const std::vector<PairSrc>& srcs, int side);
+// SPIN_PRODUCT k is S_k . S_new = -sqrt(3) [S_k x S_new]^0 of carried spin k
+// (slot B_ENV, layout make_layout(prev, 2, 0), k-th operator) and the new
+// site's spin (the exchange bond of Get_hamiltonian_ndia).
 struct TrackSrc {
-  enum What { CARRY, SITE, PRODUCT } what;
+  enum What { CARRY, SITE, PRODUCT, SPIN_PRODUCT } what;
   int k;
 };
 TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,

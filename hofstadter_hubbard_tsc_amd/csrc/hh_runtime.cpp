@@ -1130,15 +1130,15 @@ void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Track
   }
 }
 
-// <psi| term |psi> for every sys-side i and env-side j (centre sites
-// included): the two-point term amp [A_i (x) B_j]^0 of operator kind `kind`
-// (env operator read as the annihilator partner when `tilde`), compiled by
-// compile_matvec's single-term mode and applied to psi.  val(i, j) receives
-// the complex expectation; i = sys_len + 1 and j = env_len + 1 are the centre
-// sites.
+// <psi| term |psi> for every sys-side i (or only i = only_i) and env-side j
+// (centre sites included): the two-point term amp [A_i (x) B_j]^0 of
+// operator kind `kind` (env operator read as the annihilator partner when
+// `tilde`), compiled by compile_matvec's single-term mode and applied to psi.
+// val(i, j) receives the complex expectation; i = sys_len + 1 and
+// j = env_len + 1 are the centre sites.
 template <typename Sink>
 void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int kind, bool tilde, cplx amp,
-                 Sink val) {
+                 Sink val, int only_i = 0) {
   const ModelParams& m = ctx->model;
   const int nsite = m.nsite();
   const int env_len = nsite - sys_len - 2;
@@ -1158,6 +1158,7 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
   DevMem Y, X, ipsi;
   for (int i = 1; i <= sys_len + 1; ++i)
     for (int j = 1; j <= env_len + 1; ++j) {
+      if (only_i && i != only_i) continue;
       MeasureTerm t;
       t.amp = amp;
       t.s.kind = t.e.kind = kind;
@@ -1425,6 +1426,160 @@ void pair_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int nb,
       out[2 * ba + 1] = -im;
     }
   }
+}
+
+// One row of Get_density_cor with oper1 = oper2 = the spin: <S_r . S_j> for a
+// row site r on the sys side (1 .. sys_len + 1) and every other site j.
+// Block pairs use the scalar products S_r . S_j formed when their later site
+// joins (Get_sys2b) and carried as rank-0 operators; a block site with the
+// sys centre site is the within-block exchange term (sys_block_site_cor_ndia);
+// the env side is spin_cross's term for i = r.  out has nsite values,
+// out[r-1] = NaN.
+void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, double* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(r >= 1 && r <= sys_len + 1, "the row site must lie on the sys side of the cut (1 .. sys_len + 1)");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  for (int j = 0; j < nsite; ++j) out[j] = std::numeric_limits<double>::quiet_NaN();
+  const bool centre = r == sys_len + 1;
+  // spins of every site (C) and the products S_r . S_j (T), stepped together
+  TrackedRank C;
+  C.sec = d.sysb.at(1)->d.sec;
+  C.lay = make_layout(C.sec, 2, 0);
+  C.n = 1;
+  {
+    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
+    for (auto& sub : C.lay.subs) {
+      const cplx v = site_val(m.kind, OP_S, sub.bra, sub.ket);
+      h[sub.off] = make_double2(v.real(), v.imag());
+    }
+    C.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  }
+  Tracked T;
+  T.sec = C.sec;
+  DevMem nxt, X;
+  auto run = [&](const TrackPlan& P, BufPtrs b) {
+    DevTasks s1, s2;
+    s1.upload(P.stage1, ctx->stream);
+    s2.upload(P.stage2, ctx->stream);
+    launch_tiles(s1, b, ctx->stream, PREC_FP64);
+    launch_tiles(s2, b, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+  };
+  for (int l = 2; l <= sys_len; ++l) {
+    const hh_trunc* t = d.syst.at(l).get();
+    if (!centre) {
+      std::vector<TrackSrc> srcs;
+      std::vector<std::pair<int, int>> nl;
+      for (size_t q = 0; q < T.lbl.size(); ++q) {
+        srcs.push_back({TrackSrc::CARRY, (int)q});
+        nl.push_back(T.lbl[q]);
+      }
+      if (l == r)
+        for (int j = 1; j < r; ++j) {
+          srcs.push_back({TrackSrc::SPIN_PRODUCT, j - 1});
+          nl.push_back({j, r});
+        }
+      if (l > r) {
+        srcs.push_back({TrackSrc::SPIN_PRODUCT, r - 1});
+        nl.push_back({r, l});
+      }
+      TrackPlan P = compile_track_ops(m, T.sec, t->kept, srcs);
+      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (int64_t)srcs.size(), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = T.ops.p ? T.ops.c() : nullptr;
+      b.p[B_ENV] = C.ops.c();
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nxt.c();
+      run(P, b);
+      std::swap(T.ops.p, nxt.p);
+      std::swap(T.ops.bytes, nxt.bytes);
+      T.lbl = nl;
+    }
+    TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_S, 0);
+    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
+    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
+    BufPtrs c{};
+    c.p[B_SYS] = C.ops.c();
+    c.p[B_AUX] = t->evec.c();
+    c.p[B_X] = X.c();
+    c.p[B_OUT] = nxt.c();
+    run(Q, c);
+    std::swap(C.ops.p, nxt.p);
+    std::swap(C.ops.bytes, nxt.bytes);
+    C.sec.clear();
+    for (auto& k : t->kept) C.sec.push_back({Q.eb.sec[k.a].N, Q.eb.sec[k.a].S2, k.keep});
+    C.lay = make_layout(C.sec, 2, 0);
+    C.n += 1;
+    T.sec = C.sec;
+  }
+  T.osz = diag_op_size(T.sec);
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  // pairs inside the sys block: Re <Q_rj, rho'(psi, psi)>
+  if (!centre && !T.lbl.empty()) {
+    DevMem rho;
+    rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_OUT] = rho.c();
+    launch_on(ctx, compile_block_rho(S, 0, sb->plan.sbs, T.sec), b);
+    std::vector<double> h(T.lbl.size());
+    mdots_to_host(ctx, T.ops.c(), T.osz, (int)T.lbl.size(), rho.c(), T.osz, h.data());
+    for (size_t q = 0; q < T.lbl.size(); ++q) {
+      const auto& lb = T.lbl[q];
+      out[(lb.first == r ? lb.second : lb.first) - 1] = h[q];
+    }
+  }
+  // block site with the sys centre site: the within-block exchange term
+  {
+    BlockDesc sd, ed;
+    sd.side = 0;
+    sd.length = sys_len;
+    sd.sec = C.sec;
+    ed.side = 1;
+    ed.length = env_len;
+    ed.sec = sb->env->d.sec;
+    const Geometry g = make_geometry(m, sys_len, env_len);
+    DevMem Y, Xw;
+    for (int j = 1; j <= sys_len; ++j) {
+      if (!centre && j != r) continue;
+      MeasureTerm t;
+      t.within = true;
+      t.amp = -std::sqrt(3.0);
+      t.s.block = true;
+      t.s.kind = t.e.kind = OP_S;
+      t.s.buf = B_SYS;
+      t.s.base = (int64_t)(j - 1) * C.lay.size;
+      t.s.lay = &C.lay;
+      MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
+      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.sb.dim, 1));
+      Xw.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+      BufPtrs b{};
+      b.p[B_IN] = (double2*)psi;
+      b.p[B_OUT] = Y.c();
+      b.p[B_SYS] = C.ops.c();
+      b.p[B_X] = Xw.c();
+      DevTasks s1, s2;
+      s1.upload(P.stage1, ctx->stream);
+      s2.upload(P.stage2, ctx->stream);
+      launch_tiles(s1, b, ctx->stream, PREC_FP64);
+      launch_tiles(s2, b, ctx->stream, PREC_FP64);
+      double v = 0.0;
+      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &v);
+      out[centre ? j - 1 : sys_len] = v;
+    }
+  }
+  // env side
+  cross_terms(ctx, d, sys_len, psi, OP_S, false, -std::sqrt(3.0), [&](int, int j, cplx v) {
+    out[(j <= env_len ? nsite + 1 - j : nsite - env_len) - 1] = v.real();
+  }, r);
 }
 
 // One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
@@ -2100,6 +2255,14 @@ int hh_driver_pair_cross(hh_driver* h, int sys_len, const double* psi, int nbond
   return guard(h->d.ctx, [&] {
     need(psi && p_out && (nbond == 0 || bonds), "null argument");
     pair_cross(h->d.ctx, h->d, sys_len, (const double2*)psi, nbond, bonds, p_out);
+  });
+}
+
+int hh_driver_spin_row(hh_driver* h, int sys_len, int site, const double* psi, double* ss_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && ss_out, "null argument");
+    spin_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, ss_out);
   });
 }
 

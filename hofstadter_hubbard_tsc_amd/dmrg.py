@@ -507,6 +507,16 @@ class Driver:
                      "hh_driver_pair_cross")
         return out[..., 0] + 1j * out[..., 1]
 
+    def spin_row(self, sys_len, site, psi):
+        """<S_site . S_j> for every lattice position j (index j-1; NaN at j = site),
+        site on the sys side of the cut (1 .. sys_len + 1)."""
+        _sync()
+        out = np.zeros(self.ctx.model.Lx * self.ctx.model.Ly)
+        self.ctx._rc(L.load().hh_driver_spin_row(self.ptr, sys_len, site, _dptr(psi),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_spin_row")
+        return out
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

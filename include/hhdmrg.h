@@ -248,6 +248,13 @@ int hh_driver_density_row(hh_driver* d, int sys_len, int site, const double* psi
  * update_trun_ndia).  Returns the physical <S_i . S_j>.  ss_out as in
  * hh_driver_density_cross. */
 int hh_driver_spin_cross(hh_driver* d, int sys_len, const double* psi_dev, double* ss_out);
+/* one row of Get_density_cor (code/correlation.f90:43-195) with oper1 =
+ * oper2 = st_sd: <S_site . S_j> for every lattice position j (ss_out[j-1];
+ * NaN at j = site), row site on the sys side (1 .. sys_len + 1).  Block
+ * pairs use the product S_r . S_j carried through the truncations (Get_sys2b,
+ * :1-40); a block site with the sys centre site is sys_block_site_cor_ndia
+ * (:1335-1489); the env side as in hh_driver_spin_cross. */
+int hh_driver_spin_row(hh_driver* d, int sys_len, int site, const double* psi_dev, double* ss_out);
 /* single-particle correlations sum_sigma <c^dag_i,sigma c_j,sigma> across the
  * superblock's centre: Get_density_cor (code/correlation.f90:43-195) with
  * st_elec_up / st_elec_down and flag 'F', its pairs across the centre, with

@@ -30,6 +30,12 @@ correlations <Delta^dag_a Delta_b> of bonds in the two blocks
 (Get_mSCOP_Operator_cor0's block-block part), with each pair creator formed
 as the Edmonds rank-0 product of the carried c^dag and the new site's c^dag.
 
+``spin_row`` gives one row <S_r . S_j> over every j: block pairs through the
+carried products S_r . S_j (the Edmonds rank-0 product of the carried spin
+and the new site's spin), the block site with the system centre through the
+same product in the superblock's enlarged basis, the environment side through
+``spin_cross``.
+
 ``density_row`` gives one row <n_r n_j> of the same correlation for a row
 site r on the system side and every j, with the two-site products n_r n_j of
 block sites carried through the truncations (``tracked_pairs``).
@@ -430,4 +436,78 @@ def pair_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env
             Ue = dmrg.embed_block_op(Pe[be], ebl, H.e.basis, "env")
             U = {(y, yp): np.conj(m).T for (yp, y), m in Ue.items()}
             out[a, b] = complex(np.vdot(psi, _cross_apply(H, T, U, 0, 1.0, psi)))
+    return out
+
+
+def tracked_spin_products(model, blocks, truns, length, r, side="sys"):
+    """{j: Q_rj} with Q_rj = S_r . S_j = -sqrt(3) [S_a x S_b]^0 (a < b the two
+    sites) in the basis of blocks[length], formed as the Edmonds rank-0
+    product of the carried spin of the earlier site and the new site's spin
+    in the enlarged basis of the step where the later site joins
+    (Get_sys2b), then carried as rank-0 operators.  Also returns the carried
+    spins {k: S_k}.  r = 0 forms no products."""
+    st = dmrg.site_sectors(model.kind)
+    S = dmrg.site_ops(model.kind)["S"]
+    spins = {1: S}
+    prods = {}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        site = dmrg.embed_site_op(S, prev, eb, side)
+        newp = {j: _transform(dmrg.embed_block_op(Q, prev, eb, side), truns[l]) for j, Q in prods.items()}
+        created = []
+        if l == r:
+            created = [(j, spins[j]) for j in range(1, r)]
+        elif 1 <= r < l:
+            created = [(l, spins[r])]
+        for j, Sj in created:
+            Q = _couple0(dmrg.embed_block_op(Sj, prev, eb, side), site, eb, 2)
+            newp[j] = _transform({k: -math.sqrt(3.0) * v for k, v in Q.items()}, truns[l])
+        spins = {k: _transform_any(dmrg.embed_block_op(o, prev, eb, side), truns[l], 2) for k, o in spins.items()}
+        spins[l] = _transform_any(site, truns[l], 2)
+        prods = newp
+    return prods, spins
+
+
+def spin_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H, r):
+    """<S_r . S_j> for every lattice position j (index j-1, NaN at j = r),
+    row site r on the system side: one row of ``Get_density_cor`` with the
+    spin.  Block pairs: Tr(rho_b Q_b) / sqrt(2 S_b + 1) with the carried
+    products; block site with the system centre: the enlarged-basis product
+    -sqrt(3) [S_r x S_centre]^0 read off psi; environment side:
+    ``spin_cross``."""
+    nsite = model.nsite
+    out = np.full(nsite, np.nan)
+    cc = spin_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H)
+    out[sys_len + 1:] = cc[r - 1, sys_len + 1:]
+    prods, spins = tracked_spin_products(model, sys_blocks, sys_truns, sys_len, r if r <= sys_len else 0)
+    sbl = sys_blocks[sys_len]
+    eb = H.s.basis
+    S = dmrg.site_ops(model.kind)["S"]
+    site = dmrg.embed_site_op(S, sbl, eb, "sys")
+    pin = dmrg.psi_blocks(H.sb, psi)
+
+    def with_centre(Sj):
+        Q = _couple0(dmrg.embed_block_op(Sj, sbl, eb, "sys"), site, eb, 2)
+        v = 0.0
+        for i, (x, _) in enumerate(H.sb.pairs):
+            m = Q.get((x, x))
+            if m is not None:
+                v += -math.sqrt(3.0) * float(np.real(np.vdot(pin[i], m @ pin[i]))) / math.sqrt(eb.sectors[x][1] + 1)
+        return v
+
+    if r == sys_len + 1:
+        for j in range(1, sys_len + 1):
+            out[j - 1] = with_centre(spins[j])
+        return out
+    out[sys_len] = with_centre(spins[r])
+    rho, _ = _block_rho(H, psi, "sys")
+    secs = sbl.sectors
+    for j, Q in prods.items():
+        v = 0.0
+        for b, rb in rho.items():
+            m = Q.blk.get((b, b))
+            if m is not None:
+                v += float(np.real(np.trace(rb @ m))) / math.sqrt(secs[b][1] + 1)
+        out[j - 1] = v
     return out

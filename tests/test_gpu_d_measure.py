@@ -47,7 +47,9 @@ def _gpu_profile(ctx, m, kept, tol):
     psi = dev(dmrg.start_vector(sb.dim, 5))
     E, it = ctx.lanczos(sb, psi, 1e-13, 400)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 2, L + 1)}
-    return E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi), drv.hopping_cross(L, psi)
+    srows = {r: drv.spin_row(L, r, psi) for r in (1, 2, L + 1)}
+    return (E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi), drv.hopping_cross(L, psi),
+            srows)
 
 
 @pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
@@ -58,7 +60,7 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
     _, ss = ed.singlet_ground_spin_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
     _, G = ed.singlet_ground_hopping(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
-    E, prof, cc, rows, sc, hc = _gpu_profile(ctx, m, 2000, 1e-12)
+    E, prof, cc, rows, sc, hc, srows = _gpu_profile(ctx, m, 2000, 1e-12)
     assert abs(E - e0) < 1e-10 * abs(e0)
     assert abs(prof.sum() - m.n_elec) < 1e-10
     assert np.abs(prof - dens).max() < 1e-6, (prof, dens)
@@ -71,6 +73,10 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     assert np.array_equal(sc[x], sc.T[x]) and np.isnan(sc[np.ix_(left, left)]).all()
     assert np.abs(hc[x] - G[x]).max() < 1e-6, (hc[x], G[x])
     assert np.array_equal(hc.T[x], np.conj(hc[x]))
+    for r, row in srows.items():   # spin rows: carried products, within-block term, env side
+        others = [j for j in range(m.nsite) if j != r - 1]
+        assert np.isnan(row[r - 1])
+        assert np.abs(row[others] - ss[r - 1, others]).max() < 1e-6, (r, row, ss[r - 1])
     for r, row in rows.items():   # block-block (carried products), block-centre, env side
         others = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
@@ -102,6 +108,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     sc = drv.spin_cross(L, psi)
     hc = drv.hopping_cross(L, psi)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 3, L + 1)}
+    srows = {r: drv.spin_row(L, r, psi) for r in (2, L, L + 1)}
     assert abs(prof.sum() - m.n_elec) < 1e-9
     sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
     env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
@@ -124,6 +131,11 @@ def test_density_truncated_matches_oracle_chain(ctx):
     fin = ~np.isnan(href)
     assert np.array_equal(fin, ~np.isnan(hc))
     assert np.abs(hc[fin] - href[fin]).max() < 1e-10, np.abs(hc[fin] - href[fin]).max()
+    for r, row in srows.items():
+        rref = measure.spin_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
+        ok = ~np.isnan(rref)
+        assert np.array_equal(np.isnan(row), np.isnan(rref))
+        assert np.abs(row[ok] - rref[ok]).max() < 1e-10, (r, np.abs(row[ok] - rref[ok]).max())
     for r, row in rows.items():
         rref = measure.density_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
         assert np.array_equal(np.isnan(row), np.isnan(rref))
