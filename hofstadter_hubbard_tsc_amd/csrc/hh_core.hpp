@@ -302,9 +302,12 @@ TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& p
                               const std::vector<KeptSector>& kept, const std::vector<PairSrc>& srcs, int side);
 // SPIN_PRODUCT k is S_k . S_new = -sqrt(3) [S_k x S_new]^0 of carried spin k
 // (slot B_ENV, layout make_layout(prev, 2, 0), k-th operator) and the new
-// site's spin (the exchange bond of Get_hamiltonian_ndia).
+// site's spin (the exchange bond of Get_hamiltonian_ndia).  HOP_PRODUCT k is
+// sum_sigma c^dag_k c_new = sqrt(2) [c^dag_k x c~_new]^0 of carried c^dag k
+// (slot B_ENV, layout make_layout(prev, 1, 1)) and the new site's
+// annihilator (the hopping bond, sys side).
 struct TrackSrc {
-  enum What { CARRY, SITE, PRODUCT, SPIN_PRODUCT } what;
+  enum What { CARRY, SITE, PRODUCT, SPIN_PRODUCT, HOP_PRODUCT } what;
   int k;
 };
 TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,

@@ -343,7 +343,7 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
   const std::vector<int64_t> ooff = diag_op_offsets(prev), noff = diag_op_offsets(nsec);
   P.old_size = diag_op_size(prev);
   P.new_size = diag_op_size(nsec);
-  const OpLayout sl = make_layout(prev, 2, 0);
+  const OpLayout sl = make_layout(prev, 2, 0), cl = make_layout(prev, 1, 1);
   const int nst = (int)st.size();
   Collector s1, s2;
   for (size_t q = 0; q < srcs.size(); ++q) {
@@ -355,28 +355,33 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
       P.x_size += (int64_t)da * kp;
       auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, da, d, kp}; };
       for (auto& ss : eb.subs[a]) s1.ensure(rows(ss.pos, ss.dim));
-      if (src.what == TrackSrc::SPIN_PRODUCT) {
-        // -sqrt(3) [S_k(block) x S(site)]^0 on (b', s') <- (b, s), same J
-        for (auto& ssb : sl.subs)
+      if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT) {
+        // -sqrt(3) [S_k(block) x S(site)]^0 or sqrt(2) [c^dag_k(block) x
+        // c~(site)]^0 on (b', s') <- (b, s), same J (the bond terms of growth)
+        const bool hop = src.what == TrackSrc::HOP_PRODUCT;
+        const OpLayout& ly = hop ? cl : sl;
+        const int k2 = hop ? 1 : 2;
+        for (auto& ssb : ly.subs)
           for (auto& ss : eb.subs[a]) {
             if (ss.b != ssb.ket) continue;
             for (int sp = 0; sp < nst; ++sp) {
-              const cplx sv = site_val(m.kind, OP_S, sp, ss.s);
+              const cplx sv = hop ? site_tilde(m.kind, st, sp, ss.s) : site_val(m.kind, OP_S, sp, ss.s);
               if (sv == 0.0) continue;
               int pos_out = -1, d_out = 0;
               for (auto& t : eb.subs[a])
                 if (t.b == ssb.bra && t.s == sp) { pos_out = t.pos; d_out = t.dim; }
               if (pos_out < 0) continue;
-              const double c = x12(eb.bl[ssb.bra].S2, eb.bl[ssb.ket].S2, st[sp].S2, st[ss.s].S2, J2, 2);
+              double c = x12(eb.bl[ssb.bra].S2, eb.bl[ssb.ket].S2, st[sp].S2, st[ss.s].S2, J2, k2);
               if (c == 0.0) continue;
-              const cplx coef = -std::sqrt(3.0) * c * sv * std::sqrt((double)J2 + 1);
-              s1.add(rows(pos_out, d_out), mk(coef, B_ENV, (int64_t)src.k * sl.size + ssb.off, ssb.rows, OP_NN, B_AUX,
-                                              ks.u_off + ss.pos, ks.u_ld, OP_NN, ss.dim));
+              if (hop) c *= nsign(eb.bl[ss.b].N);
+              const cplx coef = (hop ? std::sqrt(2.0) : -std::sqrt(3.0)) * c * sv * std::sqrt((double)J2 + 1);
+              s1.add(rows(pos_out, d_out), mk(coef, B_ENV, (int64_t)src.k * ly.size + ssb.off, ssb.rows, OP_NN,
+                                              B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, ss.dim));
             }
           }
       }
       for (auto& ss : eb.subs[a]) {
-        if (src.what == TrackSrc::SPIN_PRODUCT) break;
+        if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT) break;
         const int mb = ss.dim;
         const double ns = (double)st[ss.s].N;
         if (src.what == TrackSrc::SITE) {

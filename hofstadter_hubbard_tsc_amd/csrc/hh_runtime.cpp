@@ -1428,31 +1428,37 @@ void pair_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int nb,
   }
 }
 
-// One row of Get_density_cor with oper1 = oper2 = the spin: <S_r . S_j> for a
-// row site r on the sys side (1 .. sys_len + 1) and every other site j.
-// Block pairs use the scalar products S_r . S_j formed when their later site
-// joins (Get_sys2b) and carried as rank-0 operators; a block site with the
-// sys centre site is the within-block exchange term (sys_block_site_cor_ndia);
-// the env side is spin_cross's term for i = r.  out has nsite values,
-// out[r-1] = NaN.
-void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, double* out) {
+// One row of Get_density_cor for the spin (kind OP_S: <S_r . S_j>) or the
+// one-body correlation (OP_CDAG: sum_sigma <c^dag_r c_j>), for a row site r
+// on the sys side (1 .. sys_len + 1) and every other site j.  Block pairs
+// use the scalar products formed when their later site joins (Get_sys2b:
+// S_a . S_b, or sum_sigma c^dag_a c_b with a < b) and carried as rank-0
+// operators; a block site with the sys centre site is the within-block bond
+// term (sys_block_site_cor_ndia); the env side is the cross term for i = r.
+// out has nsite complex values, out[r-1] = NaN.
+void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, int kind, cplx* out) {
   const ModelParams& m = ctx->model;
   const int nsite = m.nsite();
   const int env_len = nsite - sys_len - 2;
   need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
   need(r >= 1 && r <= sys_len + 1, "the row site must lie on the sys side of the cut (1 .. sys_len + 1)");
   need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
-  for (int j = 0; j < nsite; ++j) out[j] = std::numeric_limits<double>::quiet_NaN();
+  for (int l = 2; l <= sys_len; ++l) need(d.syst.count(l), "the driver holds no truncation for an intermediate length");
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int j = 0; j < nsite; ++j) out[j] = cplx(nan, nan);
+  const bool spin = kind == OP_S;
   const bool centre = r == sys_len + 1;
-  // spins of every site (C) and the products S_r . S_j (T), stepped together
+  const int k2 = op_k2(kind), dN = op_dN(kind);
+  const cplx amp = spin ? cplx(-std::sqrt(3.0)) : cplx(std::sqrt(2.0));
+  // the operator of every site (C) and the products with site r (T), stepped together
   TrackedRank C;
   C.sec = d.sysb.at(1)->d.sec;
-  C.lay = make_layout(C.sec, 2, 0);
+  C.lay = make_layout(C.sec, k2, dN);
   C.n = 1;
   {
     std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
     for (auto& sub : C.lay.subs) {
-      const cplx v = site_val(m.kind, OP_S, sub.bra, sub.ket);
+      const cplx v = site_val(m.kind, kind, sub.bra, sub.ket);
       h[sub.off] = make_double2(v.real(), v.imag());
     }
     C.ops.alloc(sizeof(double2) * h.size());
@@ -1470,22 +1476,23 @@ void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, do
     launch_tiles(s2, b, ctx->stream, PREC_FP64);
     check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
   };
+  const TrackSrc::What prod = spin ? TrackSrc::SPIN_PRODUCT : TrackSrc::HOP_PRODUCT;
   for (int l = 2; l <= sys_len; ++l) {
     const hh_trunc* t = d.syst.at(l).get();
     if (!centre) {
       std::vector<TrackSrc> srcs;
-      std::vector<std::pair<int, int>> nl;
+      std::vector<std::pair<int, int>> nl;   // (earlier site, later site)
       for (size_t q = 0; q < T.lbl.size(); ++q) {
         srcs.push_back({TrackSrc::CARRY, (int)q});
         nl.push_back(T.lbl[q]);
       }
       if (l == r)
         for (int j = 1; j < r; ++j) {
-          srcs.push_back({TrackSrc::SPIN_PRODUCT, j - 1});
+          srcs.push_back({prod, j - 1});
           nl.push_back({j, r});
         }
       if (l > r) {
-        srcs.push_back({TrackSrc::SPIN_PRODUCT, r - 1});
+        srcs.push_back({prod, r - 1});
         nl.push_back({r, l});
       }
       TrackPlan P = compile_track_ops(m, T.sec, t->kept, srcs);
@@ -1502,7 +1509,7 @@ void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, do
       std::swap(T.ops.bytes, nxt.bytes);
       T.lbl = nl;
     }
-    TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_S, 0);
+    TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, kind, 0);
     nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
     X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
     BufPtrs c{};
@@ -1515,29 +1522,40 @@ void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, do
     std::swap(C.ops.bytes, nxt.bytes);
     C.sec.clear();
     for (auto& k : t->kept) C.sec.push_back({Q.eb.sec[k.a].N, Q.eb.sec[k.a].S2, k.keep});
-    C.lay = make_layout(C.sec, 2, 0);
+    C.lay = make_layout(C.sec, k2, dN);
     C.n += 1;
     T.sec = C.sec;
   }
   T.osz = diag_op_size(T.sec);
   std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
   const SuperBasis& S = sb->plan.sb;
-  // pairs inside the sys block: Re <Q_rj, rho'(psi, psi)>
+  DevMem ipsi;
+  ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+  dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, S.dim, ctx->stream);
+  // pairs inside the sys block: <Q> = Re <Q, rho'> + i Re <Q, i rho'>
   if (!centre && !T.lbl.empty()) {
-    DevMem rho;
+    DevMem rho, irho;
     rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
+    irho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
     BufPtrs b{};
     b.p[B_IN] = (double2*)psi;
     b.p[B_OUT] = rho.c();
     launch_on(ctx, compile_block_rho(S, 0, sb->plan.sbs, T.sec), b);
-    std::vector<double> h(T.lbl.size());
-    mdots_to_host(ctx, T.ops.c(), T.osz, (int)T.lbl.size(), rho.c(), T.osz, h.data());
-    for (size_t q = 0; q < T.lbl.size(); ++q) {
+    dev_cscale_copy(irho.c(), rho.c(), 0.0, 1.0, T.osz, ctx->stream);
+    const int nq = (int)T.lbl.size();
+    std::vector<double> hr(nq), hi(nq);
+    mdots_to_host(ctx, T.ops.c(), T.osz, nq, rho.c(), T.osz, hr.data());
+    mdots_to_host(ctx, T.ops.c(), T.osz, nq, irho.c(), T.osz, hi.data());
+    for (int q = 0; q < nq; ++q) {
       const auto& lb = T.lbl[q];
-      out[(lb.first == r ? lb.second : lb.first) - 1] = h[q];
+      // the product is O_a O_b (a < b): the row entry of j < r is its conjugate
+      const cplx v(hr[q], hi[q]);
+      if (lb.first == r) out[lb.second - 1] = v;
+      else out[lb.first - 1] = std::conj(v);
     }
   }
-  // block site with the sys centre site: the within-block exchange term
+  // block site with the sys centre site: the within-block bond term, block
+  // operator first (O_j O_centre)
   {
     BlockDesc sd, ed;
     sd.side = 0;
@@ -1552,9 +1570,10 @@ void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, do
       if (!centre && j != r) continue;
       MeasureTerm t;
       t.within = true;
-      t.amp = -std::sqrt(3.0);
+      t.amp = amp;
       t.s.block = true;
-      t.s.kind = t.e.kind = OP_S;
+      t.s.kind = t.e.kind = kind;
+      t.e.tilde = !spin;
       t.s.buf = B_SYS;
       t.s.base = (int64_t)(j - 1) * C.lay.size;
       t.s.lay = &C.lay;
@@ -1571,15 +1590,24 @@ void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, do
       s2.upload(P.stage2, ctx->stream);
       launch_tiles(s1, b, ctx->stream, PREC_FP64);
       launch_tiles(s2, b, ctx->stream, PREC_FP64);
-      double v = 0.0;
-      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &v);
-      out[centre ? j - 1 : sys_len] = v;
+      double re = 0.0, im = 0.0;
+      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &re);
+      mdots_to_host(ctx, ipsi.c(), 0, 1, Y.c(), P.sb.dim, &im);
+      if (centre) out[j - 1] = std::conj(cplx(re, im));
+      else out[sys_len] = cplx(re, im);
     }
   }
   // env side
-  cross_terms(ctx, d, sys_len, psi, OP_S, false, -std::sqrt(3.0), [&](int, int j, cplx v) {
-    out[(j <= env_len ? nsite + 1 - j : nsite - env_len) - 1] = v.real();
+  cross_terms(ctx, d, sys_len, psi, kind, !spin, amp, [&](int, int j, cplx v) {
+    out[(j <= env_len ? nsite + 1 - j : nsite - env_len) - 1] = v;
   }, r);
+}
+
+void spin_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, double* out) {
+  const int nsite = ctx->model.nsite();
+  std::vector<cplx> v(nsite);
+  corr_row(ctx, d, sys_len, r, psi, OP_S, v.data());
+  for (int j = 0; j < nsite; ++j) out[j] = v[j].real();
 }
 
 // One row of Get_density_cor (code/correlation.f90:43-195): <n_r n_j> for a
@@ -2263,6 +2291,14 @@ int hh_driver_spin_row(hh_driver* h, int sys_len, int site, const double* psi, d
   return guard(h->d.ctx, [&] {
     need(psi && ss_out, "null argument");
     spin_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, ss_out);
+  });
+}
+
+int hh_driver_hopping_row(hh_driver* h, int sys_len, int site, const double* psi, double* g_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && g_out, "null argument");
+    corr_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, OP_CDAG, (cplx*)g_out);
   });
 }
 

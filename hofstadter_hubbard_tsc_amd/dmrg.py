@@ -517,6 +517,16 @@ class Driver:
                      "hh_driver_spin_row")
         return out
 
+    def hopping_row(self, sys_len, site, psi):
+        """sum_sigma <c^dag_site c_j> for every lattice position j (complex, index
+        j-1; NaN at j = site), site on the sys side (1 .. sys_len + 1)."""
+        _sync()
+        out = np.zeros((self.ctx.model.Lx * self.ctx.model.Ly, 2))
+        self.ctx._rc(L.load().hh_driver_hopping_row(self.ptr, sys_len, site, _dptr(psi),
+                                                    out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_hopping_row")
+        return out[:, 0] + 1j * out[:, 1]
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

@@ -263,6 +263,12 @@ int hh_driver_spin_row(hh_driver* d, int sys_len, int site, const double* psi_de
  * complex (re, im interleaved), row-major by lattice position - 1; the entry
  * (j, i) holds the conjugate; pairs on one side are NaN. */
 int hh_driver_hopping_cross(hh_driver* d, int sys_len, const double* psi_dev, double* g_out);
+/* one row of the same correlation: sum_sigma <c^dag_site,sigma c_j,sigma>
+ * for every lattice position j (complex, g_out[2(j-1)], g_out[2(j-1)+1];
+ * NaN at j = site), row site on the sys side (1 .. sys_len + 1); block pairs
+ * through the carried two-site product (Get_sys2b), block-centre as
+ * sys_block_site_cor_ndia, env side as hh_driver_hopping_cross. */
+int hh_driver_hopping_row(hh_driver* d, int sys_len, int site, const double* psi_dev, double* g_out);
 /* singlet pair correlations <Delta^dag_a Delta_b>, Delta^dag_ij =
  * (c^dag_i,up c^dag_j,down - c^dag_i,down c^dag_j,up)/sqrt(2), for bonds a
  * inside the sys block and b inside the env block: the block-block part of

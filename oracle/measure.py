@@ -36,6 +36,9 @@ and the new site's spin), the block site with the system centre through the
 same product in the superblock's enlarged basis, the environment side through
 ``spin_cross``.
 
+``hopping_row`` gives one row sum_sigma <c^dag_r c_j> the same way, with
+the fermionic products sqrt(2) [c^dag_a x c~_b]^0.
+
 ``density_row`` gives one row <n_r n_j> of the same correlation for a row
 site r on the system side and every j, with the two-site products n_r n_j of
 block sites carried through the truncations (``tracked_pairs``).
@@ -510,4 +513,74 @@ def spin_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_l
             if m is not None:
                 v += float(np.real(np.trace(rb @ m))) / math.sqrt(secs[b][1] + 1)
         out[j - 1] = v
+    return out
+
+
+def hopping_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H, r):
+    """sum_sigma <c^dag_r c_j> for every lattice position j (complex, index
+    j-1, NaN at j = r), row site r on the system side.  Block pairs a < b:
+    sum_sigma c^dag_a c_b = sqrt(2) [c^dag_a x c~_b]^0 formed as the Edmonds
+    rank-0 product of the carried c^dag_a and the new site's annihilator
+    partner when b joins, carried as a rank-0 operator (the row entry of
+    j < r is the conjugate); block site with the system centre: the same
+    product in the superblock's enlarged basis; environment side:
+    ``hopping_cross``."""
+    nsite = model.nsite
+    out = np.full(nsite, np.nan, dtype=complex)
+    cc = hopping_cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H)
+    out[sys_len + 1:] = cc[r - 1, sys_len + 1:]
+    st = dmrg.site_sectors(model.kind)
+    C = dmrg.site_ops(model.kind)["cdag"]
+    cd = {1: C}
+    prods = {}
+    for l in range(2, sys_len + 1):
+        prev = sys_blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        site = dmrg.embed_site_op(C, prev, eb, "sys")
+        site_t = dmrg._tilde_dict(site, eb.sectors)
+        newp = {k: _transform(dmrg.embed_block_op(Q, prev, eb, "sys"), sys_truns[l]) for k, Q in prods.items()}
+        created = []
+        if r <= sys_len:
+            if l == r:
+                created = [((j, r), cd[j]) for j in range(1, r)]
+            elif r < l:
+                created = [((r, l), cd[r])]
+        for key, Ca in created:
+            Q = _couple0(dmrg.embed_block_op(Ca, prev, eb, "sys"), site_t, eb, 1)
+            newp[key] = _transform({k: math.sqrt(2.0) * v for k, v in Q.items()}, sys_truns[l])
+        cd = {k: _transform_any(dmrg.embed_block_op(o, prev, eb, "sys"), sys_truns[l], 1, 1, True)
+              for k, o in cd.items()}
+        cd[l] = _transform_any(site, sys_truns[l], 1, 1, True)
+        prods = newp
+    sbl = sys_blocks[sys_len]
+    eb = H.s.basis
+    site_t = dmrg._tilde_dict(dmrg.embed_site_op(C, sbl, eb, "sys"), eb.sectors)
+    pin = dmrg.psi_blocks(H.sb, psi)
+
+    def with_centre(Ca):
+        Q = _couple0(dmrg.embed_block_op(Ca, sbl, eb, "sys"), site_t, eb, 1)
+        v = 0.0
+        for i, (x, _) in enumerate(H.sb.pairs):
+            m = Q.get((x, x))
+            if m is not None:
+                v += math.sqrt(2.0) * complex(np.vdot(pin[i], m @ pin[i])) / math.sqrt(eb.sectors[x][1] + 1)
+        return v
+
+    if r == sys_len + 1:
+        for j in range(1, sys_len + 1):
+            out[j - 1] = np.conj(with_centre(cd[j]))
+        return out
+    out[sys_len] = with_centre(cd[r])
+    rho, _ = _block_rho(H, psi, "sys")
+    secs = sbl.sectors
+    for (a, b), Q in prods.items():
+        v = 0.0
+        for blk, rb in rho.items():
+            m = Q.blk.get((blk, blk))
+            if m is not None:
+                v += complex(np.trace(rb @ m)) / math.sqrt(secs[blk][1] + 1)
+        if a == r:
+            out[b - 1] = v
+        else:
+            out[a - 1] = np.conj(v)
     return out

@@ -48,8 +48,9 @@ def _gpu_profile(ctx, m, kept, tol):
     E, it = ctx.lanczos(sb, psi, 1e-13, 400)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 2, L + 1)}
     srows = {r: drv.spin_row(L, r, psi) for r in (1, 2, L + 1)}
+    hrows = {r: drv.hopping_row(L, r, psi) for r in (1, 3, L + 1)}
     return (E, drv.density(L, psi), drv.density_cross(L, psi), rows, drv.spin_cross(L, psi), drv.hopping_cross(L, psi),
-            srows)
+            srows, hrows)
 
 
 @pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
@@ -60,7 +61,7 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
     e0, dens, nn = ed.singlet_ground_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
     _, ss = ed.singlet_ground_spin_correlations(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
     _, G = ed.singlet_ground_hopping(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
-    E, prof, cc, rows, sc, hc, srows = _gpu_profile(ctx, m, 2000, 1e-12)
+    E, prof, cc, rows, sc, hc, srows, hrows = _gpu_profile(ctx, m, 2000, 1e-12)
     assert abs(E - e0) < 1e-10 * abs(e0)
     assert abs(prof.sum() - m.n_elec) < 1e-10
     assert np.abs(prof - dens).max() < 1e-6, (prof, dens)
@@ -77,6 +78,10 @@ def test_density_untruncated_equals_ed(ctx, kind, flux):
         others = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
         assert np.abs(row[others] - ss[r - 1, others]).max() < 1e-6, (r, row, ss[r - 1])
+    for r, row in hrows.items():   # one-body rows: carried fermionic products, within-block hop, env side
+        others = [j for j in range(m.nsite) if j != r - 1]
+        assert np.isnan(row[r - 1])
+        assert np.abs(row[others] - G[r - 1, others]).max() < 1e-6, (r, row, G[r - 1])
     for r, row in rows.items():   # block-block (carried products), block-centre, env side
         others = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
@@ -109,6 +114,7 @@ def test_density_truncated_matches_oracle_chain(ctx):
     hc = drv.hopping_cross(L, psi)
     rows = {r: drv.density_row(L, r, psi) for r in (1, 3, L + 1)}
     srows = {r: drv.spin_row(L, r, psi) for r in (2, L, L + 1)}
+    hrows = {r: drv.hopping_row(L, r, psi) for r in (1, L - 1, L + 1)}
     assert abs(prof.sum() - m.n_elec) < 1e-9
     sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
     env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
@@ -133,6 +139,11 @@ def test_density_truncated_matches_oracle_chain(ctx):
     assert np.abs(hc[fin] - href[fin]).max() < 1e-10, np.abs(hc[fin] - href[fin]).max()
     for r, row in srows.items():
         rref = measure.spin_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
+        ok = ~np.isnan(rref)
+        assert np.array_equal(np.isnan(row), np.isnan(rref))
+        assert np.abs(row[ok] - rref[ok]).max() < 1e-10, (r, np.abs(row[ok] - rref[ok]).max())
+    for r, row in hrows.items():
+        rref = measure.hopping_row(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, r)
         ok = ~np.isnan(rref)
         assert np.array_equal(np.isnan(row), np.isnan(rref))
         assert np.abs(row[ok] - rref[ok]).max() < 1e-10, (r, np.abs(row[ok] - rref[ok]).max())

@@ -294,3 +294,26 @@ def test_untruncated_spin_rows_equal_ed(kind, flux):
         o = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
         assert np.abs(row[o] - ss[r - 1, o]).max() < 1e-6, r
+
+
+@pytest.mark.parametrize("kind,flux", [("hubbard", 0.15), ("tj", 0.2)])
+def test_untruncated_hopping_rows_equal_ed(kind, flux):
+    """Every row sum_sigma <c^dag_r c_j> (block pairs through the carried
+    fermionic products, block-centre, env side) equals the exact one-body
+    correlation, complex with flux."""
+    m = lat.Model(kind=kind, Lx=4, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    e0, G = ed.singlet_ground_hopping(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    run = driver.Run(m, driver.Schedule(kept_max=2000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    for r in range(1, L + 2):
+        row = measure.hopping_row(m, run.sys_blocks, run.env_blocks, run.sys_trun, run.env_trun, L, R, psi, H, r)
+        o = [j for j in range(m.nsite) if j != r - 1]
+        assert np.isnan(row[r - 1])
+        assert np.abs(row[o] - G[r - 1, o]).max() < 1e-6, r
