@@ -17,6 +17,12 @@
 
 #include "hh_engine.hpp"
 
+// edge work items run the narrow pipelines when more than this share of a
+// launch's MFMA work would fall on empty 16 x 16 sub-tiles
+#ifndef HH_EDGE_SHARE
+#define HH_EDGE_SHARE 0.2
+#endif
+
 namespace hh {
 
 void check_hip(hipError_t e, const char* what) {
@@ -133,16 +139,28 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
       all += it.first * full;
       empty += it.first * (full - used);
     }
-    edge_skip = all > 0.0 && empty > 0.2 * all;
+    edge_skip = all > 0.0 && empty > HH_EDGE_SHARE * all;
   }
-  // longest work items first (LPT)
+  // longest work items first (LPT) inside each shape class
+  auto cls = [&](const WorkItem& it) {
+    if (!edge_skip) return 0;
+    const Tile& t = tl.tiles[it.tile];
+    return (t.M - it.r0 <= 16 ? 1 : 0) | (t.N - it.c0 <= 16 ? 2 : 0);
+  };
   std::stable_sort(items.begin(), items.end(),
-                   [](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
+                   [&](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
+                     const int ca = cls(a.second), cb = cls(b.second);
+                     if (ca != cb) return ca < cb;
                      return a.first > b.first;
                    });
   std::vector<WorkItem> w;
   w.reserve(items.size());
-  for (auto& it : items) w.push_back(it.second);
+  for (int c = 0; c < 5; ++c) cls_off[c] = 0;
+  for (auto& it : items) {
+    w.push_back(it.second);
+    cls_off[cls(it.second) + 1] += 1;
+  }
+  for (int c = 1; c < 5; ++c) cls_off[c] += cls_off[c - 1];
   nwork = (int)w.size();
   nred = (int)reds.size();
   nslots = slot;
@@ -204,7 +222,7 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // other: no LDS, no barriers.  Two waves per SIMD (194 VGPRs): the register
 // double buffer of the operand pipeline does not fit three.
 // ----------------------------------------------------------------------------
-template <typename T, int TAG, typename Bufs, bool EDGE_SKIP>
+template <typename T, int TAG, typename Bufs, int NA, int NB>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
@@ -291,19 +309,21 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     o.ci = tm.ci;
     return o;
   };
+  // NA x NB of the WA x WB sub-tiles hold data (the kernels of the narrow
+  // shape classes run work items with Mv <= 16 and / or Nv <= 16: fewer loads
+  // and MFMAs, and one pipeline per kernel)
   auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], const OpAddr& o, int k0) {
     const uint32_t kc = (uint32_t)min(k0 + lk, o.K - 1);
 #pragma unroll
-    for (int a = 0; a < WA; ++a) xa[a] = o.Ab[o.pa[a] + kc * (uint32_t)o.a_ks];
+    for (int a = 0; a < NA; ++a) xa[a] = o.Ab[o.pa[a] + kc * (uint32_t)o.a_ks];
 #pragma unroll
-    for (int b = 0; b < WB; ++b) xb[b] = o.Bb[o.pb[b] + kc * (uint32_t)o.b_ks];
+    for (int b = 0; b < NB; ++b) xb[b] = o.Bb[o.pb[b] + kc * (uint32_t)o.b_ks];
   };
   auto next_gemm = [&](int ti) {
     while (ti < w.t1 && terms[ti].K == 0) ++ti;
     return ti;
   };
-  auto run = [&](auto edge_c) {
-    constexpr bool EDGE = decltype(edge_c)::value;
+  auto run = [&]() {
     int ti = next_gemm(w.t0);
     if (ti >= w.t1) return;
     OpAddr c = make_op(terms[ti]);
@@ -318,13 +338,13 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       const double kf = (kb + 4 <= c.K || kb + lk < c.K) ? 1.0 : 0.0;
       T ar[WA], ai[WA], br[WB], bi[WB];
 #pragma unroll
-      for (int a = 0; a < WA; ++a) {
+      for (int a = 0; a < NA; ++a) {
         const double xr = kf * xa[a].x, xi = kf * c.sa * xa[a].y;
         ar[a] = (T)(c.cr * xr - c.ci * xi);
         ai[a] = (T)(c.cr * xi + c.ci * xr);
       }
 #pragma unroll
-      for (int b = 0; b < WB; ++b) {
+      for (int b = 0; b < NB; ++b) {
         br[b] = (T)xb[b].x;
         bi[b] = (T)(c.sb * xb[b].y);
       }
@@ -334,10 +354,9 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       else load(ya, yb, c, kb + 4);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int a = 0; a < WA; ++a)
+      for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < WB; ++b) {
-          if (EDGE && (16 * a >= Mv || 16 * b >= Nv)) continue;   // sub-tile past the region edge
+        for (int b = 0; b < NB; ++b) {
           accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
           accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
           accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
@@ -362,8 +381,8 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       if (done) break;
     }
   };
-  if (EDGE_SKIP && (Mv < MTR || Nv < MTC)) run(std::integral_constant<bool, EDGE_SKIP>{});
-  else run(std::false_type{});
+  static_assert(NA >= 1 && NA <= WA && NB >= 1 && NB <= WB, "sub-tile counts of a shape class");
+  run();
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
@@ -417,11 +436,14 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
-template <typename T, typename Bufs, bool SKIP>
-static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  const dim3 g((t.nwork + WAVES - 1) / WAVES), blk(64 * WAVES);
+template <typename T, typename Bufs, int NA, int NB>
+static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, int c) {
+  const int n = t.cls_off[c + 1] - t.cls_off[c];
+  if (n <= 0) return;
+  const WorkItem* w = t.work + t.cls_off[c];
+  const dim3 g((n + WAVES - 1) / WAVES), blk(64 * WAVES);
 #define HH_LAUNCH(TG) \
-  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, SKIP>), g, blk, 0, s, t.tiles, t.terms, t.work, t.nwork, b, t.partial)
+  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, NA, NB>), g, blk, 0, s, t.tiles, t.terms, w, n, b, t.partial)
   if (tag == 1) HH_LAUNCH(1);
   else if (tag == 2) HH_LAUNCH(2);
   else if (tag == 3) HH_LAUNCH(3);
@@ -431,8 +453,10 @@ static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) 
 }
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  if (t.edge_skip) launch_ts<T, Bufs, true>(t, b, s, tag);
-  else launch_ts<T, Bufs, false>(t, b, s, tag);
+  launch_ts<T, Bufs, 2, 2>(t, b, s, tag, 0);
+  launch_ts<T, Bufs, 1, 2>(t, b, s, tag, 1);
+  launch_ts<T, Bufs, 2, 1>(t, b, s, tag, 2);
+  launch_ts<T, Bufs, 1, 1>(t, b, s, tag, 3);
 }
 
 template <typename Bufs>

@@ -31,7 +31,13 @@ struct DevTasks {
   double2* partial = nullptr;
   int ntiles = 0, nterms = 0, nwork = 0, nred = 0, nslots = 0;
   double flops_alg = 0.0, flops_exec = 0.0;
-  bool edge_skip = false;   // > 20 % of the MFMA work would land on empty sub-tiles
+  // work items grouped by shape class, each class run by its own kernel:
+  // 0 full 32 x 32, 1 at most 16 rows, 2 at most 16 columns, 3 both; items
+  // of class c are work[cls_off[c] .. cls_off[c + 1]).  Edge items are
+  // classed only when more than a set share of the MFMA work would fall on
+  // empty 16 x 16 sub-tiles; otherwise every item is class 0.
+  int cls_off[5] = {0, 0, 0, 0, 0};
+  bool edge_skip = false;
   void upload(const TaskList& tl, hipStream_t s);
   void release();
   ~DevTasks() { release(); }
