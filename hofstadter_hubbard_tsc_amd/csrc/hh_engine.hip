@@ -219,8 +219,9 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // grouped tile kernel: every wave owns one 32x32 macro tile and streams the
 // MFMA operands straight from HBM / L2 (lane l loads A[l%16][k] and
 // B[k][l%16] for its k = l/16 of each k-step), so waves never wait on each
-// other: no LDS, no barriers.  Two waves per SIMD (194 VGPRs): the register
-// double buffer of the operand pipeline does not fit three.
+// other: no LDS, no barriers.  Each shape class (NA x NB of the 2 x 2
+// sub-tiles of 16 x 16 hold data) is its own instantiation with a single
+// pipeline, so no wave carries the code or registers of another shape.
 // ----------------------------------------------------------------------------
 template <typename T, int TAG, typename Bufs, int NA, int NB>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
