@@ -88,9 +88,10 @@ void lattice_bonds(const ModelParams& m, int ncol, std::map<std::pair<int, int>,
       if (x < ncol) {
         put(u, u + Ly, cplx(-m.t, 0.0));
         if (m.lattice == TRIANGULAR) {
-          if (m.flux != 0.0) throw std::runtime_error("flux on the triangular lattice is not supported");
-          if (y > 1) put(u, u + Ly - 1, cplx(-m.t, 0.0));
-          else if (Ly > 2 && m.periodic_y) put(u, u + 2 * Ly - 1, cplx(-m.t, 0.0));
+          // (x, y) <- (x+1, y-1): the same gauge's straight-line phase, x + 1/2 along the bond
+          const cplx hd = -m.t * std::exp(cplx(0.0, -tw * (x + 0.5)));
+          if (y > 1) put(u, u + Ly - 1, hd);
+          else if (Ly > 2 && m.periodic_y) put(u, u + 2 * Ly - 1, hd);
         }
       }
     }

@@ -1794,7 +1794,7 @@ int hh_set_model(hh_ctx* ctx, const hh_model_params* p) {
     need(p->kind == HUBBARD || p->kind == TJ, "model kind must be 0 (Hubbard) or 1 (t-J)");
     need(p->Lx >= 2 && p->Ly >= 1, "lattice must have Lx >= 2, Ly >= 1");
     need(p->n_elec >= 0 && p->n_elec <= 2 * p->Lx * p->Ly && p->n_elec % 2 == 0, "n_elec must be even (S = 0)");
-    need(!(p->lattice == TRIANGULAR && p->flux != 0.0), "flux on the triangular lattice is not supported");
+    need(p->lattice == SQUARE || p->lattice == TRIANGULAR, "lattice must be 0 (square) or 1 (triangular)");
     ModelParams m;
     m.kind = p->kind;
     m.Lx = p->Lx;

@@ -13,8 +13,12 @@ Restates the reference's lattice setup:
     triangular lattice adds the (x,y)-(x+1,y-1) diagonal (:497-505).
 Flux is the Landau gauge of the benchmark configurations: x bonds real, the
 y bond from (x,y) to (x,y+1) carries t*exp(i 2 pi Phi x) for c^dag_(x,y)
-c_(x,y+1) (x is the 1-based column).  The reference hard-codes pi/2 per
-triangle instead (``get_phase``, code/quantum.f90:293-438); its couplings are
+c_(x,y+1) (x is the 1-based column); on the triangular lattice the diagonal
+from (x+1,y-1) to (x,y) carries the same gauge's straight-line phase,
+t*exp(-i 2 pi Phi (x + 1/2)) for c^dag_(x,y) c_(x+1,y-1), so every triangle
+holds Phi/2.  The reference hard-codes its own phase pattern on the
+triangular lattice instead (``get_phase``, code/quantum.f90:293-438); its
+couplings are
 jt = -sqrt(2) t, jd = -sqrt(3) J, jn = -J/4 (code/quantum.f90:93-122), which
 are the reduced-tensor forms of the physical terms below.
 """
@@ -60,6 +64,11 @@ def lattice_bonds(model, ncol):
     def hop_y(x):
         return -model.t * cmath.exp(1j * tw * x)
 
+    def hop_d(x):
+        # (x, y) <- (x+1, y-1): the straight-line Peierls phase of the same
+        # Landau gauge, A_y = -2 pi flux x averaged over the bond (x + 1/2)
+        return -model.t * cmath.exp(-1j * tw * (x + 0.5))
+
     for x in range(1, ncol + 1):
         for y in range(1, Ly + 1):
             u = y + (x - 1) * Ly
@@ -74,12 +83,10 @@ def lattice_bonds(model, ncol):
             if x < ncol:
                 out.append((u, u + Ly, "x", complex(-model.t)))
                 if model.lattice == "triangular":
-                    if model.flux != 0.0:
-                        raise NotImplementedError("flux on the triangular lattice")
                     if y > 1:
-                        out.append((u, u + Ly - 1, "diag", complex(-model.t)))
+                        out.append((u, u + Ly - 1, "diag", hop_d(x)))
                     elif Ly > 2 and model.periodic_y:
-                        out.append((u, u + 2 * Ly - 1, "diag", complex(-model.t)))
+                        out.append((u, u + 2 * Ly - 1, "diag", hop_d(x)))
     return [(min(a, b), max(a, b), k, amp if a < b else amp.conjugate())
             for (a, b, k, amp) in out]
 

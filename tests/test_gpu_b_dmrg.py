@@ -169,3 +169,16 @@ def test_wave_transformation_predicts_ground_state(ctx):
     for x, y in zip(a, b):
         if x.phase >= 2:
             assert abs(x.iters - y.iters) <= 2, (x.sys_len, x.iters, y.iters)
+
+
+def test_untruncated_triangular_flux_run_equals_ed(ctx):
+    """Triangular 3 x 3 cylinder with flux through the whole GPU driver: every
+    final-sweep energy equals the exact ground state."""
+    m = lat.Model(kind="hubbard", Lx=3, Ly=3, lattice="triangular", U=8.0, t=1.0, flux=1.0 / 6, n_elec=6)
+    ctx.set_model(m)
+    e0 = _ed_gs(m)
+    recs = ctx.dmrg_run(kept_max=4000, n_sweeps=1, tol=1e-12)
+    fin = [r for r in recs if r.phase == 2]
+    assert fin
+    for r in fin:
+        assert abs(r.energy - e0) < 1e-10 * abs(e0), (r.sys_len, r.energy, e0)

@@ -69,3 +69,12 @@ def test_matvec_tasks_match_oracle(kind, Lx, Ly, flux, N, kept):
     run = _run_oracle(model, kept)
     for ns in range(1, model.nsite - 2):
         _check_step(model, run, ns)
+
+
+def test_matvec_tasks_match_oracle_triangular_flux():
+    """The C++ task compiler's H.psi on a triangular cylinder with flux (the
+    diagonal bonds' Landau phases) against the oracle's."""
+    model = lat.Model(kind="tj", Lx=3, Ly=3, lattice="triangular", J=0.4, t=1.0, flux=1.0 / 6, n_elec=6)
+    run = _run_oracle(model, 40)
+    for ns in range(1, model.nsite - 2):
+        _check_step(model, run, ns)

@@ -123,3 +123,17 @@ def test_start_vector_is_deterministic_and_normalised():
     assert np.array_equal(a, b)
     assert abs(np.vdot(a, a).real - 1.0) < 1e-14
     assert np.all(a.real > 0) and np.all(a.imag > 0)
+
+
+@pytest.mark.parametrize("kind,N", [("hubbard", 6), ("tj", 4)])
+def test_untruncated_triangular_flux_equals_ed(kind, N):
+    """Triangular 3 x 3 cylinder with flux (complex diagonal bonds, periodic
+    wrap): every finite-sweep step of an untruncated run equals the exact
+    ground state."""
+    m = lat.Model(kind=kind, Lx=3, Ly=3, lattice="triangular", U=8.0, J=0.4, t=1.0, flux=1.0 / 6, n_elec=N)
+    e0 = _ed_gs(m)
+    recs = driver.run(m, driver.Schedule(kept_max=4000, n_sweeps=1, tol=1e-12))
+    sweep = [r for r in recs if r.phase == "sweep1"]
+    assert sweep
+    for r in sweep:
+        assert abs(r.energy - e0) < 1e-10 * abs(e0), (r.sys_len, r.energy, e0)
