@@ -99,6 +99,8 @@ SIGNATURES = [
     ("hh_driver_density", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_density_cross", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_density_row", i32, [vp, i32, i32, vp, P(f64)]),
+    ("hh_driver_save", i32, [vp, C.c_char_p]),
+    ("hh_driver_load", i32, [vp, C.c_char_p]),
     ("hh_driver_spin_cross", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_spin_row", i32, [vp, i32, i32, vp, P(f64)]),
     ("hh_driver_hopping_cross", i32, [vp, i32, vp, P(f64)]),

@@ -7,6 +7,7 @@ runs in libhhdmrg.so on the context's stream.  There is no CPU fallback: every
 call raises when the library or the GPU is missing.
 """
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -526,6 +527,17 @@ class Driver:
                                                     out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_hopping_row")
         return out[:, 0] + 1j * out[:, 1]
+
+    def save(self, path):
+        """Write the driver's blocks, truncations, carried wave function and step
+        counter to `path` (checkpoint)."""
+        _sync()
+        self.ctx._rc(L.load().hh_driver_save(self.ptr, os.fsencode(path)), "hh_driver_save")
+
+    def load(self, path):
+        """Replace the driver's state by the checkpoint at `path`."""
+        _sync()
+        self.ctx._rc(L.load().hh_driver_load(self.ptr, os.fsencode(path)), "hh_driver_load")
 
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""

@@ -205,6 +205,17 @@ int hh_driver_warmup(hh_driver* d, hh_step_record* rec, int max_rec, int* nrec);
  * truncations stay resident on the device between calls. */
 int hh_driver_sweep(hh_driver* d, int dir, int bg, int end, double lanzero, double trun_err, int kept_max, int fresh,
                     int phase, hh_step_record* rec, int max_rec, int* nrec);
+/* checkpoint / restart: write every stored block (H and operators), every
+ * truncation, the carried wave function and the step counter of the driver
+ * to one file, and read it back into a driver of a context with the same
+ * model -- the content of model_to_disk / truns_to_disk / wave_to_disk
+ * (code/space_info.f90:1406-1762) for restart.dat's resume
+ * (code/quantum.f90:43-47, :207-253), in this library's own binary layout.
+ * After a load the next step starts from the carried wave function (as after
+ * hh_driver_reset_transform); a sweep resumed from a checkpoint written
+ * between sweeps repeats the uninterrupted run bit for bit. */
+int hh_driver_save(hh_driver* d, const char* path);
+int hh_driver_load(hh_driver* d, const char* path);
 /* forget the previous superblock (next first step does not transform) */
 int hh_driver_reset_transform(hh_driver* d);
 int hh_driver_block_count(const hh_driver* d, int side, int32_t* count);

@@ -1,0 +1,83 @@
+"""GPU checkpoint / restart of the sweep driver (the content of
+model_to_disk / truns_to_disk / wave_to_disk for restart.dat's resume): a
+sweep resumed from a checkpoint written between sweeps repeats the
+uninterrupted run bit for bit, and a checkpoint of another model or a
+damaged file is refused."""
+import numpy as np
+import pytest
+
+from oracle import driver
+from oracle import lattice as lat
+
+pytestmark = pytest.mark.gpu
+
+TRUN = driver.TRUN_ERR
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a device"
+    from hofstadter_hubbard_tsc_amd import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _prepare(ctx, m, kept):
+    from hofstadter_hubbard_tsc_amd import Driver
+    ctx.set_model(m)
+    n = m.nsite
+    drv = Driver(ctx, kept_max=kept, tol=1e-9, kept_max_warm=kept, seed=3)
+    drv.warmup()
+    drv.sweep(0, n // 2 - 1, n - 2, 1e-9, TRUN, kept, fresh=True, phase=1)
+    drv.reset_transform()
+    drv.sweep(1, n - 2, 1, 1e-9, TRUN, kept, phase=1)
+    drv.reset_transform()
+    return drv
+
+
+def test_resumed_sweep_repeats_the_run(ctx, tmp_path):
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=10)
+    n = m.nsite
+    a = _prepare(ctx, m, 24)
+    path = str(tmp_path / "run.ck")
+    a.save(path)
+    ra = a.sweep(0, 1, n - 2, 1e-9, TRUN, 24, phase=2)
+    b = Driver(ctx, kept_max=24, tol=1e-9, kept_max_warm=24, seed=99)
+    b.load(path)
+    rb = b.sweep(0, 1, n - 2, 1e-9, TRUN, 24, phase=2)
+    assert len(ra) == len(rb) > 0
+    assert any(r.num_state < r.dim for r in ra), "the case must truncate"
+    for x, y in zip(ra, rb):
+        assert (x.sys_len, x.iters, x.num_state, x.dim) == (y.sys_len, y.iters, y.num_state, y.dim)
+        assert x.energy == y.energy and x.trunc_err == y.trunc_err
+
+
+def test_checkpoint_of_another_model_or_damaged_is_refused(ctx, tmp_path):
+    from hofstadter_hubbard_tsc_amd import Driver
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    m = lat.Model(kind="tj", Lx=4, Ly=2, J=0.4, t=1.0, flux=0.2, n_elec=6)
+    a = _prepare(ctx, m, 16)
+    path = str(tmp_path / "tj.ck")
+    a.save(path)
+    ctx.set_model(lat.Model(kind="tj", Lx=4, Ly=2, J=0.5, t=1.0, flux=0.2, n_elec=6))
+    b = Driver(ctx, kept_max=16, tol=1e-9, kept_max_warm=16, seed=3)
+    with pytest.raises(HHError, match="another model"):
+        b.load(path)
+    ctx.set_model(m)
+    c = Driver(ctx, kept_max=16, tol=1e-9, kept_max_warm=16, seed=3)
+    data = open(path, "rb").read()
+    short = str(tmp_path / "short.ck")
+    open(short, "wb").write(data[: len(data) // 2])
+    with pytest.raises(HHError, match="truncated|corrupt"):
+        c.load(short)
+    bad = str(tmp_path / "bad.ck")
+    open(bad, "wb").write(b"NOTACKPT" + data[8:])
+    with pytest.raises(HHError, match="not a checkpoint"):
+        c.load(bad)
+    # a refused load leaves the driver usable: load the good file and sweep
+    c.load(path)
+    recs = c.sweep(0, 1, m.nsite - 2, 1e-9, TRUN, 16, phase=2)
+    assert len(recs) == m.nsite - 3 and np.isfinite([r.energy for r in recs]).all()
