@@ -306,10 +306,24 @@ TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& p
 // sum_sigma c^dag_k c_new = sqrt(2) [c^dag_k x c~_new]^0 of carried c^dag k
 // (slot B_ENV, layout make_layout(prev, 1, 1)) and the new site's
 // annihilator (the hopping bond, sys side).
+// QUAD_PRODUCT k is -[W_k x c~_new]^0 for carried rank-1/2 operator k (slot
+// B_ENV, layout make_layout(prev, 1, 1)); with W = P_12 c~_3 it is the
+// four-site scalar P_12 Delta_34 of Get_sys_two1.
 struct TrackSrc {
-  enum What { CARRY, SITE, PRODUCT, SPIN_PRODUCT, HOP_PRODUCT } what;
+  enum What { CARRY, SITE, PRODUCT, SPIN_PRODUCT, HOP_PRODUCT, QUAD_PRODUCT } what;
   int k;
 };
+// One step of the chain of rank-1/2, N + 1 operators W = P c~_new of
+// Get_sys_two1 (sys side): CARRY q embeds old operator q (slot B_SYS,
+// layout make_layout(prev, 1, 1)); PAIR_HOLE k multiplies old pair creator k
+// (slot B_ENV, layout make_layout(prev, 0, 2)) by the new site's
+// annihilator partner.  Output in make_layout(new, 1, 1), q * new_size.
+struct HoleSrc {
+  enum What { CARRY, PAIR_HOLE } what;
+  int k;
+};
+TrackPlan compile_track_hole(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, const std::vector<HoleSrc>& srcs);
 TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,
                             const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs);
 // rho'_b = (1/sqrt(2S_b+1)) sum over psi blocks and site states of the block

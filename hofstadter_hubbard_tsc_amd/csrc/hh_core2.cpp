@@ -355,10 +355,14 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
       P.x_size += (int64_t)da * kp;
       auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, da, d, kp}; };
       for (auto& ss : eb.subs[a]) s1.ensure(rows(ss.pos, ss.dim));
-      if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT) {
-        // -sqrt(3) [S_k(block) x S(site)]^0 or sqrt(2) [c^dag_k(block) x
-        // c~(site)]^0 on (b', s') <- (b, s), same J (the bond terms of growth)
-        const bool hop = src.what == TrackSrc::HOP_PRODUCT;
+      if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT ||
+          src.what == TrackSrc::QUAD_PRODUCT) {
+        // -sqrt(3) [S_k(block) x S(site)]^0, sqrt(2) [c^dag_k(block) x
+        // c~(site)]^0 or -[W_k(block) x c~(site)]^0 on (b', s') <- (b, s),
+        // same J (the bond terms of growth)
+        const bool hop = src.what != TrackSrc::SPIN_PRODUCT;
+        const double amp = src.what == TrackSrc::SPIN_PRODUCT ? -std::sqrt(3.0)
+                           : src.what == TrackSrc::HOP_PRODUCT ? std::sqrt(2.0) : -1.0;
         const OpLayout& ly = hop ? cl : sl;
         const int k2 = hop ? 1 : 2;
         for (auto& ssb : ly.subs)
@@ -374,14 +378,16 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
               double c = x12(eb.bl[ssb.bra].S2, eb.bl[ssb.ket].S2, st[sp].S2, st[ss.s].S2, J2, k2);
               if (c == 0.0) continue;
               if (hop) c *= nsign(eb.bl[ss.b].N);
-              const cplx coef = (hop ? std::sqrt(2.0) : -std::sqrt(3.0)) * c * sv * std::sqrt((double)J2 + 1);
+              const cplx coef = amp * c * sv * std::sqrt((double)J2 + 1);
               s1.add(rows(pos_out, d_out), mk(coef, B_ENV, (int64_t)src.k * ly.size + ssb.off, ssb.rows, OP_NN,
                                               B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, ss.dim));
             }
           }
       }
       for (auto& ss : eb.subs[a]) {
-        if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT) break;
+        if (src.what == TrackSrc::SPIN_PRODUCT || src.what == TrackSrc::HOP_PRODUCT ||
+            src.what == TrackSrc::QUAD_PRODUCT)
+          break;
         const int mb = ss.dim;
         const double ns = (double)st[ss.s].N;
         if (src.what == TrackSrc::SITE) {
@@ -541,6 +547,76 @@ TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& p
           }
       }
       // P_new[bra <- ket] = U_bra^dag X
+      Region r{B_OUT, (int64_t)q * P.new_size + nsb.off, nsb.rows, nsb.rows, nsb.cols};
+      s2.add(r, mk(1.0, B_AUX, kb.u_off, kb.u_ld, OP_C, B_X, xo, dap, OP_NN, dap));
+    }
+  }
+  s1.emit(P.stage1);
+  s2.emit(P.stage2);
+  return P;
+}
+
+TrackPlan compile_track_hole(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, const std::vector<HoleSrc>& srcs) {
+  TrackPlan P;
+  const auto st = site_sectors(m.kind);
+  const int nst = (int)st.size();
+  P.eb = enlarge_basis(prev, st);
+  const EnlBasis& eb = P.eb;
+  std::vector<Sector> nsec;
+  for (auto& k : kept) nsec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
+  const OpLayout ol = make_layout(prev, 1, 1), pl = make_layout(prev, 0, 2), nl = make_layout(nsec, 1, 1);
+  P.old_size = ol.size;
+  P.new_size = nl.size;
+  Collector s1, s2;
+  for (size_t q = 0; q < srcs.size(); ++q) {
+    const HoleSrc& src = srcs[q];
+    for (auto& nsb : nl.subs) {
+      const KeptSector& kb = kept[nsb.bra];
+      const KeptSector& kk = kept[nsb.ket];
+      const int ap = kb.a, a = kk.a;
+      const int J2p = eb.sec[ap].S2, J2 = eb.sec[a].S2, dap = eb.sec[ap].dim, kp = kk.keep;
+      const int64_t xo = P.x_size;
+      P.x_size += (int64_t)dap * kp;
+      auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, dap, d, kp}; };
+      for (auto& ss : eb.subs[ap]) s1.ensure(rows(ss.pos, ss.dim));
+      if (src.what == HoleSrc::CARRY) {
+        // carried rank-1/2 operator of the sys block: e1 on (b', s) <- (b, s)
+        for (auto& osb : ol.subs)
+          for (auto& ss : eb.subs[a]) {
+            if (ss.b != osb.ket) continue;
+            int pos_out = -1, d_out = 0;
+            for (auto& t : eb.subs[ap])
+              if (t.b == osb.bra && t.s == ss.s) { pos_out = t.pos; d_out = t.dim; }
+            if (pos_out < 0) continue;
+            const double c = e1(eb.bl[osb.bra].S2, eb.bl[osb.ket].S2, st[ss.s].S2, J2p, J2, 1);
+            if (c == 0.0) continue;
+            s1.add(rows(pos_out, d_out), mk(c, B_SYS, (int64_t)src.k * P.old_size + osb.off, osb.rows, OP_NN, B_AUX,
+                                            kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
+          }
+      } else {
+        // P(block, rank 0) c~(site) on (b', s') <- (b, s): P acts as
+        // P_b'b / sqrt(2S_b+1), c~ as the e2-recoupled site element with the
+        // string (-1)^N_b of a site operator passing the sys block
+        for (auto& psb : pl.subs)
+          for (auto& ss : eb.subs[a]) {
+            if (ss.b != psb.ket) continue;
+            for (int sp = 0; sp < nst; ++sp) {
+              const cplx sv = site_tilde(m.kind, st, sp, ss.s);
+              if (sv == 0.0) continue;
+              int pos_out = -1, d_out = 0;
+              for (auto& t : eb.subs[ap])
+                if (t.b == psb.bra && t.s == sp) { pos_out = t.pos; d_out = t.dim; }
+              if (pos_out < 0) continue;
+              double c = e2(st[sp].S2, st[ss.s].S2, eb.bl[ss.b].S2, J2p, J2, 1);
+              if (c == 0.0) continue;
+              c *= nsign(eb.bl[ss.b].N) / std::sqrt((double)eb.bl[ss.b].S2 + 1);
+              s1.add(rows(pos_out, d_out), mk(c * sv, B_ENV, (int64_t)src.k * pl.size + psb.off, psb.rows, OP_NN,
+                                              B_AUX, kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
+            }
+          }
+      }
+      // W_new[bra <- ket] = U_bra^dag X
       Region r{B_OUT, (int64_t)q * P.new_size + nsb.off, nsb.rows, nsb.rows, nsb.cols};
       s2.add(r, mk(1.0, B_AUX, kb.u_off, kb.u_ld, OP_C, B_X, xo, dap, OP_NN, dap));
     }

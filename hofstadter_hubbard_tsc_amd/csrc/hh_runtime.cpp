@@ -8,6 +8,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1427,6 +1428,181 @@ void pair_cross(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int nb,
   }
 }
 
+// <Delta^dag_a Delta_b> for pairs of bonds a = (s1, s2), b = (s3, s4) inside
+// the sys block with s2 < s3 (Get_sys_two1, code/correlation.f90:370-428,
+// measured by measure_sys_block, :611-673 of measure.f90): four chains
+// stepped together through the stored truncations -- c^dag of every site,
+// the pair creators P_a (formed when s2 joins), W = P_a c~_3 (formed when s3
+// joins) and the scalars Z = -[W x c~_4]^0 = P_a Delta_b (formed when s4
+// joins) -- and Z read off the block's reduced density matrix.  out[q] is
+// the complex value of quads[q] = {s1, s2, s3, s4} (sys-block labels).
+void pair_same_block(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, const std::vector<std::array<int, 4>>& quads,
+                     cplx* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  for (int l = 2; l <= sys_len; ++l) need(d.syst.count(l), "the driver holds no truncation for an intermediate length");
+  for (auto& q : quads)
+    need(q[0] >= 1 && q[0] < q[1] && q[1] < q[2] && q[2] < q[3] && q[3] <= sys_len,
+         "bonds must lie in the sys block with the first bond before the second");
+  const int nq = (int)quads.size();
+  TrackedRank C;   // c^dag of every site
+  C.sec = d.sysb.at(1)->d.sec;
+  C.lay = make_layout(C.sec, 1, 1);
+  C.n = 1;
+  {
+    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
+    for (auto& sub : C.lay.subs) {
+      const cplx v = site_val(m.kind, OP_CDAG, sub.bra, sub.ket);
+      h[sub.off] = make_double2(v.real(), v.imag());
+    }
+    C.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  }
+  // P, W, Z: per quad, slot index in each chain (-1 while not formed)
+  DevMem Pops, Wops, Zops, nxt, X;
+  std::vector<int> pq, wq, zq;   // quad of each slot
+  std::vector<Sector> sec = C.sec;
+  auto run = [&](const TrackPlan& P, BufPtrs b) {
+    DevTasks s1, s2;
+    s1.upload(P.stage1, ctx->stream);
+    s2.upload(P.stage2, ctx->stream);
+    launch_tiles(s1, b, ctx->stream, PREC_FP64);
+    launch_tiles(s2, b, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+  };
+  auto slot_of = [](const std::vector<int>& v, int q) {
+    for (size_t i = 0; i < v.size(); ++i)
+      if (v[i] == q) return (int)i;
+    return -1;
+  };
+  for (int l = 2; l <= sys_len; ++l) {
+    const hh_trunc* t = d.syst.at(l).get();
+    // Z: carry, form from W
+    std::vector<TrackSrc> zs;
+    std::vector<int> nz;
+    for (size_t i = 0; i < zq.size(); ++i) {
+      zs.push_back({TrackSrc::CARRY, (int)i});
+      nz.push_back(zq[i]);
+    }
+    for (int q = 0; q < nq; ++q)
+      if (quads[q][3] == l) {
+        zs.push_back({TrackSrc::QUAD_PRODUCT, slot_of(wq, q)});
+        nz.push_back(q);
+      }
+    // W: carry, form from P
+    std::vector<HoleSrc> ws;
+    std::vector<int> nw;
+    for (size_t i = 0; i < wq.size(); ++i)
+      if (quads[wq[i]][3] > l) {   // still needed for a later Z
+        ws.push_back({HoleSrc::CARRY, (int)i});
+        nw.push_back(wq[i]);
+      }
+    for (int q = 0; q < nq; ++q)
+      if (quads[q][2] == l) {
+        ws.push_back({HoleSrc::PAIR_HOLE, slot_of(pq, q)});
+        nw.push_back(q);
+      }
+    // P: carry, form from c^dag
+    std::vector<PairSrc> ps;
+    std::vector<int> np;
+    for (size_t i = 0; i < pq.size(); ++i)
+      if (quads[pq[i]][2] > l) {
+        ps.push_back({PairSrc::CARRY, (int)i});
+        np.push_back(pq[i]);
+      }
+    for (int q = 0; q < nq; ++q)
+      if (quads[q][1] == l) {
+        ps.push_back({PairSrc::CREATE, quads[q][0] - 1});
+        np.push_back(q);
+      }
+    DevMem nZ, nW, nP;
+    {
+      TrackPlan Pl = compile_track_ops(m, sec, t->kept, zs);
+      nZ.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.new_size * (int64_t)zs.size(), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = Zops.p ? Zops.c() : nullptr;
+      b.p[B_ENV] = Wops.p ? Wops.c() : nullptr;
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nZ.c();
+      run(Pl, b);
+    }
+    {
+      TrackPlan Pl = compile_track_hole(m, sec, t->kept, ws);
+      nW.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.new_size * (int64_t)ws.size(), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = Wops.p ? Wops.c() : nullptr;
+      b.p[B_ENV] = Pops.p ? Pops.c() : nullptr;
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nW.c();
+      run(Pl, b);
+    }
+    {
+      TrackPlan Pl = compile_track_pairs(m, sec, t->kept, ps, 0);
+      nP.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.new_size * (int64_t)ps.size(), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Pl.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = Pops.p ? Pops.c() : nullptr;
+      b.p[B_ENV] = C.ops.c();
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nP.c();
+      run(Pl, b);
+    }
+    {
+      TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_CDAG, 0);
+      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
+      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
+      BufPtrs b{};
+      b.p[B_SYS] = C.ops.c();
+      b.p[B_AUX] = t->evec.c();
+      b.p[B_X] = X.c();
+      b.p[B_OUT] = nxt.c();
+      run(Q, b);
+      std::swap(C.ops.p, nxt.p);
+      std::swap(C.ops.bytes, nxt.bytes);
+      C.n += 1;
+    }
+    std::swap(Zops.p, nZ.p);
+    std::swap(Zops.bytes, nZ.bytes);
+    std::swap(Wops.p, nW.p);
+    std::swap(Wops.bytes, nW.bytes);
+    std::swap(Pops.p, nP.p);
+    std::swap(Pops.bytes, nP.bytes);
+    zq = nz;
+    wq = nw;
+    pq = np;
+    sec.clear();
+    const EnlBasis eb = enlarge_basis(C.sec, site_sectors(m.kind));
+    for (auto& k : t->kept) sec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
+    C.sec = sec;
+    C.lay = make_layout(C.sec, 1, 1);
+  }
+  need((int)zq.size() == nq, "four-site operator missing");
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  const int64_t osz = diag_op_size(sec);
+  DevMem rho, irho;
+  rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(osz, 1));
+  irho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(osz, 1));
+  BufPtrs b{};
+  b.p[B_IN] = (double2*)psi;
+  b.p[B_OUT] = rho.c();
+  launch_on(ctx, compile_block_rho(S, 0, sb->plan.sbs, sec), b);
+  dev_cscale_copy(irho.c(), rho.c(), 0.0, 1.0, osz, ctx->stream);
+  std::vector<double> hr(nq), hi(nq);
+  mdots_to_host(ctx, Zops.c(), osz, nq, rho.c(), osz, hr.data());
+  mdots_to_host(ctx, Zops.c(), osz, nq, irho.c(), osz, hi.data());
+  for (int i = 0; i < nq; ++i) out[zq[i]] = cplx(hr[i], hi[i]);
+}
+
 // One row of Get_density_cor for the spin (kind OP_S: <S_r . S_j>) or the
 // one-body correlation (OP_CDAG: sum_sigma <c^dag_r c_j>), for a row site r
 // on the sys side (1 .. sys_len + 1) and every other site j.  Block pairs
@@ -2528,6 +2704,21 @@ int hh_driver_load(hh_driver* h, const char* path) {
   return guard(h->d.ctx, [&] {
     need(path != nullptr, "null path");
     load_driver(h->d, path);
+  });
+}
+
+int hh_driver_pair_block(hh_driver* h, int sys_len, const double* psi, int nquad, const int32_t* quads,
+                         double* p_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && p_out && (nquad == 0 || quads), "null argument");
+    need(nquad >= 0, "negative count");
+    std::vector<std::array<int, 4>> q(nquad);
+    for (int i = 0; i < nquad; ++i) {
+      int a0 = quads[4 * i], a1 = quads[4 * i + 1], b0 = quads[4 * i + 2], b1 = quads[4 * i + 3];
+      q[i] = {std::min(a0, a1), std::max(a0, a1), std::min(b0, b1), std::max(b0, b1)};
+    }
+    pair_same_block(h->d.ctx, h->d, sys_len, (const double2*)psi, q, (cplx*)p_out);
   });
 }
 

@@ -539,6 +539,18 @@ class Driver:
         _sync()
         self.ctx._rc(L.load().hh_driver_load(self.ptr, os.fsencode(path)), "hh_driver_load")
 
+    def pair_block(self, sys_len, psi, quads):
+        """<Delta^dag_a Delta_b> (complex) for each ((s1, s2), (s3, s4)) in quads:
+        two bonds inside the sys block, every site of a before every site of b."""
+        _sync()
+        q = np.ascontiguousarray(np.asarray(quads, dtype=np.int32).reshape(-1, 4))
+        out = np.zeros((len(q), 2))
+        self.ctx._rc(L.load().hh_driver_pair_block(self.ptr, sys_len, _dptr(psi), len(q),
+                                                   q.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))),
+                     "hh_driver_pair_block")
+        return out[:, 0] + 1j * out[:, 1]
+
     def block(self, side, length):
         """Borrowed handle of a stored block (do not destroy)."""
         p = C.c_void_p()

@@ -292,6 +292,17 @@ int hh_driver_hopping_row(hh_driver* d, int sys_len, int site, const double* psi
  * (sys block, env block) are NaN. */
 int hh_driver_pair_cross(hh_driver* d, int sys_len, const double* psi_dev, int nbond, const int32_t* bonds,
                          double* p_out);
+/* the same-block part of Get_mSCOP_Operator_cor0: <Delta^dag_a Delta_b>
+ * for bonds a = (s1, s2) and b = (s3, s4) inside the sys block (lattice
+ * positions 1 .. sys_len) with every site of a before every site of b, built
+ * as Get_sys_two1 (code/correlation.f90:370-428) does: the pair creator of a
+ * carried through the truncations, multiplied by the annihilator of s3 when
+ * s3 joins and coupled with that of s4 to a scalar when s4 joins, then read
+ * off the block (measure_sys_block, code/measure.f90:611-673).  quads holds
+ * nquad x 4 positions (s1, s2, s3, s4); p_out receives nquad complex values
+ * (re, im interleaved). */
+int hh_driver_pair_block(hh_driver* d, int sys_len, const double* psi_dev, int nquad, const int32_t* quads,
+                         double* p_out);
 
 /* ---- timing of the H.psi kernels (HIP events on the context stream) ------ */
 typedef struct {

@@ -584,3 +584,65 @@ def hopping_row(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, en
         else:
             out[a - 1] = np.conj(v)
     return out
+
+
+def _scalar_times(P, U, eb):
+    """(P U) for a rank-0 operator P (any N shift) and an operator U of any
+    rank on one enlarged basis: P acts as P_red / sqrt(2 J + 1) on the
+    multiplet, so (P U)_red[a', a] = sum_a'' P_red[a', a''] U_red[a'', a] /
+    sqrt(2 J_a' + 1)."""
+    out = {}
+    for (ap, am), p in P.items():
+        for (am2, a), u in U.items():
+            if am2 != am:
+                continue
+            c = 1.0 / math.sqrt(eb.sectors[ap][1] + 1)
+            out[(ap, a)] = out.get((ap, a), 0.0) + c * (p @ u)
+    return out
+
+
+def pair_same_block(model, sys_blocks, sys_truns, sys_len, psi, H, quads):
+    """<Delta^dag_12 Delta_34> for bonds (s1, s2) and (s3, s4) of the system
+    block with s2 < s3 < s4 (Get_sys_two1, code/correlation.f90:370-428,
+    measured as measure_sys_block): the pair creator P_12 is formed when s2
+    joins; at s3 it is multiplied by the new site's annihilator partner
+    (a rank-1/2, N + 1 operator W = P_12 c~_3), at s4 W is coupled with the
+    next site's annihilator partner to the scalar Z = [W x c~_4]^0 =
+    -P_12 Delta_34 (the sign of the annihilator tensor convention), and Z is
+    carried to the measured block and read off its reduced density matrix.
+    Returns a complex array over `quads`."""
+    st = dmrg.site_sectors(model.kind)
+    C = dmrg.site_ops(model.kind)["cdag"]
+    cd = {1: C}
+    P, W, Z = {}, {}, {}
+    for l in range(2, sys_len + 1):
+        prev = sys_blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        tr = sys_truns[l]
+        site = dmrg.embed_site_op(C, prev, eb, "sys")
+        site_t = dmrg._tilde_dict(site, eb.sectors)
+        emb = lambda o: dmrg.embed_block_op(o, prev, eb, "sys")
+        nP = {k: _transform_any(emb(o), tr, 0, 2, False) for k, o in P.items()}
+        nW = {k: _transform_any(emb(o), tr, 1, 1, True) for k, o in W.items()}
+        nZ = {k: _transform(emb(o), tr) for k, o in Z.items()}
+        for q, ((s1, s2), (s3, s4)) in enumerate(quads):
+            if s2 == l:
+                nP[q] = _transform_any(_couple0(emb(cd[s1]), site, eb, 1), tr, 0, 2, False)
+            if s3 == l:
+                nW[q] = _transform_any(_scalar_times(emb(P[q]), site_t, eb), tr, 1, 1, True)
+            if s4 == l:
+                nZ[q] = _transform({k: -v for k, v in _couple0(emb(W[q]), site_t, eb, 1).items()}, tr)
+        cd = {k: _transform_any(emb(o), tr, 1, 1, True) for k, o in cd.items()}
+        cd[l] = _transform_any(site, tr, 1, 1, True)
+        P, W, Z = nP, nW, nZ
+    rho, _ = _block_rho(H, psi, "sys")
+    secs = sys_blocks[sys_len].sectors
+    out = np.zeros(len(quads), complex)
+    for q in range(len(quads)):
+        v = 0.0
+        for b, rb in rho.items():
+            m = Z[q].blk.get((b, b))
+            if m is not None:
+                v += complex(np.trace(rb @ m)) / math.sqrt(secs[b][1] + 1)
+        out[q] = v
+    return out

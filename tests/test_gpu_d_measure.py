@@ -216,3 +216,36 @@ def test_pair_cross_truncated_matches_oracle(ctx):
     ref = measure.pair_cross(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, sys_b, env_b)
     assert np.abs(ref).max() > 1e-4
     assert np.abs(P - ref).max() < 1e-10, np.abs(P - ref).max()
+
+
+def test_pair_same_block_untruncated_equals_ed(ctx):
+    """<Delta^dag_a Delta_b> for bonds a, b inside the sys block (Get_sys_two1's
+    four-site operators) against exact diagonalisation: t-J 6 x 2, 4 electrons."""
+    m = lat.Model(kind="tj", Lx=6, Ly=2, J=0.4, t=1.0, flux=0.2, n_elec=4)
+    drv, L, R, E, psi = _sweep(ctx, m, 4000, 1e-12)
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    quads = [((a, b), (c, d)) for a in range(1, L + 1) for b in range(a + 1, L + 1)
+             for c in range(b + 1, L + 1) for d in range(c + 1, L + 1)]
+    e0, D = ed.singlet_ground_pairing(m, list(range(1, m.nsite + 1)), bonds, m.n_elec,
+                                      [q[0] for q in quads], [q[1] for q in quads])
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    got = drv.pair_block(L, psi, quads)
+    assert np.abs(got - np.diag(D)).max() < 1e-6, (got, np.diag(D))
+
+
+def test_pair_same_block_truncated_matches_oracle(ctx):
+    """Hubbard 6 x 2 ladder with flux at m = 24: the GPU four-site chains against
+    the oracle's on the GPU run's own blocks, truncations and ground state."""
+    from .gpuhelp import host, oracle_block, oracle_superblock
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=4.0, t=1.0, flux=0.1, n_elec=10)
+    drv, L, R, E, psi = _sweep(ctx, m, 24, 1e-11)
+    quads = [((a, b), (c, d)) for a in range(1, L + 1) for b in range(a + 1, L + 1)
+             for c in range(b + 1, L + 1) for d in range(c + 1, L + 1)]
+    got = drv.pair_block(L, psi, quads)
+    sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
+    sys_truns = {l: drv.trunc(0, l).matrices() for l in range(2, L + 1)}
+    H = oracle_superblock(ctx, drv.block(0, L), drv.block(1, R), m, m.n_elec)
+    ref = measure.pair_same_block(m, sys_blocks, sys_truns, L, host(psi), H, quads)
+    assert np.abs(ref).max() > 1e-4
+    assert np.abs(got - ref).max() < 1e-10, np.abs(got - ref).max()

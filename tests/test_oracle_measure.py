@@ -317,3 +317,28 @@ def test_untruncated_hopping_rows_equal_ed(kind, flux):
         o = [j for j in range(m.nsite) if j != r - 1]
         assert np.isnan(row[r - 1])
         assert np.abs(row[o] - G[r - 1, o]).max() < 1e-6, r
+
+
+@pytest.mark.parametrize("kind,flux", [("tj", 0.2), ("hubbard", 0.15)])
+def test_untruncated_pair_same_block_equals_ed(kind, flux):
+    """<Delta^dag_a Delta_b> for every pair of bonds inside the system block
+    (the four-site operators of Get_sys_two1) equals the exact pair
+    correlation; the sign of the annihilator coupling is fixed by it."""
+    m = lat.Model(kind=kind, Lx=6, Ly=2, U=4.0, J=0.4, t=1.0, flux=flux, n_elec=4)
+    L = m.nsite // 2 - 1
+    R = m.nsite - L - 2
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    quads = [((a, b), (c, d)) for a in range(1, L + 1) for b in range(a + 1, L + 1)
+             for c in range(b + 1, L + 1) for d in range(c + 1, L + 1)]
+    e0, D = ed.singlet_ground_pairing(m, list(range(1, m.nsite + 1)), bonds, m.n_elec,
+                                      [q[0] for q in quads], [q[1] for q in quads])
+    run = driver.Run(m, driver.Schedule(kept_max=4000, n_sweeps=1, tol=1e-12))
+    run.run()
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(run.sys_blocks[L], m, "sys", sg),
+                        dmrg.enlarge_in_geometry(run.env_blocks[R], m, "env", sg), sg, m, m.n_elec)
+    E, psi, _ = dmrg.lanczos(H.matvec, dmrg.start_vector(H.sb.dim, 3), 1e-13, 300)
+    v = measure.pair_same_block(m, run.sys_blocks, run.sys_trun, L, psi, H, quads)
+    want = np.diag(D)
+    assert len(quads) == 5 and np.abs(want).min() > 1e-3
+    assert np.abs(v - want).max() < 1e-6
