@@ -316,18 +316,23 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
   // v1 = psi / |psi|
   dev_mdots(psi, 0, 1, psi, n, dots, part, ctx->stream);
   const double nrm0 = std::sqrt(fetch(1)[0]);
-  need(nrm0 > 0.0, "zero start vector");
+  need(std::isfinite(nrm0) && nrm0 > 0.0, "zero or non-finite start vector");
   dev_scale_copy(vec(0), psi, 1.0 / nrm0, n, ctx->stream);
   hpsi(vec(0), w);
   dev_mdots(vec(0), 0, 1, w, n, dots, part, ctx->stream);
-  std::vector<double> alpha{fetch(1)[0]}, beta;
+  // a non-finite H.psi (NaN or Inf in a block or in psi) stops the solve with an error
+  auto finite = [](double x) {
+    if (!std::isfinite(x)) throw HHError(-4, "non-finite value in Lanczos: H.psi produced NaN or Inf");
+    return x;
+  };
+  std::vector<double> alpha{finite(fetch(1)[0])}, beta;
   double ritz = alpha[0];
   int last = 1;
   while (last < max_iter) {
     // w -= alpha v_last - beta v_{last-1}, with |w|^2 in the same pass
     dev_lanczos_update_norm(w, vec(last - 1), alpha[last - 1], last >= 2 ? vec(last - 2) : nullptr,
                             last >= 2 ? beta[last - 2] : 0.0, n, dots, part, ctx->stream);
-    const double b = std::sqrt(fetch(1)[0]);
+    const double b = std::sqrt(finite(fetch(1)[0]));
     beta.push_back(b);
     if (std::fabs(b) <= lanzero) break;
     // normalise, then classical Gram-Schmidt against every stored vector with
@@ -340,7 +345,7 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
     ++last;
     hpsi(vec(last - 1), w);
     dev_mdots(vec(last - 1), 0, 1, w, n, dots, part, ctx->stream);
-    alpha.push_back(fetch(1)[0]);
+    alpha.push_back(finite(fetch(1)[0]));
     std::vector<double> d(alpha), e(beta);
     e.resize(d.size());
     tridiag_ql(d, e, nullptr);

@@ -122,3 +122,25 @@ def test_lanczos_matches_oracle(ctx, kind, Lx, Ly, flux, N, kept):
     assert abs(E - E_ref) < 1e-10 * abs(E_ref)
     got = host(psi)
     assert abs(abs(np.vdot(got, psi_ref)) - 1.0) < 1e-8
+
+
+def test_lanczos_refuses_non_finite_blocks(ctx):
+    """A block holding a NaN (damaged input) stops the Lanczos solve with an
+    error instead of returning a meaningless energy."""
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    kind, Lx, Ly, flux, N, kept = CASES[0]
+    model = lat.Model(kind=kind, Lx=Lx, Ly=Ly, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=N)
+    ctx.set_model(model)
+    run = _oracle_run(model, kept)
+    n = model.nsite
+    ns = n // 2 - 1
+    sysb, envb = run.sys_blocks[ns], run.env_blocks[n - ns - 2]
+    ps = upload_block(ctx, sysb, 0, 1)
+    pe = upload_block(ctx, envb, 1, 1)
+    arena = ps.download()
+    arena[0] = np.nan
+    ps.upload(arena)
+    sb = ctx.bind(ps, pe, N)
+    psi = dev(_rand(np.random.default_rng(0), sb.dim))
+    with pytest.raises(HHError, match="non-finite"):
+        ctx.lanczos(sb, psi, 1e-10, 50)
