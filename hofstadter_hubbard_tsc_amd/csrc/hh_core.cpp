@@ -382,11 +382,6 @@ void compile_merges(const std::vector<MergeSpec>& specs, const BlockDesc* blocks
   }
 }
 
-int site_index_of(const BlockDesc& b, int k) {
-  for (size_t i = 0; i < b.sites.size(); ++i)
-    if (b.sites[i] == k) return (int)i;
-  return -1;
-}
 }  // namespace
 
 // ============================================================================
@@ -461,7 +456,6 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
     sys_pair[sb.pairs[i].first] = i;
     env_pair[sb.pairs[i].second] = i;
   }
-  const bool tj = m.kind == TJ;
   MergeBuilder mb(P.merges, P.merged_size);
 
   auto site_fn = [&](int kind) { return [&, kind](int sp, int s) { return site_val(m.kind, kind, sp, s); }; };
@@ -530,12 +524,11 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
                 // operand descriptors
                 int64_t a_off = 0, b_off = 0;
                 int a_ld = 0, b_ld = 0;
-                uint8_t a_op = OP_NN, b_op = OP_NN;
+                uint8_t b_op = OP_NN;
                 if (Ps.block) {
                   const SubBlk& sub = Ps.lay->subs[tr.sub];
                   a_off = Ps.base + sub.off;
                   a_ld = sub.rows;
-                  a_op = OP_NN;
                 }
                 if (Qe.block) {
                   const SubBlk& sub = Qe.lay->subs[tq1.sub];

@@ -11,7 +11,6 @@ namespace hh {
 
 namespace {
 inline double nsign(int N) { return (N & 1) ? -1.0 : 1.0; }
-inline bool tri(int a, int b, int c) { return std::abs(a - b) <= c && c <= a + b && ((a + b + c) & 1) == 0; }
 
 cplx site_val(int model_kind, int kind, int sp, int s) {
   const double r2 = std::sqrt(2.0);
@@ -712,7 +711,7 @@ TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb
 // ============================================================================
 TransPlan compile_wave_trans_lr(const SuperBasis& old_sb, const EnlBasis& old_sys, const EnlBasis& old_env,
                                 const std::vector<KeptSector>& sys_trun, const std::vector<KeptSector>& env_trun_of_E,
-                                const EnlBasis& new_sys, const EnlBasis& new_env, const SuperBasis& new_sb) {
+                                const EnlBasis& new_sys, const EnlBasis& /*new_env*/, const SuperBasis& new_sb) {
   (void)old_sys;
   TransPlan P;
   Collector s1, s2;
@@ -763,7 +762,7 @@ TransPlan compile_wave_trans_lr(const SuperBasis& old_sb, const EnlBasis& old_sy
 
 TransPlan compile_wave_trans_rl(const SuperBasis& old_sb, const EnlBasis& old_sys, const EnlBasis& old_env,
                                 const std::vector<KeptSector>& sys_trun_of_A, const std::vector<KeptSector>& env_trun,
-                                const EnlBasis& new_sys, const EnlBasis& new_env, const SuperBasis& new_sb) {
+                                const EnlBasis& /*new_sys*/, const EnlBasis& new_env, const SuperBasis& new_sb) {
   TransPlan P;
   Collector s1, s2;
   std::map<int, int> etrun;

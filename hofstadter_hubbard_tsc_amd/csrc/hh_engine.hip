@@ -39,7 +39,10 @@ static constexpr int MTC = 16 * WB;     // macro tile columns
 static constexpr int WAVES = 4;        // independent waves per workgroup
 // Term lists are cut so that the launch holds about this many equal work
 // items: enough that longest-first dispatch leaves a short tail.
-static constexpr int TARGET_ITEMS = 65536;
+#ifndef HH_TARGET_ITEMS
+#define HH_TARGET_ITEMS 65536
+#endif
+static constexpr int TARGET_ITEMS = HH_TARGET_ITEMS;
 // Macro tiles are listed in bands of SWZ macro rows, column by column, and
 // runs of XCHUNK consecutive workgroups are kept on one XCD: neighbouring
 // items then share operand panels in that XCD's L2.
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(VB) void k_rdot_part(const double2* V, int64_t stri
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = VB / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[j * NPART + blockIdx.x] = red[0];
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(VB) void k_rdot_final(const double* partial, double
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = VB / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) out[j] = red[0];
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(VB) void k_mdot_part(const double2* __restrict__ V,
       if (lane == 0) red[jj][wv] = acc;
     }
     __syncthreads();
-    if (threadIdx.x < jn) {
+    if ((int)threadIdx.x < jn) {
       double t = 0.0;
 #pragma unroll
       for (int k = 0; k < VB / 64; ++k) t += red[threadIdx.x][k];

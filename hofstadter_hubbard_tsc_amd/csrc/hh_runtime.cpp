@@ -1090,48 +1090,66 @@ struct TrackedRank {
   int n = 0;
 };
 
+// stage 1 and stage 2 of a chain step, run to completion on the context stream
+void run_plan(hh_ctx* ctx, const TrackPlan& P, const BufPtrs& b) {
+  DevTasks s1, s2;
+  s1.upload(P.stage1, ctx->stream);
+  s2.upload(P.stage2, ctx->stream);
+  launch_tiles(s1, b, ctx->stream, PREC_FP64);
+  launch_tiles(s2, b, ctx->stream, PREC_FP64);
+  check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
+}
+
+// the sectors of the block that truncation t makes from a block with sectors prev
+std::vector<Sector> kept_sectors(const EnlBasis& eb, const hh_trunc* t) {
+  std::vector<Sector> out;
+  for (auto& k : t->kept) out.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
+  return out;
+}
+
+// T := the site operator of `kind` on the one-site block (site multiplets
+// have dimension 1, so each layout sub-block is one element)
+void rank_start(hh_ctx* ctx, const std::vector<Sector>& site_sec, int kind, TrackedRank& T) {
+  T.sec = site_sec;
+  T.lay = make_layout(T.sec, op_k2(kind), op_dN(kind));
+  T.n = 1;
+  std::vector<double2> h((size_t)std::max<int64_t>(T.lay.size, 1), make_double2(0.0, 0.0));
+  for (auto& sub : T.lay.subs) {
+    const cplx v = site_val(ctx->model.kind, kind, sub.bra, sub.ket);
+    h[sub.off] = make_double2(v.real(), v.imag());
+  }
+  T.ops.alloc(sizeof(double2) * h.size());
+  check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+            "h2d");
+}
+
+// one growth step of T through truncation t: every carried operator and the
+// new site's (compile_track_rank)
+void rank_step(hh_ctx* ctx, const hh_trunc* t, int kind, int side, TrackedRank& T) {
+  TrackPlan P = compile_track_rank(ctx->model, T.sec, t->kept, T.n, kind, side);
+  DevMem nxt, X;
+  nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (T.n + 1), 1));
+  X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+  BufPtrs b{};
+  b.p[B_SYS] = T.ops.c();
+  b.p[B_AUX] = t->evec.c();
+  b.p[B_X] = X.c();
+  b.p[B_OUT] = nxt.c();
+  run_plan(ctx, P, b);
+  std::swap(T.ops.p, nxt.p);
+  std::swap(T.ops.bytes, nxt.bytes);
+  T.sec = kept_sectors(P.eb, t);
+  T.lay = make_layout(T.sec, op_k2(kind), op_dN(kind));
+  T.n += 1;
+}
+
 void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, TrackedRank& T) {
-  const ModelParams& m = ctx->model;
   const auto& blocks = side == 0 ? d.sysb : d.envb;
   const auto& truns = side == 0 ? d.syst : d.envt;
-  const int k2 = op_k2(kind), dN = op_dN(kind);
-  T.sec = blocks.at(1)->d.sec;
-  T.lay = make_layout(T.sec, k2, dN);
-  T.n = 1;
-  {
-    std::vector<double2> h((size_t)std::max<int64_t>(T.lay.size, 1), make_double2(0.0, 0.0));
-    for (auto& sub : T.lay.subs) {   // site multiplets have dimension 1
-      const cplx v = site_val(m.kind, kind, sub.bra, sub.ket);
-      h[sub.off] = make_double2(v.real(), v.imag());
-    }
-    T.ops.alloc(sizeof(double2) * h.size());
-    check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-              "h2d");
-  }
-  DevMem nxt, X;
+  rank_start(ctx, blocks.at(1)->d.sec, kind, T);
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
-    const hh_trunc* t = truns.at(l).get();
-    TrackPlan P = compile_track_rank(m, T.sec, t->kept, T.n, kind, side);
-    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (T.n + 1), 1));
-    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
-    BufPtrs b{};
-    b.p[B_SYS] = T.ops.c();
-    b.p[B_AUX] = t->evec.c();
-    b.p[B_X] = X.c();
-    b.p[B_OUT] = nxt.c();
-    DevTasks s1, s2;
-    s1.upload(P.stage1, ctx->stream);
-    s2.upload(P.stage2, ctx->stream);
-    launch_tiles(s1, b, ctx->stream, PREC_FP64);
-    launch_tiles(s2, b, ctx->stream, PREC_FP64);
-    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
-    std::swap(T.ops.p, nxt.p);
-    std::swap(T.ops.bytes, nxt.bytes);
-    T.sec.clear();
-    for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
-    T.lay = make_layout(T.sec, k2, dN);
-    T.n += 1;
+    rank_step(ctx, truns.at(l).get(), kind, side, T);
   }
 }
 
@@ -1264,23 +1282,10 @@ void track_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, const std::vect
   }
   // c^dag of every site, one step behind the pairs
   TrackedRank C;
-  C.sec = blocks.at(1)->d.sec;
-  C.lay = make_layout(C.sec, 1, 1);
-  C.n = 1;
-  {
-    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
-    for (auto& sub : C.lay.subs) {
-      const cplx v = site_val(m.kind, OP_CDAG, sub.bra, sub.ket);
-      h[sub.off] = make_double2(v.real(), v.imag());
-    }
-    C.ops.alloc(sizeof(double2) * h.size());
-    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-              "h2d");
-  }
+  rank_start(ctx, blocks.at(1)->d.sec, OP_CDAG, C);
   T.sec = C.sec;
   T.lay = make_layout(T.sec, 0, 2);
   T.bonds.clear();
-  DevMem nxt, X;
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
     const hh_trunc* t = truns.at(l).get();
@@ -1296,6 +1301,7 @@ void track_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, const std::vect
         nb.push_back(b);
       }
     TrackPlan P = compile_track_pairs(m, T.sec, t->kept, srcs, side);
+    DevMem nxt, X;
     nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (int64_t)srcs.size(), 1));
     X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
     BufPtrs b{};
@@ -1304,39 +1310,13 @@ void track_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, const std::vect
     b.p[B_AUX] = t->evec.c();
     b.p[B_X] = X.c();
     b.p[B_OUT] = nxt.c();
-    DevTasks s1, s2;
-    s1.upload(P.stage1, ctx->stream);
-    s2.upload(P.stage2, ctx->stream);
-    launch_tiles(s1, b, ctx->stream, PREC_FP64);
-    launch_tiles(s2, b, ctx->stream, PREC_FP64);
-    check_hip(hipStreamSynchronize(ctx->stream), "pair chain");
+    run_plan(ctx, P, b);
     std::swap(T.ops.p, nxt.p);
     std::swap(T.ops.bytes, nxt.bytes);
-    T.sec.clear();
-    for (auto& k : t->kept) T.sec.push_back({P.eb.sec[k.a].N, P.eb.sec[k.a].S2, k.keep});
+    T.sec = kept_sectors(P.eb, t);
     T.lay = make_layout(T.sec, 0, 2);
     T.bonds = nb;
-    if (l < last) {   // advance c^dag for the bonds still to be formed
-      TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_CDAG, side);
-      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
-      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
-      BufPtrs c{};
-      c.p[B_SYS] = C.ops.c();
-      c.p[B_AUX] = t->evec.c();
-      c.p[B_X] = X.c();
-      c.p[B_OUT] = nxt.c();
-      DevTasks q1, q2;
-      q1.upload(Q.stage1, ctx->stream);
-      q2.upload(Q.stage2, ctx->stream);
-      launch_tiles(q1, c, ctx->stream, PREC_FP64);
-      launch_tiles(q2, c, ctx->stream, PREC_FP64);
-      check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
-      std::swap(C.ops.p, nxt.p);
-      std::swap(C.ops.bytes, nxt.bytes);
-      C.sec = T.sec;
-      C.lay = make_layout(C.sec, 1, 1);
-      C.n += 1;
-    }
+    if (l < last) rank_step(ctx, t, OP_CDAG, side, C);   // c^dag for the bonds still to be formed
   }
 }
 
@@ -1454,31 +1434,12 @@ void pair_same_block(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, co
          "bonds must lie in the sys block with the first bond before the second");
   const int nq = (int)quads.size();
   TrackedRank C;   // c^dag of every site
-  C.sec = d.sysb.at(1)->d.sec;
-  C.lay = make_layout(C.sec, 1, 1);
-  C.n = 1;
-  {
-    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
-    for (auto& sub : C.lay.subs) {
-      const cplx v = site_val(m.kind, OP_CDAG, sub.bra, sub.ket);
-      h[sub.off] = make_double2(v.real(), v.imag());
-    }
-    C.ops.alloc(sizeof(double2) * h.size());
-    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-              "h2d");
-  }
+  rank_start(ctx, d.sysb.at(1)->d.sec, OP_CDAG, C);
   // P, W, Z: per quad, slot index in each chain (-1 while not formed)
-  DevMem Pops, Wops, Zops, nxt, X;
+  DevMem Pops, Wops, Zops, X;
   std::vector<int> pq, wq, zq;   // quad of each slot
   std::vector<Sector> sec = C.sec;
-  auto run = [&](const TrackPlan& P, BufPtrs b) {
-    DevTasks s1, s2;
-    s1.upload(P.stage1, ctx->stream);
-    s2.upload(P.stage2, ctx->stream);
-    launch_tiles(s1, b, ctx->stream, PREC_FP64);
-    launch_tiles(s2, b, ctx->stream, PREC_FP64);
-    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
-  };
+  auto run = [&](const TrackPlan& P, const BufPtrs& b) { run_plan(ctx, P, b); };
   auto slot_of = [](const std::vector<int>& v, int q) {
     for (size_t i = 0; i < v.size(); ++i)
       if (v[i] == q) return (int)i;
@@ -1561,20 +1522,7 @@ void pair_same_block(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, co
       b.p[B_OUT] = nP.c();
       run(Pl, b);
     }
-    {
-      TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, OP_CDAG, 0);
-      nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
-      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
-      BufPtrs b{};
-      b.p[B_SYS] = C.ops.c();
-      b.p[B_AUX] = t->evec.c();
-      b.p[B_X] = X.c();
-      b.p[B_OUT] = nxt.c();
-      run(Q, b);
-      std::swap(C.ops.p, nxt.p);
-      std::swap(C.ops.bytes, nxt.bytes);
-      C.n += 1;
-    }
+    rank_step(ctx, t, OP_CDAG, 0, C);
     std::swap(Zops.p, nZ.p);
     std::swap(Zops.bytes, nZ.bytes);
     std::swap(Wops.p, nW.p);
@@ -1584,11 +1532,7 @@ void pair_same_block(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, co
     zq = nz;
     wq = nw;
     pq = np;
-    sec.clear();
-    const EnlBasis eb = enlarge_basis(C.sec, site_sectors(m.kind));
-    for (auto& k : t->kept) sec.push_back({eb.sec[k.a].N, eb.sec[k.a].S2, k.keep});
-    C.sec = sec;
-    C.lay = make_layout(C.sec, 1, 1);
+    sec = C.sec;
   }
   need((int)zq.size() == nq, "four-site operator missing");
   std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
@@ -1628,34 +1572,13 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
   for (int j = 0; j < nsite; ++j) out[j] = cplx(nan, nan);
   const bool spin = kind == OP_S;
   const bool centre = r == sys_len + 1;
-  const int k2 = op_k2(kind), dN = op_dN(kind);
   const cplx amp = spin ? cplx(-std::sqrt(3.0)) : cplx(std::sqrt(2.0));
   // the operator of every site (C) and the products with site r (T), stepped together
   TrackedRank C;
-  C.sec = d.sysb.at(1)->d.sec;
-  C.lay = make_layout(C.sec, k2, dN);
-  C.n = 1;
-  {
-    std::vector<double2> h((size_t)std::max<int64_t>(C.lay.size, 1), make_double2(0.0, 0.0));
-    for (auto& sub : C.lay.subs) {
-      const cplx v = site_val(m.kind, kind, sub.bra, sub.ket);
-      h[sub.off] = make_double2(v.real(), v.imag());
-    }
-    C.ops.alloc(sizeof(double2) * h.size());
-    check_hip(hipMemcpyAsync(C.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-              "h2d");
-  }
+  rank_start(ctx, d.sysb.at(1)->d.sec, kind, C);
   Tracked T;
   T.sec = C.sec;
   DevMem nxt, X;
-  auto run = [&](const TrackPlan& P, BufPtrs b) {
-    DevTasks s1, s2;
-    s1.upload(P.stage1, ctx->stream);
-    s2.upload(P.stage2, ctx->stream);
-    launch_tiles(s1, b, ctx->stream, PREC_FP64);
-    launch_tiles(s2, b, ctx->stream, PREC_FP64);
-    check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
-  };
   const TrackSrc::What prod = spin ? TrackSrc::SPIN_PRODUCT : TrackSrc::HOP_PRODUCT;
   for (int l = 2; l <= sys_len; ++l) {
     const hh_trunc* t = d.syst.at(l).get();
@@ -1684,26 +1607,12 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
       b.p[B_AUX] = t->evec.c();
       b.p[B_X] = X.c();
       b.p[B_OUT] = nxt.c();
-      run(P, b);
+      run_plan(ctx, P, b);
       std::swap(T.ops.p, nxt.p);
       std::swap(T.ops.bytes, nxt.bytes);
       T.lbl = nl;
     }
-    TrackPlan Q = compile_track_rank(m, C.sec, t->kept, C.n, kind, 0);
-    nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.new_size * (C.n + 1), 1));
-    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(Q.x_size, 1));
-    BufPtrs c{};
-    c.p[B_SYS] = C.ops.c();
-    c.p[B_AUX] = t->evec.c();
-    c.p[B_X] = X.c();
-    c.p[B_OUT] = nxt.c();
-    run(Q, c);
-    std::swap(C.ops.p, nxt.p);
-    std::swap(C.ops.bytes, nxt.bytes);
-    C.sec.clear();
-    for (auto& k : t->kept) C.sec.push_back({Q.eb.sec[k.a].N, Q.eb.sec[k.a].S2, k.keep});
-    C.lay = make_layout(C.sec, k2, dN);
-    C.n += 1;
+    rank_step(ctx, t, kind, 0, C);
     T.sec = C.sec;
   }
   T.osz = diag_op_size(T.sec);
