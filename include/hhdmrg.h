@@ -147,7 +147,6 @@ int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in_dev, doub
  * counterpart in the reference, which solves one superblock at a time; it is
  * model_wave (code/mat_vec.f90:3476) applied to each member.  At most 31
  * superblocks; they must stay bound while the batch exists. */
-typedef struct hh_batch hh_batch;
 int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** out);
 int hh_batch_destroy(hh_batch* b);
 /* total psi length over the members and reference-order flops of one call */

@@ -5,6 +5,8 @@ import ctypes as C
 import os
 import re
 
+import pytest
+
 from hofstadter_hubbard_tsc_amd import _lib as L
 
 HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "hhdmrg.h")
@@ -49,3 +51,21 @@ def test_ctx_create_without_device_reports_error():
     rc = lib.hh_ctx_create(0, C.byref(p))
     assert rc != 0
     assert lib.hh_last_error(None)
+
+
+def test_header_compiles_as_c99_and_cxx(tmp_path):
+    """include/hhdmrg.h is a plain C header: it compiles as pedantic C99 (no
+    duplicate typedefs, no C++ in the declarations) and as C++11."""
+    import shutil
+    import subprocess
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src_c = tmp_path / "h.c"
+    src_c.write_text('#include "hhdmrg.h"\nint main(void) { return 0; }\n')
+    src_cc = tmp_path / "h.cc"
+    src_cc.write_text('#include "hhdmrg.h"\nint main() { return 0; }\n')
+    for cc, std, src in (("gcc", "-std=c99", src_c), ("g++", "-std=c++11", src_cc)):
+        if shutil.which(cc) is None:
+            pytest.skip("%s not available" % cc)
+        r = subprocess.run([cc, std, "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", inc, "-c", str(src), "-o",
+                            str(tmp_path / "h.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
