@@ -69,6 +69,51 @@ def cpu_baseline(ctx, drv, model, sys_len, budget_s):
                       % (0, sys_len, reps, t_mv, t_build)}
 
 
+def measure_at_scale(ctx, drv, model, sys_len):
+    """Time the measurement entry points on the superblock of the last timed
+    step (not part of the timed steps): its ground state by Lanczos, then the
+    site densities, the correlation rows of the reference's row site
+    xi1 = (Lx/6 - 1) Ly + 1 (quantum.f90's Measurement), and pair
+    correlations of x bonds on the row site's leg."""
+    import torch
+    n, Ly = model.nsite, model.Ly
+    R = n - sys_len - 2
+    sb = ctx.bind(drv.block(0, sys_len), drv.block(1, R), model.n_elec)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    psi = torch.randn(sb.dim, dtype=torch.complex128, device="cuda", generator=g)
+    psi /= torch.linalg.vector_norm(psi)
+    out = {"sys_len": sys_len, "superblock_dim": int(sb.dim)}
+    t0 = time.time()
+    E, it = ctx.lanczos(sb, psi, 1e-8, 100)
+    out["lanczos_seconds"] = round(time.time() - t0, 3)
+    out["energy"] = E
+    del sb
+    xi1 = (model.Lx // 6 - 1) * Ly + 1
+    env0 = n + 1 - R
+    sys_bonds = [(xi1, xi1 + Ly), (xi1 + 2 * Ly, xi1 + 3 * Ly)]
+    env_bonds = [(env0 + Ly, env0 + 2 * Ly), (env0 + 3 * Ly, env0 + 4 * Ly)]
+    calls = [("density_profile", lambda: drv.density(sys_len, psi)),
+             ("density_row", lambda: drv.density_row(sys_len, xi1, psi)),
+             ("spin_row", lambda: drv.spin_row(sys_len, xi1, psi)),
+             ("hopping_row", lambda: drv.hopping_row(sys_len, xi1, psi)),
+             ("pair_cross", lambda: drv.pair_cross(sys_len, psi, sys_bonds + env_bonds)),
+             ("pair_block", lambda: drv.pair_block(sys_len, psi, [(sys_bonds[0], sys_bonds[1])]))]
+    for name, fn in calls:
+        t0 = time.time()
+        try:
+            v = fn()
+            out[name + "_seconds"] = round(time.time() - t0, 3)
+            if name == "density_profile":
+                out["density_sum"] = float(v.sum())
+            if name == "pair_block":
+                out["pair_block_value"] = [float(v[0].real), float(v[0].imag)]
+        except Exception as e:  # reported, never required
+            out[name + "_error"] = str(e)[:200]
+        print("# measure %s done" % name, file=sys.stderr, flush=True)
+    return out
+
+
 def run_c3(args):
     """Batched H.psi on 16 synthetic superblocks (one grouped launch per stage)
     against the same 16 H.psi launched one superblock at a time."""
@@ -152,6 +197,8 @@ def run_c3(args):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if measure is not None:
+        out["config"]["measure"] = measure
     print(json.dumps(out), flush=True)
 
 
@@ -168,6 +215,9 @@ def main():
     ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c3: batched H.psi micro-benchmark")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
+    ap.add_argument("--measure", action="store_true",
+                    help="c2: after the timed steps, time the measurement entry points on the last superblock "
+                         "(reported under config.measure, outside the timed region)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -257,6 +307,9 @@ def main():
                                                    "dim": d_.tolist(), "sites": b.sites().tolist()}
         with open(args.dump_structure, "w") as f:
             json.dump(dump, f)
+    measure = None
+    if args.measure and recs:
+        measure = measure_at_scale(ctx, drv, model, recs[-1].sys_len + 1)
     cpu = None
     if not args.no_cpu:
         try:
@@ -287,6 +340,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if measure is not None:
+        out["config"]["measure"] = measure
     print(json.dumps(out), flush=True)
 
 

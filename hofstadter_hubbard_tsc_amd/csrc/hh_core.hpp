@@ -287,6 +287,11 @@ TrackPlan compile_track_spin(const ModelParams& m, const std::vector<Sector>& pr
 // update_site_ndia_* for `side`), layout make_layout(sectors, k2, dN).
 TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold, int kind, int side);
+// The same step carrying only the old operators `carry` (in that order) and,
+// when add_site, the new site's operator after them.
+TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, const std::vector<int>& carry, bool add_site,
+                             int kind, int side);
 // One step of the chain of singlet pair creators P_12 = [c^dag_1 x
 // c^dag_2]^0 of block sites (Get_sys_two, code/correlation.f90:331-367):
 // CARRY q embeds old pair operator q (slot B_SYS, layout make_layout(prev, 0,

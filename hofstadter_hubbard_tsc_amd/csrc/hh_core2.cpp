@@ -417,6 +417,14 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
 
 TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold, int kind, int side) {
+  std::vector<int> all(nold);
+  for (int q = 0; q < nold; ++q) all[q] = q;
+  return compile_track_rank(m, prev, kept, all, true, kind, side);
+}
+
+TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& prev,
+                             const std::vector<KeptSector>& kept, const std::vector<int>& carry, bool add_site,
+                             int kind, int side) {
   TrackPlan P;
   const auto st = site_sectors(m.kind);
   const int nst = (int)st.size();
@@ -430,7 +438,9 @@ TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& pr
   P.old_size = ol.size;
   P.new_size = nl.size;
   Collector s1, s2;
-  for (int q = 0; q <= nold; ++q) {
+  const int nout = (int)carry.size() + (add_site ? 1 : 0);
+  for (int q = 0; q < nout; ++q) {
+    const int src = q < (int)carry.size() ? carry[q] : -1;   // -1: the new site's operator
     for (auto& nsb : nl.subs) {
       const KeptSector& kb = kept[nsb.bra];
       const KeptSector& kk = kept[nsb.ket];
@@ -440,7 +450,7 @@ TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& pr
       P.x_size += (int64_t)dap * kp;
       auto rows = [&](int pos, int d) { return Region{B_X, xo + pos, dap, d, kp}; };
       for (auto& ss : eb.subs[ap]) s1.ensure(rows(ss.pos, ss.dim));
-      if (q < nold) {
+      if (src >= 0) {
         // carried block operator: e1 recoupling on every (b', s) <- (b, s);
         // an env block follows its site in the fermion order
         for (auto& osb : ol.subs)
@@ -453,7 +463,7 @@ TrackPlan compile_track_rank(const ModelParams& m, const std::vector<Sector>& pr
             double c = e1(eb.bl[osb.bra].S2, eb.bl[osb.ket].S2, st[ss.s].S2, J2p, J2, k2);
             if (c == 0.0) continue;
             if (side == 1 && ferm) c *= nsign(st[ss.s].N);
-            s1.add(rows(pos_out, d_out), mk(c, B_SYS, (int64_t)q * P.old_size + osb.off, osb.rows, OP_NN, B_AUX,
+            s1.add(rows(pos_out, d_out), mk(c, B_SYS, (int64_t)src * P.old_size + osb.off, osb.rows, OP_NN, B_AUX,
                                             kk.u_off + ss.pos, kk.u_ld, OP_NN, ss.dim));
           }
       } else {

@@ -1123,12 +1123,20 @@ void rank_start(hh_ctx* ctx, const std::vector<Sector>& site_sec, int kind, Trac
             "h2d");
 }
 
-// one growth step of T through truncation t: every carried operator and the
-// new site's (compile_track_rank)
-void rank_step(hh_ctx* ctx, const hh_trunc* t, int kind, int side, TrackedRank& T) {
-  TrackPlan P = compile_track_rank(ctx->model, T.sec, t->kept, T.n, kind, side);
+// one growth step of T through truncation t (compile_track_rank): the
+// operators `carry` (all when empty and keep_all) and, when add_site, the
+// new site's
+void rank_step(hh_ctx* ctx, const hh_trunc* t, int kind, int side, TrackedRank& T, bool keep_all = true,
+               const std::vector<int>& carry = {}, bool add_site = true) {
+  std::vector<int> c = carry;
+  if (keep_all) {
+    c.resize(T.n);
+    for (int q = 0; q < T.n; ++q) c[q] = q;
+  }
+  TrackPlan P = compile_track_rank(ctx->model, T.sec, t->kept, c, add_site, kind, side);
+  const int nout = (int)c.size() + (add_site ? 1 : 0);
   DevMem nxt, X;
-  nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * (T.n + 1), 1));
+  nxt.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.new_size * nout, 1));
   X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
   BufPtrs b{};
   b.p[B_SYS] = T.ops.c();
@@ -1140,16 +1148,23 @@ void rank_step(hh_ctx* ctx, const hh_trunc* t, int kind, int side, TrackedRank& 
   std::swap(T.ops.bytes, nxt.bytes);
   T.sec = kept_sectors(P.eb, t);
   T.lay = make_layout(T.sec, op_k2(kind), op_dN(kind));
-  T.n += 1;
+  T.n = nout;
 }
 
-void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, TrackedRank& T) {
+// only = 0: the operator of every site; only = k >= 1: the operator of site
+// k alone (nothing is carried before k joins)
+void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, TrackedRank& T, int only = 0) {
   const auto& blocks = side == 0 ? d.sysb : d.envb;
   const auto& truns = side == 0 ? d.syst : d.envt;
   rank_start(ctx, blocks.at(1)->d.sec, kind, T);
+  if (only > 1) T.n = 0;
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
-    rank_step(ctx, truns.at(l).get(), kind, side, T);
+    const hh_trunc* t = truns.at(l).get();
+    if (only == 0) rank_step(ctx, t, kind, side, T);
+    else if (l < only) rank_step(ctx, t, kind, side, T, false, {}, false);
+    else if (l == only) rank_step(ctx, t, kind, side, T, false, {}, true);
+    else rank_step(ctx, t, kind, side, T, false, {0}, false);
   }
 }
 
@@ -1167,8 +1182,11 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
   const int env_len = nsite - sys_len - 2;
   need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
   need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  // with only_i the sys chain carries that one site's operator (none for the centre)
   TrackedRank TS, TE;
-  track_rank_chain(ctx, d, 0, sys_len, kind, TS);
+  if (only_i == 0) track_rank_chain(ctx, d, 0, sys_len, kind, TS);
+  else if (only_i <= sys_len) track_rank_chain(ctx, d, 0, sys_len, kind, TS, only_i);
+  else TS.sec = d.sysb.at(sys_len)->d.sec;
   track_rank_chain(ctx, d, 1, env_len, kind, TE);
   BlockDesc sd, ed;
   sd.side = 0;
@@ -1189,7 +1207,7 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
       if (i <= sys_len) {
         t.s.block = true;
         t.s.buf = B_SYS;
-        t.s.base = (int64_t)(i - 1) * TS.lay.size;
+        t.s.base = only_i ? 0 : (int64_t)(i - 1) * TS.lay.size;
         t.s.lay = &TS.lay;
       }
       if (j <= env_len) {
@@ -1594,8 +1612,8 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
           srcs.push_back({prod, j - 1});
           nl.push_back({j, r});
         }
-      if (l > r) {
-        srcs.push_back({prod, r - 1});
+      if (l > r) {   // C holds only the row site's operator from step r on
+        srcs.push_back({prod, 0});
         nl.push_back({r, l});
       }
       TrackPlan P = compile_track_ops(m, T.sec, t->kept, srcs);
@@ -1612,7 +1630,9 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
       std::swap(T.ops.bytes, nxt.bytes);
       T.lbl = nl;
     }
-    rank_step(ctx, t, kind, 0, C);
+    if (centre || l < r) rank_step(ctx, t, kind, 0, C);   // every site's operator
+    else if (l == r) rank_step(ctx, t, kind, 0, C, false, {}, true);
+    else rank_step(ctx, t, kind, 0, C, false, {0}, false);
     T.sec = C.sec;
   }
   T.osz = diag_op_size(T.sec);
@@ -1664,7 +1684,7 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
       t.s.kind = t.e.kind = kind;
       t.e.tilde = !spin;
       t.s.buf = B_SYS;
-      t.s.base = (int64_t)(j - 1) * C.lay.size;
+      t.s.base = centre ? (int64_t)(j - 1) * C.lay.size : 0;   // C holds only the row site's operator
       t.s.lay = &C.lay;
       MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
       Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.sb.dim, 1));
