@@ -5,4 +5,5 @@ include/hhdmrg.h); this package is its host-side mirror of the reference
 subroutine interface.
 """
 from ._lib import LIB_PATH, load  # noqa: F401
-from .dmrg import Batch, Block, Context, Driver, HHError, StepRecord, Superblock, Trunc  # noqa: F401
+from .dmrg import (Batch, Block, Context, Driver, HHError, StepRecord, Superblock, Trunc,  # noqa: F401
+                   structure_factor_cut)

@@ -107,6 +107,9 @@ SIGNATURES = [
     ("hh_driver_hopping_row", i32, [vp, i32, i32, vp, P(f64)]),
     ("hh_driver_pair_cross", i32, [vp, i32, vp, i32, P(i32), P(f64)]),
     ("hh_driver_pair_block", i32, [vp, i32, vp, i32, P(i32), P(f64)]),
+    ("hh_driver_density_all", i32, [vp, i32, vp, P(f64)]),
+    ("hh_driver_spin_all", i32, [vp, i32, vp, P(f64)]),
+    ("hh_driver_hopping_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_trunc", i32, [vp, i32, i32, P(vp)]),
     ("hh_trunc_layout", i32, [vp, P(i64), P(i64), P(i32)]),
     ("hh_trunc_download", i32, [vp, vp, vp]),

@@ -694,7 +694,7 @@ MatvecPlan compile_matvec(const ModelParams& m, const Geometry& g, const BlockDe
       p.lay = q.lay; p.k2 = op_k2(q.kind); p.fermion = op_fermion(q.kind);
       return p;
     };
-    if (only->within) within(0, piece(only->s, 0), piece(only->e, 0), only->amp, false);
+    if (only->within) within(only->side, piece(only->s, only->side), piece(only->e, only->side), only->amp, false);
     else cross(piece(only->s, 0), piece(only->e, 1), only->amp, false);
   } else {
     // ---------- block Hamiltonians (block_vec_dia / vec_block_dia) -----------------

@@ -210,10 +210,11 @@ struct MeasurePiece {
 struct MeasureTerm {
   MeasurePiece s, e;
   cplx amp = 1.0;
-  // within = true: [A_block (x) B_site]^0 inside the enlarged sys block (s is
-  // the block operator, e the sys centre site's operator), the form of the
-  // block-site bond of H.psi
+  // within = true: [A_block (x) B_site]^0 inside the enlarged block of
+  // `side` (s is the block operator, e that side's centre site operator),
+  // the form of the block-site bond of H.psi
   bool within = false;
+  int side = 0;
 };
 // With only != nullptr the plan holds that one term instead of the model's H
 // (block Hamiltonians, U and every bond left out); sys.sec / env.sec give the
@@ -270,10 +271,11 @@ struct TrackPlan {
 TrackPlan compile_track_step(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, int nold);
 // The same step for an explicit list of new operators: CARRY k embeds old
-// operator k, SITE is the density of the new site, PRODUCT k is old operator
-// k times the new site's density (the two-site operator n_k n_new of
-// Get_sys2b, created when its second site joins the block).  New operator q
-// goes to B_OUT at q * new_size.
+// operator k, SITE is the density of the new site (k = 0) or its double
+// occupancy n_up n_down (k = 1), PRODUCT k is old operator k times the new
+// site's density (the two-site operator n_k n_new of Get_sys2b, created when
+// its second site joins the block).  New operator q goes to B_OUT at
+// q * new_size.
 // One step of the chain of a rank-1 operator (the spin S of every block
 // site): the nold operators of block `prev` (layout make_layout(prev, 2, 0),
 // slot B_SYS, old_size apart) embedded in the enlarged basis
@@ -310,7 +312,8 @@ TrackPlan compile_track_pairs(const ModelParams& m, const std::vector<Sector>& p
 // site's spin (the exchange bond of Get_hamiltonian_ndia).  HOP_PRODUCT k is
 // sum_sigma c^dag_k c_new = sqrt(2) [c^dag_k x c~_new]^0 of carried c^dag k
 // (slot B_ENV, layout make_layout(prev, 1, 1)) and the new site's
-// annihilator (the hopping bond, sys side).
+// annihilator (the hopping bond), with the fermion sign of `side`: a sys
+// site follows its block, an env site precedes it.
 // QUAD_PRODUCT k is -[W_k x c~_new]^0 for carried rank-1/2 operator k (slot
 // B_ENV, layout make_layout(prev, 1, 1)); with W = P_12 c~_3 it is the
 // four-site scalar P_12 Delta_34 of Get_sys_two1.
@@ -330,7 +333,7 @@ struct HoleSrc {
 TrackPlan compile_track_hole(const ModelParams& m, const std::vector<Sector>& prev,
                              const std::vector<KeptSector>& kept, const std::vector<HoleSrc>& srcs);
 TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,
-                            const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs);
+                            const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs, int side = 0);
 // rho'_b = (1/sqrt(2S_b+1)) sum over psi blocks and site states of the block
 // part of psi (B_IN) for the block sectors `blk` of `side`, to B_OUT in the
 // sector-diagonal layout: <O> = Re <O, rho'> for a tracked Hermitian O.  With
@@ -347,8 +350,9 @@ TaskList compile_block_rho(const SuperBasis& sb, int side, const EnlBasis& eb, c
 TaskList compile_apply_sys(const SuperBasis& sb, const EnlBasis& eb, const std::vector<Sector>& blk, int64_t op_off,
                            bool site);
 // B_OUT += n_site psi on the rows (side 0) or columns (side 1) of the centre
-// site of `side`: <n_site> = Re <psi, out> with out zeroed beforehand.
-TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb);
+// site of `side`: <n_site> = Re <psi, out> with out zeroed beforehand.  With
+// what = 1 the site operator is the double occupancy n_up n_down.
+TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb, int what = 0);
 
 TransPlan compile_wave_trans_rl(const SuperBasis& old_sb, const EnlBasis& old_sys, const EnlBasis& old_env,
                                 const std::vector<KeptSector>& sys_trun_of_A,

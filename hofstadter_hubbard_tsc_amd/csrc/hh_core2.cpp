@@ -332,7 +332,7 @@ std::vector<int64_t> diag_op_offsets(const std::vector<Sector>& sec) {
 }
 
 TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& prev,
-                             const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs) {
+                             const std::vector<KeptSector>& kept, const std::vector<TrackSrc>& srcs, int side) {
   TrackPlan P;
   const auto st = site_sectors(m.kind);
   P.eb = enlarge_basis(prev, st);
@@ -376,7 +376,9 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
               if (pos_out < 0) continue;
               double c = x12(eb.bl[ssb.bra].S2, eb.bl[ssb.ket].S2, st[sp].S2, st[ss.s].S2, J2, k2);
               if (c == 0.0) continue;
-              if (hop) c *= nsign(eb.bl[ss.b].N);
+              // c~ of a sys site passes the block (ket state); c^dag of the
+              // block passes an env site after c~ has acted (N_s - 1)
+              if (hop) c *= side == 0 ? nsign(eb.bl[ss.b].N) : -nsign(st[ss.s].N);
               const cplx coef = amp * c * sv * std::sqrt((double)J2 + 1);
               s1.add(rows(pos_out, d_out), mk(coef, B_ENV, (int64_t)src.k * ly.size + ssb.off, ssb.rows, OP_NN,
                                               B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, ss.dim));
@@ -390,9 +392,11 @@ TrackPlan compile_track_ops(const ModelParams& m, const std::vector<Sector>& pre
         const int mb = ss.dim;
         const double ns = (double)st[ss.s].N;
         if (src.what == TrackSrc::SITE) {
-          // density of the new site: sqrt(2J+1) n_s (update_site_dia)
-          if (ns != 0.0)
-            s1.add(rows(ss.pos, mb), mk(ns * std::sqrt((double)J2 + 1), B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, 0,
+          // density (or double occupancy) of the new site: sqrt(2J+1) f_s
+          // (update_site_dia)
+          const double f = src.k == 1 ? (st[ss.s].N == 2 ? 1.0 : 0.0) : ns;
+          if (f != 0.0)
+            s1.add(rows(ss.pos, mb), mk(f * std::sqrt((double)J2 + 1), B_AUX, ks.u_off + ss.pos, ks.u_ld, OP_NN, 0,
                                         0, 0, OP_NN, 0));
           continue;
         }
@@ -696,13 +700,13 @@ TaskList compile_apply_sys(const SuperBasis& sb, const EnlBasis& eb, const std::
   return tl;
 }
 
-TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb) {
+TaskList compile_site_density(const SuperBasis& sb, int side, const EnlBasis& eb, int what) {
   TaskList tl;
   for (int i = 0; i < (int)sb.pairs.size(); ++i) {
     const int a = side == 0 ? sb.pairs[i].first : sb.pairs[i].second;
     const int r = sb.rows[i], c = sb.cols[i];
     for (auto& ss : eb.subs[a]) {
-      const int n = eb.st[ss.s].N;
+      const int n = what == 1 ? (eb.st[ss.s].N == 2 ? 1 : 0) : eb.st[ss.s].N;
       if (n == 0) continue;
       if (side == 0) {
         tl.add_tile(B_OUT, sb.off[i] + ss.pos, r, ss.dim, c, 0);

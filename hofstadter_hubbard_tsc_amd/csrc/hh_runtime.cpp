@@ -891,10 +891,11 @@ struct Tracked {
   }
 };
 
-// row = 0: the density of every site.  row = r >= 1: the density of site r
-// and every product n_r n_j (j != r) of block sites; the densities n_j with
-// j < r are carried only until site r joins.
-void track_chain(hh_ctx* ctx, Driver& d, int side, int len, int row, Tracked& T) {
+// row = 0: the density of every site (what = 1: its double occupancy
+// n_up n_down).  row = r >= 1: the density of site r and every product
+// n_r n_j (j != r) of block sites; the densities n_j with j < r are carried
+// only until site r joins.
+void track_chain(hh_ctx* ctx, Driver& d, int side, int len, int row, Tracked& T, int what = 0) {
   const ModelParams& m = ctx->model;
   const auto& blocks = side == 0 ? d.sysb : d.envb;
   const auto& truns = side == 0 ? d.syst : d.envt;
@@ -903,8 +904,10 @@ void track_chain(hh_ctx* ctx, Driver& d, int side, int len, int row, Tracked& T)
   T.lbl = {{1, 0}};
   {
     std::vector<double2> h(T.sec.size());
-    for (size_t i = 0; i < T.sec.size(); ++i)
-      h[i] = make_double2(std::sqrt((double)T.sec[i].S2 + 1) * T.sec[i].N, 0.0);
+    for (size_t i = 0; i < T.sec.size(); ++i) {
+      const double f = what == 1 ? (T.sec[i].N == 2 ? 1.0 : 0.0) : T.sec[i].N;
+      h[i] = make_double2(std::sqrt((double)T.sec[i].S2 + 1) * f, 0.0);
+    }
     T.ops.alloc(sizeof(double2) * h.size());
     check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
               "h2d");
@@ -920,7 +923,7 @@ void track_chain(hh_ctx* ctx, Driver& d, int side, int len, int row, Tracked& T)
         srcs.push_back({TrackSrc::CARRY, (int)q});
         nl.push_back(T.lbl[q]);
       }
-      srcs.push_back({TrackSrc::SITE, 0});
+      srcs.push_back({TrackSrc::SITE, row == 0 ? what : 0});
       nl.push_back({l, 0});
     } else if (l == row) {
       srcs.push_back({TrackSrc::SITE, 0});
@@ -981,8 +984,10 @@ void mdots_to_host(hh_ctx* ctx, const double2* X, int64_t stride, int nv, const 
 // (sys block sys_len, env block nsite - sys_len - 2) of the driver's stored
 // blocks: measure_operator_dia (code/measure.f90:3-61) with the operator
 // chain of Get_sys_oper_dia / Get_env_oper_dia.  out has nsite entries,
-// indexed by lattice position - 1.
-void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out) {
+// indexed by lattice position - 1.  what = 1 gives the double occupancy
+// <n_i,up n_i,down> instead (the same chain with st_double as the site
+// operator).
+void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, double* out, int what = 0) {
   const ModelParams& m = ctx->model;
   const int nsite = m.nsite();
   const int env_len = nsite - sys_len - 2;
@@ -993,7 +998,7 @@ void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, do
   for (int side = 0; side < 2; ++side) {
     const int len = side == 0 ? sys_len : env_len;
     Tracked T;
-    track_chain(ctx, d, side, len, 0, T);
+    track_chain(ctx, d, side, len, 0, T, what);
     const EnlBasis& eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
     // block sites: <O_k> = Re <O_k, rho'>
     DevMem rho, Y;
@@ -1008,7 +1013,7 @@ void density_profile(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, do
     // centre site: <n> = Re <psi, n_site psi>
     dev_zero(Y.c(), S.dim, ctx->stream);
     b.p[B_OUT] = Y.c();
-    launch_on(ctx, compile_site_density(S, side, eb), b);
+    launch_on(ctx, compile_site_density(S, side, eb, what), b);
     mdots_to_host(ctx, psi, 0, 1, Y.c(), S.dim, &h[len]);
     for (int k = 1; k <= len; ++k) out[(side == 0 ? k : nsite + 1 - k) - 1] = h[k - 1];
     out[(side == 0 ? sys_len + 1 : nsite - env_len) - 1] = h[len];
@@ -1168,12 +1173,58 @@ void track_rank_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Track
   }
 }
 
+// A task list repeated n times: copy j writes its outputs (slot B_OUT) j *
+// out_stride further on and reads the operator in slot op_buf j * op_stride
+// further on.  Used for one measurement term against a run of carried
+// operators of one layout, whose plans differ only in the operator's offset.
+TaskList replicate(const TaskList& src, int n, int64_t out_stride, uint8_t op_buf, int64_t op_stride) {
+  TaskList dst;
+  for (const Term& tm : src.terms) need(tm.a_buf != B_OUT && tm.b_buf != B_OUT, "replicated term reads its output");
+  for (int j = 0; j < n; ++j) {
+    const int t0 = (int)dst.terms.size();
+    for (Term tm : src.terms) {
+      if (tm.a_buf == op_buf) tm.a_off += (int64_t)j * op_stride;
+      if (tm.b_buf == op_buf) tm.b_off += (int64_t)j * op_stride;
+      dst.terms.push_back(tm);
+    }
+    for (Tile tl : src.tiles) {
+      tl.t0 += t0;
+      tl.t1 += t0;
+      if (tl.c_buf == B_OUT) tl.c_off += (int64_t)j * out_stride;
+      dst.tiles.push_back(tl);
+    }
+  }
+  dst.flops_alg = src.flops_alg * n;
+  dst.flops_exec = src.flops_exec * n;
+  return dst;
+}
+
+// Y_j = term_j psi for the n single-term plans term_j = P with its operator
+// in slot op_buf moved by j * op_stride: stage 1 once (it reads only the
+// other operator and psi), stage 2 repeated; Y_j at Y + j * P.sb.dim.
+void run_term_family(hh_ctx* ctx, const MatvecPlan& P, int n, uint8_t op_buf, int64_t op_stride, BufPtrs b,
+                     double2* Y) {
+  DevMem X;
+  X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
+  b.p[B_X] = X.c();
+  b.p[B_OUT] = Y;
+  for (const Term& tm : P.stage1.terms) need(tm.a_buf != op_buf && tm.b_buf != op_buf, "stage 1 reads the operator");
+  DevTasks s1, s2;
+  s1.upload(P.stage1, ctx->stream);
+  s2.upload(n == 1 ? P.stage2 : replicate(P.stage2, n, P.sb.dim, op_buf, op_stride), ctx->stream);
+  launch_tiles(s1, b, ctx->stream, PREC_FP64);
+  launch_tiles(s2, b, ctx->stream, PREC_FP64);
+  check_hip(hipStreamSynchronize(ctx->stream), "measurement");
+}
+
 // <psi| term |psi> for every sys-side i (or only i = only_i) and env-side j
 // (centre sites included): the two-point term amp [A_i (x) B_j]^0 of
 // operator kind `kind` (env operator read as the annihilator partner when
 // `tilde`), compiled by compile_matvec's single-term mode and applied to psi.
-// val(i, j) receives the complex expectation; i = sys_len + 1 and
-// j = env_len + 1 are the centre sites.
+// The env block partners of one i share one plan (run_term_family); the
+// expectations of all of them come from one multi-dot.  val(i, j, v)
+// receives the complex expectation; i = sys_len + 1 and j = env_len + 1 are
+// the centre sites.
 template <typename Sink>
 void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int kind, bool tilde, cplx amp,
                  Sink val, int only_i = 0) {
@@ -1196,10 +1247,11 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
   ed.length = env_len;
   ed.sec = TE.sec;
   const Geometry g = make_geometry(m, sys_len, env_len);
-  DevMem Y, X, ipsi;
-  for (int i = 1; i <= sys_len + 1; ++i)
-    for (int j = 1; j <= env_len + 1; ++j) {
-      if (only_i && i != only_i) continue;
+  DevMem Y, ipsi;
+  std::vector<double> re(env_len + 1), im(env_len + 1);
+  for (int i = 1; i <= sys_len + 1; ++i) {
+    if (only_i && i != only_i) continue;
+    auto term = [&](bool env_block) {
       MeasureTerm t;
       t.amp = amp;
       t.s.kind = t.e.kind = kind;
@@ -1210,36 +1262,33 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
         t.s.base = only_i ? 0 : (int64_t)(i - 1) * TS.lay.size;
         t.s.lay = &TS.lay;
       }
-      if (j <= env_len) {
+      if (env_block) {
         t.e.block = true;
         t.e.buf = B_ENV;
-        t.e.base = (int64_t)(j - 1) * TE.lay.size;
+        t.e.base = 0;
         t.e.lay = &TE.lay;
       }
-      MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
-      const int64_t n = P.sb.dim;
-      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
-      X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
-      if (!ipsi.p) {   // i psi: Re <i psi, y> = Im <psi, y>
-        ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
-        dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, n, ctx->stream);
-      }
-      BufPtrs b{};
-      b.p[B_IN] = (double2*)psi;
-      b.p[B_OUT] = Y.c();
-      b.p[B_SYS] = TS.ops.c();
-      b.p[B_ENV] = TE.ops.c();
-      b.p[B_X] = X.c();
-      DevTasks s1, s2;
-      s1.upload(P.stage1, ctx->stream);
-      s2.upload(P.stage2, ctx->stream);
-      launch_tiles(s1, b, ctx->stream, PREC_FP64);
-      launch_tiles(s2, b, ctx->stream, PREC_FP64);
-      double re = 0.0, im = 0.0;
-      mdots_to_host(ctx, psi, 0, 1, Y.c(), n, &re);
-      mdots_to_host(ctx, ipsi.c(), 0, 1, Y.c(), n, &im);
-      val(i, j, cplx(re, im));
+      return t;
+    };
+    const MeasureTerm tb = term(true), tc = term(false);
+    const MatvecPlan Pb = compile_matvec(m, g, sd, ed, m.n_elec, &tb);
+    const MatvecPlan Pc = compile_matvec(m, g, sd, ed, m.n_elec, &tc);
+    const int64_t n = Pb.sb.dim;
+    Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n * (env_len + 1), 1));
+    if (!ipsi.p) {   // i psi: Re <y, i psi> = Im <psi, y>
+      ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
+      dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, n, ctx->stream);
     }
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_SYS] = TS.ops.c();
+    b.p[B_ENV] = TE.ops.c();
+    run_term_family(ctx, Pb, env_len, B_ENV, TE.lay.size, b, Y.c());          // env block sites 1 .. env_len
+    run_term_family(ctx, Pc, 1, B_ENV, 0, b, Y.c() + (int64_t)env_len * n);   // env centre site
+    mdots_to_host(ctx, Y.c(), n, env_len + 1, psi, n, re.data());
+    mdots_to_host(ctx, Y.c(), n, env_len + 1, ipsi.c(), n, im.data());
+    for (int j = 1; j <= env_len + 1; ++j) val(i, j, cplx(re[j - 1], im[j - 1]));
+  }
 }
 
 // <S_i . S_j> for i on the sys side (block sites and the sys centre site) and
@@ -1673,38 +1722,33 @@ void corr_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi, in
     ed.side = 1;
     ed.length = env_len;
     ed.sec = sb->env->d.sec;
-    const Geometry g = make_geometry(m, sys_len, env_len);
-    DevMem Y, Xw;
-    for (int j = 1; j <= sys_len; ++j) {
-      if (!centre && j != r) continue;
-      MeasureTerm t;
-      t.within = true;
-      t.amp = amp;
-      t.s.block = true;
-      t.s.kind = t.e.kind = kind;
-      t.e.tilde = !spin;
-      t.s.buf = B_SYS;
-      t.s.base = centre ? (int64_t)(j - 1) * C.lay.size : 0;   // C holds only the row site's operator
-      t.s.lay = &C.lay;
-      MatvecPlan P = compile_matvec(m, g, sd, ed, m.n_elec, &t);
-      Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.sb.dim, 1));
-      Xw.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P.x_size, 1));
-      BufPtrs b{};
-      b.p[B_IN] = (double2*)psi;
-      b.p[B_OUT] = Y.c();
-      b.p[B_SYS] = C.ops.c();
-      b.p[B_X] = Xw.c();
-      DevTasks s1, s2;
-      s1.upload(P.stage1, ctx->stream);
-      s2.upload(P.stage2, ctx->stream);
-      launch_tiles(s1, b, ctx->stream, PREC_FP64);
-      launch_tiles(s2, b, ctx->stream, PREC_FP64);
-      double re = 0.0, im = 0.0;
-      mdots_to_host(ctx, psi, 0, 1, Y.c(), P.sb.dim, &re);
-      mdots_to_host(ctx, ipsi.c(), 0, 1, Y.c(), P.sb.dim, &im);
-      if (centre) out[j - 1] = std::conj(cplx(re, im));
-      else out[sys_len] = cplx(re, im);
-    }
+    // one plan for the first carried operator; with the centre as row site
+    // the other block sites follow by moving the operator offset
+    MeasureTerm t;
+    t.within = true;
+    t.amp = amp;
+    t.s.block = true;
+    t.s.kind = t.e.kind = kind;
+    t.e.tilde = !spin;
+    t.s.buf = B_SYS;
+    t.s.base = 0;
+    t.s.lay = &C.lay;
+    const MatvecPlan P = compile_matvec(m, make_geometry(m, sys_len, env_len), sd, ed, m.n_elec, &t);
+    const int nj = centre ? sys_len : 1;   // otherwise C holds only the row site's operator
+    const int64_t n = P.sb.dim;
+    DevMem Y;
+    Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n * nj, 1));
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_SYS] = C.ops.c();
+    run_term_family(ctx, P, nj, B_SYS, C.lay.size, b, Y.c());
+    std::vector<double> re(nj), im(nj);
+    mdots_to_host(ctx, Y.c(), n, nj, psi, n, re.data());
+    mdots_to_host(ctx, Y.c(), n, nj, ipsi.c(), n, im.data());
+    if (centre)
+      for (int j = 1; j <= sys_len; ++j) out[j - 1] = std::conj(cplx(re[j - 1], im[j - 1]));
+    else
+      out[sys_len] = cplx(re[0], im[0]);
   }
   // env side
   cross_terms(ctx, d, sys_len, psi, kind, !spin, amp, [&](int, int j, cplx v) {
@@ -1768,6 +1812,228 @@ void density_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi,
     apply_sys(ctx, sb.get(), TS, TS.find(r, 0), psi, Y.c());
   }
   env_entries(ctx, sb.get(), TE, env_len, psi, Y.c(), out);
+}
+
+// One chain step for a long source list: compile_track_ops in chunks whose
+// stage-1 scratch stays within TRACK_X_ELEMS complex elements, every chunk
+// writing its operators at its place in `out`.  Returns the new sectors.
+constexpr int64_t TRACK_X_ELEMS = int64_t(1) << 27;   // 2 GiB of complex128
+std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, const hh_trunc* t,
+                                   const std::vector<TrackSrc>& srcs, int side, const double2* carried,
+                                   const double2* rank_ops, DevMem& out) {
+  const ModelParams& m = ctx->model;
+  const TrackPlan P1 = compile_track_ops(m, sec, t->kept, {srcs.at(0)}, side);
+  const int64_t per = std::max<int64_t>(P1.x_size, 1), nsrc = (int64_t)srcs.size();
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsrc, TRACK_X_ELEMS / per));
+  out.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P1.new_size * nsrc, 1));
+  DevMem X;
+  X.alloc(sizeof(double2) * (size_t)(per * chunk));
+  for (int64_t q0 = 0; q0 < nsrc; q0 += chunk) {
+    const std::vector<TrackSrc> part(srcs.begin() + q0, srcs.begin() + std::min(nsrc, q0 + chunk));
+    const TrackPlan P = part.size() == 1 && q0 == 0 ? P1 : compile_track_ops(m, sec, t->kept, part, side);
+    BufPtrs b{};
+    b.p[B_SYS] = (double2*)carried;
+    b.p[B_ENV] = (double2*)rank_ops;
+    b.p[B_AUX] = t->evec.c();
+    b.p[B_X] = X.c();
+    b.p[B_OUT] = out.c() + q0 * P1.new_size;
+    run_plan(ctx, P, b);
+  }
+  return kept_sectors(P1.eb, t);
+}
+
+// Every pair a < b of block sites of one side, formed when b joins and
+// carried to length len (Get_sys_oper_cor_ndia1 / Get_env_oper_cor_ndia1,
+// code/correlation.f90:851-974, :1097-1216).  Density (kind OP_N): T holds
+// the densities {a, 0} and the products n_a n_b {a, b}.  Spin and hopping: T
+// holds the products S_a . S_b or sum_sigma c^dag_a c_b {a, b}, C the
+// rank-k operator of every site.
+void block_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Tracked& T, TrackedRank& C) {
+  const auto& blocks = side == 0 ? d.sysb : d.envb;
+  const auto& truns = side == 0 ? d.syst : d.envt;
+  const bool dens = kind == OP_N;
+  T.lbl.clear();
+  if (dens) {
+    T.sec = blocks.at(1)->d.sec;
+    T.lbl = {{1, 0}};
+    std::vector<double2> h(T.sec.size());
+    for (size_t i = 0; i < T.sec.size(); ++i) h[i] = make_double2(std::sqrt((double)T.sec[i].S2 + 1) * T.sec[i].N, 0.0);
+    T.ops.alloc(sizeof(double2) * h.size());
+    check_hip(hipMemcpyAsync(T.ops.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+  } else {
+    rank_start(ctx, blocks.at(1)->d.sec, kind, C);
+    T.sec = C.sec;
+  }
+  DevMem nxt;
+  const TrackSrc::What prod = kind == OP_S ? TrackSrc::SPIN_PRODUCT : TrackSrc::HOP_PRODUCT;
+  for (int l = 2; l <= len; ++l) {
+    need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
+    const hh_trunc* t = truns.at(l).get();
+    std::vector<TrackSrc> srcs;
+    std::vector<std::pair<int, int>> nl;
+    for (size_t q = 0; q < T.lbl.size(); ++q) {
+      srcs.push_back({TrackSrc::CARRY, (int)q});
+      nl.push_back(T.lbl[q]);
+    }
+    if (dens) {
+      srcs.push_back({TrackSrc::SITE, 0});
+      nl.push_back({l, 0});
+      for (size_t q = 0; q < T.lbl.size(); ++q)
+        if (T.lbl[q].second == 0) {
+          srcs.push_back({TrackSrc::PRODUCT, (int)q});
+          nl.push_back({T.lbl[q].first, l});
+        }
+    } else {
+      for (int a = 1; a < l; ++a) {   // C holds the operators of sites 1 .. l-1 in order
+        srcs.push_back({prod, a - 1});
+        nl.push_back({a, l});
+      }
+    }
+    T.sec = track_ops_step(ctx, T.sec, t, srcs, side, T.ops.p ? T.ops.c() : nullptr, dens ? nullptr : C.ops.c(),
+                           nxt);
+    std::swap(T.ops.p, nxt.p);
+    std::swap(T.ops.bytes, nxt.bytes);
+    T.lbl = nl;
+    if (!dens) rank_step(ctx, t, kind, side, C);
+  }
+  T.osz = diag_op_size(T.sec);
+}
+
+// Get_oper_cor_ndia1 (code/correlation.f90:2891-3107) for oper = oper1 = n
+// (kind OP_N), the spin (OP_S) or st_elec_up / st_elec_down (OP_CDAG): the
+// correlation of every pair of sites of the superblock and the on-site
+// values, out[(p-1) * nsite + (q-1)] for lattice positions p, q.
+//   * across the cut: cross_density / cross_terms;
+//   * pairs of block sites of one side: the carried products of
+//     block_pair_chain read off that block's reduced density matrix;
+//   * a block site with its side's centre site: the transition matrix of
+//     (n_centre psi, psi) for the density, the within-block bond term
+//     (single-term compile_matvec, one plan per side) for spin and hopping;
+//   * on site: <n^2> = <n> + 2 <n_up n_down>, <S.S> = 3/4 <n - 2 n_up n_down>,
+//     sum_sigma <c^dag c> = <n> (st_oper x st_oper1^dag of one site).
+// Spin and density are real and symmetric; hopping is Hermitian,
+// out[p][q] = sum_sigma <c^dag_p c_q>.
+void corr_all(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int kind, cplx* out) {
+  const ModelParams& m = ctx->model;
+  const int nsite = m.nsite();
+  const int env_len = nsite - sys_len - 2;
+  need(sys_len >= 1 && env_len >= 1, "superblock lengths out of range");
+  need(d.sysb.count(sys_len) && d.envb.count(env_len), "the driver holds no block of that length");
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int64_t i = 0; i < (int64_t)nsite * nsite; ++i) out[i] = cplx(nan, nan);
+  auto put = [&](int p, int q, cplx v) {   // lattice positions; the mirrored entry holds the conjugate
+    out[(int64_t)(p - 1) * nsite + (q - 1)] = v;
+    out[(int64_t)(q - 1) * nsite + (p - 1)] = std::conj(v);
+  };
+  const bool dens = kind == OP_N, spin = kind == OP_S;
+  const cplx amp = spin ? cplx(-std::sqrt(3.0)) : cplx(std::sqrt(2.0));
+  // across the cut
+  if (dens) {
+    std::vector<double> c((size_t)nsite * nsite);
+    cross_density(ctx, d, sys_len, psi, c.data());
+    for (int p = 1; p <= sys_len + 1; ++p)
+      for (int q = sys_len + 2; q <= nsite; ++q) put(p, q, c[(size_t)(p - 1) * nsite + (q - 1)]);
+  } else {
+    cross_terms(ctx, d, sys_len, psi, kind, !spin, amp, [&](int i, int j, cplx v) {
+      put(i, j <= env_len ? nsite + 1 - j : nsite - env_len, v);
+    });
+  }
+  // on site
+  {
+    std::vector<double> n(nsite), dd(nsite, 0.0);
+    density_profile(ctx, d, sys_len, psi, n.data(), 0);
+    if (m.kind == HUBBARD && kind != OP_CDAG) density_profile(ctx, d, sys_len, psi, dd.data(), 1);
+    for (int p = 0; p < nsite; ++p)
+      out[(int64_t)p * nsite + p] = dens ? n[p] + 2.0 * dd[p] : spin ? 0.75 * (n[p] - 2.0 * dd[p]) : n[p];
+  }
+  std::unique_ptr<hh_superblock> sb(bind(ctx, d.sysb.at(sys_len).get(), d.envb.at(env_len).get(), m.n_elec));
+  const SuperBasis& S = sb->plan.sb;
+  DevMem ipsi, Y;
+  ipsi.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim, 1));
+  dev_cscale_copy(ipsi.c(), psi, 0.0, 1.0, S.dim, ctx->stream);
+  for (int side = 0; side < 2; ++side) {
+    const int len = side == 0 ? sys_len : env_len;
+    auto pos = [&](int k) { return side == 0 ? k : nsite + 1 - k; };   // k = len + 1: the centre site
+    const EnlBasis& eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
+    Tracked T;
+    TrackedRank C;
+    block_pair_chain(ctx, d, side, len, kind, T, C);
+    // pairs of block sites: <Q> = Re <Q, rho'> + i Re <Q, i rho'>
+    const int nq = (int)T.lbl.size();
+    DevMem rho, irho;
+    rho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_OUT] = rho.c();
+    launch_on(ctx, compile_block_rho(S, side, eb, T.sec), b);
+    std::vector<double> hr(nq), hi(nq, 0.0);
+    if (nq > 0) {
+      mdots_to_host(ctx, T.ops.c(), T.osz, nq, rho.c(), T.osz, hr.data());
+      if (!dens && !spin) {
+        irho.alloc(sizeof(double2) * (size_t)std::max<int64_t>(T.osz, 1));
+        dev_cscale_copy(irho.c(), rho.c(), 0.0, 1.0, T.osz, ctx->stream);
+        mdots_to_host(ctx, T.ops.c(), T.osz, nq, irho.c(), T.osz, hi.data());
+      }
+    }
+    for (int q = 0; q < nq; ++q)
+      if (T.lbl[q].second != 0) put(pos(T.lbl[q].first), pos(T.lbl[q].second), cplx(hr[q], hi[q]));
+    // a block site with this side's centre site
+    Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(S.dim * std::max(len, 1), 1));
+    if (dens) {
+      // Y = n_centre psi; <n_a n_centre> = Re <O_a, rho'(Y, psi)>
+      dev_zero(Y.c(), S.dim, ctx->stream);
+      BufPtrs w{};
+      w.p[B_IN] = (double2*)psi;
+      w.p[B_OUT] = Y.c();
+      launch_on(ctx, compile_site_density(S, side, eb), w);
+      BufPtrs r{};
+      r.p[B_IN] = (double2*)psi;
+      r.p[B_X] = Y.c();
+      r.p[B_OUT] = rho.c();
+      launch_on(ctx, compile_block_rho(S, side, eb, T.sec, B_X, B_IN), r);
+      mdots_to_host(ctx, T.ops.c(), T.osz, nq, rho.c(), T.osz, hr.data());
+      for (int q = 0; q < nq; ++q)
+        if (T.lbl[q].second == 0) put(pos(T.lbl[q].first), pos(len + 1), hr[q]);
+    } else {
+      // amp [O_a (x) O~_centre]^0 inside the enlarged block: one plan for
+      // a = 1, the others by moving the operator offset
+      BlockDesc sd, ed;
+      sd.side = 0;
+      sd.length = sys_len;
+      sd.sec = side == 0 ? C.sec : sb->sys->d.sec;
+      ed.side = 1;
+      ed.length = env_len;
+      ed.sec = side == 1 ? C.sec : sb->env->d.sec;
+      MeasureTerm t;
+      t.within = true;
+      t.side = side;
+      t.amp = amp;
+      t.s.block = true;
+      t.s.kind = t.e.kind = kind;
+      t.e.tilde = !spin;
+      t.s.buf = B_SYS;
+      t.s.base = 0;
+      t.s.lay = &C.lay;
+      const MatvecPlan P = compile_matvec(m, make_geometry(m, sys_len, env_len), sd, ed, m.n_elec, &t);
+      need(P.sb.dim == S.dim, "measurement superblock differs from the bound one");
+      BufPtrs w{};
+      w.p[B_IN] = (double2*)psi;
+      w.p[B_SYS] = C.ops.c();
+      run_term_family(ctx, P, len, B_SYS, C.lay.size, w, Y.c());
+      std::vector<double> yr(len), yi(len);
+      mdots_to_host(ctx, Y.c(), S.dim, len, psi, S.dim, yr.data());
+      mdots_to_host(ctx, Y.c(), S.dim, len, ipsi.c(), S.dim, yi.data());
+      for (int a = 1; a <= len; ++a) put(pos(a), pos(len + 1), cplx(yr[a - 1], yi[a - 1]));
+    }
+  }
+}
+
+void corr_all_real(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int kind, double* out) {
+  const int nsite = ctx->model.nsite();
+  std::vector<cplx> v((size_t)nsite * nsite);
+  corr_all(ctx, d, sys_len, psi, kind, v.data());
+  for (size_t i = 0; i < v.size(); ++i) out[i] = v[i].real();
 }
 
 // message of a failed hh_ctx_create (the context no longer exists)
@@ -2622,6 +2888,30 @@ int hh_driver_hopping_row(hh_driver* h, int sys_len, int site, const double* psi
   return guard(h->d.ctx, [&] {
     need(psi && g_out, "null argument");
     corr_row(h->d.ctx, h->d, sys_len, site, (const double2*)psi, OP_CDAG, (cplx*)g_out);
+  });
+}
+
+int hh_driver_density_all(hh_driver* h, int sys_len, const double* psi, double* nn_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && nn_out, "null argument");
+    corr_all_real(h->d.ctx, h->d, sys_len, (const double2*)psi, OP_N, nn_out);
+  });
+}
+
+int hh_driver_spin_all(hh_driver* h, int sys_len, const double* psi, double* ss_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && ss_out, "null argument");
+    corr_all_real(h->d.ctx, h->d, sys_len, (const double2*)psi, OP_S, ss_out);
+  });
+}
+
+int hh_driver_hopping_all(hh_driver* h, int sys_len, const double* psi, double* g_out) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(psi && g_out, "null argument");
+    corr_all(h->d.ctx, h->d, sys_len, (const double2*)psi, OP_CDAG, (cplx*)g_out);
   });
 }
 

@@ -38,6 +38,40 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def structure_factor_cut(cor, Lx, Ly, nx_cut=0):
+    """Get_structure_factor_cut (code/measure.f90:2159-2236): the structure
+    factor of a correlation matrix over the sites of columns nx_cut + 1 ..
+    Lx - nx_cut (N' sites, site label y + (x-1) Ly as in Get_Lattice,
+    code/warm_up.f90:796-812),
+
+        S(kx, ky) = (1/N') [ sum_x C(x,x) + sum_{x != y} e^{i k.(r_x - r_y)} C(x,y) ],
+
+    on the grid kx = 2 pi i / (Lx - 2 nx_cut), i = 0 .. Lx - 2 nx_cut, and
+    ky = 2 pi j / Ly, j = 0 .. Ly, with C the real part of `cor` (the
+    reference passes the correlation to a real array).  Returns (kx, ky, S),
+    S real with shape (Lx - 2 nx_cut + 1, Ly + 1)."""
+    c = np.real(np.asarray(cor))
+    n = Lx * Ly
+    if c.shape != (n, n):
+        raise HHError("correlation matrix must be (Lx*Ly, Lx*Ly)")
+    if nx_cut < 0 or 2 * nx_cut >= Lx:
+        raise HHError("nx_cut leaves no column")
+    kxn = Lx - 2 * nx_cut
+    kx = 2.0 * np.pi * np.arange(kxn + 1) / kxn
+    ky = 2.0 * np.pi * np.arange(Ly + 1) / Ly
+    sites = np.arange(nx_cut * Ly, n - nx_cut * Ly)
+    x = sites // Ly + 1.0
+    y = sites % Ly + 1.0
+    cs = c[np.ix_(sites, sites)]
+    off = cs - np.diag(np.diag(cs))
+    S = np.empty((kxn + 1, Ly + 1))
+    for i, qx in enumerate(kx):
+        for j, qy in enumerate(ky):
+            ph = np.exp(1j * (qx * x + qy * y))
+            S[i, j] = np.real(np.trace(cs) + ph @ off @ np.conj(ph))
+    return kx, ky, S / len(sites)
+
+
 class Batch:
     """Independent superblocks whose H.psi runs as one grouped launch per stage."""
 
@@ -527,6 +561,28 @@ class Driver:
                                                     out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_hopping_row")
         return out[:, 0] + 1j * out[:, 1]
+
+    def _all(self, fn, sys_len, psi, cplx):
+        _sync()
+        n = self.ctx.model.Lx * self.ctx.model.Ly
+        out = np.zeros((n, n, 2) if cplx else (n, n))
+        self.ctx._rc(getattr(L.load(), fn)(self.ptr, sys_len, _dptr(psi), out.ctypes.data_as(C.POINTER(C.c_double))),
+                     fn)
+        return out[:, :, 0] + 1j * out[:, :, 1] if cplx else out
+
+    def density_all(self, sys_len, psi):
+        """<n_i n_j> for every pair of lattice positions and <n_i^2> on the
+        diagonal (Get_oper_cor_ndia1 with n): (Nsite, Nsite), symmetric."""
+        return self._all("hh_driver_density_all", sys_len, psi, False)
+
+    def spin_all(self, sys_len, psi):
+        """<S_i . S_j> for every pair and <S_i . S_i> on the diagonal."""
+        return self._all("hh_driver_spin_all", sys_len, psi, False)
+
+    def hopping_all(self, sys_len, psi):
+        """sum_sigma <c^dag_i c_j> for every pair and <n_i> on the diagonal
+        (complex, Hermitian)."""
+        return self._all("hh_driver_hopping_all", sys_len, psi, True)
 
     def save(self, path):
         """Write the driver's blocks, truncations, carried wave function and step

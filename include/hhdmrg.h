@@ -279,6 +279,23 @@ int hh_driver_hopping_cross(hh_driver* d, int sys_len, const double* psi_dev, do
  * through the carried two-site product (Get_sys2b), block-centre as
  * sys_block_site_cor_ndia, env side as hh_driver_hopping_cross. */
 int hh_driver_hopping_row(hh_driver* d, int sys_len, int site, const double* psi_dev, double* g_out);
+/* every pair of sites and the on-site values at once: Get_oper_cor_ndia1
+ * (code/correlation.f90:2891-3107, the measbl path of Measurement,
+ * code/quantum.f90:556-575) with oper = oper1 = n.  Pairs inside one block
+ * as Get_sys_oper_cor_ndia1 / Get_env_oper_cor_ndia1 (:851-974,
+ * :1097-1216): each product formed when its later site joins and carried
+ * through the stored truncations; a block site with its side's centre site
+ * as sys_block_site_cor_ndia / env_block_site_cor_ndia; pairs across the
+ * centre as hh_driver_density_cross; on-site <n_i^2>.  nn_out is Nsite x
+ * Nsite, symmetric, row-major by lattice position - 1. */
+int hh_driver_density_all(hh_driver* d, int sys_len, const double* psi_dev, double* nn_out);
+/* the same for oper = oper1 = the spin: <S_i . S_j> for every pair, on-site
+ * <S_i . S_i> = 3/4 <n_i - 2 n_i,up n_i,down>. */
+int hh_driver_spin_all(hh_driver* d, int sys_len, const double* psi_dev, double* ss_out);
+/* the same for st_elec_up / st_elec_down, flag 'F': sum_sigma
+ * <c^dag_i,sigma c_j,sigma> for every pair, on-site <n_i>.  g_out is Nsite x
+ * Nsite complex (re, im interleaved), Hermitian. */
+int hh_driver_hopping_all(hh_driver* d, int sys_len, const double* psi_dev, double* g_out);
 /* singlet pair correlations <Delta^dag_a Delta_b>, Delta^dag_ij =
  * (c^dag_i,up c^dag_j,down - c^dag_i,down c^dag_j,up)/sqrt(2), for bonds a
  * inside the sys block and b inside the env block: the block-block part of

@@ -227,15 +227,13 @@ def singlet_ground_spin_correlations(model, positions, bonds, n_elec):
 
 def singlet_ground_hopping(model, positions, bonds, n_elec):
     """(E0, G) with G[i-1, j-1] = sum_sigma <c^dag_i,sigma c_j,sigma> in the
-    lowest total-S = 0 state (i != j; lattice positions)."""
+    lowest total-S = 0 state (lattice positions; the diagonal is <n_i>)."""
     e, psi, fs = _singlet_ground(model, positions, bonds, n_elec)
     n = max(positions)
     out = np.zeros((n, n), complex)
     order = {p: i for i, p in enumerate(positions)}
     for pa in positions:
         for pb in positions:
-            if pa == pb:
-                continue
             i, j = order[pa], order[pb]
             v = 0.0
             for sgm in (0, 1):

@@ -646,3 +646,142 @@ def pair_same_block(model, sys_blocks, sys_truns, sys_len, psi, H, quads):
                 v += complex(np.trace(rb @ m)) / math.sqrt(secs[b][1] + 1)
         out[q] = v
     return out
+
+
+def _onsite_op(kind, name):
+    """Rank-0 site operator whose expectation is the on-site entry of
+    Get_oper_cor_ndia1 (st_oper x st_oper1^dag of one site): n^2 for the
+    density, S.S = S(S+1) for the spin, sum_sigma c^dag c = n for the
+    one-body correlation; reduced elements sqrt(2S+1) f(s)."""
+    op = dmrg.Op(0, 0, False)
+    for i, (N, S2) in enumerate(dmrg.site_sectors(kind)):
+        f = {"n": N * N, "S": S2 * (S2 + 2) / 4.0, "cdag": N}[name]
+        if f:
+            op.add((i, i), [[math.sqrt(S2 + 1) * f]])
+    return op
+
+
+def _carry_site_op(model, blocks, truns, length, side, site_op):
+    """{k: f_k in the basis of blocks[length]} for a rank-0 site operator."""
+    st = dmrg.site_sectors(model.kind)
+    ops = {1: site_op}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        new = {k: _transform(dmrg.embed_block_op(o, prev, eb, side), truns[l]) for k, o in ops.items()}
+        new[l] = _transform(dmrg.embed_site_op(site_op, prev, eb, side), truns[l])
+        ops = new
+    return ops
+
+
+def _expect_block(rho, op, secs):
+    """<O> = sum_b Tr(rho_b O_b) / sqrt(2 S_b + 1) for a rank-0 block operator."""
+    v = 0j
+    for b, r in rho.items():
+        m = op.blk.get((b, b))
+        if m is not None:
+            v += complex(np.trace(r @ m)) / math.sqrt(secs[b][1] + 1)
+    return v
+
+
+def _pair_product(name, e, site, eb):
+    """Two-site scalar of block operator e and new-site operator site, both in
+    one enlarged basis: n_a n_b, S_a . S_b = -sqrt(3) [S_a x S_b]^0 or
+    sum_sigma c^dag_a c_b = sqrt(2) [c^dag_a x c~_b]^0."""
+    if name == "n":
+        return _product(e, site, eb)
+    if name == "S":
+        return {k: -math.sqrt(3.0) * v for k, v in _couple0(e, site, eb, 2).items()}
+    return {k: math.sqrt(2.0) * v for k, v in _couple0(e, dmrg._tilde_dict(site, eb.sectors), eb, 1).items()}
+
+
+def tracked_all_products(model, blocks, truns, length, side, name):
+    """({(a, b): Q_ab}, {k: O_k}) in the basis of blocks[length] for every pair
+    a < b of block sites of one side (Get_sys_oper_cor_ndia1 /
+    Get_env_oper_cor_ndia1, code/correlation.f90:851-974, :1097-1216): Q_ab
+    is formed in the enlarged basis of the step where b joins, from the
+    carried O_a and the new site's operator (``_pair_product``), then carried
+    as a rank-0 operator; O_k is the carried single-site operator."""
+    st = dmrg.site_sectors(model.kind)
+    O = dmrg.site_ops(model.kind)[name]
+    ones = {1: O}
+    prods = {}
+    for l in range(2, length + 1):
+        prev = blocks[l - 1]
+        eb = dmrg.enlarge_basis(prev.sectors, prev.dims, st)
+        site = dmrg.embed_site_op(O, prev, eb, side)
+        emb = {k: dmrg.embed_block_op(o, prev, eb, side) for k, o in ones.items()}
+        newp = {key: _transform(dmrg.embed_block_op(Q, prev, eb, side), truns[l]) for key, Q in prods.items()}
+        for a, e in emb.items():
+            newp[(a, l)] = _transform(_pair_product(name, e, site, eb), truns[l])
+        if name == "n":
+            ones = {k: _transform(e, truns[l]) for k, e in emb.items()}
+            ones[l] = _transform(site, truns[l])
+        else:
+            ones = {k: _transform_any(e, truns[l], O.k2, O.dN, O.fermion) for k, e in emb.items()}
+            ones[l] = _transform_any(site, truns[l], O.k2, O.dN, O.fermion)
+        prods = newp
+    return prods, ones
+
+
+def corr_all(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H, name):
+    """Get_oper_cor_ndia1 (code/correlation.f90:2891-3107) for name "n"
+    (<n_i n_j>), "S" (<S_i . S_j>) or "cdag" (sum_sigma <c^dag_i c_j>): every
+    pair of lattice positions and the on-site values, (nsite, nsite), index
+    position - 1; real for "n" and "S", complex Hermitian for "cdag".
+
+    Pairs across the centre: ``cross_density`` / ``spin_cross`` /
+    ``hopping_cross``.  Pairs of block sites of one side: the products of
+    ``tracked_all_products`` read off that side's block density matrix.  A
+    block site with its side's centre site: the same two-site scalar formed
+    in the superblock's enlarged basis and applied to psi (rows of the system
+    part, columns of the environment part).  On site: the carried
+    ``_onsite_op`` of every block site, and sum f(s) |psi_(b,s)|^2 for the
+    centre sites."""
+    nsite = model.nsite
+    out = np.full((nsite, nsite), np.nan, dtype=complex)
+    cross = {"n": cross_density, "S": spin_cross, "cdag": hopping_cross}[name]
+    cc = cross(model, sys_blocks, env_blocks, sys_truns, env_truns, sys_len, env_len, psi, H)
+    fin = ~np.isnan(cc)
+    out[fin] = cc[fin]
+    pin = dmrg.psi_blocks(H.sb, psi)
+    O = dmrg.site_ops(model.kind)[name]
+    fop = _onsite_op(model.kind, name)
+    fval = [fop.blk[(i, i)][0, 0] / math.sqrt(S2 + 1) if (i, i) in fop.blk else 0.0
+            for i, (_, S2) in enumerate(dmrg.site_sectors(model.kind))]
+    for side, blocks, truns, length in (("sys", sys_blocks, sys_truns, sys_len),
+                                        ("env", env_blocks, env_truns, env_len)):
+        pos = (lambda k: k) if side == "sys" else (lambda k, n=nsite: n + 1 - k)
+        centre = sys_len + 1 if side == "sys" else nsite - env_len
+        blk = blocks[length]
+        secs = blk.sectors
+        rho, _ = _block_rho(H, psi, side)
+        prods, ones = tracked_all_products(model, blocks, truns, length, side, name)
+        for (a, b), Q in prods.items():
+            v = _expect_block(rho, Q, secs)
+            out[pos(a) - 1, pos(b) - 1] = v
+            out[pos(b) - 1, pos(a) - 1] = np.conj(v)
+        for k, f in _carry_site_op(model, blocks, truns, length, side, fop).items():
+            out[pos(k) - 1, pos(k) - 1] = _expect_block(rho, f, secs).real
+        eb = H.s.basis if side == "sys" else H.e.basis
+        site = dmrg.embed_site_op(O, blk, eb, side)
+        diag = 0.0
+        for i, (x, y) in enumerate(H.sb.pairs):
+            z = x if side == "sys" else y
+            for (b, s, p0, d) in eb.subs[z]:
+                part = pin[i][p0:p0 + d, :] if side == "sys" else pin[i][:, p0:p0 + d]
+                diag += fval[s] * float(np.real(np.vdot(part, part)))
+        out[centre - 1, centre - 1] = diag
+        for a, o in ones.items():
+            Q = _pair_product(name, dmrg.embed_block_op(o, blk, eb, side), site, eb)
+            v = 0j
+            for i, (x, y) in enumerate(H.sb.pairs):
+                z = x if side == "sys" else y
+                m = Q.get((z, z))
+                if m is None:
+                    continue
+                w = pin[i]
+                v += complex(np.vdot(w, m @ w if side == "sys" else w @ m.T)) / math.sqrt(eb.sectors[z][1] + 1)
+            out[pos(a) - 1, centre - 1] = v
+            out[centre - 1, pos(a) - 1] = np.conj(v)
+    return out if name == "cdag" else out.real

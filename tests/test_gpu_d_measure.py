@@ -160,7 +160,7 @@ def _pair_bonds(m, L, R):
     return sys_b, env_b, [(m.nsite + 1 - b, m.nsite + 1 - a) for (a, b) in env_b]
 
 
-def _sweep(ctx, m, kept, tol):
+def _sweep(ctx, m, kept, tol, L=None):
     from hofstadter_hubbard_tsc_amd import Driver
     ctx.set_model(m)
     n = m.nsite
@@ -169,7 +169,7 @@ def _sweep(ctx, m, kept, tol):
     drv.sweep(0, n // 2 - 1, n - 2, tol, TRUN, kept, fresh=True, phase=1)
     drv.reset_transform()
     drv.sweep(1, n - 2, 1, tol, TRUN, kept, phase=1)
-    L = n // 2 - 1
+    L = n // 2 - 1 if L is None else L
     R = n - L - 2
     sb = ctx.bind(drv.block(0, L), drv.block(1, R), m.n_elec)
     psi = dev(dmrg.start_vector(sb.dim, 5))
@@ -249,3 +249,50 @@ def test_pair_same_block_truncated_matches_oracle(ctx):
     ref = measure.pair_same_block(m, sys_blocks, sys_truns, L, host(psi), H, quads)
     assert np.abs(ref).max() > 1e-4
     assert np.abs(got - ref).max() < 1e-10, np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("kind,flux,Lx,cut", [("hubbard", 0.15, 4, None), ("hubbard", 0.15, 4, 2),
+                                              ("tj", 0.2, 5, None), ("tj", 0.2, 5, 5)])
+def test_all_pairs_untruncated_equal_ed(ctx, kind, flux, Lx, cut):
+    """Get_oper_cor_ndia1 for the density, the spin and the one-body
+    correlation (every pair and the on-site values) against exact
+    diagonalisation at symmetric and asymmetric cuts; symmetry, Hermiticity
+    and the sum rules sum_j <n_i n_j> = N <n_i>, sum_j <S_i . S_j> = 0."""
+    m = lat.Model(kind=kind, Lx=Lx, Ly=2, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=6)
+    drv, L, R, E, psi = _sweep(ctx, m, 4000, 1e-12, cut)
+    sg = dmrg.StepGeometry(m, L, R)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    pos = list(range(1, m.nsite + 1))
+    e0, dens, nn = ed.singlet_ground_correlations(m, pos, bonds, m.n_elec)
+    _, ss = ed.singlet_ground_spin_correlations(m, pos, bonds, m.n_elec)
+    _, G = ed.singlet_ground_hopping(m, pos, bonds, m.n_elec)
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    A, B, C = drv.density_all(L, psi), drv.spin_all(L, psi), drv.hopping_all(L, psi)
+    assert not np.isnan(A).any() and not np.isnan(B).any() and not np.isnan(C).any()
+    assert np.abs(A - nn).max() < 1e-6, np.abs(A - nn).max()
+    assert np.abs(B - ss).max() < 1e-6, np.abs(B - ss).max()
+    assert np.abs(C - G).max() < 1e-6, np.abs(C - G).max()
+    assert np.array_equal(A, A.T) and np.array_equal(B, B.T) and np.array_equal(C, np.conj(C.T))
+    assert np.abs(A.sum(axis=1) - m.n_elec * np.diag(C).real).max() < 1e-9
+    assert np.abs(B.sum(axis=1)).max() < 1e-9
+
+
+@pytest.mark.parametrize("cut", [None, 3])
+def test_all_pairs_truncated_match_oracle(ctx, cut):
+    """Hubbard 6 x 2 ladder with flux at m = 24 (truncated): every entry of
+    the three correlation matrices against the oracle's all-pairs chains fed
+    with the GPU run's own blocks, truncations and ground state."""
+    from .gpuhelp import host, oracle_block, oracle_superblock
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=4.0, t=1.0, flux=0.1, n_elec=10)
+    drv, L, R, E, psi = _sweep(ctx, m, 24, 1e-11, cut)
+    got = {"n": drv.density_all(L, psi), "S": drv.spin_all(L, psi), "cdag": drv.hopping_all(L, psi)}
+    sys_blocks = {l: oracle_block(ctx, drv.block(0, l), m) for l in range(1, L + 1)}
+    env_blocks = {l: oracle_block(ctx, drv.block(1, l), m) for l in range(1, R + 1)}
+    sys_truns = {l: drv.trunc(0, l).matrices() for l in range(2, L + 1)}
+    env_truns = {l: drv.trunc(1, l).matrices() for l in range(2, R + 1)}
+    assert any(sum(u.shape[1] for _, u in t) < sum(u.shape[0] for _, u in t)
+               for t in list(sys_truns.values()) + list(env_truns.values())), "the case must truncate"
+    H = oracle_superblock(ctx, drv.block(0, L), drv.block(1, R), m, m.n_elec)
+    for name, g in got.items():
+        ref = measure.corr_all(m, sys_blocks, env_blocks, sys_truns, env_truns, L, R, host(psi), H, name)
+        assert np.abs(g - ref).max() < 1e-10, (name, np.abs(g - ref).max())
