@@ -73,8 +73,9 @@ def measure_at_scale(ctx, drv, model, sys_len):
     """Time the measurement entry points on the superblock of the last timed
     step (not part of the timed steps): its ground state by Lanczos, then the
     site densities, the correlation rows of the reference's row site
-    xi1 = (Lx/6 - 1) Ly + 1 (quantum.f90's Measurement), and pair
-    correlations of x bonds on the row site's leg."""
+    xi1 = (Lx/6 - 1) Ly + 1 (quantum.f90's Measurement), pair correlations
+    of x bonds on the row site's leg, and the every-pair matrices of the
+    measbl path."""
     import torch
     n, Ly = model.nsite, model.Ly
     R = n - sys_len - 2
@@ -98,7 +99,11 @@ def measure_at_scale(ctx, drv, model, sys_len):
              ("spin_row", lambda: drv.spin_row(sys_len, xi1, psi)),
              ("hopping_row", lambda: drv.hopping_row(sys_len, xi1, psi)),
              ("pair_cross", lambda: drv.pair_cross(sys_len, psi, sys_bonds + env_bonds)),
-             ("pair_block", lambda: drv.pair_block(sys_len, psi, [(sys_bonds[0], sys_bonds[1])]))]
+             ("pair_block", lambda: drv.pair_block(sys_len, psi, [(sys_bonds[0], sys_bonds[1])])),
+             ("density_all", lambda: drv.density_all(sys_len, psi)),
+             ("spin_all", lambda: drv.spin_all(sys_len, psi)),
+             ("hopping_all", lambda: drv.hopping_all(sys_len, psi))]
+    dens = None
     for name, fn in calls:
         t0 = time.time()
         try:
@@ -108,6 +113,14 @@ def measure_at_scale(ctx, drv, model, sys_len):
                 out["density_sum"] = float(v.sum())
             if name == "pair_block":
                 out["pair_block_value"] = [float(v[0].real), float(v[0].imag)]
+            if name == "density_profile":
+                dens = v
+            # sum rules, exact only without truncation: sum_j <n_i n_j> = N <n_i>,
+            # sum_j <S_i . S_j> = 0 in a singlet; reported as the truncation's effect
+            if name == "density_all" and dens is not None:
+                out["density_all_rowsum_dev"] = float(abs(v.sum(axis=1) - model.n_elec * dens).max())
+            if name == "spin_all":
+                out["spin_all_rowsum_max"] = float(abs(v.sum(axis=1)).max())
         except Exception as e:  # reported, never required
             out[name + "_error"] = str(e)[:200]
         print("# measure %s done" % name, file=sys.stderr, flush=True)

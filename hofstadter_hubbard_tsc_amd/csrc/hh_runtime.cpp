@@ -639,6 +639,12 @@ hh_block* site_block(hh_ctx* ctx, int side) {
 // ----------------------------------------------------------------------------
 // sweep driver: Get_Ground_State (code/quantum.f90:179-288)
 // ----------------------------------------------------------------------------
+// HHDMRG_VERBOSE=1: progress lines on stderr (sweep steps, measurement chains)
+bool verbose() {
+  static const bool v = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
+  return v;
+}
+
 struct Driver {
   hh_ctx* ctx;
   hh_schedule S;
@@ -691,9 +697,7 @@ struct Driver {
     r.grow_seconds = ph[4];
     (void)mv0;
     recs.push_back(r);
-    // HHDMRG_VERBOSE=1: one progress line per step on stderr
-    static const bool verbose = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
-    if (verbose) {
+    if (verbose()) {
       std::fprintf(stderr,
                    "# step phase=%d sys=%d env=%d dim=%lld iters=%d kept=%d E=%.12f %.2fs "
                    "(bind %.3f wave %.3f lanczos %.3f [H.psi %.3f] trunc %.3f grow %.3f)\n",
@@ -1288,6 +1292,10 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
     mdots_to_host(ctx, Y.c(), n, env_len + 1, psi, n, re.data());
     mdots_to_host(ctx, Y.c(), n, env_len + 1, ipsi.c(), n, im.data());
     for (int j = 1; j <= env_len + 1; ++j) val(i, j, cplx(re[j - 1], im[j - 1]));
+    if (verbose() && !only_i && (i % 8 == 0 || i == sys_len + 1)) {
+      std::fprintf(stderr, "# cross terms: sys site %d/%d done\n", i, sys_len + 1);
+      std::fflush(stderr);
+    }
   }
 }
 
@@ -1896,6 +1904,10 @@ void block_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Track
     std::swap(T.ops.bytes, nxt.bytes);
     T.lbl = nl;
     if (!dens) rank_step(ctx, t, kind, side, C);
+    if (verbose() && (l % 8 == 0 || l == len)) {
+      std::fprintf(stderr, "# pair chain side=%d length=%d/%d: %zu operators\n", side, l, len, T.lbl.size());
+      std::fflush(stderr);
+    }
   }
   T.osz = diag_op_size(T.sec);
 }

@@ -4,7 +4,7 @@ correlations <n_i n_j> (Get_density_cor: the pairs across the superblock
 centre, and whole rows with the two-site products carried in the system
 block), and of the spin correlations <S_i . S_j> and one-body correlations
 sum_sigma <c^dag_i c_j> and singlet pair correlations <Delta^dag_a Delta_b>
-across the centre:
+across the centre, and the every-pair matrices of Get_oper_cor_ndia1:
 untruncated runs against exact diagonalisation, truncated
 runs against the oracle's chain, and the sum rule sum_i <n_i> = N."""
 import numpy as np
