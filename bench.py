@@ -210,8 +210,6 @@ def run_c3(args):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if measure is not None:
-        out["config"]["measure"] = measure
     print(json.dumps(out), flush=True)
 
 
