@@ -37,6 +37,18 @@ static constexpr int WA = 2, WB = 2;   // 16x16 MFMA sub-tiles per wave: rows, c
 static constexpr int MTR = 16 * WA;     // macro tile rows (one wave)
 static constexpr int MTC = 16 * WB;     // macro tile columns
 static constexpr int WAVES = 4;        // independent waves per workgroup
+// FP64 complex products by the 3M scheme (3 real MFMAs per complex product)
+// in the full 32 x 32 shape class of task lists whose flop-weighted mean K is
+// at least HH_M3_MIN_K (HH_CPLX3 = 0 turns it off).  The 3M loop holds a third
+// accumulator set and runs at two waves per SIMD; it pays for long K and
+// costs a little on short K, where the four-product form at four waves per
+// SIMD hides latency better.
+#ifndef HH_CPLX3
+#define HH_CPLX3 1
+#endif
+#ifndef HH_M3_MIN_K
+#define HH_M3_MIN_K 64
+#endif
 // Term lists are cut so that the launch holds about this many equal work
 // items: enough that longest-first dispatch leaves a short tail.
 #ifndef HH_TARGET_ITEMS
@@ -144,6 +156,17 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
     }
     edge_skip = all > 0.0 && empty > HH_EDGE_SHARE * all;
   }
+  // 3M products for the full class when the flop-weighted mean K is long
+  {
+    double fk = 0.0, fkk = 0.0;
+    for (const Tile& t : tl.tiles)
+      for (int k = t.t0; k < t.t1; ++k) {
+        const double f = (double)t.M * t.N * tl.terms[k].K;
+        fk += f;
+        fkk += f * tl.terms[k].K;
+      }
+    cplx3 = HH_CPLX3 != 0 && fk > 0.0 && fkk >= HH_M3_MIN_K * fk;
+  }
   // longest work items first (LPT) inside each shape class
   auto cls = [&](const WorkItem& it) {
     if (!edge_skip) return 0;
@@ -226,7 +249,7 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // sub-tiles of 16 x 16 hold data) is its own instantiation with a single
 // pipeline, so no wave carries the code or registers of another shape.
 // ----------------------------------------------------------------------------
-template <typename T, int TAG, typename Bufs, int NA, int NB>
+template <typename T, int TAG, typename Bufs, int NA, int NB, bool CPLX3>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
@@ -278,6 +301,22 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
             accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
           }
         }
+  }
+  // FP64 complex products by the 3M scheme: with P1 = sum ar br, P2 = sum ai
+  // bi and P3 = sum (ar + ai)(br + bi), Re = P1 - P2 and Im = P3 - P1 - P2.
+  // During the GEMM terms accR holds P1, accP holds P2 and accI holds P3 (the
+  // elementwise terms above enter as P1 = Re, P3 = Re + Im).  FP32 keeps the
+  // four-product form.
+  constexpr bool M3 = std::is_same<T, double>::value && CPLX3;
+  V accP[M3 ? WA : 1][M3 ? WB : 1];
+  if constexpr (M3) {
+#pragma unroll
+    for (int a = 0; a < WA; ++a)
+#pragma unroll
+      for (int b = 0; b < WB; ++b) {
+        accP[a][b] = V{0, 0, 0, 0};
+        accI[a][b] += accR[a][b];
+      }
   }
   // GEMM terms (K > 0): one software pipeline over the whole term range.  The
   // operands of k-step kb are converted to MFMA inputs, the loads of the next
@@ -357,15 +396,31 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       if (last && tn < w.t1) load(ya, yb, make_op(dn), 0);
       else load(ya, yb, c, kb + 4);
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (M3) {
+        T sa[WA], sb[WB];
 #pragma unroll
-      for (int a = 0; a < NA; ++a)
+        for (int a = 0; a < NA; ++a) sa[a] = ar[a] + ai[a];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
-          accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
-          accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
-          accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
-        }
+        for (int b = 0; b < NB; ++b) sb[b] = br[b] + bi[b];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
+            accP[a][b] = Acc<T>::mfma(ai[a], bi[b], accP[a][b]);
+            accI[a][b] = Acc<T>::mfma(sa[a], sb[b], accI[a][b]);
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
+            accR[a][b] = Acc<T>::mfma(-ai[a], bi[b], accR[a][b]);
+            accI[a][b] = Acc<T>::mfma(ar[a], bi[b], accI[a][b]);
+            accI[a][b] = Acc<T>::mfma(ai[a], br[b], accI[a][b]);
+          }
+      }
       if (!last) {
         kb += 4;
       } else if (tn >= w.t1) {
@@ -387,6 +442,15 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   };
   static_assert(NA >= 1 && NA <= WA && NB >= 1 && NB <= WB, "sub-tile counts of a shape class");
   run();
+  if constexpr (M3) {
+#pragma unroll
+    for (int a = 0; a < WA; ++a)
+#pragma unroll
+      for (int b = 0; b < WB; ++b) {
+        accI[a][b] = accI[a][b] - accR[a][b] - accP[a][b];
+        accR[a][b] = accR[a][b] - accP[a][b];
+      }
+  }
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
@@ -440,14 +504,14 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
-template <typename T, typename Bufs, int NA, int NB>
+template <typename T, typename Bufs, int NA, int NB, bool CPLX3 = false>
 static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, int c) {
   const int n = t.cls_off[c + 1] - t.cls_off[c];
   if (n <= 0) return;
   const WorkItem* w = t.work + t.cls_off[c];
   const dim3 g((n + WAVES - 1) / WAVES), blk(64 * WAVES);
 #define HH_LAUNCH(TG) \
-  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, NA, NB>), g, blk, 0, s, t.tiles, t.terms, w, n, b, t.partial)
+  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, NA, NB, CPLX3>), g, blk, 0, s, t.tiles, t.terms, w, n, b, t.partial)
   if (tag == 1) HH_LAUNCH(1);
   else if (tag == 2) HH_LAUNCH(2);
   else if (tag == 3) HH_LAUNCH(3);
@@ -457,7 +521,8 @@ static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, 
 }
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  launch_ts<T, Bufs, 2, 2>(t, b, s, tag, 0);
+  if (std::is_same<T, double>::value && t.cplx3) launch_ts<T, Bufs, 2, 2, std::is_same<T, double>::value>(t, b, s, tag, 0);
+  else launch_ts<T, Bufs, 2, 2>(t, b, s, tag, 0);
   launch_ts<T, Bufs, 1, 2>(t, b, s, tag, 1);
   launch_ts<T, Bufs, 2, 1>(t, b, s, tag, 2);
   launch_ts<T, Bufs, 1, 1>(t, b, s, tag, 3);
