@@ -282,41 +282,42 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       accR[a][b] = V{0, 0, 0, 0};
       accI[a][b] = V{0, 0, 0, 0};
     }
-  // elementwise (K = 0) terms first, straight into the accumulators
-  for (int ti = w.t0; ti < w.t1; ++ti) {
-    const Term tm = terms[ti];
-    if (tm.K != 0) continue;
-    const double2* __restrict__ A = bufs.p[tm.a_buf];
-    const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
+  // elementwise (K = 0) terms, straight into the accumulators (before the
+  // GEMM terms in the four-product form, after them in the 3M form)
+  auto elementwise = [&]() {
+    for (int ti = w.t0; ti < w.t1; ++ti) {
+      const Term tm = terms[ti];
+      if (tm.K != 0) continue;
+      const double2* __restrict__ A = bufs.p[tm.a_buf];
+      const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
 #pragma unroll
-    for (int a = 0; a < WA; ++a)
+      for (int a = 0; a < WA; ++a)
 #pragma unroll
-      for (int b = 0; b < WB; ++b)
+        for (int b = 0; b < WB; ++b)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
-          if (r < Mv && c < Nv) {
-            double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
-            accR[a][b][v] += (T)(tm.cr * x.x - tm.ci * x.y);
-            accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
+          for (int v = 0; v < 4; ++v) {
+            const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
+            if (r < Mv && c < Nv) {
+              double2 x = ld_op(A, tm.a_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tm.a_ld, cj);
+              accR[a][b][v] += (T)(tm.cr * x.x - tm.ci * x.y);
+              accI[a][b][v] += (T)(tm.cr * x.y + tm.ci * x.x);
+            }
           }
-        }
-  }
+    }
+  };
   // FP64 complex products by the 3M scheme: with P1 = sum ar br, P2 = sum ai
   // bi and P3 = sum (ar + ai)(br + bi), Re = P1 - P2 and Im = P3 - P1 - P2.
-  // During the GEMM terms accR holds P1, accP holds P2 and accI holds P3 (the
-  // elementwise terms above enter as P1 = Re, P3 = Re + Im).  FP32 keeps the
-  // four-product form.
+  // During the GEMM terms accR holds P1, accP holds P2 and accI holds P3, all
+  // starting from zero.  FP32 keeps the four-product form.
   constexpr bool M3 = std::is_same<T, double>::value && CPLX3;
   V accP[M3 ? WA : 1][M3 ? WB : 1];
   if constexpr (M3) {
 #pragma unroll
     for (int a = 0; a < WA; ++a)
 #pragma unroll
-      for (int b = 0; b < WB; ++b) {
-        accP[a][b] = V{0, 0, 0, 0};
-        accI[a][b] += accR[a][b];
-      }
+      for (int b = 0; b < WB; ++b) accP[a][b] = V{0, 0, 0, 0};
+  } else {
+    elementwise();
   }
   // GEMM terms (K > 0): one software pipeline over the whole term range.  The
   // operands of k-step kb are converted to MFMA inputs, the loads of the next
@@ -433,11 +434,24 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
         kb = 0;
       }
     };
-    while (true) {
-      step(xa0, xb0, xa1, xb1);
-      if (done) break;
-      step(xa1, xb1, xa0, xb0);
-      if (done) break;
+    if constexpr (M3) {
+      // one step per iteration; the next step's operands move into the
+      // current buffer (the accumulators then keep one home across the loop)
+      while (true) {
+        step(xa0, xb0, xa1, xb1);
+        if (done) break;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) xa0[a] = xa1[a];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) xb0[b] = xb1[b];
+      }
+    } else {
+      while (true) {
+        step(xa0, xb0, xa1, xb1);
+        if (done) break;
+        step(xa1, xb1, xa0, xb0);
+        if (done) break;
+      }
     }
   };
   static_assert(NA >= 1 && NA <= WA && NB >= 1 && NB <= WB, "sub-tile counts of a shape class");
@@ -450,6 +464,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
         accI[a][b] = accI[a][b] - accR[a][b] - accP[a][b];
         accR[a][b] = accR[a][b] - accP[a][b];
       }
+    elementwise();
   }
   // write the macro tile once (or its partial sum)
   double2* __restrict__ C = bufs.p[tile.c_buf];
