@@ -308,9 +308,8 @@ def main():
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
             "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(s2_launches, 1),
-            "complex_products": "3M in the full 32x32 tiles (lists with flop-weighted mean K >= 64): 3 real FP64 "
-                                "MFMAs per complex multiply-add, so the MFMA units execute 3/4 of the 8 flops per "
-                                "complex multiply-add that achieved counts"}
+            "complex_products": "3M: 3 real FP64 MFMAs per complex multiply-add, so the MFMA units execute 3/4 "
+                                "of the 8 flops per complex multiply-add that achieved counts"}
     if args.dump_structure and recs:
         ls = recs[-1].sys_len + 1
         dump = {}

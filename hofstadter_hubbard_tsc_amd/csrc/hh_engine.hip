@@ -38,16 +38,10 @@ static constexpr int MTR = 16 * WA;     // macro tile rows (one wave)
 static constexpr int MTC = 16 * WB;     // macro tile columns
 static constexpr int WAVES = 4;        // independent waves per workgroup
 // FP64 complex products by the 3M scheme (3 real MFMAs per complex product)
-// in the full 32 x 32 shape class of task lists whose flop-weighted mean K is
-// at least HH_M3_MIN_K (HH_CPLX3 = 0 turns it off).  The 3M loop holds a third
-// accumulator set and runs at two waves per SIMD; it pays for long K and
-// costs a little on short K, where the four-product form at four waves per
-// SIMD hides latency better.
+// in every shape class; HH_CPLX3 = 0 builds the four-product form instead.
+// FP32 always uses four products.
 #ifndef HH_CPLX3
 #define HH_CPLX3 1
-#endif
-#ifndef HH_M3_MIN_K
-#define HH_M3_MIN_K 64
 #endif
 // Term lists are cut so that the launch holds about this many equal work
 // items: enough that longest-first dispatch leaves a short tail.
@@ -156,17 +150,6 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
     }
     edge_skip = all > 0.0 && empty > HH_EDGE_SHARE * all;
   }
-  // 3M products for the full class when the flop-weighted mean K is long
-  {
-    double fk = 0.0, fkk = 0.0;
-    for (const Tile& t : tl.tiles)
-      for (int k = t.t0; k < t.t1; ++k) {
-        const double f = (double)t.M * t.N * tl.terms[k].K;
-        fk += f;
-        fkk += f * tl.terms[k].K;
-      }
-    cplx3 = HH_CPLX3 != 0 && fk > 0.0 && fkk >= HH_M3_MIN_K * fk;
-  }
   // longest work items first (LPT) inside each shape class
   auto cls = [&](const WorkItem& it) {
     if (!edge_skip) return 0;
@@ -248,6 +231,7 @@ __device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t 
 // other: no LDS, no barriers.  Each shape class (NA x NB of the 2 x 2
 // sub-tiles of 16 x 16 hold data) is its own instantiation with a single
 // pipeline, so no wave carries the code or registers of another shape.
+// CPLX3 selects three real MFMAs per complex product (FP64) over four.
 // ----------------------------------------------------------------------------
 template <typename T, int TAG, typename Bufs, int NA, int NB, bool CPLX3>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
@@ -519,7 +503,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
-template <typename T, typename Bufs, int NA, int NB, bool CPLX3 = false>
+template <typename T, typename Bufs, int NA, int NB, bool CPLX3>
 static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, int c) {
   const int n = t.cls_off[c + 1] - t.cls_off[c];
   if (n <= 0) return;
@@ -536,11 +520,11 @@ static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, 
 }
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  if (std::is_same<T, double>::value && t.cplx3) launch_ts<T, Bufs, 2, 2, std::is_same<T, double>::value>(t, b, s, tag, 0);
-  else launch_ts<T, Bufs, 2, 2>(t, b, s, tag, 0);
-  launch_ts<T, Bufs, 1, 2>(t, b, s, tag, 1);
-  launch_ts<T, Bufs, 2, 1>(t, b, s, tag, 2);
-  launch_ts<T, Bufs, 1, 1>(t, b, s, tag, 3);
+  constexpr bool m3 = std::is_same<T, double>::value && HH_CPLX3 != 0;
+  launch_ts<T, Bufs, 2, 2, m3>(t, b, s, tag, 0);
+  launch_ts<T, Bufs, 1, 2, m3>(t, b, s, tag, 1);
+  launch_ts<T, Bufs, 2, 1, m3>(t, b, s, tag, 2);
+  launch_ts<T, Bufs, 1, 1, m3>(t, b, s, tag, 3);
 }
 
 template <typename Bufs>

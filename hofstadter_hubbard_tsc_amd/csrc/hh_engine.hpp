@@ -38,9 +38,6 @@ struct DevTasks {
   // empty 16 x 16 sub-tiles; otherwise every item is class 0.
   int cls_off[5] = {0, 0, 0, 0, 0};
   bool edge_skip = false;
-  // full-class items in FP64 use 3 real MFMAs per complex product (set when
-  // the flop-weighted mean K of the list is long)
-  bool cplx3 = false;
   void upload(const TaskList& tl, hipStream_t s);
   void release();
   ~DevTasks() { release(); }
