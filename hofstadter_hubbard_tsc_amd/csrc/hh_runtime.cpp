@@ -63,6 +63,13 @@ struct DevMem {
   double2* c() const { return (double2*)p; }
 };
 
+// HHDMRG_VERBOSE=1: progress lines on stderr (sweep steps, truncation phases,
+// measurement chains)
+bool verbose() {
+  static const bool v = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
+  return v;
+}
+
 double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -375,7 +382,11 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
 // (about 20-30 us per column) with host synchronisations inside, so the
 // sectors are spread over the context stream and its side streams (largest
 // first, each to the stream with the fewest columns so far) and every stream
-// is driven by its own host thread.
+// is driven by its own host thread.  Sectors are not batched: a strided-
+// batched zheevd of sectors padded to one size returns NaN eigenvectors
+// (with info = 0) for every padded member once the size reaches the
+// divide-and-conquer range, whatever the pad block, because the padded
+// matrix is block diagonal.
 template <typename OnFail>
 void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, const std::vector<int>& rdim,
                   double* evals, double* work, int64_t work_stride, rocblas_int* info, OnFail on_fail) {
@@ -441,6 +452,7 @@ void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, 
 hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi, int kept_min, int kept_max,
                    double trun_err) {
   const SuperBasis& S = sb->plan.sb;
+  const double t0 = wall();
   std::vector<int> rsec, rdim;
   std::vector<int64_t> roff;
   TaskList tl = compile_density(S, side, rsec, roff, rdim);
@@ -464,6 +476,11 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   evals.alloc(sizeof(double) * (size_t)std::max<int64_t>(nev, 1));
   work.alloc(sizeof(double) * (size_t)dmax * (hh_ctx::NSIDE + 1));
   info.alloc(sizeof(rocblas_int) * rdim.size() + 16);
+  double t1 = 0.0;
+  if (verbose()) {
+    check_hip(hipStreamSynchronize(ctx->stream), "density matrix");
+    t1 = wall();
+  }
   heev_sectors(ctx, t->evec.c(), roff, rdim, (double*)evals.p, (double*)work.p, dmax, (rocblas_int*)info.p,
                [&]() { delete t; });
   std::vector<double> ev(nev);
@@ -472,10 +489,22 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   check_hip(hipMemcpyAsync(inf.data(), info.p, sizeof(rocblas_int) * rdim.size(), hipMemcpyDeviceToHost, ctx->stream),
             "d2h");
   check_hip(hipStreamSynchronize(ctx->stream), "eig");
+  if (verbose()) {
+    int64_t cols = 0;
+    for (int d : rdim) cols += d;
+    std::fprintf(stderr, "# truncation side=%d: density matrix %.4f s, zheevd %.4f s (%zu sectors, largest %d, %lld "
+                 "columns)\n", side, t1 - t0, wall() - t1, rdim.size(), dmax, (long long)cols);
+    std::fflush(stderr);
+  }
   for (auto x : inf)
     if (x != 0) {
       delete t;
       throw HHError(-2, "rocsolver_zheevd did not converge");
+    }
+  for (double x : ev)
+    if (!std::isfinite(x)) {
+      delete t;
+      throw HHError(-2, "rocsolver_zheevd returned a non-finite eigenvalue");
     }
   // global descending sort, ties by position in the concatenated ascending
   // per-sector spectra (deterministic stand-in for QuickSort_Double)
@@ -639,12 +668,6 @@ hh_block* site_block(hh_ctx* ctx, int side) {
 // ----------------------------------------------------------------------------
 // sweep driver: Get_Ground_State (code/quantum.f90:179-288)
 // ----------------------------------------------------------------------------
-// HHDMRG_VERBOSE=1: progress lines on stderr (sweep steps, measurement chains)
-bool verbose() {
-  static const bool v = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
-  return v;
-}
-
 struct Driver {
   hh_ctx* ctx;
   hh_schedule S;
