@@ -108,6 +108,8 @@ SIGNATURES = [
     ("hh_driver_pair_cross", i32, [vp, i32, vp, i32, P(i32), P(f64)]),
     ("hh_driver_pair_block", i32, [vp, i32, vp, i32, P(i32), P(f64)]),
     ("hh_driver_density_all", i32, [vp, i32, vp, P(f64)]),
+    ("hh_density_matrix_layout", i32, [vp, i32, P(i32), P(i64), P(i32), P(i32), P(i64)]),
+    ("hh_density_matrix", i32, [vp, vp, i32, vp, vp]),
     ("hh_driver_spin_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_hopping_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_trunc", i32, [vp, i32, i32, P(vp)]),

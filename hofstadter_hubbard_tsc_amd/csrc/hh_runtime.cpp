@@ -2488,6 +2488,37 @@ int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi, double lanzero, int 
   });
 }
 
+int hh_density_matrix_layout(const hh_superblock* sb, int side, int32_t* nblk, int64_t* size, int32_t* sector,
+                             int32_t* dim, int64_t* off) {
+  if (!sb || !nblk || !size || (side != 0 && side != 1)) return 1;
+  std::vector<int> rsec, rdim;
+  std::vector<int64_t> roff;
+  (void)compile_density(sb->plan.sb, side, rsec, roff, rdim);
+  *nblk = (int32_t)rsec.size();
+  *size = 0;
+  for (size_t i = 0; i < rsec.size(); ++i) {
+    *size += (int64_t)rdim[i] * rdim[i];
+    if (sector) sector[i] = rsec[i];
+    if (dim) dim[i] = rdim[i];
+    if (off) off[i] = roff[i];
+  }
+  return 0;
+}
+
+int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi, double* rho) {
+  return guard(ctx, [&] {
+    need(sb && psi && rho, "null argument");
+    need(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
+    std::vector<int> rsec, rdim;
+    std::vector<int64_t> roff;
+    TaskList tl = compile_density(sb->plan.sb, side, rsec, roff, rdim);
+    BufPtrs b{};
+    b.p[B_IN] = (double2*)psi;
+    b.p[B_OUT] = (double2*)rho;
+    launch_on(ctx, tl, b);
+  });
+}
+
 int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi, int kept_min, int kept_max,
                 double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy) {
   return guard(ctx, [&] {

@@ -367,6 +367,28 @@ class Context:
                  "hh_lanczos")
         return e.value, it.value
 
+    def density_matrix(self, sb, side, psi):
+        """Get_density_matrix_sys_new / _env_new (code/get_den.f90:106-177):
+        [(enlarged sector, rho)] in super-basis order, rho a host ndarray."""
+        import torch
+        lib = L.load()
+        nb, size = L.i32(), C.c_int64()
+        self._rc(lib.hh_density_matrix_layout(sb.ptr, side, C.byref(nb), C.byref(size), None, None, None),
+                 "hh_density_matrix_layout")
+        sec = np.zeros(max(nb.value, 1), np.int32)
+        dim = np.zeros(max(nb.value, 1), np.int32)
+        off = np.zeros(max(nb.value, 1), np.int64)
+        self._rc(lib.hh_density_matrix_layout(sb.ptr, side, C.byref(nb), C.byref(size),
+                                              sec.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              dim.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              off.ctypes.data_as(C.POINTER(C.c_int64))), "hh_density_matrix_layout")
+        rho = torch.zeros(max(size.value, 1), dtype=torch.complex128, device="cuda")
+        _sync()
+        self._rc(lib.hh_density_matrix(self.ptr, sb.ptr, side, _dptr(psi), _dptr(rho)), "hh_density_matrix")
+        h = rho.cpu().numpy()
+        return [(int(sec[i]), h[off[i]:off[i] + dim[i] * dim[i]].reshape((dim[i], dim[i]), order="F"))
+                for i in range(nb.value)]
+
     def truncate(self, sb, side, psi, kept_min, kept_max, trun_err):
         """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90)."""
         _sync()

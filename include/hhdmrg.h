@@ -159,6 +159,18 @@ int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in_de
 int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi_dev, double lanzero, int max_iter, double* e0,
                int* iters);
 
+/* Get_density_matrix_sys_new / Get_density_matrix_env_new
+ * (code/get_den.f90:106-177, levels = 1): the reduced density matrix of the
+ * enlarged block of `side` (0 sys: psi psi^dag, 1 env: psi^T conj(psi)), one
+ * block per psi sector in super-basis order.  hh_density_matrix_layout gives
+ * the block count and the total size in complex elements; when sector, dim
+ * and off are not null it also fills, per block, the enlarged sector index,
+ * the dimension and the element offset.  hh_density_matrix writes the blocks
+ * (complex128, column-major, dim x dim at off) to rho_dev. */
+int hh_density_matrix_layout(const hh_superblock* sb, int side, int32_t* nblk, int64_t* size, int32_t* sector,
+                             int32_t* dim, int64_t* off);
+int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, double* rho_dev);
+
 /* Reduced density matrix + Get_truncation_operator
  * (code/get_den.f90:4-99, :184-428): side 0 sys (psi psi^dag), 1 env. */
 int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, int kept_min, int kept_max,

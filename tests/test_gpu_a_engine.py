@@ -144,3 +144,31 @@ def test_lanczos_refuses_non_finite_blocks(ctx):
     psi = dev(_rand(np.random.default_rng(0), sb.dim))
     with pytest.raises(HHError, match="non-finite"):
         ctx.lanczos(sb, psi, 1e-10, 50)
+
+
+@pytest.mark.parametrize("kind,Lx,Ly,flux,N,kept", CASES[:2])
+def test_density_matrix_matches_oracle(ctx, kind, Lx, Ly, flux, N, kept):
+    """Get_density_matrix_sys_new / _env_new through the C ABI: every block
+    equals the oracle's psi psi^dag (sys) or psi^T conj(psi) (env) of the same
+    psi, in super-basis order, and the traces sum to |psi|^2."""
+    model = lat.Model(kind=kind, Lx=Lx, Ly=Ly, U=8.0, J=0.4, t=1.0, flux=flux, n_elec=N)
+    ctx.set_model(model)
+    run = _oracle_run(model, kept)
+    nk = 1 if kind == "hubbard" else 3
+    n = model.nsite
+    ns = n // 2 - 1
+    sysb, envb = run.sys_blocks[ns], run.env_blocks[n - ns - 2]
+    sg = dmrg.StepGeometry(model, sysb.length, envb.length)
+    H = dmrg.Superblock(dmrg.enlarge_in_geometry(sysb, model, "sys", sg),
+                        dmrg.enlarge_in_geometry(envb, model, "env", sg), sg, model, N)
+    sb = ctx.bind(upload_block(ctx, sysb, 0, nk), upload_block(ctx, envb, 1, nk), N)
+    psi = _rand(np.random.default_rng(7), H.sb.dim)
+    psi /= np.linalg.norm(psi)
+    for side, name in ((0, "sys"), (1, "env")):
+        got = ctx.density_matrix(sb, side, dev(psi))
+        ref = dmrg.density_matrix(H.sb, psi, name)
+        assert [a for a, _ in got] == [a for a, _ in ref]
+        for (_, g), (_, r) in zip(got, ref):
+            assert g.shape == r.shape
+            assert np.abs(g - r).max() < 1e-14 * max(1.0, np.abs(r).max())
+        assert abs(sum(np.trace(g).real for _, g in got) - 1.0) < 1e-12
