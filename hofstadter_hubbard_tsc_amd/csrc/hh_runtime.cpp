@@ -1276,6 +1276,8 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
   const Geometry g = make_geometry(m, sys_len, env_len);
   DevMem Y, ipsi;
   std::vector<double> re(env_len + 1), im(env_len + 1);
+  const double t_start = wall();
+  double t_compile = 0.0;
   for (int i = 1; i <= sys_len + 1; ++i) {
     if (only_i && i != only_i) continue;
     auto term = [&](bool env_block) {
@@ -1298,8 +1300,10 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
       return t;
     };
     const MeasureTerm tb = term(true), tc = term(false);
+    const double t0 = wall();
     const MatvecPlan Pb = compile_matvec(m, g, sd, ed, m.n_elec, &tb);
     const MatvecPlan Pc = compile_matvec(m, g, sd, ed, m.n_elec, &tc);
+    t_compile += wall() - t0;
     const int64_t n = Pb.sb.dim;
     Y.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n * (env_len + 1), 1));
     if (!ipsi.p) {   // i psi: Re <y, i psi> = Im <psi, y>
@@ -1316,7 +1320,8 @@ void cross_terms(hh_ctx* ctx, Driver& d, int sys_len, const double2* psi, int ki
     mdots_to_host(ctx, Y.c(), n, env_len + 1, ipsi.c(), n, im.data());
     for (int j = 1; j <= env_len + 1; ++j) val(i, j, cplx(re[j - 1], im[j - 1]));
     if (verbose() && !only_i && (i % 8 == 0 || i == sys_len + 1)) {
-      std::fprintf(stderr, "# cross terms: sys site %d/%d done\n", i, sys_len + 1);
+      std::fprintf(stderr, "# cross terms: sys site %d/%d done, %.2f s (%.2f s compiling)\n", i, sys_len + 1,
+                   wall() - t_start, t_compile);
       std::fflush(stderr);
     }
   }
@@ -1851,8 +1856,9 @@ void density_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi,
 constexpr int64_t TRACK_X_ELEMS = int64_t(1) << 27;   // 2 GiB of complex128
 std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, const hh_trunc* t,
                                    const std::vector<TrackSrc>& srcs, int side, const double2* carried,
-                                   const double2* rank_ops, DevMem& out) {
+                                   const double2* rank_ops, DevMem& out, double* compile_seconds = nullptr) {
   const ModelParams& m = ctx->model;
+  double tc = wall();
   const TrackPlan P1 = compile_track_ops(m, sec, t->kept, {srcs.at(0)}, side);
   const int64_t per = std::max<int64_t>(P1.x_size, 1), nsrc = (int64_t)srcs.size();
   const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsrc, TRACK_X_ELEMS / per));
@@ -1861,7 +1867,9 @@ std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, 
   X.alloc(sizeof(double2) * (size_t)(per * chunk));
   for (int64_t q0 = 0; q0 < nsrc; q0 += chunk) {
     const std::vector<TrackSrc> part(srcs.begin() + q0, srcs.begin() + std::min(nsrc, q0 + chunk));
+    if (q0 > 0) tc = wall();
     const TrackPlan P = part.size() == 1 && q0 == 0 ? P1 : compile_track_ops(m, sec, t->kept, part, side);
+    if (compile_seconds) *compile_seconds += wall() - tc;
     BufPtrs b{};
     b.p[B_SYS] = (double2*)carried;
     b.p[B_ENV] = (double2*)rank_ops;
@@ -1898,6 +1906,8 @@ void block_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Track
   }
   DevMem nxt;
   const TrackSrc::What prod = kind == OP_S ? TrackSrc::SPIN_PRODUCT : TrackSrc::HOP_PRODUCT;
+  const double t_start = wall();
+  double t_compile = 0.0;
   for (int l = 2; l <= len; ++l) {
     need(blocks.count(l) && truns.count(l), "the driver holds no truncation for an intermediate length");
     const hh_trunc* t = truns.at(l).get();
@@ -1922,13 +1932,14 @@ void block_pair_chain(hh_ctx* ctx, Driver& d, int side, int len, int kind, Track
       }
     }
     T.sec = track_ops_step(ctx, T.sec, t, srcs, side, T.ops.p ? T.ops.c() : nullptr, dens ? nullptr : C.ops.c(),
-                           nxt);
+                           nxt, &t_compile);
     std::swap(T.ops.p, nxt.p);
     std::swap(T.ops.bytes, nxt.bytes);
     T.lbl = nl;
     if (!dens) rank_step(ctx, t, kind, side, C);
     if (verbose() && (l % 8 == 0 || l == len)) {
-      std::fprintf(stderr, "# pair chain side=%d length=%d/%d: %zu operators\n", side, l, len, T.lbl.size());
+      std::fprintf(stderr, "# pair chain side=%d length=%d/%d: %zu operators, %.2f s (%.2f s compiling)\n", side, l,
+                   len, T.lbl.size(), wall() - t_start, t_compile);
       std::fflush(stderr);
     }
   }
