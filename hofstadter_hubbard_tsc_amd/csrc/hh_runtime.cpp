@@ -1852,7 +1852,11 @@ void density_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi,
 
 // One chain step for a long source list: compile_track_ops in chunks whose
 // stage-1 scratch stays within TRACK_X_ELEMS complex elements, every chunk
-// writing its operators at its place in `out`.  Returns the new sectors.
+// writing its operators at its place in `out`.  The host compiles chunk q + 1
+// while the GPU runs chunk q: the launches are asynchronous on the context
+// stream (in order, so the chunks can share one scratch buffer), and the
+// task lists of chunk q stay alive until the upload of chunk q + 1 has
+// synchronised the stream.  Returns the new sectors.
 constexpr int64_t TRACK_X_ELEMS = int64_t(1) << 27;   // 2 GiB of complex128
 std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, const hh_trunc* t,
                                    const std::vector<TrackSrc>& srcs, int side, const double2* carried,
@@ -1865,19 +1869,27 @@ std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, 
   out.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P1.new_size * nsrc, 1));
   DevMem X;
   X.alloc(sizeof(double2) * (size_t)(per * chunk));
+  std::unique_ptr<DevTasks> run1, run2;   // the chunk in flight
   for (int64_t q0 = 0; q0 < nsrc; q0 += chunk) {
     const std::vector<TrackSrc> part(srcs.begin() + q0, srcs.begin() + std::min(nsrc, q0 + chunk));
     if (q0 > 0) tc = wall();
     const TrackPlan P = part.size() == 1 && q0 == 0 ? P1 : compile_track_ops(m, sec, t->kept, part, side);
     if (compile_seconds) *compile_seconds += wall() - tc;
+    auto s1 = std::make_unique<DevTasks>(), s2 = std::make_unique<DevTasks>();
+    s1->upload(P.stage1, ctx->stream);   // synchronises the stream: the previous chunk is done
+    s2->upload(P.stage2, ctx->stream);
+    run1 = std::move(s1);
+    run2 = std::move(s2);
     BufPtrs b{};
     b.p[B_SYS] = (double2*)carried;
     b.p[B_ENV] = (double2*)rank_ops;
     b.p[B_AUX] = t->evec.c();
     b.p[B_X] = X.c();
     b.p[B_OUT] = out.c() + q0 * P1.new_size;
-    run_plan(ctx, P, b);
+    launch_tiles(*run1, b, ctx->stream, PREC_FP64);
+    launch_tiles(*run2, b, ctx->stream, PREC_FP64);
   }
+  check_hip(hipStreamSynchronize(ctx->stream), "operator chain");
   return kept_sectors(P1.eb, t);
 }
 
