@@ -18,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1852,32 +1853,56 @@ void density_row(hh_ctx* ctx, Driver& d, int sys_len, int r, const double2* psi,
 
 // One chain step for a long source list: compile_track_ops in chunks whose
 // stage-1 scratch stays within TRACK_X_ELEMS complex elements, every chunk
-// writing its operators at its place in `out`.  The host compiles chunk q + 1
-// while the GPU runs chunk q: the launches are asynchronous on the context
-// stream (in order, so the chunks can share one scratch buffer), and the
-// task lists of chunk q stay alive until the upload of chunk q + 1 has
-// synchronised the stream.  Returns the new sectors.
+// writing its operators at its place in `out`.  The chunks are compiled on up
+// to TRACK_THREADS host threads (the compilers are pure functions of their
+// inputs), then uploaded and launched in order on the context stream; the
+// chunks share one scratch buffer, and the task lists of a chunk stay alive
+// until the next upload has synchronised the stream.  Returns the new sectors.
 constexpr int64_t TRACK_X_ELEMS = int64_t(1) << 27;   // 2 GiB of complex128
+constexpr int TRACK_THREADS = 16;
 std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, const hh_trunc* t,
                                    const std::vector<TrackSrc>& srcs, int side, const double2* carried,
                                    const double2* rank_ops, DevMem& out, double* compile_seconds = nullptr) {
   const ModelParams& m = ctx->model;
-  double tc = wall();
+  const double tc = wall();
   const TrackPlan P1 = compile_track_ops(m, sec, t->kept, {srcs.at(0)}, side);
   const int64_t per = std::max<int64_t>(P1.x_size, 1), nsrc = (int64_t)srcs.size();
   const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsrc, TRACK_X_ELEMS / per));
+  const int64_t nchunks = (nsrc + chunk - 1) / chunk;
+  std::vector<TrackPlan> plans((size_t)nchunks);
+  {
+    std::atomic<int64_t> next{0};
+    std::atomic<bool> failed{false};
+    std::string why;
+    std::mutex mu;
+    auto work = [&]() {
+      for (int64_t c; (c = next++) < nchunks && !failed;) {
+        try {
+          const std::vector<TrackSrc> part(srcs.begin() + c * chunk, srcs.begin() + std::min(nsrc, (c + 1) * chunk));
+          plans[(size_t)c] = compile_track_ops(m, sec, t->kept, part, side);
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!failed) why = e.what();
+          failed = true;
+        }
+      }
+    };
+    const int nth = (int)std::min<int64_t>(nchunks, TRACK_THREADS);
+    std::vector<std::thread> th;
+    for (int k = 1; k < nth; ++k) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+    need(!failed, "operator chain compilation: " + why);
+  }
+  if (compile_seconds) *compile_seconds += wall() - tc;
   out.alloc(sizeof(double2) * (size_t)std::max<int64_t>(P1.new_size * nsrc, 1));
   DevMem X;
   X.alloc(sizeof(double2) * (size_t)(per * chunk));
   std::unique_ptr<DevTasks> run1, run2;   // the chunk in flight
-  for (int64_t q0 = 0; q0 < nsrc; q0 += chunk) {
-    const std::vector<TrackSrc> part(srcs.begin() + q0, srcs.begin() + std::min(nsrc, q0 + chunk));
-    if (q0 > 0) tc = wall();
-    const TrackPlan P = part.size() == 1 && q0 == 0 ? P1 : compile_track_ops(m, sec, t->kept, part, side);
-    if (compile_seconds) *compile_seconds += wall() - tc;
+  for (int64_t c = 0; c < nchunks; ++c) {
     auto s1 = std::make_unique<DevTasks>(), s2 = std::make_unique<DevTasks>();
-    s1->upload(P.stage1, ctx->stream);   // synchronises the stream: the previous chunk is done
-    s2->upload(P.stage2, ctx->stream);
+    s1->upload(plans[(size_t)c].stage1, ctx->stream);   // synchronises the stream: the previous chunk is done
+    s2->upload(plans[(size_t)c].stage2, ctx->stream);
     run1 = std::move(s1);
     run2 = std::move(s2);
     BufPtrs b{};
@@ -1885,7 +1910,7 @@ std::vector<Sector> track_ops_step(hh_ctx* ctx, const std::vector<Sector>& sec, 
     b.p[B_ENV] = (double2*)rank_ops;
     b.p[B_AUX] = t->evec.c();
     b.p[B_X] = X.c();
-    b.p[B_OUT] = out.c() + q0 * P1.new_size;
+    b.p[B_OUT] = out.c() + c * chunk * P1.new_size;
     launch_tiles(*run1, b, ctx->stream, PREC_FP64);
     launch_tiles(*run2, b, ctx->stream, PREC_FP64);
   }
