@@ -252,7 +252,8 @@ def test_pair_same_block_truncated_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("kind,flux,Lx,cut", [("hubbard", 0.15, 4, None), ("hubbard", 0.15, 4, 2),
-                                              ("tj", 0.2, 5, None), ("tj", 0.2, 5, 5)])
+                                              ("tj", 0.2, 5, None), ("tj", 0.2, 5, 5), ("hubbard", 0.15, 4, 1),
+                                              ("tj", 0.2, 5, 7)])
 def test_all_pairs_untruncated_equal_ed(ctx, kind, flux, Lx, cut):
     """Get_oper_cor_ndia1 for the density, the spin and the one-body
     correlation (every pair and the on-site values) against exact

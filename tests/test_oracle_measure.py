@@ -345,7 +345,7 @@ def test_untruncated_pair_same_block_equals_ed(kind, flux):
 
 
 @pytest.mark.parametrize("kind,flux,Lx,L", [("hubbard", 0.15, 4, 2), ("hubbard", 0.15, 4, 3), ("tj", 0.2, 5, 2),
-                                            ("tj", 0.2, 5, 5)])
+                                            ("tj", 0.2, 5, 5), ("hubbard", 0.15, 4, 1), ("tj", 0.2, 5, 7)])
 def test_untruncated_all_pairs_equal_ed(kind, flux, Lx, L):
     """Every pair and every on-site value of Get_oper_cor_ndia1 for the
     density, the spin and the one-body correlation (products of block sites
