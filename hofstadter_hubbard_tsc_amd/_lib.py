@@ -110,6 +110,8 @@ SIGNATURES = [
     ("hh_driver_density_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_density_matrix_layout", i32, [vp, i32, P(i32), P(i64), P(i32), P(i32), P(i64)]),
     ("hh_density_matrix", i32, [vp, vp, i32, vp, vp]),
+    ("hh_wave_to_disk", i32, [vp, vp, vp, C.c_char_p]),
+    ("hh_wave_from_disk", i32, [vp, vp, C.c_char_p, vp]),
     ("hh_driver_spin_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_hopping_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_trunc", i32, [vp, i32, i32, P(vp)]),

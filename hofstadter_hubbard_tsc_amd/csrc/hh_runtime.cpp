@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <map>
 #include <memory>
@@ -2412,6 +2413,113 @@ int hh_superblock_pairs(const hh_superblock* sb, int32_t* sys_sec, int32_t* env_
   }
   return 0;
 }
+// ---- wave_to_disk / wave_from_disk (code/space_info.f90:1406-1468) --------
+// The reference's wavefunction file: Fortran sequential unformatted records,
+// each framed by its byte length as a 4-byte integer before and after.
+// Records: sys_len, env_len, num, dim (default integers; the lengths are
+// those of the enlarged blocks), then per super sector in super-basis order
+// an empty record, (N_sys, 2S_sys), (N_env, 2S_env), (sys_dim, env_dim) and
+// the sys_dim x env_dim complex*16 block, column-major.
+namespace {
+struct FortranOut {
+  FILE* f;
+  void record(const void* p, size_t n) {
+    need(n < (size_t(1) << 31), "wave file record too large for 4-byte record markers");
+    const int32_t m = (int32_t)n;
+    need(std::fwrite(&m, 4, 1, f) == 1 && (n == 0 || std::fwrite(p, 1, n, f) == n) && std::fwrite(&m, 4, 1, f) == 1,
+         "cannot write the wave file");
+  }
+  void ints(std::initializer_list<int64_t> v) {
+    std::vector<int32_t> a;
+    for (int64_t x : v) a.push_back((int32_t)x);
+    record(a.data(), 4 * a.size());
+  }
+};
+struct FortranIn {
+  FILE* f;
+  std::vector<char> record() {
+    int32_t m = 0, m2 = 0;
+    need(std::fread(&m, 4, 1, f) == 1 && m >= 0, "wave file: truncated or not a Fortran unformatted file");
+    std::vector<char> b((size_t)m);
+    need((m == 0 || std::fread(b.data(), 1, (size_t)m, f) == (size_t)m) && std::fread(&m2, 4, 1, f) == 1 && m2 == m,
+         "wave file: damaged record");
+    return b;
+  }
+  std::vector<int32_t> ints(size_t n) {
+    std::vector<char> b = record();
+    need(b.size() == 4 * n, "wave file: unexpected record length");
+    std::vector<int32_t> v(n);
+    std::memcpy(v.data(), b.data(), 4 * n);
+    return v;
+  }
+};
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+int hh_wave_to_disk(hh_ctx* ctx, const hh_superblock* sb, const double* psi, const char* path) {
+  return guard(ctx, [&] {
+    need(sb && psi && path, "null argument");
+    const SuperBasis& S = sb->plan.sb;
+    std::vector<double2> h((size_t)std::max<int64_t>(S.dim, 1));
+    check_hip(hipMemcpyAsync(h.data(), psi, sizeof(double2) * (size_t)S.dim, hipMemcpyDeviceToHost, ctx->stream),
+              "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "d2h");
+    need(S.dim < (int64_t(1) << 31), "superblock too large for the reference's default-integer dim");
+    FileCloser fc{std::fopen(path, "wb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    FortranOut o{fc.f};
+    o.ints({sb->sys->d.length + 1});
+    o.ints({sb->env->d.length + 1});
+    o.ints({(int64_t)S.pairs.size()});
+    o.ints({S.dim});
+    for (size_t i = 0; i < S.pairs.size(); ++i) {
+      const Sector& a = sb->plan.sbs.sec[S.pairs[i].first];
+      const Sector& b = sb->plan.ebs.sec[S.pairs[i].second];
+      o.record(nullptr, 0);
+      o.ints({a.N, a.S2});
+      o.ints({b.N, b.S2});
+      o.ints({S.rows[i], S.cols[i]});
+      o.record(h.data() + S.off[i], sizeof(double2) * (size_t)S.rows[i] * S.cols[i]);
+    }
+    need(std::fflush(fc.f) == 0, "cannot write the wave file");
+  });
+}
+
+int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, double* psi) {
+  return guard(ctx, [&] {
+    need(sb && psi && path, "null argument");
+    const SuperBasis& S = sb->plan.sb;
+    FileCloser fc{std::fopen(path, "rb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    FortranIn in{fc.f};
+    need(in.ints(1)[0] == sb->sys->d.length + 1 && in.ints(1)[0] == sb->env->d.length + 1,
+         "wave file: block lengths differ from the superblock's");
+    need(in.ints(1)[0] == (int32_t)S.pairs.size() && in.ints(1)[0] == (int32_t)S.dim,
+         "wave file: sector count or dimension differs from the superblock's");
+    std::vector<double2> h((size_t)std::max<int64_t>(S.dim, 1));
+    for (size_t i = 0; i < S.pairs.size(); ++i) {
+      const Sector& a = sb->plan.sbs.sec[S.pairs[i].first];
+      const Sector& b = sb->plan.ebs.sec[S.pairs[i].second];
+      need(in.record().empty(), "wave file: expected the empty record before a sector");
+      const auto qa = in.ints(2), qb = in.ints(2), dm = in.ints(2);
+      need(qa[0] == a.N && qa[1] == a.S2 && qb[0] == b.N && qb[1] == b.S2 && dm[0] == S.rows[i] &&
+               dm[1] == S.cols[i],
+           "wave file: super sector " + std::to_string(i + 1) + " differs from the superblock's");
+      const std::vector<char> v = in.record();
+      need(v.size() == sizeof(double2) * (size_t)S.rows[i] * S.cols[i], "wave file: block of the wrong size");
+      std::memcpy(h.data() + S.off[i], v.data(), v.size());
+    }
+    check_hip(hipMemcpyAsync(psi, h.data(), sizeof(double2) * (size_t)S.dim, hipMemcpyHostToDevice, ctx->stream),
+              "h2d");
+    check_hip(hipStreamSynchronize(ctx->stream), "h2d");
+  });
+}
+
 int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim) {
   if (!sb) return 1;
   const EnlBasis& E = side == 0 ? sb->plan.sbs : sb->plan.ebs;

@@ -389,6 +389,17 @@ class Context:
         return [(int(sec[i]), h[off[i]:off[i] + dim[i] * dim[i]].reshape((dim[i], dim[i]), order="F"))
                 for i in range(nb.value)]
 
+    def wave_to_disk(self, sb, psi, path):
+        """wave_to_disk (code/space_info.f90:1406-1436): psi in the reference's
+        Fortran unformatted wavefunction file."""
+        _sync()
+        self._rc(L.load().hh_wave_to_disk(self.ptr, sb.ptr, _dptr(psi), os.fsencode(path)), "hh_wave_to_disk")
+
+    def wave_from_disk(self, sb, path, psi):
+        """wave_from_disk (code/space_info.f90:1440-1468) into the device vector psi."""
+        _sync()
+        self._rc(L.load().hh_wave_from_disk(self.ptr, sb.ptr, os.fsencode(path), _dptr(psi)), "hh_wave_from_disk")
+
     def truncate(self, sb, side, psi, kept_min, kept_max, trun_err):
         """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90)."""
         _sync()

@@ -138,6 +138,16 @@ int hh_superblock_pairs(const hh_superblock* sb, int32_t* sys_sec, int32_t* env_
 /* enlarged basis of one side: sectors (N, 2S, dim) */
 int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim);
 
+/* wave_to_disk / wave_from_disk (code/space_info.f90:1406-1468): psi of a
+ * bound superblock in the reference's own wavefunction file, Fortran
+ * sequential unformatted records with 4-byte length markers: sys_len,
+ * env_len (enlarged blocks), num, dim, then per super sector an empty record,
+ * (N_sys, 2S_sys), (N_env, 2S_env), (sys_dim, env_dim) and the complex*16
+ * block.  Reading checks every header field against the superblock and
+ * refuses a file that differs or is damaged, leaving psi_dev untouched. */
+int hh_wave_to_disk(hh_ctx* ctx, const hh_superblock* sb, const double* psi_dev, const char* path);
+int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, double* psi_dev);
+
 /* H.psi: replaces model_wave(sys, env, sys_bs, env_bs, invec, outvec)
  * (code/mat_vec.f90:3476); psi_out is overwritten. */
 int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in_dev, double* psi_out_dev);
