@@ -112,6 +112,8 @@ SIGNATURES = [
     ("hh_density_matrix", i32, [vp, vp, i32, vp, vp]),
     ("hh_wave_to_disk", i32, [vp, vp, vp, C.c_char_p]),
     ("hh_wave_from_disk", i32, [vp, vp, C.c_char_p, vp]),
+    ("hh_trunc_to_disk", i32, [vp, vp, i32, C.c_char_p]),
+    ("hh_trunc_from_disk", i32, [vp, vp, i32, C.c_char_p, P(i32), P(vp)]),
     ("hh_driver_spin_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_hopping_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_trunc", i32, [vp, i32, i32, P(vp)]),

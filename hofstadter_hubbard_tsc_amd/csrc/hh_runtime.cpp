@@ -2520,6 +2520,99 @@ int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, do
   });
 }
 
+// truns_to_disk / truns_from_disk (code/space_info.f90:1621-1661), i.e.
+// block_to_disk / block_from_disk (:1470-1521) of the truncation operator
+// Get_truncation_operator builds (code/get_den.f90:362-421): records len, num,
+// dim, up_dif, down_dif, then per kept sector an empty record, (N, 2S,
+// down_dif = 0), (row_dim, sdim) and the row_dim x sdim complex*16 block of
+// the kept eigenvectors (the last sdim columns of the ascending solve).
+int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* path) {
+  return guard(ctx, [&] {
+    need(t && path, "null argument");
+    need(length >= 1, "length must be the enlarged block's length (>= 1)");
+    std::vector<double2> h((size_t)std::max<size_t>(t->evec.bytes / sizeof(double2), 1));
+    check_hip(hipMemcpyAsync(h.data(), t->evec.p, t->evec.bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "d2h");
+    int64_t dim = 0;
+    for (const KeptSector& k : t->kept) dim += k.keep;
+    FileCloser fc{std::fopen(path, "wb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    FortranOut o{fc.f};
+    o.ints({length});
+    o.ints({(int64_t)t->kept.size()});
+    o.ints({dim});
+    o.ints({0});
+    o.ints({0});
+    std::vector<double2> blk;
+    for (const KeptSector& k : t->kept) {
+      const Sector& a = t->eb.sec[k.a];
+      o.record(nullptr, 0);
+      o.ints({a.N, a.S2, 0});
+      o.ints({a.dim, k.keep});
+      blk.resize((size_t)a.dim * k.keep);
+      for (int c = 0; c < k.keep; ++c)
+        std::memcpy(blk.data() + (size_t)c * a.dim, h.data() + k.u_off + (int64_t)c * k.u_ld, sizeof(double2) * a.dim);
+      o.record(blk.data(), sizeof(double2) * blk.size());
+    }
+    need(std::fflush(fc.f) == 0, "cannot write the truncation file");
+  });
+}
+
+int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* path, int32_t* length,
+                       hh_trunc** out) {
+  if (out) *out = nullptr;
+  return guard(ctx, [&] {
+    need(prev && path && out, "null argument");
+    need(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
+    FileCloser fc{std::fopen(path, "rb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    FortranIn in{fc.f};
+    const int32_t len = in.ints(1)[0], num = in.ints(1)[0], dim = in.ints(1)[0];
+    need(in.ints(1)[0] == 0 && in.ints(1)[0] == 0, "truncation file: the operator must be a scalar with no N shift");
+    need(len == prev->d.length + 1, "truncation file: its length is not the enlarged length of this block");
+    need(num >= 0, "truncation file: negative sector count");
+    std::unique_ptr<hh_trunc> t(new hh_trunc());
+    t->side = side;
+    t->eb = enlarge_basis(prev->d.sec, site_sectors(ctx->model.kind));
+    std::vector<double2> arena;
+    std::vector<bool> seen(t->eb.sec.size(), false);
+    int64_t total = 0;
+    for (int32_t i = 0; i < num; ++i) {
+      need(in.record().empty(), "truncation file: expected the empty record before a sector");
+      const auto q = in.ints(3), d = in.ints(2);
+      need(q[2] == 0, "truncation file: sector with a spin change");
+      int a = -1;
+      for (size_t j = 0; j < t->eb.sec.size(); ++j)
+        if (t->eb.sec[j].N == q[0] && t->eb.sec[j].S2 == q[1]) a = (int)j;
+      need(a >= 0 && !seen[a], "truncation file: sector (" + std::to_string(q[0]) + ", " + std::to_string(q[1]) +
+                                   ") is not an enlarged sector of this block, or repeats");
+      seen[a] = true;
+      need(d[0] == t->eb.sec[a].dim && d[1] >= 1 && d[1] <= d[0], "truncation file: block dimensions do not fit");
+      const std::vector<char> v = in.record();
+      need(v.size() == sizeof(double2) * (size_t)d[0] * d[1], "truncation file: block of the wrong size");
+      KeptSector k;
+      k.a = a;
+      k.keep = d[1];
+      k.u_off = (int64_t)arena.size();
+      k.u_ld = d[0];
+      arena.resize(arena.size() + (size_t)d[0] * d[1]);
+      std::memcpy(arena.data() + k.u_off, v.data(), v.size());
+      t->kept.push_back(k);
+      total += d[1];
+    }
+    need(total == dim, "truncation file: dim differs from the sum of the kept columns");
+    t->num_state = (int)total;
+    t->evec.alloc(sizeof(double2) * std::max<size_t>(arena.size(), 1));
+    if (!arena.empty()) {
+      check_hip(hipMemcpyAsync(t->evec.p, arena.data(), sizeof(double2) * arena.size(), hipMemcpyHostToDevice,
+                               ctx->stream), "h2d");
+      check_hip(hipStreamSynchronize(ctx->stream), "h2d");
+    }
+    if (length) *length = len;
+    *out = t.release();
+  });
+}
+
 int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim) {
   if (!sb) return 1;
   const EnlBasis& E = side == 0 ? sb->plan.sbs : sb->plan.ebs;

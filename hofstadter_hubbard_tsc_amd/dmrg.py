@@ -389,6 +389,19 @@ class Context:
         return [(int(sec[i]), h[off[i]:off[i] + dim[i] * dim[i]].reshape((dim[i], dim[i]), order="F"))
                 for i in range(nb.value)]
 
+    def trunc_to_disk(self, trunc, length, path):
+        """truns_to_disk (code/space_info.f90:1621-1636): the truncation that
+        produces the block of `length` sites, in the reference's file layout."""
+        self._rc(L.load().hh_trunc_to_disk(self.ptr, trunc.ptr, length, os.fsencode(path)), "hh_trunc_to_disk")
+
+    def trunc_from_disk(self, prev, side, path):
+        """truns_from_disk (code/space_info.f90:1642-1661) for the block grown
+        from `prev`: (Trunc, length)."""
+        p, n = C.c_void_p(), L.i32()
+        self._rc(L.load().hh_trunc_from_disk(self.ptr, prev.ptr, side, os.fsencode(path), C.byref(n), C.byref(p)),
+                 "hh_trunc_from_disk")
+        return Trunc(self, p), n.value
+
     def wave_to_disk(self, sb, psi, path):
         """wave_to_disk (code/space_info.f90:1406-1436): psi in the reference's
         Fortran unformatted wavefunction file."""

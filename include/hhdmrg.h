@@ -185,6 +185,17 @@ int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* ps
  * (code/get_den.f90:4-99, :184-428): side 0 sys (psi psi^dag), 1 env. */
 int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, int kept_min, int kept_max,
                 double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy);
+/* truns_to_disk / truns_from_disk (code/space_info.f90:1621-1661, the
+ * block_to_disk / block_from_disk records of :1470-1521) of the truncation
+ * operator of Get_truncation_operator (code/get_den.f90:362-421): len, num,
+ * dim, up_dif, down_dif, then per kept sector an empty record, (N, 2S, 0),
+ * (row_dim, sdim) and the kept eigenvector block.  length is the length of
+ * the block the truncation produces.  Reading needs the block it was grown
+ * from (its enlarged basis); sectors are matched by (N, 2S) and every field
+ * is checked.  hh_grow_block then rebuilds the reference's block from it. */
+int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* path);
+int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* path, int32_t* length,
+                       hh_trunc** out);
 /* identity truncation of the warm-up (code/warm_up.f90:1421-1432) */
 int hh_trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side, hh_trunc** out);
 int hh_trunc_destroy(hh_trunc* t);

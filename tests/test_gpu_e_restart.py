@@ -148,3 +148,49 @@ def test_wave_file_in_the_reference_layout(ctx, tmp_path):
     with pytest.raises(HHError):
         ctx.wave_from_disk(sb, bad, back)
     assert torch.equal(back, keep)
+
+
+def test_truncation_file_in_the_reference_layout(ctx, tmp_path):
+    """truns_to_disk / truns_from_disk (code/space_info.f90:1621-1661 with the
+    block_to_disk records of :1470-1521): the file holds len, num, dim, up_dif,
+    down_dif and per kept sector an empty record, (N, 2S, 0), (row_dim, sdim)
+    and the kept eigenvector block; reading it back gives the same matrices
+    bit for bit, growing the previous block with it rebuilds the driver's own
+    block bit for bit (sys and env), and another block's file is refused."""
+    import struct
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    drv = _prepare(ctx, m, 12)
+    n = m.nsite
+    ints = lambda b: list(struct.unpack("<%di" % (len(b) // 4), b))
+    l = 4
+    truncated = False
+    for side in (0, 1):
+        tr = drv.trunc(side, l)
+        mats = tr.matrices()
+        path = tmp_path / ("trun%d.dat" % side)
+        ctx.trunc_to_disk(tr, l, path)
+        recs = _fortran_records(path.read_bytes())
+        kept = sum(u.shape[1] for _, u in mats)
+        assert [ints(r) for r in recs[:5]] == [[l], [len(mats)], [kept], [0], [0]]
+        assert len(recs) == 5 + 4 * len(mats)
+        prev = drv.block(side, l - 1)
+        other = drv.block(1 - side, n - l - 1)
+        sb = ctx.bind(prev, other, m.n_elec) if side == 0 else ctx.bind(other, prev, m.n_elec)
+        N, S2, dims = sb.enlarged(side)
+        for i, (a, u) in enumerate(mats):
+            r = recs[5 + 4 * i:9 + 4 * i]
+            assert r[0] == b"" and ints(r[1]) == [N[a], S2[a], 0] and ints(r[2]) == [dims[a], u.shape[1]]
+            assert np.array_equal(np.frombuffer(r[3], dtype=np.complex128), u.reshape(-1, order="F"))
+        truncated |= kept < int(sum(dims[a] for a, _ in mats))
+        back, length = ctx.trunc_from_disk(prev, side, path)
+        assert length == l
+        got = back.matrices()
+        assert [a for a, _ in got] == [a for a, _ in mats]
+        assert all(np.array_equal(g, u) for (_, g), (_, u) in zip(got, mats))
+        rebuilt = ctx.grow(sb, side, back)
+        own = drv.block(side, l)
+        assert all(np.array_equal(x, y) for x, y in zip(rebuilt.sectors(), own.sectors()))
+        assert np.array_equal(rebuilt.download(), own.download())
+        with pytest.raises(Exception):
+            ctx.trunc_from_disk(drv.block(side, l - 2), side, path)
+    assert truncated, "the case must truncate"
