@@ -2525,7 +2525,11 @@ int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, do
 // Get_truncation_operator builds (code/get_den.f90:362-421): records len, num,
 // dim, up_dif, down_dif, then per kept sector an empty record, (N, 2S,
 // down_dif = 0), (row_dim, sdim) and the row_dim x sdim complex*16 block of
-// the kept eigenvectors (the last sdim columns of the ascending solve).
+// the kept eigenvectors (the last sdim columns of the ascending solve).  The
+// reference's environment density matrix is psi^dag psi
+// (Get_density_matrix_env, code/get_den.f90:84-91), the complex conjugate of
+// this library's psi^T conj(psi), so environment truncations are written and
+// read conjugated.
 int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* path) {
   return guard(ctx, [&] {
     need(t && path, "null argument");
@@ -2552,6 +2556,8 @@ int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* pat
       blk.resize((size_t)a.dim * k.keep);
       for (int c = 0; c < k.keep; ++c)
         std::memcpy(blk.data() + (size_t)c * a.dim, h.data() + k.u_off + (int64_t)c * k.u_ld, sizeof(double2) * a.dim);
+      if (t->side == 1)
+        for (auto& x : blk) x.y = -x.y;
       o.record(blk.data(), sizeof(double2) * blk.size());
     }
     need(std::fflush(fc.f) == 0, "cannot write the truncation file");
@@ -2597,6 +2603,8 @@ int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* 
       k.u_ld = d[0];
       arena.resize(arena.size() + (size_t)d[0] * d[1]);
       std::memcpy(arena.data() + k.u_off, v.data(), v.size());
+      if (side == 1)
+        for (size_t e = 0; e < (size_t)d[0] * d[1]; ++e) arena[k.u_off + e].y = -arena[k.u_off + e].y;
       t->kept.push_back(k);
       total += d[1];
     }

@@ -192,7 +192,9 @@ int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev,
  * (row_dim, sdim) and the kept eigenvector block.  length is the length of
  * the block the truncation produces.  Reading needs the block it was grown
  * from (its enlarged basis); sectors are matched by (N, 2S) and every field
- * is checked.  hh_grow_block then rebuilds the reference's block from it. */
+ * is checked.  hh_grow_block then rebuilds the reference's block from it.
+ * Environment truncations are stored conjugated, as the reference's
+ * environment density matrix is psi^dag psi (code/get_den.f90:84-91). */
 int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* path);
 int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* path, int32_t* length,
                        hh_trunc** out);

@@ -154,8 +154,9 @@ def test_truncation_file_in_the_reference_layout(ctx, tmp_path):
     """truns_to_disk / truns_from_disk (code/space_info.f90:1621-1661 with the
     block_to_disk records of :1470-1521): the file holds len, num, dim, up_dif,
     down_dif and per kept sector an empty record, (N, 2S, 0), (row_dim, sdim)
-    and the kept eigenvector block; reading it back gives the same matrices
-    bit for bit, growing the previous block with it rebuilds the driver's own
+    and the kept eigenvector block (conjugated for the environment, whose
+    reference density matrix is psi^dag psi); reading it back gives the same
+    matrices bit for bit, growing the previous block with it rebuilds the driver's own
     block bit for bit (sys and env), and another block's file is refused."""
     import struct
     m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
@@ -180,7 +181,9 @@ def test_truncation_file_in_the_reference_layout(ctx, tmp_path):
         for i, (a, u) in enumerate(mats):
             r = recs[5 + 4 * i:9 + 4 * i]
             assert r[0] == b"" and ints(r[1]) == [N[a], S2[a], 0] and ints(r[2]) == [dims[a], u.shape[1]]
-            assert np.array_equal(np.frombuffer(r[3], dtype=np.complex128), u.reshape(-1, order="F"))
+            # the reference's environment density matrix is psi^dag psi: its U is the conjugate
+            ref = u if side == 0 else np.conj(u)
+            assert np.array_equal(np.frombuffer(r[3], dtype=np.complex128), ref.reshape(-1, order="F"))
         truncated |= kept < int(sum(dims[a] for a, _ in mats))
         back, length = ctx.trunc_from_disk(prev, side, path)
         assert length == l
