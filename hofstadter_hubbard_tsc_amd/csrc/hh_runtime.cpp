@@ -2423,11 +2423,12 @@ int hh_superblock_pairs(const hh_superblock* sb, int32_t* sys_sec, int32_t* env_
 namespace {
 struct FortranOut {
   FILE* f;
+  const char* what;   // "wave file" or "truncation file", for the messages
   void record(const void* p, size_t n) {
-    need(n < (size_t(1) << 31), "wave file record too large for 4-byte record markers");
+    need(n < (size_t(1) << 31), std::string(what) + ": record too large for 4-byte record markers");
     const int32_t m = (int32_t)n;
     need(std::fwrite(&m, 4, 1, f) == 1 && (n == 0 || std::fwrite(p, 1, n, f) == n) && std::fwrite(&m, 4, 1, f) == 1,
-         "cannot write the wave file");
+         std::string("cannot write the ") + what);
   }
   void ints(std::initializer_list<int64_t> v) {
     std::vector<int32_t> a;
@@ -2437,17 +2438,21 @@ struct FortranOut {
 };
 struct FortranIn {
   FILE* f;
-  std::vector<char> record() {
+  const char* what;   // "wave file" or "truncation file", for the messages
+  // one record whose length must be `expect` bytes; the leading marker is
+  // checked before anything is allocated, so a damaged marker cannot ask for
+  // a large buffer
+  std::vector<char> record(size_t expect) {
     int32_t m = 0, m2 = 0;
-    need(std::fread(&m, 4, 1, f) == 1 && m >= 0, "wave file: truncated or not a Fortran unformatted file");
+    need(std::fread(&m, 4, 1, f) == 1 && m >= 0, std::string(what) + ": truncated or not a Fortran unformatted file");
+    need((size_t)m == expect, std::string(what) + ": unexpected record length");
     std::vector<char> b((size_t)m);
     need((m == 0 || std::fread(b.data(), 1, (size_t)m, f) == (size_t)m) && std::fread(&m2, 4, 1, f) == 1 && m2 == m,
-         "wave file: damaged record");
+         std::string(what) + ": damaged record");
     return b;
   }
   std::vector<int32_t> ints(size_t n) {
-    std::vector<char> b = record();
-    need(b.size() == 4 * n, "wave file: unexpected record length");
+    std::vector<char> b = record(4 * n);
     std::vector<int32_t> v(n);
     std::memcpy(v.data(), b.data(), 4 * n);
     return v;
@@ -2472,7 +2477,7 @@ int hh_wave_to_disk(hh_ctx* ctx, const hh_superblock* sb, const double* psi, con
     need(S.dim < (int64_t(1) << 31), "superblock too large for the reference's default-integer dim");
     FileCloser fc{std::fopen(path, "wb")};
     need(fc.f != nullptr, std::string("cannot open ") + path);
-    FortranOut o{fc.f};
+    FortranOut o{fc.f, "wave file"};
     o.ints({sb->sys->d.length + 1});
     o.ints({sb->env->d.length + 1});
     o.ints({(int64_t)S.pairs.size()});
@@ -2496,7 +2501,7 @@ int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, do
     const SuperBasis& S = sb->plan.sb;
     FileCloser fc{std::fopen(path, "rb")};
     need(fc.f != nullptr, std::string("cannot open ") + path);
-    FortranIn in{fc.f};
+    FortranIn in{fc.f, "wave file"};
     need(in.ints(1)[0] == sb->sys->d.length + 1 && in.ints(1)[0] == sb->env->d.length + 1,
          "wave file: block lengths differ from the superblock's");
     need(in.ints(1)[0] == (int32_t)S.pairs.size() && in.ints(1)[0] == (int32_t)S.dim,
@@ -2505,13 +2510,12 @@ int hh_wave_from_disk(hh_ctx* ctx, const hh_superblock* sb, const char* path, do
     for (size_t i = 0; i < S.pairs.size(); ++i) {
       const Sector& a = sb->plan.sbs.sec[S.pairs[i].first];
       const Sector& b = sb->plan.ebs.sec[S.pairs[i].second];
-      need(in.record().empty(), "wave file: expected the empty record before a sector");
+      in.record(0);   // the empty record before each sector
       const auto qa = in.ints(2), qb = in.ints(2), dm = in.ints(2);
       need(qa[0] == a.N && qa[1] == a.S2 && qb[0] == b.N && qb[1] == b.S2 && dm[0] == S.rows[i] &&
                dm[1] == S.cols[i],
            "wave file: super sector " + std::to_string(i + 1) + " differs from the superblock's");
-      const std::vector<char> v = in.record();
-      need(v.size() == sizeof(double2) * (size_t)S.rows[i] * S.cols[i], "wave file: block of the wrong size");
+      const std::vector<char> v = in.record(sizeof(double2) * (size_t)S.rows[i] * S.cols[i]);
       std::memcpy(h.data() + S.off[i], v.data(), v.size());
     }
     check_hip(hipMemcpyAsync(psi, h.data(), sizeof(double2) * (size_t)S.dim, hipMemcpyHostToDevice, ctx->stream),
@@ -2541,7 +2545,7 @@ int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* pat
     for (const KeptSector& k : t->kept) dim += k.keep;
     FileCloser fc{std::fopen(path, "wb")};
     need(fc.f != nullptr, std::string("cannot open ") + path);
-    FortranOut o{fc.f};
+    FortranOut o{fc.f, "truncation file"};
     o.ints({length});
     o.ints({(int64_t)t->kept.size()});
     o.ints({dim});
@@ -2572,7 +2576,7 @@ int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* 
     need(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
     FileCloser fc{std::fopen(path, "rb")};
     need(fc.f != nullptr, std::string("cannot open ") + path);
-    FortranIn in{fc.f};
+    FortranIn in{fc.f, "truncation file"};
     const int32_t len = in.ints(1)[0], num = in.ints(1)[0], dim = in.ints(1)[0];
     need(in.ints(1)[0] == 0 && in.ints(1)[0] == 0, "truncation file: the operator must be a scalar with no N shift");
     need(len == prev->d.length + 1, "truncation file: its length is not the enlarged length of this block");
@@ -2584,7 +2588,7 @@ int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* 
     std::vector<bool> seen(t->eb.sec.size(), false);
     int64_t total = 0;
     for (int32_t i = 0; i < num; ++i) {
-      need(in.record().empty(), "truncation file: expected the empty record before a sector");
+      in.record(0);   // the empty record before each sector
       const auto q = in.ints(3), d = in.ints(2);
       need(q[2] == 0, "truncation file: sector with a spin change");
       int a = -1;
@@ -2594,8 +2598,7 @@ int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* 
                                    ") is not an enlarged sector of this block, or repeats");
       seen[a] = true;
       need(d[0] == t->eb.sec[a].dim && d[1] >= 1 && d[1] <= d[0], "truncation file: block dimensions do not fit");
-      const std::vector<char> v = in.record();
-      need(v.size() == sizeof(double2) * (size_t)d[0] * d[1], "truncation file: block of the wrong size");
+      const std::vector<char> v = in.record(sizeof(double2) * (size_t)d[0] * d[1]);
       KeptSector k;
       k.a = a;
       k.keep = d[1];

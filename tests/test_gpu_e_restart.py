@@ -148,6 +148,13 @@ def test_wave_file_in_the_reference_layout(ctx, tmp_path):
     with pytest.raises(HHError):
         ctx.wave_from_disk(sb, bad, back)
     assert torch.equal(back, keep)
+    # a record marker claiming ~2 GiB is refused from the marker alone
+    raw = bytearray(path.read_bytes())
+    raw[0:4] = np.int32(0x7FFFFFF0).tobytes()
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(HHError, match="unexpected record length"):
+        ctx.wave_from_disk(sb, bad, back)
+    assert torch.equal(back, keep)
 
 
 def test_truncation_file_in_the_reference_layout(ctx, tmp_path):
