@@ -159,10 +159,10 @@ struct hh_ctx {
   rocblas_handle rb = nullptr;
   // side streams (default priority) for independent per-sector eigensolver
   // calls, each with its own rocBLAS handle; joined back with events
-  static constexpr int NSIDE = 3;
-  hipStream_t side[NSIDE] = {nullptr, nullptr, nullptr};
-  rocblas_handle rbs[NSIDE] = {nullptr, nullptr, nullptr};
-  hipEvent_t fork = nullptr, join[NSIDE] = {nullptr, nullptr, nullptr};
+  static constexpr int NSIDE = 7;
+  hipStream_t side[NSIDE] = {};
+  rocblas_handle rbs[NSIDE] = {};
+  hipEvent_t fork = nullptr, join[NSIDE] = {};
   ModelParams model;
   bool model_set = false;
   int precision = PREC_FP64;
