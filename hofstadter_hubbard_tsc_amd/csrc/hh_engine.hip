@@ -737,10 +737,14 @@ __device__ __forceinline__ void block_norm(double acc, double* __restrict__ part
   }
 }
 
-// w = w - a v1 - b v2 (v2 may be null); partial |w|^2 per block
-__global__ __launch_bounds__(VB) void k_lupd_norm(double2* __restrict__ w, const double2* __restrict__ v1, double a,
+// w = w - a v1 - b v2 (v2 may be null); partial |w|^2 per block.  With a_dev
+// set, a is read from device memory (a scalar the stream wrote earlier)
+// instead of the argument.
+__global__ __launch_bounds__(VB) void k_lupd_norm(double2* __restrict__ w, const double2* __restrict__ v1,
+                                                  double a_arg, const double* __restrict__ a_dev,
                                                   const double2* __restrict__ v2, double b, int64_t n,
                                                   double* __restrict__ partial) {
+  const double a = a_dev ? *a_dev : a_arg;
   const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
   double acc = 0.0;
 #pragma unroll
@@ -855,9 +859,9 @@ void dev_mdots(const double2* V, int64_t stride, int nv, const double2* w, int64
   hipLaunchKernelGGL(k_block_sum, dim3(nv), dim3(VB), 0, s, partial_dev, nb, out_dev);
 }
 void dev_lanczos_update_norm(double2* w, const double2* v1, double a, const double2* v2, double b, int64_t n,
-                             double* out_dev, double* partial_dev, hipStream_t s) {
+                             double* out_dev, double* partial_dev, hipStream_t s, const double* a_dev) {
   const int nb = vec_blocks(n);
-  hipLaunchKernelGGL(k_lupd_norm, dim3(nb), dim3(VB), 0, s, w, v1, a, v2, b, n, partial_dev);
+  hipLaunchKernelGGL(k_lupd_norm, dim3(nb), dim3(VB), 0, s, w, v1, a, a_dev, v2, b, n, partial_dev);
   hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(VB), 0, s, partial_dev, nb, out_dev);
 }
 void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, int nv, const double* c_dev, double cs,

@@ -74,9 +74,10 @@ int vec_blocks(int64_t n);   // partial-sum slots per vector of length n
 // out[j] = Re <V_j, w>, j < nv; w is read once
 void dev_mdots(const double2* V, int64_t stride, int nv, const double2* w, int64_t n, double* out_dev,
                double* partial_dev, hipStream_t s);
-// w = w - a*v1 - b*v2 (v2 may be null); out[0] = |w|^2
+// w = w - a*v1 - b*v2 (v2 may be null); out[0] = |w|^2; a is read from a_dev
+// on the device when a_dev is set
 void dev_lanczos_update_norm(double2* w, const double2* v1, double a, const double2* v2, double b, int64_t n,
-                             double* out_dev, double* partial_dev, hipStream_t s);
+                             double* out_dev, double* partial_dev, hipStream_t s, const double* a_dev = nullptr);
 // w = sw*w + cs * sum_j c_j V_j (c on the device); out[0] = |w|^2
 void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, int nv, const double* c_dev, double cs,
                       int64_t n, double* out_dev, double* partial_dev, hipStream_t s);

@@ -328,20 +328,34 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
   need(std::isfinite(nrm0) && nrm0 > 0.0, "zero or non-finite start vector");
   dev_scale_copy(vec(0), psi, 1.0 / nrm0, n, ctx->stream);
   hpsi(vec(0), w);
-  dev_mdots(vec(0), 0, 1, w, n, dots, part, ctx->stream);
   // a non-finite H.psi (NaN or Inf in a block or in psi) stops the solve with an error
   auto finite = [](double x) {
     if (!std::isfinite(x)) throw HHError(-4, "non-finite value in Lanczos: H.psi produced NaN or Inf");
     return x;
   };
-  std::vector<double> alpha{finite(fetch(1)[0])}, beta;
+  // alpha_j = Re<v_j, H v_j> stays on the device and the three-term update
+  //   w -= alpha_j v_j + beta_{j-1} v_{j-1}   (|w|^2 in the same pass)
+  // reads it there, so alpha_j and |w|^2 come back in one read.  When the
+  // stopping test then ends the solve, that update was spare work on the
+  // scratch vector w only.
+  double* wsq = dots + max_iter + 2;   // past the Gram-Schmidt coefficients
+  std::vector<double> alpha, beta;
+  auto alpha_and_update = [&](int j) {
+    dev_mdots(vec(j - 1), 0, 1, w, n, dots, part, ctx->stream);
+    dev_lanczos_update_norm(w, vec(j - 1), 0.0, j >= 2 ? vec(j - 2) : nullptr, j >= 2 ? beta[j - 2] : 0.0, n, wsq,
+                            part, ctx->stream, dots);
+    double h[2];
+    check_hip(hipMemcpyAsync(&h[0], dots, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipMemcpyAsync(&h[1], wsq, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+    return std::make_pair(h[0], h[1]);
+  };
+  auto r = alpha_and_update(1);
+  alpha.push_back(finite(r.first));
   double ritz = alpha[0];
   int last = 1;
   while (last < max_iter) {
-    // w -= alpha v_last - beta v_{last-1}, with |w|^2 in the same pass
-    dev_lanczos_update_norm(w, vec(last - 1), alpha[last - 1], last >= 2 ? vec(last - 2) : nullptr,
-                            last >= 2 ? beta[last - 2] : 0.0, n, dots, part, ctx->stream);
-    const double b = std::sqrt(finite(fetch(1)[0]));
+    const double b = std::sqrt(finite(r.second));
     beta.push_back(b);
     if (std::fabs(b) <= lanzero) break;
     // normalise, then classical Gram-Schmidt against every stored vector with
@@ -353,8 +367,8 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
     dev_scale_copy(vec(last), w, 1.0 / nw, n, ctx->stream);
     ++last;
     hpsi(vec(last - 1), w);
-    dev_mdots(vec(last - 1), 0, 1, w, n, dots, part, ctx->stream);
-    alpha.push_back(finite(fetch(1)[0]));
+    r = alpha_and_update(last);
+    alpha.push_back(finite(r.first));
     std::vector<double> d(alpha), e(beta);
     e.resize(d.size());
     tridiag_ql(d, e, nullptr);
