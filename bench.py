@@ -32,41 +32,55 @@ HBM_PEAK_GBS = 8000.0
 TRUN_ERR = 1.0000000180025095e-18
 
 
-def workload(m_kept):
-    from oracle import lattice as lat   # lattice description only (a dataclass)
-    model = lat.Model(kind="tj", Lx=32, Ly=4, lattice="square", periodic_y=True, t=1.0, J=0.4, flux=1.0 / 8,
-                      n_elec=112)
-    return model
+def workload(name):
+    from hofstadter_hubbard_tsc_amd import WORKLOADS
+    return WORKLOADS[name]
 
 
-def cpu_baseline(ctx, drv, model, sys_len, budget_s, m_kept):
-    """numpy oracle H.psi on the superblock of the last timed step."""
+def cpu_baseline(ctx, drv, model, budget_s, m_kept, energy):
+    """The numpy oracle on the superblock of the last timed step: its H.psi
+    timed on the host cores (the baseline), and -- as the checker -- its H.psi
+    of that step's GPU ground state against hh_model_wave, the oracle's
+    <psi|H|psi> against the step energy and its |H psi - E psi|."""
+    import numpy as np
     from oracle import dmrg as odmrg
+    from oracle import lattice as lat
     from tests.gpuhelp import oracle_block
-    n = model.nsite
-    sysb = oracle_block(ctx, drv.block(0, sys_len), model)
-    envb = oracle_block(ctx, drv.block(1, n - sys_len - 2), model)
-    sg = odmrg.StepGeometry(model, sysb.length, envb.length)
+    om = lat.Model(kind=model.kind, Lx=model.Lx, Ly=model.Ly, lattice=model.lattice, periodic_y=model.periodic_y,
+                   t=model.t, U=model.U, J=model.J, flux=model.flux, n_elec=model.n_elec)
+    sys_len, env_len, psi_dev = drv.wave()
+    sysb = oracle_block(ctx, drv.block(0, sys_len), om)
+    envb = oracle_block(ctx, drv.block(1, env_len), om)
+    sg = odmrg.StepGeometry(om, sysb.length, envb.length)
     t0 = time.time()
-    H = odmrg.Superblock(odmrg.enlarge_in_geometry(sysb, model, "sys", sg),
-                         odmrg.enlarge_in_geometry(envb, model, "env", sg), sg, model, model.n_elec)
+    H = odmrg.Superblock(odmrg.enlarge_in_geometry(sysb, om, "sys", sg),
+                         odmrg.enlarge_in_geometry(envb, om, "env", sg), sg, om, om.n_elec)
     t_build = time.time() - t0
-    psi = odmrg.start_vector(H.sb.dim, 5)
-    # product's own algorithmic flop count for this superblock
-    sb = ctx.bind(drv.block(0, sys_len), drv.block(1, n - sys_len - 2), model.n_elec)
+    # product's own algorithmic flop count and H.psi of this superblock
+    sb = ctx.bind(drv.block(0, sys_len), drv.block(1, env_len), om.n_elec)
     flops = sb.info()["flops_alg"]
+    hpsi = psi_dev.clone()
+    ctx.model_wave(sb, psi_dev, hpsi)
     del sb
-    reps, t_mv = 0, 0.0
+    x = psi_dev.cpu().numpy()
+    got = hpsi.cpu().numpy()
+    reps, t_mv, ref = 0, 0.0, None
     while reps == 0 or (t_mv < budget_s and reps < 50):
         t1 = time.time()
-        H.matvec(psi)
+        ref = H.matvec(x)
         t_mv += time.time() - t1
         reps += 1
+    e_ref = float(np.vdot(x, ref).real / np.vdot(x, x).real)
+    check = {"superblock": [sys_len, env_len], "dim": int(x.size),
+             "matvec_max_err_rel": float(np.abs(got - ref).max() / np.abs(ref).max()),
+             "dE_vs_oracle": float(energy - e_ref),
+             "residual_oracle": float(np.linalg.norm(ref - e_ref * x) / np.linalg.norm(x))}
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": flops * reps / t_mv / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": "numpy oracle H.psi (dense enlarged-block form) on the m=%d superblock of step sys_len=%d, "
-                      "%d matvecs in %.1f s (+%.1f s operator setup), same reference-order flop count"
+    base = {"value": flops * reps / t_mv / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": "numpy oracle H.psi (dense enlarged-block form) on the m=%d superblock of the last timed step "
+                      "(sys_len=%d), %d matvecs in %.1f s (+%.1f s operator setup), same reference-order flop count"
                       % (m_kept, sys_len, reps, t_mv, t_build)}
+    return base, check
 
 
 def measure_at_scale(ctx, drv, model, sys_len):
@@ -125,6 +139,97 @@ def measure_at_scale(ctx, drv, model, sys_len):
             out[name + "_error"] = str(e)[:200]
         print("# measure %s done" % name, file=sys.stderr, flush=True)
     return out
+
+
+def run_c1(args):
+    """The C1 schedule (Hubbard 16 x 2 ladder, U = 8, N = 28, m = 256): warm-up,
+    the two warm-up half sweeps at 15x tolerance, then 6 sweeps at 1e-10, all
+    timed; every sweep's wall time is measured (steady clock, stream
+    synchronised at both ends)."""
+    import torch
+    from hofstadter_hubbard_tsc_amd import Context, Driver
+    wl = workload("c1")
+    model = wl.model
+    n = model.nsite
+    ctx = Context(0)
+    ctx.set_model(model)
+    drv = Driver(ctx, kept_max=wl.kept_max, tol=wl.tol, kept_max_warm=wl.kept_max_warm, seed=77)
+    ctx.reset_stats()
+    torch.cuda.synchronize()
+    t_all = time.time()
+    recs = list(drv.warmup())
+    t_warm = time.time() - t_all
+    t1 = time.time()
+    recs += drv.sweep(0, n // 2 - 1, n - 2, wl.tol * 15, TRUN_ERR * 15, wl.kept_max, fresh=True, phase=1)
+    drv.reset_transform()
+    recs += drv.sweep(1, n - 2, 1, wl.tol * 15, TRUN_ERR * 15, wl.kept_max, phase=1)
+    t_half = time.time() - t1
+    sweep_s = []
+    for k in range(wl.n_sweeps):
+        t1 = time.time()
+        drv.reset_transform()
+        recs += drv.sweep(0, 1, n - 2, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
+        drv.reset_transform()
+        recs += drv.sweep(1, n - 2, 1, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
+        torch.cuda.synchronize()
+        sweep_s.append(round(time.time() - t1, 3))
+        print("# c1 sweep %d: %.3f s, lowest E %.10f" % (k + 1, sweep_s[-1],
+              min(r.energy for r in recs if r.phase == k + 2)), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.time() - t_all
+    mv_time = sum(r.matvec_seconds for r in recs)
+    mv_flops = sum(r.matvec_flops for r in recs)
+    last = [r for r in recs if r.phase == wl.n_sweeps + 1]
+    e_final = min(r.energy for r in last)
+    cpu = None
+    if not args.no_cpu:
+        cpu = c1_cpu_baseline(ctx, drv, model, args.cpu_seconds)
+    out = {
+        "metric": "sweep wall-time + matvec GFLOP/s at |dE|<=1e-8, chosen form vs plain FP64 GPU",
+        "value": round(mv_flops / max(mv_time, 1e-30) / 1e9, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": len(recs), "warmup": 0, "ms_per_step": round(1e3 * total / max(len(recs), 1), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (FP64 MFMA)",
+        "data": "synthetic: deterministic counter-based start vectors, no external data",
+        "config": {"workload": wl.description, "m": wl.kept_max, "form": "plain FP64",
+                   "total_seconds": round(total, 3), "warmup_seconds": round(t_warm, 3),
+                   "warmup_half_sweeps_seconds": round(t_half, 3), "sweep_seconds": sweep_s,
+                   "energy": e_final, "lowest_energy_per_sweep": [min(r.energy for r in recs if r.phase == k + 2)
+                                                                  for k in range(wl.n_sweeps)],
+                   "steps_at_iteration_cap": sum(1 for r in recs if not r.converged),
+                   "max_trunc_err_last_sweep": max(r.trunc_err for r in last),
+                   "matvec_share": round(mv_time / max(total, 1e-30), 3)},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def c1_cpu_baseline(ctx, drv, model, budget_s):
+    """The numpy oracle's Lanczos (lanczos_memo_new restated) on the centre
+    superblock of the finished C1 run, from the same start vector as a GPU
+    Lanczos on the same device blocks: timed on the host cores, and the two
+    energies compared (the checker)."""
+    from oracle import dmrg as odmrg
+    from oracle import lattice as lat
+    from tests.gpuhelp import dev, oracle_superblock
+    om = lat.Model(kind=model.kind, Lx=model.Lx, Ly=model.Ly, lattice=model.lattice, periodic_y=model.periodic_y,
+                   t=model.t, U=model.U, J=model.J, flux=model.flux, n_elec=model.n_elec)
+    n = model.nsite
+    ls = n // 2 - 1
+    H = oracle_superblock(ctx, drv.block(0, ls), drv.block(1, n - ls - 2), om, om.n_elec)
+    sb = ctx.bind(drv.block(0, ls), drv.block(1, n - ls - 2), om.n_elec)
+    flops = sb.info()["flops_alg"]
+    v0 = odmrg.start_vector(H.sb.dim, 3)
+    t0 = time.time()
+    e_ref, _, it_ref = odmrg.lanczos(H.matvec, v0, 1e-10, 100)
+    t_cpu = time.time() - t0
+    e_gpu, it_gpu = ctx.lanczos(sb, dev(v0), 1e-10, 100)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": flops * it_ref / t_cpu / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": "numpy oracle Lanczos (tol 1e-10) on the centre superblock (sys_len=%d, dim %d) of the "
+                      "finished run: %d H.psi in %.2f s; the GPU Lanczos from the same start vector on the same "
+                      "blocks: %d H.psi, E_GPU - E_oracle = %.3e"
+                      % (ls, H.sb.dim, it_ref, t_cpu, it_gpu, e_gpu - e_ref)}
 
 
 def run_c3(args):
@@ -223,8 +328,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
-                    help="c2: DMRG steps of the flux t-J cylinder at m (default); c3: batched H.psi micro-benchmark")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c2",
+                    help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
+                         "schedule at m = 256; c3: batched H.psi micro-benchmark")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--measure", action="store_true",
                     help="c2: after the timed steps, time the measurement entry points on the last superblock "
@@ -254,10 +360,14 @@ def main():
     if args.workload == "c3":
         run_c3(args)
         return
+    if args.workload == "c1":
+        run_c1(args)
+        return
     os.environ.setdefault("HHDMRG_VERBOSE", "1")   # progress line per DMRG step on stderr
     from hofstadter_hubbard_tsc_amd import Context, Driver
 
-    model = workload(args.m)
+    wl = workload("c2")
+    model = wl.model
     n = model.nsite
     ctx = Context(0)
     ctx.set_model(model)
@@ -298,6 +408,11 @@ def main():
     s2_avg_ms = st["stage2_ms"] / max(s2_launches, 1)
     s2_tflops = st["stage2_flops_alg"] / max(st["stage2_ms"], 1e-30) / 1e9
     s1_tflops = st["stage1_flops_alg"] / max(st["stage1_ms"], 1e-30) / 1e9
+    # executed flops: the engine's own association order (8 flops per complex
+    # multiply-add); with 3M products the MFMA units run 3/4 of those
+    s2_exec_tflops = st["stage2_flops_exec"] / max(st["stage2_ms"], 1e-30) / 1e9
+    s1_exec_tflops = st["stage1_flops_exec"] / max(st["stage1_ms"], 1e-30) / 1e9
+    bytes_alg = st["bytes_alg"] / max(s2_launches, 1)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hpsi_stage2_tj_cyl32x4_m%d_traffic.json" % args.m)
     if os.path.exists(tpath):
@@ -308,8 +423,15 @@ def main():
             "launches": int(s2_launches), "flops_per_launch": st["stage2_flops_alg"] / max(s2_launches, 1),
             "stage1_tflops": round(s1_tflops, 3), "stage1_ms_per_launch": st["stage1_ms"] / max(s2_launches, 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(s2_launches, 1),
+            "flops_exec_per_launch": st["stage2_flops_exec"] / max(s2_launches, 1),
+            "exec_tflops": round(s2_exec_tflops, 3),
+            "mfma_tflops": round(0.75 * s2_exec_tflops, 3),
+            "mfma_frac": round(0.75 * s2_exec_tflops / FP64_MFMA_PEAK_TFLOPS, 4),
+            "stage1_exec_tflops": round(s1_exec_tflops, 3),
+            "bytes_alg_per_matvec": bytes_alg,
+            "traffic_over_bytes_alg": round(traffic / bytes_alg, 2) if traffic and bytes_alg else None,
             "complex_products": "3M: 3 real FP64 MFMAs per complex multiply-add, so the MFMA units execute 3/4 "
-                                "of the 8 flops per complex multiply-add that achieved counts"}
+                                "of the 8 flops per complex multiply-add that achieved counts (mfma_tflops)"}
     if args.dump_structure and recs:
         ls = recs[-1].sys_len + 1
         dump = {}
@@ -323,10 +445,10 @@ def main():
     measure = None
     if args.measure and recs:
         measure = measure_at_scale(ctx, drv, model, recs[-1].sys_len + 1)
-    cpu = None
+    cpu, check = None, None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(ctx, drv, model, recs[-1].sys_len + 1, args.cpu_seconds, args.m)
+            cpu, check = cpu_baseline(ctx, drv, model, args.cpu_seconds, args.m, recs[-1].energy)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "GFLOP/s", "cores": None, "kind": "port", "sample": "failed: %s" % e}
     out = {
@@ -348,7 +470,12 @@ def main():
                    "sweep_seconds_est": round(ms_per_step * 2 * (n - 3) / 1e3, 1),
                    "matvec_share": round(mv_time / max(wall, 1e-30), 3),
                    "superblock_dim": int(recs[-1].dim) if recs else 0,
-                   "energy": recs[-1].energy if recs else None},
+                   "energy": recs[-1].energy if recs else None,
+                   "lanczos_iters": [r.iters for r in recs],
+                   "steps_at_iteration_cap": sum(1 for r in recs if not r.converged),
+                   "max_final_ritz_change": max((r.ritz_change for r in recs), default=None),
+                   "max_residual": max((r.residual for r in recs), default=None),
+                   "oracle_check": check},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -7,3 +7,4 @@ subroutine interface.
 from ._lib import LIB_PATH, load  # noqa: F401
 from .dmrg import (Batch, Block, Context, Driver, HHError, StepRecord, Superblock, Trunc,  # noqa: F401
                    structure_factor_cut)
+from .model import WORKLOADS, Model, Workload  # noqa: F401

@@ -29,16 +29,18 @@ class Schedule(C.Structure):
 
 class StepRecord(C.Structure):
     _fields_ = [("phase", i32), ("sys_len", i32), ("env_len", i32), ("iters", i32), ("num_state", i32),
-                ("pad", i32), ("dim", i64), ("energy", f64), ("trunc_err", f64), ("entropy", f64),
+                ("converged", i32), ("dim", i64), ("energy", f64), ("trunc_err", f64), ("entropy", f64),
                 ("seconds", f64), ("matvec_seconds", f64), ("matvec_flops", f64),
                 ("bind_seconds", f64), ("wave_seconds", f64), ("lanczos_seconds", f64), ("trunc_seconds", f64),
-                ("grow_seconds", f64)]
+                ("grow_seconds", f64), ("ritz_change", f64), ("residual", f64), ("hpsi_fp64", i32),
+                ("hpsi_fp32", i32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("matvecs", i64), ("matvec_ms", f64), ("stage1_ms", f64), ("stage2_ms", f64),
                 ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64), ("stage1_flops_alg", f64),
-                ("stage2_flops_alg", f64), ("reduce_ms", f64)]
+                ("stage2_flops_alg", f64), ("reduce_ms", f64), ("stage1_flops_exec", f64),
+                ("stage2_flops_exec", f64)]
 
 
 # numpy views of the engine's task structs (hh_core.hpp: Term, Tile)
@@ -80,6 +82,7 @@ SIGNATURES = [
     ("hh_batch_info", i32, [vp, P(i64), P(f64)]),
     ("hh_batch_model_wave", i32, [vp, vp, P(vp), P(vp)]),
     ("hh_lanczos", i32, [vp, vp, vp, f64, i32, P(f64), P(i32)]),
+    ("hh_lanczos_status", i32, [vp, P(i32), P(f64), P(f64)]),
     ("hh_truncate", i32, [vp, vp, i32, vp, i32, i32, f64, P(vp), P(i32), P(f64), P(f64)]),
     ("hh_trunc_identity", i32, [vp, vp, i32, P(vp)]),
     ("hh_trunc_destroy", i32, [vp]),
@@ -92,6 +95,7 @@ SIGNATURES = [
     ("hh_driver_warmup", i32, [vp, P(StepRecord), i32, P(i32)]),
     ("hh_driver_sweep", i32, [vp, i32, i32, i32, f64, f64, i32, i32, i32, P(StepRecord), i32, P(i32)]),
     ("hh_driver_reset_transform", i32, [vp]),
+    ("hh_driver_wave", i32, [vp, P(i32), P(i32), P(i64), vp]),
     ("hh_driver_block_count", i32, [vp, i32, P(i32)]),
     ("hh_driver_block", i32, [vp, i32, i32, P(vp)]),
     ("hh_set_profiling", i32, [vp, i32]),

@@ -174,6 +174,9 @@ struct hh_ctx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hh_stats stats{};
   std::vector<DevMem*> user_allocs;
+  // outcome of the last ground-state solve (hh_lanczos_status)
+  int last_converged = 0;
+  double last_ritz_change = 0.0, last_residual = 0.0;
 };
 
 struct hh_block {
@@ -199,6 +202,7 @@ struct hh_batch {
   DevMem table;
   std::vector<double2*> host_table;
   double flops_alg = 0.0, stage1_flops = 0.0, stage2_flops = 0.0, flops_exec = 0.0;
+  double stage1_exec = 0.0, stage2_exec = 0.0;
   int64_t total_dim = 0;
 };
 
@@ -280,6 +284,8 @@ void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
   ctx->stats.stage1_flops_alg += sb->plan.stage1.flops_alg;
   ctx->stats.stage2_flops_alg += sb->plan.stage2.flops_alg;
   ctx->stats.flops_exec += sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
+  ctx->stats.stage1_flops_exec += sb->plan.stage1.flops_exec;
+  ctx->stats.stage2_flops_exec += sb->plan.stage2.flops_exec;
   ctx->stats.bytes_alg += sb->plan.bytes_alg;
 }
 
@@ -297,9 +303,24 @@ void ensure_ws(hh_ctx* ctx, int64_t n, int nvec) {
   ctx->partial.alloc(sizeof(double) * (size_t)std::max(rdot_partials(), vec_blocks(n)) * (nvec + 2));
 }
 
+// Outcome of one ground-state solve.  converged: the reference's stopping
+// rule held (|change of the lowest Ritz value| <= lanzero, or beta <=
+// lanzero, code/lanczos.f90:77-80, :102) before the vector budget ran out;
+// ritz_change is the last change of the lowest Ritz value; residual is
+// |H psi - E psi| of the returned unit vector (for Lanczos the estimate
+// beta_k |z_k| of the Krylov relation, exact up to the loss of orthogonality).
+struct SolveOut {
+  double e0 = 0.0;
+  int iters = 0;
+  double mv_seconds = 0.0;
+  bool converged = false;
+  double ritz_change = 0.0;
+  double residual = 0.0;
+  int hpsi_fp64 = 0, hpsi_fp32 = 0;
+};
+
 // Lanczos, lanczos_memo_new (code/lanczos.f90:4-176), levels = 1.
-void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter, double* e0, int* iters,
-             double* mv_seconds) {
+SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter) {
   const int64_t n = sb->plan.sb.dim;
   need(n > 0, "empty superblock");
   need(max_iter >= 1, "max_iter must be >= 1");
@@ -350,14 +371,21 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
     check_hip(hipStreamSynchronize(ctx->stream), "sync");
     return std::make_pair(h[0], h[1]);
   };
+  SolveOut out;
+  int nhpsi = 1;
   auto r = alpha_and_update(1);
   alpha.push_back(finite(r.first));
   double ritz = alpha[0];
   int last = 1;
+  double b_last = std::sqrt(std::max(r.second, 0.0));   // norm of the Krylov remainder
   while (last < max_iter) {
     const double b = std::sqrt(finite(r.second));
     beta.push_back(b);
-    if (std::fabs(b) <= lanzero) break;
+    b_last = b;
+    if (std::fabs(b) <= lanzero) {
+      out.converged = true;
+      break;
+    }
     // normalise, then classical Gram-Schmidt against every stored vector with
     // real coefficients (Gram_Schmidt_process, code/space_info.f90:1176-1214;
     // the vectors are unit norm):  w <- (w - sum_j Re<v_j,w> v_j) / b
@@ -367,12 +395,16 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
     dev_scale_copy(vec(last), w, 1.0 / nw, n, ctx->stream);
     ++last;
     hpsi(vec(last - 1), w);
+    ++nhpsi;
     r = alpha_and_update(last);
     alpha.push_back(finite(r.first));
+    b_last = std::sqrt(std::max(r.second, 0.0));
     std::vector<double> d(alpha), e(beta);
     e.resize(d.size());
     tridiag_ql(d, e, nullptr);
-    if (std::fabs(d[0] - ritz) <= lanzero || last == max_iter) {
+    out.ritz_change = std::fabs(d[0] - ritz);
+    if (out.ritz_change <= lanzero) out.converged = true;
+    if (out.converged || last == max_iter) {
       ritz = d[0];
       break;
     }
@@ -387,9 +419,15 @@ void lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int m
   const double np = std::sqrt(fetch(1)[0]);
   dev_scale(psi, 1.0 / np, n, ctx->stream);
   check_hip(hipStreamSynchronize(ctx->stream), "lanczos");
-  *e0 = d[0];
-  *iters = last;
-  if (mv_seconds) *mv_seconds = mv;
+  out.e0 = d[0];
+  out.iters = last;
+  out.mv_seconds = mv;
+  out.residual = b_last * std::fabs(z[last - 1]);
+  out.hpsi_fp64 = nhpsi;
+  ctx->last_converged = out.converged ? 1 : 0;
+  ctx->last_ritz_change = out.ritz_change;
+  ctx->last_residual = out.residual;
+  return out;
 }
 
 // Eigendecomposition of every sector matrix (ZHEEV, code/get_den.f90:242),
@@ -714,13 +752,22 @@ struct Driver {
     ph_t = t;
   }
 
-  void record(int phase, hh_superblock* sb, double E, int it, const hh_trunc* t, double secs, double mvs, int64_t mv0,
+  // the step's ground-state solve (the plain form: Lanczos with FP64 H.psi)
+  SolveOut solve(hh_superblock* sb, double2* p, double lanzero) { return lanczos(ctx, sb, p, lanzero, S.max_iter); }
+
+  void record(int phase, hh_superblock* sb, const SolveOut& so, const hh_trunc* t, double secs, int64_t mv0,
               double fl0) {
+    const double E = so.e0, mvs = so.mv_seconds;
     hh_step_record r{};
     r.phase = phase;
     r.sys_len = sb->sys->d.length;
     r.env_len = sb->env->d.length;
-    r.iters = it;
+    r.iters = so.iters;
+    r.converged = so.converged ? 1 : 0;
+    r.ritz_change = so.ritz_change;
+    r.residual = so.residual;
+    r.hpsi_fp64 = so.hpsi_fp64;
+    r.hpsi_fp32 = so.hpsi_fp32;
     r.num_state = t ? t->num_state : 0;
     r.dim = sb->plan.sb.dim;
     r.energy = E;
@@ -738,9 +785,10 @@ struct Driver {
     recs.push_back(r);
     if (verbose()) {
       std::fprintf(stderr,
-                   "# step phase=%d sys=%d env=%d dim=%lld iters=%d kept=%d E=%.12f %.2fs "
+                   "# step phase=%d sys=%d env=%d dim=%lld iters=%d%s kept=%d E=%.12f res=%.1e %.2fs "
                    "(bind %.3f wave %.3f lanczos %.3f [H.psi %.3f] trunc %.3f grow %.3f)\n",
-                   r.phase, r.sys_len, r.env_len, (long long)r.dim, r.iters, r.num_state, r.energy, r.seconds,
+                   r.phase, r.sys_len, r.env_len, (long long)r.dim, r.iters, r.converged ? "" : "(cap)", r.num_state,
+                   r.energy, r.residual, r.seconds,
                    r.bind_seconds, r.wave_seconds, r.lanczos_seconds, r.matvec_seconds, r.trunc_seconds,
                    r.grow_seconds);
       std::fflush(stderr);
@@ -773,9 +821,7 @@ struct Driver {
       dev_start_vector(p, n, S.seed + step, ctx->stream);
       ++step;
       lap(1);
-      double E, mvs;
-      int it;
-      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      const SolveOut so = solve(sb.get(), p, lanzero);
       lap(2);
       std::unique_ptr<hh_trunc> ts, te;
       if (sb->plan.sbs.total_dim() <= kmax) ts.reset(trunc_identity(ctx, sb.get(), 0));
@@ -786,7 +832,7 @@ struct Driver {
       std::unique_ptr<hh_block> ns(grow(ctx, sb.get(), 0, ts.get()));
       std::unique_ptr<hh_block> ne(grow(ctx, sb.get(), 1, te.get()));
       lap(4);
-      record(0, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
+      record(0, sb.get(), so, ts.get(), wall() - t0, mv0, fl0);
       sysb[label + 1] = std::move(ns);
       envb[label + 1] = std::move(ne);
       syst[label + 1] = std::move(ts);
@@ -830,16 +876,14 @@ struct Driver {
       }
       ++step;
       lap(1);
-      double E, mvs;
-      int it;
-      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      const SolveOut so = solve(sb.get(), p, lanzero);
       lap(2);
       psi_dim = n;
       std::unique_ptr<hh_trunc> ts(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
       lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 0, ts.get()));
       lap(4);
-      record(phase, sb.get(), E, it, ts.get(), wall() - t0, mvs, mv0, fl0);
+      record(phase, sb.get(), so, ts.get(), wall() - t0, mv0, fl0);
       // the previous superblock references blocks that stay stored
       prev_sb = std::move(sb);
       sysb[label + 1] = std::move(nb);
@@ -882,16 +926,14 @@ struct Driver {
       }
       ++step;
       lap(1);
-      double E, mvs;
-      int it;
-      lanczos(ctx, sb.get(), p, lanzero, S.max_iter, &E, &it, &mvs);
+      const SolveOut so = solve(sb.get(), p, lanzero);
       lap(2);
       psi_dim = n;
       std::unique_ptr<hh_trunc> te(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
       lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 1, te.get()));
       lap(4);
-      record(phase, sb.get(), E, it, te.get(), wall() - t0, mvs, mv0, fl0);
+      record(phase, sb.get(), so, te.get(), wall() - t0, mv0, fl0);
       prev_sb = std::move(sb);
       envb[ns_ - label] = std::move(nb);
       envt[ns_ - label] = std::move(te);
@@ -2699,6 +2741,8 @@ int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** 
     b->stage2_flops = s2.flops_alg;
     b->flops_alg = s1.flops_alg + s2.flops_alg;
     b->flops_exec = s1.flops_exec + s2.flops_exec;
+    b->stage1_exec = s1.flops_exec;
+    b->stage2_exec = s2.flops_exec;
     b->table.alloc(sizeof(double2*) * 8 * nsb);
     b->host_table.assign(8 * nsb, nullptr);
     *out = b.release();
@@ -2751,6 +2795,8 @@ int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in, d
     ctx->stats.stage1_flops_alg += b->stage1_flops;
     ctx->stats.stage2_flops_alg += b->stage2_flops;
     ctx->stats.flops_exec += b->flops_exec;
+    ctx->stats.stage1_flops_exec += b->stage1_exec;
+    ctx->stats.stage2_flops_exec += b->stage2_exec;
     check_hip(hipStreamSynchronize(ctx->stream), "batch model_wave");
   });
 }
@@ -2758,8 +2804,18 @@ int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in, d
 int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi, double lanzero, int max_iter, double* e0, int* iters) {
   return guard(ctx, [&] {
     need(psi && e0 && iters, "null argument");
-    lanczos(ctx, sb, (double2*)psi, lanzero, max_iter, e0, iters, nullptr);
+    const SolveOut so = lanczos(ctx, sb, (double2*)psi, lanzero, max_iter);
+    *e0 = so.e0;
+    *iters = so.iters;
   });
+}
+
+int hh_lanczos_status(const hh_ctx* ctx, int32_t* converged, double* ritz_change, double* residual) {
+  if (!ctx) return 1;
+  if (converged) *converged = ctx->last_converged;
+  if (ritz_change) *ritz_change = ctx->last_ritz_change;
+  if (residual) *residual = ctx->last_residual;
+  return 0;
 }
 
 int hh_density_matrix_layout(const hh_superblock* sb, int side, int32_t* nblk, int64_t* size, int32_t* sector,
@@ -3140,6 +3196,23 @@ int hh_driver_sweep(hh_driver* h, int dir, int bg, int end, double lanzero, doub
 int hh_driver_reset_transform(hh_driver* h) {
   h->d.prev_sb.reset();
   return 0;
+}
+
+int hh_driver_wave(const hh_driver* h, int32_t* sys_len, int32_t* env_len, int64_t* dim, double* psi) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    const hh_superblock* sb = h->d.prev_sb.get();
+    need(sb != nullptr && h->d.psi_dim == sb->plan.sb.dim, "no sweep step since the driver was created or reset");
+    if (sys_len) *sys_len = sb->sys->d.length;
+    if (env_len) *env_len = sb->env->d.length;
+    if (dim) *dim = sb->plan.sb.dim;
+    if (psi) {
+      hh_ctx* ctx = h->d.ctx;
+      check_hip(hipMemcpyAsync(psi, h->d.psi.p, sizeof(double2) * (size_t)sb->plan.sb.dim, hipMemcpyDeviceToDevice,
+                               ctx->stream), "d2d");
+      check_hip(hipStreamSynchronize(ctx->stream), "sync");
+    }
+  });
 }
 
 int hh_driver_block_count(const hh_driver* h, int side, int32_t* count) {

@@ -26,10 +26,16 @@ def _check(ctx_ptr, rc, what):
         raise HHError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
 
-def _dptr(t):
-    """Device address of a contiguous torch tensor."""
-    if not t.is_cuda or not t.is_contiguous():
+def _dptr(t, n=None):
+    """Device address of a contiguous complex128 torch tensor of at least n
+    elements (the C side reads and writes n complex128 values through it)."""
+    import torch
+    if not hasattr(t, "is_cuda") or not t.is_cuda or not t.is_contiguous():
         raise HHError("expected a contiguous device tensor")
+    if t.dtype != torch.complex128:
+        raise HHError("expected a complex128 device tensor, got %s" % t.dtype)
+    if n is not None and t.numel() < n:
+        raise HHError("device tensor holds %d complex128 elements, %d needed" % (t.numel(), n))
     return C.c_void_p(t.data_ptr())
 
 
@@ -140,11 +146,14 @@ class Block:
     def upload(self, arena):
         """Whole arena from a complex128 numpy array or contiguous device tensor."""
         lib = L.load()
+        size = self.info()[0]
         if hasattr(arena, "is_cuda") and arena.is_cuda:
             _sync()
-            src = _dptr(arena)
+            src = _dptr(arena, size)
         else:
             arena = np.ascontiguousarray(arena, dtype=np.complex128)
+            if arena.size < size:
+                raise HHError("arena holds %d complex128 elements, the block needs %d" % (arena.size, size))
             src = arena.ctypes.data_as(C.c_void_p)
         _check(self.ctx.ptr, lib.hh_block_upload(self.ctx.ptr, self.ptr, src), "hh_block_upload")
 
@@ -252,12 +261,18 @@ class StepRecord:
     lanczos_seconds: float = 0.0
     trunc_seconds: float = 0.0
     grow_seconds: float = 0.0
+    converged: bool = True
+    ritz_change: float = 0.0
+    residual: float = 0.0
+    hpsi_fp64: int = 0
+    hpsi_fp32: int = 0
 
     @classmethod
     def from_c(cls, r):
         return cls(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy, r.trunc_err, r.entropy,
                    r.seconds, r.matvec_seconds, r.matvec_flops, r.bind_seconds, r.wave_seconds, r.lanczos_seconds,
-                   r.trunc_seconds, r.grow_seconds)
+                   r.trunc_seconds, r.grow_seconds, bool(r.converged), r.ritz_change, r.residual, r.hpsi_fp64,
+                   r.hpsi_fp32)
 
 
 class Context:
@@ -340,7 +355,7 @@ class Context:
     def model_wave(self, sb, psi_in, psi_out):
         """model_wave (code/mat_vec.f90:3476): psi_out = H psi_in."""
         _sync()
-        self._rc(L.load().hh_model_wave(self.ptr, sb.ptr, _dptr(psi_in), _dptr(psi_out)), "hh_model_wave")
+        self._rc(L.load().hh_model_wave(self.ptr, sb.ptr, _dptr(psi_in, sb.dim), _dptr(psi_out, sb.dim)), "hh_model_wave")
 
     def batch(self, sbs):
         """Group bound superblocks for one-launch-per-stage H.psi (at most 31)."""
@@ -355,17 +370,23 @@ class Context:
         n = len(batch._keep)
         if len(psi_in) != n or len(psi_out) != n:
             raise HHError("one input and one output vector per batch member")
-        a = (C.c_void_p * n)(*[_dptr(t).value for t in psi_in])
-        b = (C.c_void_p * n)(*[_dptr(t).value for t in psi_out])
+        a = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psi_in, batch._keep)])
+        b = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psi_out, batch._keep)])
         self._rc(L.load().hh_batch_model_wave(self.ptr, batch.ptr, a, b), "hh_batch_model_wave")
 
     def lanczos(self, sb, psi, lanzero, max_iter=100):
         """lanczos_memo_new (code/lanczos.f90:4-176), levels=1; psi updated in place."""
         _sync()
         e, it = L.f64(), L.i32()
-        self._rc(L.load().hh_lanczos(self.ptr, sb.ptr, _dptr(psi), lanzero, max_iter, C.byref(e), C.byref(it)),
+        self._rc(L.load().hh_lanczos(self.ptr, sb.ptr, _dptr(psi, sb.dim), lanzero, max_iter, C.byref(e), C.byref(it)),
                  "hh_lanczos")
         return e.value, it.value
+
+    def lanczos_status(self):
+        """(converged, last Ritz-value change, |H psi - E psi|) of the last lanczos call."""
+        c, d, r = L.i32(), L.f64(), L.f64()
+        self._rc(L.load().hh_lanczos_status(self.ptr, C.byref(c), C.byref(d), C.byref(r)), "hh_lanczos_status")
+        return bool(c.value), d.value, r.value
 
     def density_matrix(self, sb, side, psi):
         """Get_density_matrix_sys_new / _env_new (code/get_den.f90:106-177):
@@ -384,7 +405,7 @@ class Context:
                                               off.ctypes.data_as(C.POINTER(C.c_int64))), "hh_density_matrix_layout")
         rho = torch.zeros(max(size.value, 1), dtype=torch.complex128, device="cuda")
         _sync()
-        self._rc(lib.hh_density_matrix(self.ptr, sb.ptr, side, _dptr(psi), _dptr(rho)), "hh_density_matrix")
+        self._rc(lib.hh_density_matrix(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), _dptr(rho, size.value)), "hh_density_matrix")
         h = rho.cpu().numpy()
         return [(int(sec[i]), h[off[i]:off[i] + dim[i] * dim[i]].reshape((dim[i], dim[i]), order="F"))
                 for i in range(nb.value)]
@@ -406,19 +427,19 @@ class Context:
         """wave_to_disk (code/space_info.f90:1406-1436): psi in the reference's
         Fortran unformatted wavefunction file."""
         _sync()
-        self._rc(L.load().hh_wave_to_disk(self.ptr, sb.ptr, _dptr(psi), os.fsencode(path)), "hh_wave_to_disk")
+        self._rc(L.load().hh_wave_to_disk(self.ptr, sb.ptr, _dptr(psi, sb.dim), os.fsencode(path)), "hh_wave_to_disk")
 
     def wave_from_disk(self, sb, path, psi):
         """wave_from_disk (code/space_info.f90:1440-1468) into the device vector psi."""
         _sync()
-        self._rc(L.load().hh_wave_from_disk(self.ptr, sb.ptr, os.fsencode(path), _dptr(psi)), "hh_wave_from_disk")
+        self._rc(L.load().hh_wave_from_disk(self.ptr, sb.ptr, os.fsencode(path), _dptr(psi, sb.dim)), "hh_wave_from_disk")
 
     def truncate(self, sb, side, psi, kept_min, kept_max, trun_err):
         """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90)."""
         _sync()
         p = C.c_void_p()
         ns, er, en = L.i32(), L.f64(), L.f64()
-        self._rc(L.load().hh_truncate(self.ptr, sb.ptr, side, _dptr(psi), kept_min, kept_max, trun_err, C.byref(p),
+        self._rc(L.load().hh_truncate(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), kept_min, kept_max, trun_err, C.byref(p),
                                       C.byref(ns), C.byref(er), C.byref(en)), "hh_truncate")
         return Trunc(self, p, ns.value, er.value, en.value)
 
@@ -436,8 +457,8 @@ class Context:
     def wave_trans(self, direction, old_sb, psi_old, sys_t, env_t, new_sb, psi_new):
         """wave_trans_LR (direction 0) / wave_trans_RL (1), code/warm_up.f90:1855-2170."""
         _sync()
-        self._rc(L.load().hh_wave_trans(self.ptr, direction, old_sb.ptr, _dptr(psi_old), sys_t.ptr, env_t.ptr,
-                                        new_sb.ptr, _dptr(psi_new)), "hh_wave_trans")
+        self._rc(L.load().hh_wave_trans(self.ptr, direction, old_sb.ptr, _dptr(psi_old, old_sb.dim), sys_t.ptr, env_t.ptr,
+                                        new_sb.ptr, _dptr(psi_new, new_sb.dim)), "hh_wave_trans")
 
     def start_vector(self, psi, seed):
         _sync()
@@ -496,6 +517,32 @@ class Driver:
     def _recs(buf, n):
         return [StepRecord.from_c(r) for r in buf[:n]]
 
+    def _psi(self, sys_len, psi):
+        """Device address of psi after checking it against the dimension of
+        the superblock of the stored sys block of sys_len sites and the env
+        block of Nsite - sys_len - 2 sites (the C side reads that many
+        complex128 values)."""
+        n = self.ctx.model.Lx * self.ctx.model.Ly
+        sysb, envb = self.block(0, sys_len), self.block(1, n - sys_len - 2)
+        key = (sys_len, sysb.ptr.value, envb.ptr.value)
+        if getattr(self, "_dims", None) is None:
+            self._dims = {}
+        if key not in self._dims:
+            self._dims[key] = self.ctx.bind(sysb, envb, self.ctx.model.n_elec).dim
+        return _dptr(psi, self._dims[key])
+
+    def wave(self):
+        """(sys_len, env_len, psi): the ground state of the last sweep step as a
+        new device tensor, with the lengths of the blocks of its superblock."""
+        import torch
+        ls, le, d = L.i32(), L.i32(), L.i64()
+        self.ctx._rc(L.load().hh_driver_wave(self.ptr, C.byref(ls), C.byref(le), C.byref(d), None), "hh_driver_wave")
+        psi = torch.empty(d.value, dtype=torch.complex128, device="cuda")
+        _sync()
+        self.ctx._rc(L.load().hh_driver_wave(self.ptr, C.byref(ls), C.byref(le), C.byref(d), _dptr(psi, d.value)),
+                     "hh_driver_wave")
+        return ls.value, le.value, psi
+
     def warmup(self):
         cap = 4096
         buf = (L.StepRecord * cap)()
@@ -530,7 +577,7 @@ class Driver:
         superblock of the stored sys block of sys_len sites (measure_operator_dia)."""
         _sync()
         out = np.zeros(self.ctx.model.Lx * self.ctx.model.Ly)
-        self.ctx._rc(L.load().hh_driver_density(self.ptr, sys_len, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_density(self.ptr, sys_len, self._psi(sys_len, psi),
                                                 out.ctypes.data_as(C.POINTER(C.c_double))), "hh_driver_density")
         return out
 
@@ -539,7 +586,7 @@ class Driver:
         _sync()
         n = self.ctx.model.Lx * self.ctx.model.Ly
         out = np.zeros((n, n))
-        self.ctx._rc(L.load().hh_driver_density_cross(self.ptr, sys_len, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_density_cross(self.ptr, sys_len, self._psi(sys_len, psi),
                                                       out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_density_cross")
         return out
@@ -549,7 +596,7 @@ class Driver:
         site on the sys side of the cut (1 .. sys_len + 1)."""
         _sync()
         out = np.zeros(self.ctx.model.Lx * self.ctx.model.Ly)
-        self.ctx._rc(L.load().hh_driver_density_row(self.ptr, sys_len, site, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_density_row(self.ptr, sys_len, site, self._psi(sys_len, psi),
                                                     out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_density_row")
         return out
@@ -559,7 +606,7 @@ class Driver:
         _sync()
         n = self.ctx.model.Lx * self.ctx.model.Ly
         out = np.zeros((n, n))
-        self.ctx._rc(L.load().hh_driver_spin_cross(self.ptr, sys_len, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_spin_cross(self.ptr, sys_len, self._psi(sys_len, psi),
                                                    out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_spin_cross")
         return out
@@ -570,7 +617,7 @@ class Driver:
         _sync()
         n = self.ctx.model.Lx * self.ctx.model.Ly
         out = np.zeros((n, n, 2))
-        self.ctx._rc(L.load().hh_driver_hopping_cross(self.ptr, sys_len, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_hopping_cross(self.ptr, sys_len, self._psi(sys_len, psi),
                                                       out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_hopping_cross")
         return out[..., 0] + 1j * out[..., 1]
@@ -582,7 +629,7 @@ class Driver:
         bonds = np.ascontiguousarray(np.asarray(bonds, dtype=np.int32).reshape(-1, 2))
         nb = len(bonds)
         out = np.zeros((nb, nb, 2))
-        self.ctx._rc(L.load().hh_driver_pair_cross(self.ptr, sys_len, _dptr(psi), nb,
+        self.ctx._rc(L.load().hh_driver_pair_cross(self.ptr, sys_len, self._psi(sys_len, psi), nb,
                                                    bonds.ctypes.data_as(C.POINTER(C.c_int32)),
                                                    out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_pair_cross")
@@ -593,7 +640,7 @@ class Driver:
         site on the sys side of the cut (1 .. sys_len + 1)."""
         _sync()
         out = np.zeros(self.ctx.model.Lx * self.ctx.model.Ly)
-        self.ctx._rc(L.load().hh_driver_spin_row(self.ptr, sys_len, site, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_spin_row(self.ptr, sys_len, site, self._psi(sys_len, psi),
                                                  out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_spin_row")
         return out
@@ -603,7 +650,7 @@ class Driver:
         j-1; NaN at j = site), site on the sys side (1 .. sys_len + 1)."""
         _sync()
         out = np.zeros((self.ctx.model.Lx * self.ctx.model.Ly, 2))
-        self.ctx._rc(L.load().hh_driver_hopping_row(self.ptr, sys_len, site, _dptr(psi),
+        self.ctx._rc(L.load().hh_driver_hopping_row(self.ptr, sys_len, site, self._psi(sys_len, psi),
                                                     out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_hopping_row")
         return out[:, 0] + 1j * out[:, 1]
@@ -612,7 +659,7 @@ class Driver:
         _sync()
         n = self.ctx.model.Lx * self.ctx.model.Ly
         out = np.zeros((n, n, 2) if cplx else (n, n))
-        self.ctx._rc(getattr(L.load(), fn)(self.ptr, sys_len, _dptr(psi), out.ctypes.data_as(C.POINTER(C.c_double))),
+        self.ctx._rc(getattr(L.load(), fn)(self.ptr, sys_len, self._psi(sys_len, psi), out.ctypes.data_as(C.POINTER(C.c_double))),
                      fn)
         return out[:, :, 0] + 1j * out[:, :, 1] if cplx else out
 
@@ -647,7 +694,7 @@ class Driver:
         _sync()
         q = np.ascontiguousarray(np.asarray(quads, dtype=np.int32).reshape(-1, 4))
         out = np.zeros((len(q), 2))
-        self.ctx._rc(L.load().hh_driver_pair_block(self.ptr, sys_len, _dptr(psi), len(q),
+        self.ctx._rc(L.load().hh_driver_pair_block(self.ptr, sys_len, self._psi(sys_len, psi), len(q),
                                                    q.ctypes.data_as(C.POINTER(C.c_int32)),
                                                    out.ctypes.data_as(C.POINTER(C.c_double))),
                      "hh_driver_pair_block")

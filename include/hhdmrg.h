@@ -65,7 +65,8 @@ typedef struct {
   int32_t sys_len, env_len;
   int32_t iters;
   int32_t num_state;
-  int32_t pad;
+  int32_t converged;         /* 1: the Lanczos stopping rule held (code/lanczos.f90:77-80,
+                                :102); 0: it stopped at the vector budget (loop_in = 100) */
   int64_t dim;               /* superblock dimension (multiplet space) */
   double energy;
   double trunc_err;
@@ -79,6 +80,11 @@ typedef struct {
   double lanczos_seconds;    /* lanczos_memo_new, H.psi included */
   double trunc_seconds;      /* density matrix, eigendecomposition, kept-count rule */
   double grow_seconds;       /* Get_Hamiltonian_Sys/Env with update_trun_* */
+  double ritz_change;        /* last change of the lowest Ritz value */
+  double residual;           /* |H psi - E psi| of the unit ground-state vector (Lanczos:
+                                beta_k |z_k|, the remainder of the Krylov relation) */
+  int32_t hpsi_fp64;         /* H.psi applications with FP64 arithmetic in the solve */
+  int32_t hpsi_fp32;         /* H.psi applications with FP32 arithmetic in the solve */
 } hh_step_record;
 
 /* ---- context ---------------------------------------------------------- */
@@ -168,6 +174,12 @@ int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in_de
  * psi_dev holds the start vector on entry and the ground state on exit. */
 int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi_dev, double lanzero, int max_iter, double* e0,
                int* iters);
+/* Outcome of the last hh_lanczos call on ctx: whether the stopping rule of
+ * lanczos_memo_new (code/lanczos.f90:77-80, :102) held before max_iter, the
+ * last change of the lowest Ritz value and |H psi - E psi| of the returned
+ * unit vector (beta_k |z_k|).  The reference has no such output: it writes
+ * the energy only (Energy_All.dat, :170-174). */
+int hh_lanczos_status(const hh_ctx* ctx, int32_t* converged, double* ritz_change, double* residual);
 
 /* Get_density_matrix_sys_new / Get_density_matrix_env_new
  * (code/get_den.f90:106-177, levels = 1): the reduced density matrix of the
@@ -252,6 +264,13 @@ int hh_driver_save(hh_driver* d, const char* path);
 int hh_driver_load(hh_driver* d, const char* path);
 /* forget the previous superblock (next first step does not transform) */
 int hh_driver_reset_transform(hh_driver* d);
+/* The ground state of the last sweep step (the eigvec that left_to_right /
+ * right_to_left hand to wave_trans_LR / wave_trans_RL, code/warm_up.f90:1607,
+ * :1786): the lengths of the sys and env blocks its superblock was bound from
+ * and its dimension; when psi_dev is not null the vector (complex128, super-
+ * basis order) is copied there.  Fails when no sweep step has run since the
+ * driver was created, loaded or reset. */
+int hh_driver_wave(const hh_driver* d, int32_t* sys_len, int32_t* env_len, int64_t* dim, double* psi_dev);
 int hh_driver_block_count(const hh_driver* d, int side, int32_t* count);
 /* borrowed handle of a stored block (side 0 sys / 1 env, by length); valid
  * until the driver replaces or frees it */
@@ -367,6 +386,8 @@ typedef struct {
   double stage1_flops_alg;   /* part of flops_alg done by the stage-1 launches */
   double stage2_flops_alg;   /* part of flops_alg done by the stage-2 launches */
   double reduce_ms;          /* kernel time of the split-sum reductions of both stages */
+  double stage1_flops_exec;  /* part of flops_exec run by the stage-1 launches */
+  double stage2_flops_exec;  /* part of flops_exec run by the stage-2 launches */
 } hh_stats;
 /* Arithmetic of the H.psi tile kernels for the calls that follow on this
  * context: 0 FP64 MFMA (plain form), 1 FP32 MFMA on FP64-stored operands

@@ -172,3 +172,32 @@ def test_density_matrix_matches_oracle(ctx, kind, Lx, Ly, flux, N, kept):
             assert g.shape == r.shape
             assert np.abs(g - r).max() < 1e-14 * max(1.0, np.abs(r).max())
         assert abs(sum(np.trace(g).real for _, g in got) - 1.0) < 1e-12
+
+
+def test_vector_arguments_are_checked(ctx):
+    """A device vector of the wrong dtype or shorter than the superblock is
+    refused by the binding before any pointer reaches the library, and
+    hh_lanczos_status reports the last solve."""
+    import torch
+    from hofstadter_hubbard_tsc_amd import HHError
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=6)
+    ctx.set_model(m)
+    run = driver.Run(m, driver.Schedule(kept_max=20, n_sweeps=0, tol=1e-9))
+    run.warm_up()
+    n = m.nsite
+    sysb, envb = run.sys_blocks[n // 2 - 1], run.env_blocks[n - n // 2 - 1]
+    sb = ctx.bind(upload_block(ctx, sysb, 0, 1), upload_block(ctx, envb, 1, 1), m.n_elec)
+    good = torch.zeros(sb.dim, dtype=torch.complex128, device="cuda")
+    with pytest.raises(HHError):
+        ctx.model_wave(sb, good[:-1].contiguous(), good.clone())
+    with pytest.raises(HHError):
+        ctx.model_wave(sb, torch.zeros(sb.dim, dtype=torch.complex64, device="cuda"), good.clone())
+    with pytest.raises(HHError):
+        ctx.lanczos(sb, torch.zeros(2 * sb.dim, dtype=torch.float64, device="cuda"), 1e-9, 10)
+    psi = dev(dmrg.start_vector(sb.dim, 3))
+    E, it = ctx.lanczos(sb, psi, 1e-12, 100)
+    conv, dritz, res = ctx.lanczos_status()
+    assert conv and dritz <= 1e-12 and res < 1e-5, (conv, dritz, res)
+    E2, it2 = ctx.lanczos(sb, dev(dmrg.start_vector(sb.dim, 3)), 1e-12, 3)
+    conv, dritz, res = ctx.lanczos_status()
+    assert it2 == 3 and not conv and res > 0.0
