@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
+#include <string>
 #include <stdexcept>
 #include <type_traits>
 
@@ -56,11 +58,14 @@ static constexpr int SWZ = 4;
 static constexpr int XCHUNK = 8;
 
 void DevTasks::release() {
-  if (tiles) (void)hipFree(tiles);
-  if (terms) (void)hipFree(terms);
-  if (work) (void)hipFree(work);
-  if (red) (void)hipFree(red);
-  if (partial) (void)hipFree(partial);
+  if (!borrowed) {
+    if (tiles) (void)hipFree(tiles);
+    if (terms) (void)hipFree(terms);
+    if (work) (void)hipFree(work);
+    if (red) (void)hipFree(red);
+    if (partial) (void)hipFree(partial);
+  }
+  borrowed = false;
   tiles = nullptr;
   terms = nullptr;
   work = nullptr;
@@ -69,8 +74,10 @@ void DevTasks::release() {
   ntiles = nterms = nwork = nred = nslots = 0;
 }
 
-void DevTasks::upload(const TaskList& tl, hipStream_t s) {
+void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap) {
   release();
+  if (place && (sizeof(Tile) * tl.tiles.size() + sizeof(Term) * tl.terms.size() > cap))
+    throw HHCapacity("task list tables exceed the fixed scratch");
   ntiles = (int)tl.tiles.size();
   nterms = (int)tl.terms.size();
   flops_alg = tl.flops_alg;
@@ -173,6 +180,38 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
   nwork = (int)w.size();
   nred = (int)reds.size();
   nslots = slot;
+  if (place) {
+    // carve the tables out of the given memory, 256-byte aligned; nothing is
+    // allocated and release() leaves the memory alone
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_tiles = 0, o_terms = o_tiles + up(sizeof(Tile) * ntiles);
+    const size_t o_work = o_terms + up(sizeof(Term) * nterms);
+    const size_t o_red = o_work + up(sizeof(WorkItem) * nwork);
+    const size_t o_part = o_red + up(sizeof(ReduceItem) * nred);
+    const size_t end = o_part + sizeof(double2) * (size_t)nslots * MTR * MTC;
+    if (end > cap) {
+      nwork = nred = nslots = ntiles = nterms = 0;
+      throw HHCapacity("task list needs " + std::to_string(end) + " bytes of device tables and partial sums, the "
+                       "fixed scratch holds " + std::to_string(cap));
+    }
+    char* base = (char*)place;
+    borrowed = true;
+    tiles = ntiles ? (Tile*)(base + o_tiles) : nullptr;
+    terms = nterms ? (Term*)(base + o_terms) : nullptr;
+    work = nwork ? (WorkItem*)(base + o_work) : nullptr;
+    red = nred ? (ReduceItem*)(base + o_red) : nullptr;
+    partial = nred ? (double2*)(base + o_part) : nullptr;
+    if (ntiles)
+      check_hip(hipMemcpyAsync(tiles, tl.tiles.data(), sizeof(Tile) * ntiles, hipMemcpyHostToDevice, s), "copy");
+    if (nterms)
+      check_hip(hipMemcpyAsync(terms, tl.terms.data(), sizeof(Term) * nterms, hipMemcpyHostToDevice, s), "copy");
+    if (nwork)
+      check_hip(hipMemcpyAsync(work, w.data(), sizeof(WorkItem) * nwork, hipMemcpyHostToDevice, s), "copy");
+    if (nred)
+      check_hip(hipMemcpyAsync(red, reds.data(), sizeof(ReduceItem) * nred, hipMemcpyHostToDevice, s), "copy");
+    check_hip(hipStreamSynchronize(s), "task upload");
+    return;
+  }
   if (ntiles) {
     check_hip(hipMalloc(&tiles, sizeof(Tile) * ntiles), "hipMalloc tiles");
     check_hip(hipMemcpyAsync(tiles, tl.tiles.data(), sizeof(Tile) * ntiles, hipMemcpyHostToDevice, s), "copy tiles");
@@ -191,6 +230,37 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s) {
     check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * MTR * MTC), "hipMalloc partial");
   }
   check_hip(hipStreamSynchronize(s), "task upload");
+}
+
+// ----------------------------------------------------------------------------
+// fixed scratch of the allocation-free entry point
+// ----------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+void* noalloc_scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) throw std::runtime_error("device ordinal out of range for the scratch");
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_ptr[device]) {
+    // the array's name is a host-side stand-in; its device address on the
+    // current device comes from the runtime
+    void* p = nullptr;
+    check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned no address");
+    scratch_ptr[device] = p;
+  }
+  return scratch_ptr[device];
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  return scratch_ptr[device];
 }
 
 // ----------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -38,10 +39,30 @@ struct DevTasks {
   // empty 16 x 16 sub-tiles; otherwise every item is class 0.
   int cls_off[5] = {0, 0, 0, 0, 0};
   bool edge_skip = false;
-  void upload(const TaskList& tl, hipStream_t s);
+  // borrowed: the arrays live in caller-provided device memory (never freed here)
+  bool borrowed = false;
+  // Device arrays from hipMalloc, or -- when `place` is given -- carved out of
+  // the `cap` bytes at `place` (throws HHCapacity when they do not fit).
+  void upload(const TaskList& tl, hipStream_t s, void* place = nullptr, size_t cap = 0);
   void release();
   ~DevTasks() { release(); }
 };
+
+// A task list whose device tables do not fit the memory given to upload().
+struct HHCapacity : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Fixed device scratch of the allocation-free entry point
+// (hh_run_tasks_noalloc): a file-scope device array of this many bytes in
+// the engine's code object, present on every GPU the library runs on until
+// the process ends.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+// Address of that array on `device`, which must be the current device: taken
+// with hipGetSymbolAddress on the first call for the device and kept.
+void* noalloc_scratch_init(int device);
+// The address found by noalloc_scratch_init for `device` (nullptr before it).
+void* noalloc_scratch(int device);
 
 enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
 

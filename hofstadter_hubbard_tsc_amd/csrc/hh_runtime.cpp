@@ -2246,6 +2246,9 @@ int hh_ctx_create(int device, hh_ctx** out) {
     check_hip(hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (device < 0 || device >= n) throw HHError(-1, "no such HIP device");
     check_hip(hipSetDevice(device), "hipSetDevice");
+    // device address of the allocation-free entry point's fixed scratch,
+    // once per device, with the device current
+    (void)noalloc_scratch_init(device);
     check_hip(hipStreamCreate(&c->stream), "hipStreamCreate");
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw HHError(-2, "rocblas_create_handle");
     rocblas_set_stream(c->rb, c->stream);
@@ -3555,29 +3558,68 @@ int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// A caller's task list checked against its buffer slots (hh_run_tasks).
+TaskList checked_tasks(const void* tiles, int32_t ntiles, const void* terms, int32_t nterms, void* const* bufs_dev,
+                       int precision) {
+  need(ntiles >= 0 && nterms >= 0 && bufs_dev, "bad task list");
+  need(ntiles == 0 || tiles, "null tile array");
+  need(nterms == 0 || terms, "null term array");
+  need(precision == PREC_FP64 || precision == PREC_FP32, "precision must be 0 or 1");
+  TaskList tl;
+  tl.tiles.assign((const Tile*)tiles, (const Tile*)tiles + ntiles);
+  tl.terms.assign((const Term*)terms, (const Term*)terms + nterms);
+  for (const Tile& t : tl.tiles) {
+    need(t.t0 >= 0 && t.t1 <= nterms && t.t0 <= t.t1 && t.M >= 0 && t.N >= 0, "tile out of range");
+    need(t.c_buf >= 0 && t.c_buf < 8 && bufs_dev[t.c_buf], "tile buffer slot");
+    for (int k = t.t0; k < t.t1; ++k) {
+      const Term& m = tl.terms[k];
+      need(m.a_buf < 8 && bufs_dev[m.a_buf] && m.a_op <= 3 && m.b_op <= 3 && m.K >= 0, "term operand");
+      if (m.K > 0) need(m.b_buf < 8 && bufs_dev[m.b_buf], "term operand");
+    }
+  }
+  return tl;
+}
+}  // namespace
+
+extern "C" {
+
 int hh_run_tasks(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
                  void* const* bufs_dev, int precision) {
   return guard(ctx, [&] {
-    need(ntiles >= 0 && nterms >= 0 && bufs_dev, "bad task list");
-    need(precision == PREC_FP64 || precision == PREC_FP32, "precision must be 0 or 1");
-    TaskList tl;
-    tl.tiles.assign((const Tile*)tiles, (const Tile*)tiles + ntiles);
-    tl.terms.assign((const Term*)terms, (const Term*)terms + nterms);
-    for (const Tile& t : tl.tiles) {
-      need(t.t0 >= 0 && t.t1 <= nterms && t.t0 <= t.t1 && t.M >= 0 && t.N >= 0, "tile out of range");
-      need(t.c_buf >= 0 && t.c_buf < 8 && bufs_dev[t.c_buf], "tile buffer slot");
-      for (int k = t.t0; k < t.t1; ++k) {
-        const Term& m = tl.terms[k];
-        need(m.a_buf < 8 && bufs_dev[m.a_buf] && m.a_op <= 3 && m.b_op <= 3 && m.K >= 0, "term operand");
-        if (m.K > 0) need(m.b_buf < 8 && bufs_dev[m.b_buf], "term operand");
-      }
-    }
+    const TaskList tl = checked_tasks(tiles, ntiles, terms, nterms, bufs_dev, precision);
     DevTasks dt;
     dt.upload(tl, ctx->stream);
     BufPtrs b;
     for (int i = 0; i < 8; ++i) b.p[i] = (double2*)bufs_dev[i];
     launch_tiles(dt, b, ctx->stream, precision);
     check_hip(hipStreamSynchronize(ctx->stream), "hh_run_tasks");
+  });
+}
+
+// hh_run_tasks with the task tables, work items and split-sum partials in
+// the fixed device scratch (its address on this context's device was taken
+// at hh_ctx_create): no device memory is allocated or freed.  Every launch
+// rewrites the parts of the scratch it reads, so nothing needs clearing
+// between calls; calls that overlap on one GPU share the array and race.
+int hh_run_tasks_noalloc(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
+                         void* const* bufs_dev, int precision) {
+  return guard(ctx, [&] {
+    const TaskList tl = checked_tasks(tiles, ntiles, terms, nterms, bufs_dev, precision);
+    void* scr = noalloc_scratch(ctx->device);
+    need(scr != nullptr, "no fixed scratch on this device");
+    DevTasks dt;
+    try {
+      dt.upload(tl, ctx->stream, scr, NOALLOC_SCRATCH_BYTES);
+    } catch (const HHCapacity& e) {
+      throw HHError(1, e.what());
+    }
+    BufPtrs b;
+    for (int i = 0; i < 8; ++i) b.p[i] = (double2*)bufs_dev[i];
+    launch_tiles(dt, b, ctx->stream, precision);
+    check_hip(hipStreamSynchronize(ctx->stream), "hh_run_tasks_noalloc");
   });
 }
 

@@ -438,6 +438,17 @@ int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_
 int hh_run_tasks(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
                  void* const* bufs_dev, int precision);
 
+/* hh_run_tasks without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The task tables, work items and split-sum partial buffer go
+ * into a fixed 64 MiB device array of the library (held on every GPU the
+ * library runs on until the process ends); a task list that needs more
+ * returns status 1.  The caller still passes every input and output buffer.
+ * All calls on one GPU share that array: calls that overlap on one GPU race. */
+int hh_run_tasks_noalloc(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* terms, int32_t nterms,
+                         void* const* bufs_dev, int precision);
+
 /* Counter-based start vector (splitmix64 stream) written to psi_dev. */
 int hh_start_vector(hh_ctx* ctx, double* psi_dev, int64_t n, uint64_t seed);
 
