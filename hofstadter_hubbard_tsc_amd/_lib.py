@@ -135,6 +135,7 @@ SIGNATURES = [
     ("hh_plan_block", i32, [vp, i32, P(i64), P(i32), P(i32), P(i32), P(i32), P(i64), P(i32), P(i32)]),
     ("hh_plan_layout", i32, [vp, i32, i32, P(i32), P(i32), P(i32), P(i64)]),
     ("hh_plan_op_offset", i32, [vp, i32, i32, i32, P(i64)]),
+    ("hh_plan_enlarged", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
     ("hh_plan_super", i32, [vp, P(i64), P(i32), P(i32), P(i32), P(i32), P(i32), P(i64)]),
     ("hh_run_tasks", i32, [vp, vp, i32, vp, i32, P(vp), i32]),
     ("hh_run_tasks_noalloc", i32, [vp, vp, i32, vp, i32, P(vp), i32]),

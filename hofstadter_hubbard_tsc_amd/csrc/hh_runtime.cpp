@@ -638,8 +638,20 @@ hh_trunc* trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side) {
   return t;
 }
 
+// The truncation's enlarged basis is the one it is applied to: same sectors
+// (N, 2S, dim) in the same order.  Guards the kernels against a truncation
+// of another block (its offsets and leading dimensions index that basis).
+bool same_sectors(const EnlBasis& a, const EnlBasis& b) {
+  if (a.sec.size() != b.sec.size()) return false;
+  for (size_t i = 0; i < a.sec.size(); ++i)
+    if (a.sec[i].N != b.sec[i].N || a.sec[i].S2 != b.sec[i].S2 || a.sec[i].dim != b.sec[i].dim) return false;
+  return true;
+}
+
 hh_block* grow(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t) {
   need(t->side == side, "truncation side mismatch");
+  need(same_sectors(t->eb, side == 0 ? sb->plan.sbs : sb->plan.ebs),
+       "the truncation was made for another enlarged block than this superblock's");
   const hh_block* blk = side == 0 ? sb->sys : sb->env;
   GrowPlan gp = compile_grow(ctx->model, sb->g, blk->d, t->kept);
   auto* nb = new hh_block();
@@ -662,6 +674,14 @@ hh_block* grow(hh_ctx* ctx, hh_superblock* sb, int side, const hh_trunc* t) {
 
 void wave_trans(hh_ctx* ctx, int dir, hh_superblock* osb, const double2* psi_old, const hh_trunc* st,
                 const hh_trunc* et, hh_superblock* nsb, double2* psi_new) {
+  need(st->side == 0 && et->side == 1, "wave_trans needs a sys and an env truncation");
+  // LR: st truncates the old sys enlarged block, et built the old env block
+  // from the new env enlarged block; RL: st built the old sys block from the
+  // new sys enlarged block, et truncates the old env enlarged block
+  const EnlBasis& sb_basis = dir == 0 ? osb->plan.sbs : nsb->plan.sbs;
+  const EnlBasis& eb_basis = dir == 0 ? nsb->plan.ebs : osb->plan.ebs;
+  need(same_sectors(st->eb, sb_basis) && same_sectors(et->eb, eb_basis),
+       "wave_trans: a truncation does not belong to these superblocks");
   TransPlan tp = dir == 0 ? compile_wave_trans_lr(osb->plan.sb, osb->plan.sbs, osb->plan.ebs, st->kept, et->kept,
                                                   nsb->plan.sbs, nsb->plan.ebs, nsb->plan.sb)
                           : compile_wave_trans_rl(osb->plan.sb, osb->plan.sbs, osb->plan.ebs, st->kept, et->kept,
@@ -717,6 +737,16 @@ hh_block* site_block(hh_ctx* ctx, int side) {
             "h2d");
   check_hip(hipStreamSynchronize(ctx->stream), "sync");
   return b;
+}
+
+// A sweep schedule the driver can run (from the caller or from a checkpoint).
+void check_schedule(const hh_schedule& s) {
+  need(s.kept_max >= 1 && s.kept_max_warm >= 0 && s.n_sweeps >= 0, "bad schedule: kept counts / sweeps");
+  need(s.max_iter >= 2 && s.max_iter <= 1000, "bad schedule: max_iter must be in [2, 1000]");
+  need(std::isfinite(s.tol) && s.tol >= 0.0 && std::isfinite(s.warm_tol_factor) && s.warm_tol_factor > 0.0 &&
+           std::isfinite(s.first_tol_factor) && s.first_tol_factor > 0.0,
+       "bad schedule: tolerances");
+  need(s.precision == PREC_FP64, "only the FP64 form (precision 0) is available");
 }
 
 // ----------------------------------------------------------------------------
@@ -2914,8 +2944,8 @@ int hh_wave_trans(hh_ctx* ctx, int dir, hh_superblock* old_sb, const double* psi
 int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_rec, int* nrec) {
   return guard(ctx, [&] {
     need(ctx->model_set, "hh_set_model first");
-    need(s && s->kept_max >= 1 && s->max_iter >= 2, "bad schedule");
-    need(s->precision == PREC_FP64, "only the FP64 form (precision 0) is available in hh_dmrg_run");
+    need(s != nullptr, "null schedule");
+    check_schedule(*s);
     need(ctx->model.nsite() >= 4, "lattice too small for two-site DMRG");
     Driver d;
     d.ctx = ctx;
@@ -3078,6 +3108,7 @@ void load_driver(Driver& d, const char* path) {
   Driver fresh;   // build the whole state aside; the driver is replaced only on success
   fresh.ctx = ctx;
   fresh.S = in.get<hh_schedule>();
+  check_schedule(fresh.S);
   fresh.step = in.get<uint64_t>();
   fresh.psi_dim = in.get<int64_t>();
   in.device(ctx, fresh.psi, fresh.psi_dim > 0 ? fresh.psi_dim : 0);
@@ -3131,6 +3162,7 @@ void load_driver(Driver& d, const char* path) {
       truns[key] = std::move(tr);
     }
   }
+  need(std::fgetc(f.get()) == EOF, "checkpoint file is corrupt (data after the last record)");
   d.S = fresh.S;
   d.step = fresh.step;
   d.psi_dim = fresh.psi_dim;
@@ -3154,8 +3186,8 @@ extern "C" {
 int hh_driver_create(hh_ctx* ctx, const hh_schedule* s, hh_driver** out) {
   return guard(ctx, [&] {
     need(ctx->model_set, "hh_set_model first");
-    need(s && s->kept_max >= 1 && s->max_iter >= 2, "bad schedule");
-    need(s->precision == PREC_FP64, "only the FP64 form (precision 0) is available");
+    need(s != nullptr, "null schedule");
+    check_schedule(*s);
     auto* h = new hh_driver();
     h->d.ctx = ctx;
     h->d.S = *s;
@@ -3540,6 +3572,32 @@ int hh_plan_op_offset(const hh_plan* p, int which, int site_index, int kind, int
   if (site_index < 0 || site_index >= (int)d.sites.size() || kind < 0 || kind > 2) return 1;
   *off = d.op_off[site_index][kind];
   return *off < 0 ? 1 : 0;
+}
+
+int hh_plan_enlarged(const hh_plan* p, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim, int32_t* nsub,
+                     int32_t* subs) {
+  if (!p || p->is_grow || (side != 0 && side != 1) || !nsec) return 1;
+  const EnlBasis& e = side == 0 ? p->mv.sbs : p->mv.ebs;
+  *nsec = (int32_t)e.sec.size();
+  int32_t k = 0;
+  for (size_t a = 0; a < e.sec.size(); ++a) {
+    if (N) N[a] = e.sec[a].N;
+    if (S2) S2[a] = e.sec[a].S2;
+    if (dim) dim[a] = e.sec[a].dim;
+    for (const SubSector& q : e.subs[a]) {
+      if (subs) {
+        int32_t* r = subs + 5 * (size_t)k;
+        r[0] = (int32_t)a;
+        r[1] = q.b;
+        r[2] = q.s;
+        r[3] = q.pos;
+        r[4] = q.dim;
+      }
+      ++k;
+    }
+  }
+  if (nsub) *nsub = k;
+  return 0;
 }
 
 int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,

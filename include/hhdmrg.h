@@ -429,6 +429,13 @@ int hh_plan_block(const hh_plan* p, int which, int64_t* arena, int32_t* nsec, in
                   int64_t* h_off, int32_t* nsites, int32_t* sites);
 int hh_plan_layout(const hh_plan* p, int which, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off);
 int hh_plan_op_offset(const hh_plan* p, int which, int site_index, int kind, int64_t* off);
+/* Enlarged basis of side 0 (sys) / 1 (env) of a matvec plan, as Get_basis
+ * (code/update.f90:11-112) orders it: sectors (N, 2S, dim) in QuickSort_Integer
+ * order (2S descending, then N descending, code/update.f90:240-273), and per
+ * subsector {enlarged sector, block sector, site sector, spos, sdim} in
+ * block-major, site-minor order (subs: nsub x 5 int32; NULL to query). */
+int hh_plan_enlarged(const hh_plan* p, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim, int32_t* nsub,
+                     int32_t* subs);
 int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,
                   int32_t* cols, int64_t* off);
 

@@ -235,6 +235,13 @@ def test_wave_trans_matches_oracle_on_truncated_steps(ctx, kind, Lx, Ly, flux, N
         got = host(out)
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 1e-12, (direction, np.abs(got - ref).max())
+        # truncations of other blocks are refused before any kernel runs
+        from hofstadter_hubbard_tsc_amd import HHError
+        with pytest.raises(HHError, match="sys and an env truncation"):
+            ctx.wave_trans(direction, old_sb, psi, env_t, env_t, new_sb, out)
+        wrong = drv.trunc(0, 2)   # made from the enlarged one-site block
+        with pytest.raises(HHError, match="does not belong"):
+            ctx.wave_trans(direction, old_sb, psi, wrong, env_t, new_sb, out)
         # the prediction is not trivial: it carries most of the old state's weight
         assert abs(np.vdot(got, got).real - 1.0) < 1e-12
         checked += 1

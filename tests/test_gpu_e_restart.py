@@ -77,6 +77,10 @@ def test_checkpoint_of_another_model_or_damaged_is_refused(ctx, tmp_path):
     open(bad, "wb").write(b"NOTACKPT" + data[8:])
     with pytest.raises(HHError, match="not a checkpoint"):
         c.load(bad)
+    longer = str(tmp_path / "long.ck")
+    open(longer, "wb").write(data + b"\0")
+    with pytest.raises(HHError, match="after the last record"):
+        c.load(longer)
     # a refused load leaves the driver usable: load the good file and sweep
     c.load(path)
     recs = c.sweep(0, 1, m.nsite - 2, 1e-9, TRUN, 16, phase=2)
