@@ -6,16 +6,28 @@ doping 1/8), S = 0, flux 1/8 per plaquette in the Landau gauge, at the top of
 the 500 -> 2000 multiplet ramp: m = 2000 kept SU(2) multiplets.
 
 Untimed preparation: the infinite-system warm-up to two 64-site blocks at
-m = 2000 (warm_up_point) and `--warmup` left-to-right steps from the centre.
-Timed: `--steps` further left-to-right finite-sweep steps (wave
-transformation, Lanczos with FP64 H.psi, density matrix, eigendecomposition,
-truncation, growth + renormalisation), all on the GPU with the blocks resident
-in HBM.  One step = one DMRG step; a full sweep is 2*(Nsite-3) steps.
+m = 2000 (warm_up_point) and `--warmup` left-to-right steps from the centre,
+solved by the chosen form's Davidson with the fixed diagonal preconditioner
+while training triples of the learned preconditioner are recorded; then the
+preconditioner network is trained on the GPU.
+Timed, from identical device copies of that state: `--steps` further
+left-to-right finite-sweep steps (or with --sweep one whole sweep) of
+  * the plain FP64 form (Lanczos with FP64 H.psi; the headline), then
+  * the chosen form (Davidson with FP32-storage H.psi, FP64 restarts, the
+    learned per-sector preconditioner, fallback to mixed Lanczos), and
+  * the chosen form without the network (fixed diagonal preconditioner).
+Each step is wave transformation, ground-state solve, density matrix,
+eigendecomposition, truncation, growth + renormalisation, all on the GPU with
+the blocks resident in HBM.  One step = one DMRG step; a full sweep is
+2*(Nsite-3) steps.
 
-Prints one JSON line.  value = H.psi GFLOP/s (reference-order algorithmic
-flops / H.psi wall time) over the timed steps; ms_per_step is the DMRG step
-wall time.  roofline is the dominant kernel (H.psi stage 2, FP64 MFMA);
-cpu_baseline times the numpy oracle's H.psi on the same superblock.
+Prints one JSON line.  value = the plain form's H.psi GFLOP/s
+(reference-order algorithmic flops / H.psi wall time) over its timed steps;
+ms_per_step is its DMRG step wall time; config.forms holds the chosen forms'
+step times, speedups, H.psi counts and energies relative to the plain form.
+roofline is the dominant kernel (H.psi stage 2, FP64 MFMA); cpu_baseline
+times the numpy oracle's H.psi on the last step's superblock and, as the
+checker, compares it with the GPU's.
 """
 import argparse
 import json
@@ -28,8 +40,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 TRUN_ERR = 1.0000000180025095e-18
+
+
+def run_leg(d, start, args, n):
+    """The timed steps of one form: --steps left-to-right steps from `start`,
+    or with --sweep one whole sweep from `start` (to the right end, back to
+    the left end, and right again to `start`: 2 (Nsite - 3) steps)."""
+    if not args.sweep:
+        return d.sweep(0, start, start + args.steps, args.tol, TRUN_ERR, args.m, fresh=(args.warmup == 0), phase=2)
+    recs = d.sweep(0, start, n - 2, args.tol, TRUN_ERR, args.m, fresh=(args.warmup == 0), phase=2)
+    d.reset_transform()
+    recs += d.sweep(1, n - 2, 1, args.tol, TRUN_ERR, args.m, phase=2)
+    d.reset_transform()
+    recs += d.sweep(0, 1, start, args.tol, TRUN_ERR, args.m, phase=2)
+    return recs
 
 
 def workload(name):
@@ -322,11 +349,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--m", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chosen", action="store_true", help="time the plain FP64 form only")
+    ap.add_argument("--sweep", action="store_true",
+                    help="c2: time one whole sweep per form (2 (Nsite - 3) steps) instead of --steps steps")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c2",
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
@@ -377,18 +407,43 @@ def main():
     print("# warm-up: %d steps, %.1f s, %d Lanczos vectors, E0 = %.10f"
           % (len(warm), time.time() - t_prep, sum(r.iters for r in warm), warm[-1].energy), file=sys.stderr, flush=True)
     bg = n // 2 - 1
-    kmin = min(3 * args.m // 4, 3600)
+    chosen = not args.no_chosen
+    train = None
     if args.warmup > 0:
+        if chosen:
+            # the untimed steps run the chosen form's Davidson with the fixed
+            # diagonal preconditioner and record training triples at its FP64
+            # restarts
+            drv.capture(16)
+            drv.set_solver(2)
+            drv.set_precond(1)
         w = drv.sweep(0, bg, bg + args.warmup, args.tol, TRUN_ERR, args.m, fresh=True, phase=1)
+        drv.set_solver(0)
+        drv.set_precond(0)
         for r in w:
             print("# untimed step sys_len=%d dim=%d iters=%d E=%.10f %.2fs" % (r.sys_len, r.dim, r.iters,
                   r.energy, r.seconds), file=sys.stderr, flush=True)
+        if chosen:
+            t1 = time.time()
+            losses, ntrip = drv.train_precond(epochs=100, lr=1e-3)
+            train = {"triples": ntrip, "epochs": 100, "seconds": round(time.time() - t1, 3),
+                     "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])}
+            print("# preconditioner trained on %d triples in %.2f s: loss %.4f -> %.4f"
+                  % (ntrip, train["seconds"], losses[0], losses[-1]), file=sys.stderr, flush=True)
+    forks = {}
+    if chosen:
+        # identical starting states for the other forms (device copies)
+        for name, mode in (("chosen", 0), ("chosen_no_network", 1)):
+            f = drv.fork()
+            f.set_solver(2)
+            f.set_precond(mode)
+            forks[name] = f
     start = bg + args.warmup
     ctx.reset_stats()
     ctx.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.time()
-    recs = drv.sweep(0, start, start + args.steps, args.tol, TRUN_ERR, args.m, fresh=(args.warmup == 0), phase=2)
+    recs = run_leg(drv, start, args, n)
     torch.cuda.synchronize()
     wall = time.time() - t0
     ctx.set_profiling(False)
@@ -445,6 +500,40 @@ def main():
     measure = None
     if args.measure and recs:
         measure = measure_at_scale(ctx, drv, model, recs[-1].sys_len + 1)
+    forms = {}
+    for name, f in forks.items():
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        rc = run_leg(f, start, args, n)
+        torch.cuda.synchronize()
+        wall_f = time.time() - t1
+        ctx.set_profiling(False)
+        sf = ctx.stats()
+        for r in rc:
+            print("# %s step sys_len=%d dim=%d H.psi fp64 %d fp32 %d%s E=%.10f res=%.2e %.3fs (H.psi %.3fs)"
+                  % (name, r.sys_len, r.dim, r.hpsi_fp64, r.hpsi_fp32, " fallback" if r.fallback else "", r.energy,
+                     r.residual, r.seconds, r.matvec_seconds), file=sys.stderr, flush=True)
+        s2f = sf["stage2_flops_alg_fp32"] / max(sf["stage2_ms_fp32"], 1e-30) / 1e9
+        forms[name] = {
+            "sweep_seconds": round(wall_f, 2) if args.sweep else None,
+            "ms_per_step": round(1e3 * wall_f / max(len(rc), 1), 2),
+            "speedup_vs_plain": round(ms_per_step / max(1e3 * wall_f / max(len(rc), 1), 1e-9), 3),
+            "hpsi_fp64": [r.hpsi_fp64 for r in rc], "hpsi_fp32": [r.hpsi_fp32 for r in rc],
+            "fallbacks": sum(1 for r in rc if r.fallback),
+            "steps_not_converged": sum(1 for r in rc if not r.converged),
+            "energy_minus_plain": [r.energy - q.energy for r, q in zip(rc, recs)],
+            "max_residual": max(r.residual for r in rc),
+            "matvec_share": round(sum(r.matvec_seconds for r in rc) / max(wall_f, 1e-30), 3),
+            "fp32_stage2": {"bound": "mfma", "achieved": round(s2f, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": round(s2f / FP32_MFMA_PEAK_TFLOPS, 4),
+                            "avg_launch_ms": round(sf["stage2_ms_fp32"] / max(sf["matvecs_fp32"], 1), 4),
+                            "launches": int(sf["matvecs_fp32"]),
+                            "kernel": "hh::k_tiles<float, 4, hh::BufPtrs32, *> (FP32-storage H.psi stage 2)"},
+        }
+        del f
+    forks.clear()
     cpu, check = None, None
     if not args.no_cpu:
         try:
@@ -468,6 +557,7 @@ def main():
                                "m=%d SU(2) multiplets, plain FP64 form, finite-sweep steps from the centre" % args.m,
                    "m": args.m, "lanczos_tol": args.tol, "form": "plain FP64",
                    "sweep_seconds_est": round(ms_per_step * 2 * (n - 3) / 1e3, 1),
+                   "sweep_seconds": round(wall, 2) if args.sweep else None,
                    "matvec_share": round(mv_time / max(wall, 1e-30), 3),
                    "superblock_dim": int(recs[-1].dim) if recs else 0,
                    "energy": recs[-1].energy if recs else None,
@@ -475,7 +565,13 @@ def main():
                    "steps_at_iteration_cap": sum(1 for r in recs if not r.converged),
                    "max_final_ritz_change": max((r.ritz_change for r in recs), default=None),
                    "max_residual": max((r.residual for r in recs), default=None),
-                   "oracle_check": check},
+                   "oracle_check": check,
+                   "chosen_form": ("Davidson with FP32-storage H.psi (FP32 MFMA), FP64 H.psi restarts (exact Rayleigh "
+                                   "quotient and residual), the learned per-sector preconditioner with fallback to "
+                                   "mixed-precision Lanczos; same steps from a device copy of the same state"
+                                   if forms else None),
+                   "precond_training": train,
+                   "forms": forms},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -24,7 +24,7 @@ class ModelParams(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("kept_max", i32), ("kept_max_warm", i32), ("n_sweeps", i32), ("max_iter", i32),
                 ("tol", f64), ("warm_tol_factor", f64), ("first_tol_factor", f64), ("seed", u64),
-                ("precision", i32), ("reserved", i32)]
+                ("precision", i32), ("solver", i32)]
 
 
 class StepRecord(C.Structure):
@@ -33,14 +33,15 @@ class StepRecord(C.Structure):
                 ("seconds", f64), ("matvec_seconds", f64), ("matvec_flops", f64),
                 ("bind_seconds", f64), ("wave_seconds", f64), ("lanczos_seconds", f64), ("trunc_seconds", f64),
                 ("grow_seconds", f64), ("ritz_change", f64), ("residual", f64), ("hpsi_fp64", i32),
-                ("hpsi_fp32", i32)]
+                ("hpsi_fp32", i32), ("solver", i32), ("fallback", i32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("matvecs", i64), ("matvec_ms", f64), ("stage1_ms", f64), ("stage2_ms", f64),
                 ("flops_alg", f64), ("flops_exec", f64), ("bytes_alg", f64), ("stage1_flops_alg", f64),
                 ("stage2_flops_alg", f64), ("reduce_ms", f64), ("stage1_flops_exec", f64),
-                ("stage2_flops_exec", f64)]
+                ("stage2_flops_exec", f64), ("matvecs_fp32", i64), ("stage2_ms_fp32", f64),
+                ("stage2_flops_alg_fp32", f64)]
 
 
 # numpy views of the engine's task structs (hh_core.hpp: Term, Tile)
@@ -96,6 +97,14 @@ SIGNATURES = [
     ("hh_driver_sweep", i32, [vp, i32, i32, i32, f64, f64, i32, i32, i32, P(StepRecord), i32, P(i32)]),
     ("hh_driver_reset_transform", i32, [vp]),
     ("hh_driver_wave", i32, [vp, P(i32), P(i32), P(i64), vp]),
+    ("hh_driver_set_solver", i32, [vp, i32]),
+    ("hh_driver_fork", i32, [vp, P(vp)]),
+    ("hh_driver_capture", i32, [vp, i32]),
+    ("hh_driver_train_precond", i32, [vp, i32, f64, P(f64), P(i32)]),
+    ("hh_driver_set_precond", i32, [vp, i32]),
+    ("hh_precond_eval", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), P(f64), P(f64), P(f64), f64,
+                              P(C.c_float), P(C.c_float), P(C.c_float), P(f64), P(f64), P(C.c_float)]),
+    ("hh_precond_size", i32, []),
     ("hh_driver_block_count", i32, [vp, i32, P(i32)]),
     ("hh_driver_block", i32, [vp, i32, i32, P(vp)]),
     ("hh_set_profiling", i32, [vp, i32]),

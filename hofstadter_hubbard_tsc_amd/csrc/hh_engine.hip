@@ -288,8 +288,19 @@ template <> struct Acc<float> {
   static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
-__device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t idx, bool conj) {
-  double2 v = p[idx];
+// operand storage of a launch: complex128 buffers (BufPtrs, BufTable) or
+// complex64 buffers (BufPtrs32, the FP32-storage form)
+template <typename Bufs> struct StoreOf { typedef double2 type; };
+template <> struct StoreOf<BufPtrs32> { typedef float2 type; };
+
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ void put(double2* p, double re, double im) { *p = make_double2(re, im); }
+__device__ __forceinline__ void put(float2* p, double re, double im) { *p = make_float2((float)re, (float)im); }
+
+template <typename S>
+__device__ __forceinline__ double2 ld_op(const S* __restrict__ p, int64_t idx, bool conj) {
+  double2 v = to_d2(p[idx]);
   if (conj) v.y = -v.y;
   return v;
 }
@@ -308,6 +319,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
                                                const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
   typedef typename Acc<T>::V V;
+  typedef typename StoreOf<Bufs>::type S;   // operand element (complex128 or complex64)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int wg = blockIdx.x;
   {
@@ -342,7 +354,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     for (int ti = w.t0; ti < w.t1; ++ti) {
       const Term tm = terms[ti];
       if (tm.K != 0) continue;
-      const double2* __restrict__ A = bufs.p[tm.a_buf];
+      const S* __restrict__ A = bufs.p[tm.a_buf];
       const bool cj = tm.a_op == OP_R || tm.a_op == OP_C;
 #pragma unroll
       for (int a = 0; a < WA; ++a)
@@ -359,11 +371,11 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
           }
     }
   };
-  // FP64 complex products by the 3M scheme: with P1 = sum ar br, P2 = sum ai
-  // bi and P3 = sum (ar + ai)(br + bi), Re = P1 - P2 and Im = P3 - P1 - P2.
+  // Complex products by the 3M scheme: with P1 = sum ar br, P2 = sum ai bi
+  // and P3 = sum (ar + ai)(br + bi), Re = P1 - P2 and Im = P3 - P1 - P2.
   // During the GEMM terms accR holds P1, accP holds P2 and accI holds P3, all
-  // starting from zero.  FP32 keeps the four-product form.
-  constexpr bool M3 = std::is_same<T, double>::value && CPLX3;
+  // starting from zero.
+  constexpr bool M3 = CPLX3;
   V accP[M3 ? WA : 1][M3 ? WB : 1];
   if constexpr (M3) {
 #pragma unroll
@@ -381,8 +393,8 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   // past the region edge read the last valid row / column (never stored), k
   // past K reads k = K-1 and is zeroed in the A operand at use.
   struct OpAddr {
-    const double2* Ab;
-    const double2* Bb;
+    const S* Ab;
+    const S* Bb;
     uint32_t pa[WA], pb[WB];   // k-independent part of each lane's element offset
     int a_ks, b_ks, K;
     double sa, sb, cr, ci;
@@ -410,7 +422,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   // NA x NB of the WA x WB sub-tiles hold data (the kernels of the narrow
   // shape classes run work items with Mv <= 16 and / or Nv <= 16: fewer loads
   // and MFMAs, and one pipeline per kernel)
-  auto load = [&](double2 (&xa)[WA], double2 (&xb)[WB], const OpAddr& o, int k0) {
+  auto load = [&](S (&xa)[WA], S (&xb)[WB], const OpAddr& o, int k0) {
     const uint32_t kc = (uint32_t)min(k0 + lk, o.K - 1);
 #pragma unroll
     for (int a = 0; a < NA; ++a) xa[a] = o.Ab[o.pa[a] + kc * (uint32_t)o.a_ks];
@@ -427,24 +439,25 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     OpAddr c = make_op(terms[ti]);
     int tn = next_gemm(ti + 1);
     Term dn = terms[tn < w.t1 ? tn : ti];   // descriptor of the following GEMM term
-    double2 xa0[WA], xb0[WB], xa1[WA], xb1[WB];
+    S xa0[WA], xb0[WB], xa1[WA], xb1[WB];
     load(xa0, xb0, c, 0);
     int kb = 0;
     bool done = false;
-    auto step = [&](const double2 (&xa)[WA], const double2 (&xb)[WB], double2 (&ya)[WA], double2 (&yb)[WB]) {
+    auto step = [&](const S (&xa)[WA], const S (&xb)[WB], S (&ya)[WA], S (&yb)[WB]) {
       // only the last k-step of a term can hold k >= K
-      const double kf = (kb + 4 <= c.K || kb + lk < c.K) ? 1.0 : 0.0;
+      const T kf = (kb + 4 <= c.K || kb + lk < c.K) ? T(1) : T(0);
+      const T cr = (T)c.cr, ci = (T)c.ci, sa = (T)c.sa, sb = (T)c.sb;
       T ar[WA], ai[WA], br[WB], bi[WB];
 #pragma unroll
       for (int a = 0; a < NA; ++a) {
-        const double xr = kf * xa[a].x, xi = kf * c.sa * xa[a].y;
-        ar[a] = (T)(c.cr * xr - c.ci * xi);
-        ai[a] = (T)(c.cr * xi + c.ci * xr);
+        const T xr = kf * (T)xa[a].x, xi = kf * sa * (T)xa[a].y;
+        ar[a] = cr * xr - ci * xi;
+        ai[a] = cr * xi + ci * xr;
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         br[b] = (T)xb[b].x;
-        bi[b] = (T)(c.sb * xb[b].y);
+        bi[b] = sb * (T)xb[b].y;
       }
       const bool last = kb + 4 >= c.K;
       // both arms issue the same loads, so the wait counts stay path-independent
@@ -452,18 +465,18 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       else load(ya, yb, c, kb + 4);
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (M3) {
-        T sa[WA], sb[WB];
+        T pa[WA], pb[WB];
 #pragma unroll
-        for (int a = 0; a < NA; ++a) sa[a] = ar[a] + ai[a];
+        for (int a = 0; a < NA; ++a) pa[a] = ar[a] + ai[a];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) sb[b] = br[b] + bi[b];
+        for (int b = 0; b < NB; ++b) pb[b] = br[b] + bi[b];
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             accR[a][b] = Acc<T>::mfma(ar[a], br[b], accR[a][b]);
             accP[a][b] = Acc<T>::mfma(ai[a], bi[b], accP[a][b]);
-            accI[a][b] = Acc<T>::mfma(sa[a], sb[b], accI[a][b]);
+            accI[a][b] = Acc<T>::mfma(pa[a], pb[b], accI[a][b]);
           }
       } else {
 #pragma unroll
@@ -521,7 +534,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     elementwise();
   }
   // write the macro tile once (or its partial sum)
-  double2* __restrict__ C = bufs.p[tile.c_buf];
+  S* __restrict__ C = bufs.p[tile.c_buf];
 #pragma unroll
   for (int a = 0; a < WA; ++a)
 #pragma unroll
@@ -536,11 +549,11 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
           } else {
             const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
             if (tile.accumulate) {
-              double2 old = C[o];
+              const double2 old = to_d2(C[o]);
               re += old.x;
               im += old.y;
             }
-            C[o] = make_double2(re, im);
+            put(C + o, re, im);
           }
         }
       }
@@ -553,7 +566,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   const ReduceItem ri = red[blockIdx.x];
   const Tile tile = tiles[ri.tile];
   const int Mv = min(MTR, tile.M - ri.r0), Nv = min(MTC, tile.N - ri.c0);
-  double2* __restrict__ C = bufs.p[tile.c_buf];
+  typename StoreOf<Bufs>::type* __restrict__ C = bufs.p[tile.c_buf];
   for (int e = threadIdx.x; e < MTR * MTC; e += 256) {
     const int r = e % MTR, c = e / MTR;
     if (r >= Mv || c >= Nv) continue;
@@ -565,11 +578,11 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
     }
     const int64_t o = tile.c_off + (int64_t)(ri.r0 + r) + (int64_t)(ri.c0 + c) * tile.c_ld;
     if (tile.accumulate) {
-      const double2 old = C[o];
+      const double2 old = to_d2(C[o]);
       re += old.x;
       im += old.y;
     }
-    C[o] = make_double2(re, im);
+    put(C + o, re, im);
   }
 }
 
@@ -590,7 +603,7 @@ static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, 
 }
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
-  constexpr bool m3 = std::is_same<T, double>::value && HH_CPLX3 != 0;
+  constexpr bool m3 = HH_CPLX3 != 0;
   launch_ts<T, Bufs, 2, 2, m3>(t, b, s, tag, 0);
   launch_ts<T, Bufs, 1, 2, m3>(t, b, s, tag, 1);
   launch_ts<T, Bufs, 2, 1, m3>(t, b, s, tag, 2);
@@ -600,8 +613,13 @@ static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
 template <typename Bufs>
 static void launch_any(const DevTasks& t, const Bufs& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
   if (t.nwork > 0) {
-    if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
-    else launch_t<double>(t, b, s, tag);
+    // complex64 storage always computes in FP32
+    if constexpr (std::is_same<Bufs, BufPtrs32>::value) {
+      launch_t<float>(t, b, s, tag);
+    } else {
+      if (precision == PREC_FP32) launch_t<float>(t, b, s, tag);
+      else launch_t<double>(t, b, s, tag);
+    }
     check_hip(hipGetLastError(), "k_tiles launch");
   }
   if (mid) check_hip(hipEventRecord(mid, s), "event");
@@ -617,6 +635,9 @@ void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precis
 }
 void launch_tiles(const DevTasks& t, const BufTable& b, hipStream_t s, int precision, int tag, hipEvent_t mid) {
   launch_any(t, b, s, precision, tag, mid);
+}
+void launch_tiles(const DevTasks& t, const BufPtrs32& b, hipStream_t s, int tag, hipEvent_t mid) {
+  launch_any(t, b, s, PREC_FP32, tag, mid);
 }
 
 // ----------------------------------------------------------------------------
@@ -878,6 +899,41 @@ __global__ __launch_bounds__(VB) void k_combine_norm(double2* __restrict__ w, do
   block_norm(acc, partial);
 }
 
+// w = sum_j (cv_j V_j + cw_j W_j) (coefficients on the device); partial |w|^2
+__global__ __launch_bounds__(VB) void k_combine2_norm(double2* __restrict__ w, const double2* __restrict__ V,
+                                                      const double2* __restrict__ W, int64_t stride, int nv,
+                                                      const double* __restrict__ cv, const double* __restrict__ cw,
+                                                      int64_t n, double* __restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  double2 x[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) x[q] = make_double2(0.0, 0.0);
+  for (int j = 0; j < nv; ++j) {
+    const double a = cv[j], b = cw[j];
+    const double2* __restrict__ v = V + (int64_t)j * stride;
+    const double2* __restrict__ u = W + (int64_t)j * stride;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int64_t i = base + (int64_t)VB * q;
+      if (i < n) {
+        const double2 y = v[i], z = u[i];
+        x[q].x += a * y.x + b * z.x;
+        x[q].y += a * y.y + b * z.y;
+      }
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      w[i] = x[q];
+      acc += x[q].x * x[q].x + x[q].y * x[q].y;
+    }
+  }
+  block_norm(acc, partial);
+}
+
 __global__ __launch_bounds__(VB) void k_scale_copy(double2* __restrict__ d, const double2* __restrict__ src, double a,
                                                    int64_t n) {
   const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
@@ -940,6 +996,12 @@ void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, i
   hipLaunchKernelGGL(k_combine_norm, dim3(nb), dim3(VB), 0, s, w, sw, V, stride, nv, c_dev, cs, n, partial_dev);
   hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(VB), 0, s, partial_dev, nb, out_dev);
 }
+void dev_combine2_norm(double2* w, const double2* V, const double2* W, int64_t stride, int nv, const double* cv_dev,
+                       const double* cw_dev, int64_t n, double* out_dev, double* partial_dev, hipStream_t s) {
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_combine2_norm, dim3(nb), dim3(VB), 0, s, w, V, W, stride, nv, cv_dev, cw_dev, n, partial_dev);
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(VB), 0, s, partial_dev, nb, out_dev);
+}
 void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, a, n);
@@ -961,6 +1023,27 @@ void dev_cscale_copy(double2* d, const double2* src, double ar, double ai, int64
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cscale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, ar, ai, n);
 }
+__global__ __launch_bounds__(VB) void k_to_f32(float2* __restrict__ d, const double2* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 y = src[i];
+    d[i] = make_float2((float)y.x, (float)y.y);
+  }
+}
+__global__ __launch_bounds__(VB) void k_to_f64(double2* __restrict__ d, const float2* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 y = src[i];
+    d[i] = make_double2((double)y.x, (double)y.y);
+  }
+}
+void dev_to_f32(float2* d, const double2* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(VB), 0, s, d, src, n);
+}
+void dev_to_f64(double2* d, const float2* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_f64, dim3(grid_for(n)), dim3(VB), 0, s, d, src, n);
+}
+
 void dev_start_vector(double2* p, int64_t n, uint64_t seed, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_start, dim3(grid_for(n)), dim3(VB), 0, s, p, n, (seed & 0xFFFFFFFFull) << 32);

@@ -16,6 +16,9 @@ struct BufPtrs { double2* p[8]; };
 // Buffer pointers held in device memory (up to 256 slots): lets one launch
 // address the buffers of many superblocks.
 struct BufTable { double2* const* p; };
+// complex64 buffer slots: the FP32-storage form of a task list (operands,
+// intermediates and outputs all single precision, FP32 MFMA)
+struct BufPtrs32 { float2* p[8]; };
 // One wave's share of the work: a 32x32 macro tile of output region `tile`
 // and the term range [t0, t1); slot >= 0 sends the partial sum to the
 // partial buffer (tiles whose term list is split across several waves).
@@ -64,7 +67,10 @@ void* noalloc_scratch_init(int device);
 // The address found by noalloc_scratch_init for `device` (nullptr before it).
 void* noalloc_scratch(int device);
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
+// Arithmetic of an H.psi: FP64 MFMA on complex128 operands (the plain form),
+// FP32 MFMA on complex128 operands, or FP32 MFMA on complex64 copies of the
+// operands, intermediates and vectors (FP32 storage).
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_STORE = 2 };
 
 // C tiles = sum of terms; one wave per 32x32 macro tile and term range.
 // tag names the launch site so profiles separate them: 0 generic, 1 H.psi
@@ -75,6 +81,10 @@ void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precis
                   hipEvent_t mid = nullptr);
 void launch_tiles(const DevTasks& t, const BufTable& b, hipStream_t s, int precision = PREC_FP64, int tag = 0,
                   hipEvent_t mid = nullptr);
+void launch_tiles(const DevTasks& t, const BufPtrs32& b, hipStream_t s, int tag = 0, hipEvent_t mid = nullptr);
+// complex128 <-> complex64 copies of n elements
+void dev_to_f32(float2* d, const double2* src, int64_t n, hipStream_t s);
+void dev_to_f64(double2* d, const float2* src, int64_t n, hipStream_t s);
 
 // vector kernels (complex128 vectors of length n)
 void dev_zero(double2* p, int64_t n, hipStream_t s);
@@ -102,6 +112,10 @@ void dev_lanczos_update_norm(double2* w, const double2* v1, double a, const doub
 // w = sw*w + cs * sum_j c_j V_j (c on the device); out[0] = |w|^2
 void dev_combine_norm(double2* w, double sw, const double2* V, int64_t stride, int nv, const double* c_dev, double cs,
                       int64_t n, double* out_dev, double* partial_dev, hipStream_t s);
+// w = sum_j (cv_j V_j + cw_j W_j) over two strided vector sets (c on the
+// device); out[0] = |w|^2
+void dev_combine2_norm(double2* w, const double2* V, const double2* W, int64_t stride, int nv, const double* cv_dev,
+                       const double* cw_dev, int64_t n, double* out_dev, double* partial_dev, hipStream_t s);
 // d = a * src
 void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s);
 // d = (ar + i ai) src

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <limits>
 #include <map>
@@ -29,52 +30,12 @@
 #include "../../include/hhdmrg.h"
 #include "hh_core.hpp"
 #include "hh_engine.hpp"
+#include "hh_runtime.hpp"
+#include "hh_solver.hpp"
 
 using namespace hh;
 
 namespace {
-
-struct HHError : std::runtime_error {
-  int code;
-  HHError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-inline void need(bool ok, const std::string& msg) {
-  if (!ok) throw HHError(1, msg);
-}
-
-// device allocation owned by one object
-struct DevMem {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevMem() = default;
-  DevMem(const DevMem&) = delete;
-  DevMem& operator=(const DevMem&) = delete;
-  ~DevMem() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void alloc(size_t b) {
-    if (b <= bytes && p) return;
-    reset();
-    if (b == 0) b = 16;
-    check_hip(hipMalloc(&p, b), "hipMalloc");
-    bytes = b;
-  }
-  double2* c() const { return (double2*)p; }
-};
-
-// HHDMRG_VERBOSE=1: progress lines on stderr (sweep steps, truncation phases,
-// measurement chains)
-bool verbose() {
-  static const bool v = std::getenv("HHDMRG_VERBOSE") && std::getenv("HHDMRG_VERBOSE")[0] == '1';
-  return v;
-}
-
-double wall() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 // ----------------------------------------------------------------------------
 // symmetric tridiagonal eigenproblem (DSTEV stand-in): implicit-shift QL
@@ -150,72 +111,6 @@ void tridiag_ql(std::vector<double>& d, std::vector<double> e, std::vector<doubl
 
 }  // namespace
 
-// ============================================================================
-// objects
-// ============================================================================
-struct hh_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  rocblas_handle rb = nullptr;
-  // side streams (default priority) for independent per-sector eigensolver
-  // calls, each with its own rocBLAS handle; joined back with events
-  static constexpr int NSIDE = 7;
-  hipStream_t side[NSIDE] = {};
-  rocblas_handle rbs[NSIDE] = {};
-  hipEvent_t fork = nullptr, join[NSIDE] = {};
-  ModelParams model;
-  bool model_set = false;
-  int precision = PREC_FP64;
-  std::string err;
-  // Lanczos workspace
-  DevMem V, dots, partial, scratch_w;
-  // profiling
-  bool profile = false;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  hh_stats stats{};
-  std::vector<DevMem*> user_allocs;
-  // outcome of the last ground-state solve (hh_lanczos_status)
-  int last_converged = 0;
-  double last_ritz_change = 0.0, last_residual = 0.0;
-};
-
-struct hh_block {
-  BlockDesc d;
-  DevMem arena;
-};
-
-struct hh_superblock {
-  const hh_block* sys = nullptr;
-  const hh_block* env = nullptr;
-  int N = 0;
-  Geometry g;
-  MatvecPlan plan;
-  DevTasks merge, st1, st2;
-  DevMem merged, X;
-};
-
-// Several superblocks' H.psi task lists merged into one list per stage; the
-// buffer slots of member i are 8*i + (B_IN .. B_AUX2) in a device table.
-struct hh_batch {
-  std::vector<hh_superblock*> sbs;
-  DevTasks st1, st2;
-  DevMem table;
-  std::vector<double2*> host_table;
-  double flops_alg = 0.0, stage1_flops = 0.0, stage2_flops = 0.0, flops_exec = 0.0;
-  double stage1_exec = 0.0, stage2_exec = 0.0;
-  int64_t total_dim = 0;
-};
-
-struct hh_trunc {
-  int side = 0;
-  EnlBasis eb;
-  std::vector<KeptSector> kept;
-  std::vector<int> kept_per_rho;   // per rho block (super-basis order)
-  DevMem evec;
-  int num_state = 0;
-  double err = 0.0, entropy = 0.0;
-};
-
 namespace {
 
 BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* env, double2* X, double2* mrg,
@@ -259,7 +154,74 @@ hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>
   return b;
 }
 
+}  // namespace
+
+namespace hh {
+
+void ensure32(hh_ctx* ctx, const hh_block* b) {
+  if (b->have32) return;
+  b->arena32.alloc(sizeof(float2) * (size_t)std::max<int64_t>(b->d.size, 1));
+  dev_to_f32(b->arena32.f(), b->arena.c(), b->d.size, ctx->stream);
+  b->have32 = true;
+}
+
+// H.psi in the FP32-storage form: complex64 copies of the blocks, merged
+// operators, stage-1 intermediate and vectors; FP32 MFMA, FP32 accumulation.
+// in and out stay complex128 (converted on the way in and out).
+void matvec32(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
+  const int64_t n = sb->plan.sb.dim;
+  ensure32(ctx, sb->sys);
+  ensure32(ctx, sb->env);
+  if (!sb->have32) {
+    sb->merged32.alloc(sizeof(float2) * (size_t)std::max<int64_t>(sb->plan.merged_size, 1));
+    sb->X32.alloc(sizeof(float2) * (size_t)std::max<int64_t>(sb->plan.x_size, 1));
+    dev_to_f32(sb->merged32.f(), sb->merged.c(), sb->plan.merged_size, ctx->stream);
+    sb->have32 = true;
+  }
+  ctx->in32.alloc(sizeof(float2) * (size_t)std::max<int64_t>(n, 1));
+  ctx->out32.alloc(sizeof(float2) * (size_t)std::max<int64_t>(n, 1));
+  BufPtrs32 b{};
+  b.p[B_IN] = ctx->in32.f();
+  b.p[B_OUT] = ctx->out32.f();
+  b.p[B_SYS] = sb->sys->arena32.f();
+  b.p[B_ENV] = sb->env->arena32.f();
+  b.p[B_X] = sb->X32.f();
+  b.p[B_MRG] = sb->merged32.f();
+  const double t0 = ctx->profile ? wall() : 0.0;
+  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
+  dev_to_f32(ctx->in32.f(), in, n, ctx->stream);
+  launch_tiles(sb->st1, b, ctx->stream, ctx->profile ? 3 : 1, ctx->profile ? ctx->ev[1] : nullptr);
+  if (ctx->profile) check_hip(hipEventRecord(ctx->ev[2], ctx->stream), "event");
+  launch_tiles(sb->st2, b, ctx->stream, ctx->profile ? 4 : 2, ctx->profile ? ctx->ev[3] : nullptr);
+  dev_to_f64(out, ctx->out32.f(), n, ctx->stream);
+  if (ctx->profile) {
+    check_hip(hipEventRecord(ctx->ev[4], ctx->stream), "event");
+    check_hip(hipEventSynchronize(ctx->ev[4]), "event sync");
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; ++i) check_hip(hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]), "elapsed");
+    ctx->stats.stage1_ms += ms[0];
+    ctx->stats.reduce_ms += ms[1] + ms[3];
+    ctx->stats.stage2_ms += ms[2];
+    ctx->stats.stage2_ms_fp32 += ms[2];
+    ctx->stats.matvec_ms += (wall() - t0) * 1e3;
+  }
+  ctx->stats.matvecs += 1;
+  ctx->stats.matvecs_fp32 += 1;
+  ctx->stats.stage2_flops_alg_fp32 += sb->plan.stage2.flops_alg;
+  ctx->stats.flops_alg += sb->plan.stage1.flops_alg + sb->plan.stage2.flops_alg;
+  ctx->stats.stage1_flops_alg += sb->plan.stage1.flops_alg;
+  ctx->stats.stage2_flops_alg += sb->plan.stage2.flops_alg;
+  ctx->stats.flops_exec += sb->plan.stage1.flops_exec + sb->plan.stage2.flops_exec;
+  ctx->stats.stage1_flops_exec += sb->plan.stage1.flops_exec;
+  ctx->stats.stage2_flops_exec += sb->plan.stage2.flops_exec;
+  ctx->stats.bytes_alg += 0.5 * sb->plan.bytes_alg;
+}
+
 void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out) {
+  if (ctx->precision == PREC_FP32_STORE) {
+    matvec32(ctx, sb, in, out);
+    return;
+  }
   BufPtrs b = bufs_of((double2*)in, out, sb->sys, sb->env, sb->X.c(), sb->merged.c(), nullptr, nullptr);
   if (!ctx->profile) {
     launch_tiles(sb->st1, b, ctx->stream, ctx->precision, 1);
@@ -303,24 +265,9 @@ void ensure_ws(hh_ctx* ctx, int64_t n, int nvec) {
   ctx->partial.alloc(sizeof(double) * (size_t)std::max(rdot_partials(), vec_blocks(n)) * (nvec + 2));
 }
 
-// Outcome of one ground-state solve.  converged: the reference's stopping
-// rule held (|change of the lowest Ritz value| <= lanzero, or beta <=
-// lanzero, code/lanczos.f90:77-80, :102) before the vector budget ran out;
-// ritz_change is the last change of the lowest Ritz value; residual is
-// |H psi - E psi| of the returned unit vector (for Lanczos the estimate
-// beta_k |z_k| of the Krylov relation, exact up to the loss of orthogonality).
-struct SolveOut {
-  double e0 = 0.0;
-  int iters = 0;
-  double mv_seconds = 0.0;
-  bool converged = false;
-  double ritz_change = 0.0;
-  double residual = 0.0;
-  int hpsi_fp64 = 0, hpsi_fp32 = 0;
-};
-
 // Lanczos, lanczos_memo_new (code/lanczos.f90:4-176), levels = 1.
-SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter) {
+SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter,
+                 const LanczosOpts& opt) {
   const int64_t n = sb->plan.sb.dim;
   need(n > 0, "empty superblock");
   need(max_iter >= 1, "max_iter must be >= 1");
@@ -329,9 +276,21 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
   double2* w = V + (size_t)max_iter * n;   // last slot is the work vector
   auto vec = [&](int j) { return V + (size_t)j * n; };
   double mv = 0.0;
+  int n64 = 0, n32 = 0, calls = 0;
   auto hpsi = [&](const double2* x, double2* y) {
     const double t = wall();
-    matvec(ctx, sb, x, y);
+    const int saved = ctx->precision;
+    const int prec = calls == 0 ? opt.first_prec : opt.rest_prec;
+    if (prec >= 0) ctx->precision = prec;
+    (ctx->precision == PREC_FP64 ? n64 : n32) += 1;
+    ++calls;
+    try {
+      matvec(ctx, sb, x, y);
+    } catch (...) {
+      ctx->precision = saved;
+      throw;
+    }
+    ctx->precision = saved;
     check_hip(hipStreamSynchronize(ctx->stream), "matvec");
     mv += wall() - t;
   };
@@ -372,12 +331,27 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
     return std::make_pair(h[0], h[1]);
   };
   SolveOut out;
-  int nhpsi = 1;
   auto r = alpha_and_update(1);
   alpha.push_back(finite(r.first));
   double ritz = alpha[0];
   int last = 1;
   double b_last = std::sqrt(std::max(r.second, 0.0));   // norm of the Krylov remainder
+  if (opt.on_first) opt.on_first(vec(0), w, alpha[0]);
+  if (opt.after_first && opt.after_first(alpha[0], b_last)) {
+    dev_scale_copy(psi, vec(0), 1.0, n, ctx->stream);
+    check_hip(hipStreamSynchronize(ctx->stream), "lanczos");
+    out.e0 = alpha[0];
+    out.iters = 1;
+    out.mv_seconds = mv;
+    out.converged = true;
+    out.residual = b_last;
+    out.hpsi_fp64 = n64;
+    out.hpsi_fp32 = n32;
+    ctx->last_converged = 1;
+    ctx->last_ritz_change = 0.0;
+    ctx->last_residual = b_last;
+    return out;
+  }
   while (last < max_iter) {
     const double b = std::sqrt(finite(r.second));
     beta.push_back(b);
@@ -395,7 +369,6 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
     dev_scale_copy(vec(last), w, 1.0 / nw, n, ctx->stream);
     ++last;
     hpsi(vec(last - 1), w);
-    ++nhpsi;
     r = alpha_and_update(last);
     alpha.push_back(finite(r.first));
     b_last = std::sqrt(std::max(r.second, 0.0));
@@ -423,12 +396,81 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
   out.iters = last;
   out.mv_seconds = mv;
   out.residual = b_last * std::fabs(z[last - 1]);
-  out.hpsi_fp64 = nhpsi;
+  out.hpsi_fp64 = n64;
+  out.hpsi_fp32 = n32;
   ctx->last_converged = out.converged ? 1 : 0;
   ctx->last_ritz_change = out.ritz_change;
   ctx->last_residual = out.residual;
   return out;
 }
+
+// Mixed-precision ground state: Lanczos rounds whose first H.psi is FP64 and
+// whose other H.psi run in the FP32-storage form.  The FP64 product of the
+// round's start vector u gives its exact Rayleigh quotient alpha_1 and the
+// exact residual r = H u - alpha_1 u; the round's Krylov space is spanned by
+// u, r and the FP32 products, so its Ritz vector corrects u along the exact
+// residual (iterative refinement: each round shrinks the error of u by about
+// the relative FP32 error of H over the spectral gap).  Rounds restart from
+// the Ritz vector until two successive FP64 Rayleigh quotients agree to
+// lanzero (the reference's stopping rule applied to exact energies); the
+// returned energy is that FP64 Rayleigh quotient of the returned vector and
+// the residual its exact residual.
+SolveOut mixed_lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter, int max_rounds) {
+  SolveOut tot;
+  double prev = std::numeric_limits<double>::quiet_NaN();
+  bool stop = false;
+  double a1 = 0.0, b1 = 0.0;
+  LanczosOpts o;
+  o.first_prec = PREC_FP64;
+  o.rest_prec = PREC_FP32_STORE;
+  o.after_first = [&](double a, double b) {
+    a1 = a;
+    b1 = b;
+    stop = std::isfinite(prev) && (std::fabs(a - prev) <= lanzero || b <= lanzero);
+    return stop;
+  };
+  for (int round = 0; round < max_rounds; ++round) {
+    const SolveOut so = lanczos(ctx, sb, psi, lanzero, max_iter, o);
+    tot.iters += so.iters;
+    tot.mv_seconds += so.mv_seconds;
+    tot.hpsi_fp64 += so.hpsi_fp64;
+    tot.hpsi_fp32 += so.hpsi_fp32;
+    tot.ritz_change = std::isfinite(prev) ? std::fabs(a1 - prev) : so.ritz_change;
+    tot.e0 = a1;          // exact Rayleigh quotient of this round's start vector
+    tot.residual = b1;    // and its exact residual
+    if (stop) {
+      tot.converged = true;
+      break;
+    }
+    prev = a1;
+  }
+  if (!tot.converged) {
+    // out of rounds: one last FP64 product fixes the energy and residual of
+    // the vector that is returned
+    LanczosOpts last;
+    last.first_prec = PREC_FP64;
+    last.after_first = [&](double a, double b) {
+      a1 = a;
+      b1 = b;
+      return true;
+    };
+    const SolveOut so = lanczos(ctx, sb, psi, lanzero, max_iter, last);
+    tot.iters += so.iters;
+    tot.mv_seconds += so.mv_seconds;
+    tot.hpsi_fp64 += so.hpsi_fp64;
+    tot.ritz_change = std::fabs(a1 - prev);
+    tot.e0 = a1;
+    tot.residual = b1;
+  }
+  ctx->last_converged = tot.converged ? 1 : 0;
+  ctx->last_ritz_change = tot.ritz_change;
+  ctx->last_residual = tot.residual;
+  return tot;
+}
+
+}  // namespace hh
+
+namespace {
 
 // Eigendecomposition of every sector matrix (ZHEEV, code/get_den.f90:242),
 // ascending eigenvalues and eigenvectors in place.  The sectors are
@@ -746,7 +788,8 @@ void check_schedule(const hh_schedule& s) {
   need(std::isfinite(s.tol) && s.tol >= 0.0 && std::isfinite(s.warm_tol_factor) && s.warm_tol_factor > 0.0 &&
            std::isfinite(s.first_tol_factor) && s.first_tol_factor > 0.0,
        "bad schedule: tolerances");
-  need(s.precision == PREC_FP64, "only the FP64 form (precision 0) is available");
+  need(s.precision == PREC_FP64, "the driver's H.psi precision must be 0; the solver field selects the mixed forms");
+  need(s.solver >= 0 && s.solver <= 2, "bad schedule: solver must be 0, 1 or 2");
 }
 
 // ----------------------------------------------------------------------------
@@ -782,8 +825,77 @@ struct Driver {
     ph_t = t;
   }
 
-  // the step's ground-state solve (the plain form: Lanczos with FP64 H.psi)
-  SolveOut solve(hh_superblock* sb, double2* p, double lanzero) { return lanczos(ctx, sb, p, lanzero, S.max_iter); }
+  // learned preconditioner: network, captured training triples, mode
+  std::unique_ptr<PcNet> net;
+  std::vector<std::unique_ptr<PcSample>> samples;
+  int capture_left = 0;
+  int pc_mode = PC_LEARNED;
+  DevMem cap_u, cap_r;
+
+  // the step's ground-state solve: the plain form (Lanczos with FP64 H.psi),
+  // the mixed-precision Lanczos rounds, or the chosen Davidson form
+  SolveOut solve(hh_superblock* sb, double2* p, double lanzero) {
+    if (S.solver == 2) {
+      bool fb = false;
+      const bool have_net = net && net->trained;
+      const int mode = (pc_mode == PC_LEARNED && !have_net) ? PC_DIAGONAL : pc_mode;
+      const float* w = mode == PC_LEARNED ? (const float*)net->w.p : nullptr;
+      if (capture_left <= 0) {
+        SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb);
+        so.fallback = fb;
+        return so;
+      }
+      // record the restart points of this solve (unit vector, exact residual,
+      // Rayleigh quotient); the last CAP_PER_SOLVE + 1 are kept in a ring
+      const int64_t n = sb->plan.sb.dim;
+      constexpr int CAP_PER_SOLVE = 3;
+      struct Anc {
+        DevMem u, r;
+        double theta = 0.0;
+      };
+      std::vector<Anc> ring(CAP_PER_SOLVE + 1);
+      int count = 0;
+      AnchorHook hook = [&](const double2* u, const double2* r, double th) {
+        Anc& a = ring[count % ring.size()];
+        a.u.alloc(sizeof(double2) * (size_t)n);
+        a.r.alloc(sizeof(double2) * (size_t)n);
+        dev_scale_copy(a.u.c(), u, 1.0, n, ctx->stream);
+        dev_scale_copy(a.r.c(), r, 1.0, n, ctx->stream);
+        a.theta = th;
+        ++count;
+      };
+      SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, hook);
+      so.fallback = fb;
+      // the last restart of a converged solve is the returned vector itself
+      // (its correction is zero): the ones before it are the triples
+      const int last = (so.converged && !fb) ? count - 1 : count;
+      const int first = std::max(0, last - CAP_PER_SOLVE);
+      for (int i = first; i < last && capture_left > 0; ++i) {
+        Anc& a = ring[i % ring.size()];
+        samples.push_back(make_sample(ctx, sb, a.u.c(), a.r.c(), a.theta, p));
+        --capture_left;
+      }
+      return so;
+    }
+    if (S.solver == 1) return mixed_lanczos(ctx, sb, p, lanzero, S.max_iter, 4);
+    if (capture_left <= 0) return lanczos(ctx, sb, p, lanzero, S.max_iter);
+    // plain solve that also records a training triple of the preconditioner:
+    // the unit start vector, its FP64 residual and Rayleigh quotient
+    const int64_t n = sb->plan.sb.dim;
+    double theta0 = 0.0;
+    LanczosOpts o;
+    o.on_first = [&](const double2* v1, const double2* r1, double a1) {
+      cap_u.alloc(sizeof(double2) * (size_t)n);
+      cap_r.alloc(sizeof(double2) * (size_t)n);
+      dev_scale_copy(cap_u.c(), v1, 1.0, n, ctx->stream);
+      dev_scale_copy(cap_r.c(), r1, 1.0, n, ctx->stream);
+      theta0 = a1;
+    };
+    SolveOut so = lanczos(ctx, sb, p, lanzero, S.max_iter, o);
+    samples.push_back(make_sample(ctx, sb, cap_u.c(), cap_r.c(), theta0, p));
+    --capture_left;
+    return so;
+  }
 
   void record(int phase, hh_superblock* sb, const SolveOut& so, const hh_trunc* t, double secs, int64_t mv0,
               double fl0) {
@@ -798,6 +910,8 @@ struct Driver {
     r.residual = so.residual;
     r.hpsi_fp64 = so.hpsi_fp64;
     r.hpsi_fp32 = so.hpsi_fp32;
+    r.solver = S.solver;
+    r.fallback = so.fallback ? 1 : 0;
     r.num_state = t ? t->num_state : 0;
     r.dim = sb->plan.sb.dim;
     r.energy = E;
@@ -815,9 +929,10 @@ struct Driver {
     recs.push_back(r);
     if (verbose()) {
       std::fprintf(stderr,
-                   "# step phase=%d sys=%d env=%d dim=%lld iters=%d%s kept=%d E=%.12f res=%.1e %.2fs "
+                   "# step phase=%d sys=%d env=%d dim=%lld iters=%d%s%s kept=%d E=%.12f res=%.1e %.2fs "
                    "(bind %.3f wave %.3f lanczos %.3f [H.psi %.3f] trunc %.3f grow %.3f)\n",
-                   r.phase, r.sys_len, r.env_len, (long long)r.dim, r.iters, r.converged ? "" : "(cap)", r.num_state,
+                   r.phase, r.sys_len, r.env_len, (long long)r.dim, r.iters, r.converged ? "" : "(cap)",
+                   r.fallback ? "(fallback)" : "", r.num_state,
                    r.energy, r.residual, r.seconds,
                    r.bind_seconds, r.wave_seconds, r.lanczos_seconds, r.matvec_seconds, r.trunc_seconds,
                    r.grow_seconds);
@@ -2456,6 +2571,7 @@ int hh_block_op_offset(const hh_block* b, int site_index, int kind, int64_t* off
 int hh_block_upload(hh_ctx* ctx, hh_block* b, const double* src) {
   return guard(ctx, [&] {
     need(b && src, "null argument");
+    b->have32 = false;   // the complex64 copy, if any, is stale now
     check_hip(hipMemcpyAsync(b->arena.p, src, sizeof(double2) * b->d.size, hipMemcpyDefault, ctx->stream), "upload");
     check_hip(hipStreamSynchronize(ctx->stream), "sync");
   });
@@ -2797,6 +2913,7 @@ int hh_batch_info(const hh_batch* b, int64_t* total_dim, double* flops_alg) {
 int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in, double* const* psi_out) {
   return guard(ctx, [&] {
     need(b && psi_in && psi_out, "null argument");
+    need(ctx->precision != PREC_FP32_STORE, "the batched H.psi runs in precision 0 or 1");
     const int n = (int)b->sbs.size();
     for (int i = 0; i < n; ++i) {
       need(psi_in[i] && psi_out[i] && psi_in[i] != psi_out[i], "batch needs distinct input and output vectors");
@@ -3233,6 +3350,110 @@ int hh_driver_reset_transform(hh_driver* h) {
   return 0;
 }
 
+int hh_driver_capture(hh_driver* h, int max_samples) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(max_samples >= 0 && max_samples <= 64, "max_samples must be in [0, 64]");
+    h->d.capture_left = max_samples;
+  });
+}
+
+int hh_driver_train_precond(hh_driver* h, int epochs, double lr, double* losses, int32_t* nsamples) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    Driver& d = h->d;
+    need(std::isfinite(lr) && lr > 0.0, "learning rate must be positive");
+    if (!d.net) {
+      d.net.reset(new PcNet());
+      d.net->init(d.ctx->stream, d.S.seed, PC_SIGMA0);
+    }
+    std::vector<double> ls;
+    pc_train(d.ctx, *d.net, d.samples, epochs, lr, &ls);
+    if (losses)
+      for (size_t i = 0; i < ls.size(); ++i) losses[i] = ls[i];
+    if (nsamples) *nsamples = (int32_t)d.samples.size();
+  });
+}
+
+int hh_driver_set_precond(hh_driver* h, int mode) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(mode == PC_LEARNED || mode == PC_DIAGONAL || mode == PC_NONE, "mode must be 0, 1 or 2");
+    h->d.pc_mode = mode;
+  });
+}
+
+int hh_driver_fork(const hh_driver* src, hh_driver** out) {
+  if (!src || !out) return 1;
+  hh_ctx* ctx = src->d.ctx;
+  return guard(ctx, [&] {
+    std::unique_ptr<hh_driver> h(new hh_driver());
+    Driver& d = h->d;
+    const Driver& s = src->d;
+    d.ctx = ctx;
+    d.S = s.S;
+    d.step = s.step;
+    d.psi_dim = s.psi_dim;
+    auto copy_dev = [&](DevMem& dst, const DevMem& from) {
+      if (!from.p) return;
+      dst.alloc(from.bytes);
+      check_hip(hipMemcpyAsync(dst.p, from.p, from.bytes, hipMemcpyDeviceToDevice, ctx->stream), "d2d");
+    };
+    for (int side = 0; side < 2; ++side) {
+      const auto& sb = side == 0 ? s.sysb : s.envb;
+      auto& db = side == 0 ? d.sysb : d.envb;
+      for (const auto& kv : sb) {
+        std::unique_ptr<hh_block> b(new hh_block());
+        b->d = kv.second->d;
+        copy_dev(b->arena, kv.second->arena);
+        db[kv.first] = std::move(b);
+      }
+      const auto& st = side == 0 ? s.syst : s.envt;
+      auto& dt = side == 0 ? d.syst : d.envt;
+      for (const auto& kv : st) {
+        std::unique_ptr<hh_trunc> t(new hh_trunc());
+        const hh_trunc& o = *kv.second;
+        t->side = o.side;
+        t->eb = o.eb;
+        t->kept = o.kept;
+        t->kept_per_rho = o.kept_per_rho;
+        t->num_state = o.num_state;
+        t->err = o.err;
+        t->entropy = o.entropy;
+        copy_dev(t->evec, o.evec);
+        dt[kv.first] = std::move(t);
+      }
+    }
+    copy_dev(d.psi, s.psi);
+    d.pc_mode = s.pc_mode;
+    if (s.net) {
+      d.net.reset(new PcNet());
+      for (DevMem* m : {&d.net->w, &d.net->g, &d.net->m, &d.net->v}) m->alloc(sizeof(float) * MlpLayout::SIZE);
+      d.net->loss.alloc(sizeof(double) * 4);
+      copy_dev(d.net->w, s.net->w);
+      copy_dev(d.net->m, s.net->m);
+      copy_dev(d.net->v, s.net->v);
+      d.net->adam_t = s.net->adam_t;
+      d.net->trained = s.net->trained;
+    }
+    if (s.prev_sb) {
+      // the previous step's superblock, bound again to the copied blocks
+      d.prev_sb.reset(bind(ctx, d.sysb.at(s.prev_sb->sys->d.length).get(), d.envb.at(s.prev_sb->env->d.length).get(),
+                           s.prev_sb->N));
+    }
+    check_hip(hipStreamSynchronize(ctx->stream), "fork");
+    *out = h.release();
+  });
+}
+
+int hh_driver_set_solver(hh_driver* h, int solver) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(solver >= 0 && solver <= 2, "solver must be 0, 1 or 2");
+    h->d.S.solver = solver;
+  });
+}
+
 int hh_driver_wave(const hh_driver* h, int32_t* sys_len, int32_t* env_len, int64_t* dim, double* psi) {
   if (!h) return 1;
   return guard(h->d.ctx, [&] {
@@ -3396,7 +3617,8 @@ int hh_driver_pair_block(hh_driver* h, int sys_len, const double* psi, int nquad
 
 int hh_set_precision(hh_ctx* ctx, int precision) {
   return guard(ctx, [&] {
-    need(precision == PREC_FP64 || precision == PREC_FP32, "precision must be 0 (FP64) or 1 (FP32)");
+    need(precision == PREC_FP64 || precision == PREC_FP32 || precision == PREC_FP32_STORE,
+         "precision must be 0 (FP64), 1 (FP32 MFMA) or 2 (FP32 storage)");
     ctx->precision = precision;
   });
 }
@@ -3680,6 +3902,62 @@ int hh_run_tasks_noalloc(hh_ctx* ctx, const void* tiles, int32_t ntiles, const v
     check_hip(hipStreamSynchronize(ctx->stream), "hh_run_tasks_noalloc");
   });
 }
+
+int hh_precond_eval(hh_ctx* ctx, int np, const int32_t* rows, const int32_t* cols, const int32_t* s2,
+                    const int32_t* nq, int n_mid, const double* hs, const double* he, const double* r,
+                    const double* delta, double theta, const float* w, float* feat, float* out, double* par,
+                    double* loss, float* grad) {
+  return guard(ctx, [&] {
+    need(np >= 1 && rows && cols && s2 && nq && hs && he && r && delta && w, "null or empty argument");
+    PairTable t;
+    t.np = np;
+    int64_t n = 0;
+    int32_t so = 0, eo = 0;
+    for (int p = 0; p < np; ++p) {
+      need(rows[p] >= 1 && cols[p] >= 1, "empty sector");
+      t.off.push_back(n);
+      t.rows.push_back(rows[p]);
+      t.cols.push_back(cols[p]);
+      t.soff.push_back(so);
+      t.eoff.push_back(eo);
+      t.s2.push_back(s2[p]);
+      t.nq.push_back(nq[p]);
+      n += (int64_t)rows[p] * cols[p];
+      so += rows[p];
+      eo += cols[p];
+    }
+    t.n = n;
+    std::vector<double> vhs(hs, hs + so), vhe(he, he + eo);
+    PcDev P;
+    pc_setup(P, t, vhs, vhe, n_mid, ctx->stream);
+    DevMem dr, dd, dw, dg, dl;
+    dr.alloc(sizeof(double2) * (size_t)n);
+    dd.alloc(sizeof(double2) * (size_t)n);
+    dw.alloc(sizeof(float) * MlpLayout::SIZE);
+    dg.alloc(sizeof(float) * MlpLayout::SIZE);
+    dl.alloc(sizeof(double) * 4);
+    double dn = 0.0;
+    for (int64_t i = 0; i < 2 * n; ++i) dn += delta[i] * delta[i];
+    need(dn > 0.0, "zero target");
+    check_hip(hipMemcpyAsync(dr.p, r, sizeof(double2) * n, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    check_hip(hipMemcpyAsync(dd.p, delta, sizeof(double2) * n, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    check_hip(hipMemcpyAsync(dw.p, w, sizeof(float) * MlpLayout::SIZE, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    pc_features(P, dr.c(), theta, n_mid, ctx->stream);
+    pc_forward(P, (const float*)dw.p, ctx->stream);
+    pc_backward(P, dr.c(), dd.c(), theta, dn, (const float*)dw.p, (float*)dg.p, (double*)dl.p, ctx->stream);
+    auto down = [&](void* dst, const void* src, size_t b) {
+      if (dst) check_hip(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    };
+    down(feat, P.feat, sizeof(float) * (size_t)np * PC_NF);
+    down(out, P.out, sizeof(float) * (size_t)np * PC_NO);
+    down(par, P.par, sizeof(double) * 2 * (size_t)np);
+    down(loss, dl.p, sizeof(double));
+    down(grad, dg.p, sizeof(float) * MlpLayout::SIZE);
+    check_hip(hipStreamSynchronize(ctx->stream), "precond eval");
+  });
+}
+
+int hh_precond_size(void) { return MlpLayout::SIZE; }
 
 int hh_start_vector(hh_ctx* ctx, double* psi, int64_t n, uint64_t seed) {
   return guard(ctx, [&] {

@@ -266,13 +266,15 @@ class StepRecord:
     residual: float = 0.0
     hpsi_fp64: int = 0
     hpsi_fp32: int = 0
+    solver: int = 0
+    fallback: bool = False
 
     @classmethod
     def from_c(cls, r):
         return cls(r.phase, r.sys_len, r.env_len, r.iters, r.num_state, r.dim, r.energy, r.trunc_err, r.entropy,
                    r.seconds, r.matvec_seconds, r.matvec_flops, r.bind_seconds, r.wave_seconds, r.lanczos_seconds,
                    r.trunc_seconds, r.grow_seconds, bool(r.converged), r.ritz_change, r.residual, r.hpsi_fp64,
-                   r.hpsi_fp32)
+                   r.hpsi_fp32, r.solver, bool(r.fallback))
 
 
 class Context:
@@ -466,9 +468,11 @@ class Context:
 
     # -- whole run (Get_Ground_State, code/quantum.f90:179-288) ---------------
     def dmrg_run(self, kept_max, n_sweeps, tol, kept_max_warm=0, max_iter=100, seed=1234,
-                 warm_tol_factor=1000.0, first_tol_factor=15.0, precision=0):
+                 warm_tol_factor=1000.0, first_tol_factor=15.0, precision=0, solver=0):
+        """solver 0: plain FP64 Lanczos; 1: mixed-precision Lanczos rounds;
+        2: the chosen Davidson form (hh_schedule.solver)."""
         s = L.Schedule(kept_max, kept_max_warm, n_sweeps, max_iter, tol, warm_tol_factor, first_tol_factor, seed,
-                       precision, 0)
+                       precision, solver)
         cap = 4096
         recs = (L.StepRecord * cap)()
         n = L.i32()
@@ -498,10 +502,10 @@ class Driver:
     blocks and truncations resident on the device."""
 
     def __init__(self, ctx, kept_max, tol, kept_max_warm=0, max_iter=100, seed=1234, warm_tol_factor=1000.0,
-                 first_tol_factor=15.0, precision=0):
+                 first_tol_factor=15.0, precision=0, solver=0):
         self.ctx = ctx
         s = L.Schedule(kept_max, kept_max_warm, 0, max_iter, tol, warm_tol_factor, first_tol_factor, seed,
-                       precision, 0)
+                       precision, solver)
         p = C.c_void_p()
         ctx._rc(L.load().hh_driver_create(ctx.ptr, C.byref(s), C.byref(p)), "hh_driver_create")
         self.ptr = p
@@ -530,6 +534,39 @@ class Driver:
         if key not in self._dims:
             self._dims[key] = self.ctx.bind(sysb, envb, self.ctx.model.n_elec).dim
         return _dptr(psi, self._dims[key])
+
+    def fork(self):
+        """An independent copy of this driver's state (made on the device)."""
+        p = C.c_void_p()
+        self.ctx._rc(L.load().hh_driver_fork(self.ptr, C.byref(p)), "hh_driver_fork")
+        d = Driver.__new__(Driver)
+        d.ctx = self.ctx
+        d.ptr = p
+        return d
+
+    def capture(self, max_samples):
+        """The next max_samples plain (solver 0) steps record a training triple
+        of the learned preconditioner each."""
+        self.ctx._rc(L.load().hh_driver_capture(self.ptr, int(max_samples)), "hh_driver_capture")
+
+    def train_precond(self, epochs=60, lr=1e-3):
+        """Train the preconditioner network on the captured triples (Adam, on
+        the GPU); returns (mean loss per epoch, number of triples)."""
+        losses = np.zeros(max(int(epochs), 1))
+        ns = L.i32()
+        self.ctx._rc(L.load().hh_driver_train_precond(self.ptr, int(epochs), float(lr),
+                                                      losses.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ns)),
+                     "hh_driver_train_precond")
+        return losses, ns.value
+
+    def set_precond(self, mode):
+        """0 the trained network, 1 the fixed diagonal form, 2 none (t = r)."""
+        self.ctx._rc(L.load().hh_driver_set_precond(self.ptr, int(mode)), "hh_driver_set_precond")
+
+    def set_solver(self, solver):
+        """Ground-state solver of the following steps: 0 plain FP64 Lanczos, 1
+        mixed-precision Lanczos rounds, 2 the chosen Davidson form."""
+        self.ctx._rc(L.load().hh_driver_set_solver(self.ptr, int(solver)), "hh_driver_set_solver")
 
     def wave(self):
         """(sys_len, env_len, psi): the ground state of the last sweep step as a

@@ -55,7 +55,15 @@ typedef struct {
   double first_tol_factor;   /* reference: 15 */
   uint64_t seed;
   int32_t precision;         /* 0 FP64 MFMA (plain form) */
-  int32_t reserved;
+  int32_t solver;            /* ground-state solve of every step:
+                                0 Lanczos with FP64 H.psi (lanczos_memo_new, the plain form);
+                                1 mixed precision: Lanczos rounds with FP32-storage H.psi,
+                                  each started by an FP64 H.psi of its start vector, until two
+                                  successive FP64 Rayleigh quotients agree to the tolerance;
+                                2 the chosen form: Davidson with FP32-storage H.psi, FP64
+                                  restarts and the learned per-sector preconditioner (see
+                                  hh_driver_train_precond), falling back to solver 1 when the
+                                  FP64 residual stops falling */
 } hh_schedule;
 
 /* One DMRG step (a line of Energy_All.dat + Num_State.dat + Entropy.dat,
@@ -85,6 +93,8 @@ typedef struct {
                                 beta_k |z_k|, the remainder of the Krylov relation) */
   int32_t hpsi_fp64;         /* H.psi applications with FP64 arithmetic in the solve */
   int32_t hpsi_fp32;         /* H.psi applications with FP32 arithmetic in the solve */
+  int32_t solver;            /* hh_schedule.solver of the step */
+  int32_t fallback;          /* 1: the chosen form handed the solve to the mixed Lanczos */
 } hh_step_record;
 
 /* ---- context ---------------------------------------------------------- */
@@ -264,6 +274,52 @@ int hh_driver_save(hh_driver* d, const char* path);
 int hh_driver_load(hh_driver* d, const char* path);
 /* forget the previous superblock (next first step does not transform) */
 int hh_driver_reset_transform(hh_driver* d);
+/* the ground-state solver of the steps that follow (hh_schedule.solver);
+ * the reference always runs lanczos_memo_new (code/lanczos.f90:4-176) */
+int hh_driver_set_solver(hh_driver* d, int solver);
+/* An independent copy of a driver (blocks, truncations, carried wave
+ * function, previous superblock, step counter and schedule), made on the
+ * device; both continue on their own.  No counterpart in the reference: it
+ * lets one state be swept by two solvers. */
+int hh_driver_fork(const hh_driver* src, hh_driver** out);
+
+/* ---- learned preconditioner of the chosen form (solver 2) ----------------
+ * No counterpart in the reference, whose steps all use lanczos_memo_new
+ * (code/lanczos.f90:4-176).  The Davidson expansion vector of sector block p
+ * (element i) is t_i = alpha_p r_i / (max(d_i - theta, 0) + sigma_p), with r
+ * the residual, theta the Ritz value and d the diagonal of the sys and env
+ * block Hamiltonians in the enlarged bases; (sigma_p, alpha_p) come from an
+ * MLP (8 -> 256 -> 256 -> 2, tanh, FP32) on features of block p.  Training
+ * triples (diagonal, residual of the step's start vector, exact correction
+ * toward the step's ground state) are captured from plain solves.
+ *
+ * hh_driver_capture: the steps that follow record training triples until
+ * max_samples are kept on the device: a step solved by solver 0 records its
+ * start vector (exact correction toward the step's ground state), a step
+ * solved by solver 2 up to three of its FP64 restart points.
+ * hh_driver_train_precond: epochs passes of Adam (learning rate lr) over the
+ * captured triples with the library's own HIP forward and backward kernels;
+ * losses (epochs doubles, may be null) receives the mean loss |t - delta|^2 /
+ * |delta|^2 per epoch, *nsamples the number of triples.
+ * hh_driver_set_precond: 0 the trained network (the fixed diagonal form while
+ * none is trained), 1 the fixed diagonal form (sigma = 0.1 x the spread of
+ * d, alpha = 1), 2 no preconditioner (t = r). */
+int hh_driver_capture(hh_driver* d, int max_samples);
+int hh_driver_train_precond(hh_driver* d, int epochs, double lr, double* losses, int32_t* nsamples);
+int hh_driver_set_precond(hh_driver* d, int mode);
+/* The preconditioner's whole training pass on caller data, for checking the
+ * kernels: np sector blocks (rows x cols, 2S and N of the sys sector), the
+ * diagonal d(a, b) = hs[a] + he[b] per block (hs, he concatenated over the
+ * blocks), residual r and target delta (complex128, interleaved, blocks in
+ * order), Ritz value theta and network parameters w (hh_precond_size()
+ * floats).  Returns the features (np x 8), network outputs (np x 2), sector
+ * parameters (sigma_p, alpha_p), the loss and its gradient with respect to
+ * w; any output pointer may be null. */
+int hh_precond_eval(hh_ctx* ctx, int np, const int32_t* rows, const int32_t* cols, const int32_t* s2,
+                    const int32_t* nq, int n_mid, const double* hs, const double* he, const double* r,
+                    const double* delta, double theta, const float* w, float* feat, float* out, double* par,
+                    double* loss, float* grad);
+int hh_precond_size(void);
 /* The ground state of the last sweep step (the eigvec that left_to_right /
  * right_to_left hand to wave_trans_LR / wave_trans_RL, code/warm_up.f90:1607,
  * :1786): the lengths of the sys and env blocks its superblock was bound from
@@ -388,10 +444,16 @@ typedef struct {
   double reduce_ms;          /* kernel time of the split-sum reductions of both stages */
   double stage1_flops_exec;  /* part of flops_exec run by the stage-1 launches */
   double stage2_flops_exec;  /* part of flops_exec run by the stage-2 launches */
+  int64_t matvecs_fp32;      /* of matvecs: those in the FP32-storage form */
+  double stage2_ms_fp32;     /* of stage2_ms: the FP32-storage launches */
+  double stage2_flops_alg_fp32;  /* of stage2_flops_alg: the FP32-storage launches */
 } hh_stats;
 /* Arithmetic of the H.psi tile kernels for the calls that follow on this
  * context: 0 FP64 MFMA (plain form), 1 FP32 MFMA on FP64-stored operands
- * with FP64 output.  Lanczos vectors and reductions stay FP64. */
+ * with FP64 output, 2 FP32 storage: complex64 copies of the blocks, merged
+ * operators, stage-1 intermediate and vectors, FP32 MFMA with FP32
+ * accumulation (the vectors passed in and out stay complex128).  Lanczos
+ * vectors and reductions stay FP64. */
 int hh_set_precision(hh_ctx* ctx, int precision);
 /* profile = 1 records HIP events around every H.psi launch (adds a sync).
  * While it is on, the H.psi tile kernels run as their own instantiations
