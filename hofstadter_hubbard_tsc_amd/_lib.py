@@ -102,6 +102,7 @@ SIGNATURES = [
     ("hh_driver_capture", i32, [vp, i32]),
     ("hh_driver_train_precond", i32, [vp, i32, f64, P(f64), P(i32)]),
     ("hh_driver_set_precond", i32, [vp, i32]),
+    ("hh_driver_set_fallback", i32, [vp, f64]),
     ("hh_precond_eval", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), P(f64), P(f64), P(f64), f64,
                               P(C.c_float), P(C.c_float), P(C.c_float), P(f64), P(f64), P(C.c_float)]),
     ("hh_precond_size", i32, []),

@@ -172,7 +172,7 @@ void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P) {
 }
 
 SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_hpsi, int mode,
-                  const float* net, bool* fell_back, const AnchorHook& on_anchor) {
+                  const float* net, bool* fell_back, const AnchorHook& on_anchor, double fall_factor) {
   const int64_t n = sb->plan.sb.dim;
   need(n > 0, "empty superblock");
   need(max_hpsi >= 2, "max_iter must be >= 2");
@@ -332,7 +332,7 @@ SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, 
         break;
       }
       if (used >= max_hpsi) break;
-      if (rho > 0.9 * anchor_rho && mode != PC_NONE) {
+      if (rho > fall_factor * anchor_rho && mode != PC_NONE) {
         // the FP64 residual did not fall: finish with the mixed-precision
         // Lanczos from the current vector
         if (fell_back) *fell_back = true;

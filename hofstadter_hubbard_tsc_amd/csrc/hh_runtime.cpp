@@ -830,6 +830,7 @@ struct Driver {
   std::vector<std::unique_ptr<PcSample>> samples;
   int capture_left = 0;
   int pc_mode = PC_LEARNED;
+  double fall_factor = DAV_FALL_FACTOR;
   DevMem cap_u, cap_r;
 
   // the step's ground-state solve: the plain form (Lanczos with FP64 H.psi),
@@ -841,7 +842,7 @@ struct Driver {
       const int mode = (pc_mode == PC_LEARNED && !have_net) ? PC_DIAGONAL : pc_mode;
       const float* w = mode == PC_LEARNED ? (const float*)net->w.p : nullptr;
       if (capture_left <= 0) {
-        SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb);
+        SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, AnchorHook(), fall_factor);
         so.fallback = fb;
         return so;
       }
@@ -864,7 +865,7 @@ struct Driver {
         a.theta = th;
         ++count;
       };
-      SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, hook);
+      SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, hook, fall_factor);
       so.fallback = fb;
       // the last restart of a converged solve is the returned vector itself
       // (its correction is zero): the ones before it are the triples
@@ -3383,6 +3384,14 @@ int hh_driver_set_precond(hh_driver* h, int mode) {
   });
 }
 
+int hh_driver_set_fallback(hh_driver* h, double factor) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(std::isfinite(factor) && factor > 0.0, "the fallback factor must be positive");
+    h->d.fall_factor = factor;
+  });
+}
+
 int hh_driver_fork(const hh_driver* src, hh_driver** out) {
   if (!src || !out) return 1;
   hh_ctx* ctx = src->d.ctx;
@@ -3426,6 +3435,7 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out) {
     }
     copy_dev(d.psi, s.psi);
     d.pc_mode = s.pc_mode;
+    d.fall_factor = s.fall_factor;
     if (s.net) {
       d.net.reset(new PcNet());
       for (DevMem* m : {&d.net->w, &d.net->g, &d.net->m, &d.net->v}) m->alloc(sizeof(float) * MlpLayout::SIZE);

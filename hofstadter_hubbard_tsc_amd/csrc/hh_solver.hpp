@@ -51,8 +51,12 @@ void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P);
 typedef std::function<void(const double2*, const double2*, double)> AnchorHook;
 // Largest Davidson subspace before a restart.
 constexpr int DAV_KMAX = 32;
+// fall_factor: the FP64 residual must fall below fall_factor times its value
+// at the previous restart, or the mixed Lanczos takes over.
+constexpr double DAV_FALL_FACTOR = 0.9;
 SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_hpsi, int mode,
-                  const float* net, bool* fell_back, const AnchorHook& on_anchor = AnchorHook());
+                  const float* net, bool* fell_back, const AnchorHook& on_anchor = AnchorHook(),
+                  double fall_factor = DAV_FALL_FACTOR);
 
 // Record one training triple from a finished plain solve: u0 is the unit
 // start vector, r0 = H u0 - theta0 u0 its FP64 residual, psi the ground state.

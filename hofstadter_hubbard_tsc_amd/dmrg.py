@@ -563,6 +563,11 @@ class Driver:
         """0 the trained network, 1 the fixed diagonal form, 2 none (t = r)."""
         self.ctx._rc(L.load().hh_driver_set_precond(self.ptr, int(mode)), "hh_driver_set_precond")
 
+    def set_fallback(self, factor):
+        """Solver 2 hands the solve to the mixed Lanczos unless the FP64
+        residual falls below factor x its value at the previous restart."""
+        self.ctx._rc(L.load().hh_driver_set_fallback(self.ptr, float(factor)), "hh_driver_set_fallback")
+
     def set_solver(self, solver):
         """Ground-state solver of the following steps: 0 plain FP64 Lanczos, 1
         mixed-precision Lanczos rounds, 2 the chosen Davidson form."""

@@ -307,6 +307,11 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out);
 int hh_driver_capture(hh_driver* d, int max_samples);
 int hh_driver_train_precond(hh_driver* d, int epochs, double lr, double* losses, int32_t* nsamples);
 int hh_driver_set_precond(hh_driver* d, int mode);
+/* Fallback rule of solver 2: at every FP64 restart the exact residual must
+ * be below factor times its value at the previous restart (default 0.9), or
+ * the mixed-precision Lanczos (solver 1) finishes the solve from the current
+ * vector; the step record's fallback flag is then set. */
+int hh_driver_set_fallback(hh_driver* d, double factor);
 /* The preconditioner's whole training pass on caller data, for checking the
  * kernels: np sector blocks (rows x cols, 2S and N of the sys sector), the
  * diagonal d(a, b) = hs[a] + he[b] per block (hs, he concatenated over the
