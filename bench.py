@@ -298,8 +298,10 @@ def run_c3(args):
     ctx.set_profiling(False)
     st = ctx.stats()
     s2_tflops = st["stage2_flops_alg"] / max(st["stage2_ms"], 1e-30) / 1e9
+    tpath = os.path.join(ROOT, "profiles", "hpsi_stage2_batched_c3_traffic.json")
+    traffic = json.load(open(tpath)).get("hbm_bytes_per_launch") if os.path.exists(tpath) else None
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(s2_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
             "kernel": "hh::k_tiles<double, 2, hh::BufTable, *> (batched H.psi stage 2)",
             "avg_launch_ms": round(st["stage2_ms"] / max(st["matvecs"], 1), 4), "launches": int(st["matvecs"]),
             "stage1_ms_per_launch": st["stage1_ms"] / max(st["matvecs"], 1),
@@ -501,6 +503,8 @@ def main():
     if args.measure and recs:
         measure = measure_at_scale(ctx, drv, model, recs[-1].sys_len + 1)
     forms = {}
+    t32 = os.path.join(ROOT, "profiles", "hpsi_stage2_fp32_tj_cyl32x4_m%d_traffic.json" % args.m)
+    traffic32 = json.load(open(t32)).get("hbm_bytes_per_launch") if os.path.exists(t32) else None
     for name, f in forks.items():
         ctx.reset_stats()
         ctx.set_profiling(True)
@@ -528,6 +532,7 @@ def main():
             "matvec_share": round(sum(r.matvec_seconds for r in rc) / max(wall_f, 1e-30), 3),
             "fp32_stage2": {"bound": "mfma", "achieved": round(s2f, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
                             "unit": "TFLOP/s", "frac": round(s2f / FP32_MFMA_PEAK_TFLOPS, 4),
+                            "traffic": traffic32,
                             "avg_launch_ms": round(sf["stage2_ms_fp32"] / max(sf["matvecs_fp32"], 1), 4),
                             "launches": int(sf["matvecs_fp32"]),
                             "kernel": "hh::k_tiles<float, 4, hh::BufPtrs32, *> (FP32-storage H.psi stage 2)"},
