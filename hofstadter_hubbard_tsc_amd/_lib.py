@@ -89,6 +89,8 @@ SIGNATURES = [
     ("hh_trunc_destroy", i32, [vp]),
     ("hh_trunc_kept", i32, [vp, P(i32), P(i32), P(i32)]),
     ("hh_grow_block", i32, [vp, vp, i32, vp, P(vp)]),
+    ("hh_renormalize", i32, [vp, vp, i32, P(i32), P(i32), P(i64), vp, P(i32), P(i64), P(i32), P(i32), P(i64), vp]),
+    ("hh_block_conjugate", i32, [vp, vp]),
     ("hh_wave_trans", i32, [vp, i32, vp, vp, vp, vp, vp, vp]),
     ("hh_dmrg_run", i32, [vp, P(Schedule), P(StepRecord), i32, P(i32)]),
     ("hh_driver_create", i32, [vp, P(Schedule), P(vp)]),

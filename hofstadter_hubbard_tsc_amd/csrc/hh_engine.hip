@@ -1035,6 +1035,14 @@ __global__ __launch_bounds__(VB) void k_to_f64(double2* __restrict__ d, const fl
     d[i] = make_double2((double)y.x, (double)y.y);
   }
 }
+__global__ __launch_bounds__(VB) void k_conj(double2* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i].y = -p[i].y;
+}
+void dev_conj(double2* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_conj, dim3(grid_for(n)), dim3(VB), 0, s, p, n);
+}
 void dev_to_f32(float2* d, const double2* src, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(VB), 0, s, d, src, n);

@@ -82,6 +82,8 @@ void launch_tiles(const DevTasks& t, const BufPtrs& b, hipStream_t s, int precis
 void launch_tiles(const DevTasks& t, const BufTable& b, hipStream_t s, int precision = PREC_FP64, int tag = 0,
                   hipEvent_t mid = nullptr);
 void launch_tiles(const DevTasks& t, const BufPtrs32& b, hipStream_t s, int tag = 0, hipEvent_t mid = nullptr);
+// p = conj(p), n elements
+void dev_conj(double2* p, int64_t n, hipStream_t s);
 // complex128 <-> complex64 copies of n elements
 void dev_to_f32(float2* d, const double2* src, int64_t n, hipStream_t s);
 void dev_to_f64(double2* d, const float2* src, int64_t n, hipStream_t s);

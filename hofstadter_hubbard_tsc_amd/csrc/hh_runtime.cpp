@@ -3029,6 +3029,68 @@ int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* ke
   return 0;
 }
 
+int hh_renormalize(hh_ctx* ctx, const hh_trunc* t, int nsub, const int32_t* bra, const int32_t* ket,
+                   const int64_t* off, const double* op_dev, int32_t* out_nsub, int64_t* out_size, int32_t* out_bra,
+                   int32_t* out_ket, int64_t* out_off, double* out_dev) {
+  return guard(ctx, [&] {
+    need(t && nsub >= 0 && (nsub == 0 || (bra && ket && off)), "null argument");
+    const auto& sec = t->eb.sec;
+    std::vector<int> kidx(sec.size(), -1);
+    for (size_t j = 0; j < t->kept.size(); ++j) kidx[t->kept[j].a] = (int)j;
+    // output sub-blocks: (bra, ket) pairs with both sectors kept, in input order
+    TaskList s1, s2;
+    int64_t xo = 0, oo = 0;
+    std::vector<int32_t> ob, ok;
+    std::vector<int64_t> oof;
+    for (int i = 0; i < nsub; ++i) {
+      need(bra[i] >= 0 && bra[i] < (int)sec.size() && ket[i] >= 0 && ket[i] < (int)sec.size() && off[i] >= 0,
+           "sub-block sector out of range");
+      const int jb = kidx[bra[i]], jk = kidx[ket[i]];
+      if (jb < 0 || jk < 0) continue;
+      const KeptSector& kb = t->kept[jb];
+      const KeptSector& kk = t->kept[jk];
+      const int db = sec[bra[i]].dim, dk = sec[ket[i]].dim;
+      // X = O U_ket (db x keep_ket), then out = U_bra^dag X (keep_bra x keep_ket)
+      s1.add_tile(B_X, xo, db, db, kk.keep);
+      s1.add_term(Term{1.0, 0.0, off[i], kk.u_off, db, kk.u_ld, dk, B_IN, B_AUX, OP_NN, OP_NN});
+      s2.add_tile(B_OUT, oo, kb.keep, kb.keep, kk.keep);
+      s2.add_term(Term{1.0, 0.0, kb.u_off, xo, kb.u_ld, db, db, B_AUX, B_X, OP_C, OP_NN});
+      ob.push_back(jb);
+      ok.push_back(jk);
+      oof.push_back(oo);
+      xo += (int64_t)db * kk.keep;
+      oo += (int64_t)kb.keep * kk.keep;
+    }
+    if (out_nsub) *out_nsub = (int32_t)ob.size();
+    if (out_size) *out_size = oo;
+    for (size_t i = 0; i < ob.size(); ++i) {
+      if (out_bra) out_bra[i] = ob[i];
+      if (out_ket) out_ket[i] = ok[i];
+      if (out_off) out_off[i] = oof[i];
+    }
+    if (!out_dev || ob.empty()) return;
+    need(op_dev != nullptr, "null operator");
+    DevMem X;
+    X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(xo, 1));
+    DevTasks a, b;
+    a.upload(s1, ctx->stream);
+    b.upload(s2, ctx->stream);
+    BufPtrs bp = bufs_of((double2*)op_dev, (double2*)out_dev, nullptr, nullptr, X.c(), nullptr, t->evec.c(), nullptr);
+    launch_tiles(a, bp, ctx->stream, PREC_FP64);
+    launch_tiles(b, bp, ctx->stream, PREC_FP64);
+    check_hip(hipStreamSynchronize(ctx->stream), "renormalize");
+  });
+}
+
+int hh_block_conjugate(hh_ctx* ctx, hh_block* b) {
+  return guard(ctx, [&] {
+    need(b != nullptr, "null block");
+    dev_conj(b->arena.c(), b->d.size, ctx->stream);
+    b->have32 = false;
+    check_hip(hipStreamSynchronize(ctx->stream), "conjugate");
+  });
+}
+
 int hh_trunc_layout(const hh_trunc* t, int64_t* size, int64_t* u_off, int32_t* u_ld) {
   if (!t) return 1;
   if (size) *size = (int64_t)(t->evec.bytes / sizeof(double2));

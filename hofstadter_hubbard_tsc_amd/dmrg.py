@@ -456,6 +456,47 @@ class Context:
         self._rc(L.load().hh_grow_block(self.ptr, sb.ptr, side, trunc.ptr, C.byref(p)), "hh_grow_block")
         return Block(self, p)
 
+    def renormalize(self, trunc, subs):
+        """update_trun_dia / update_trun_ndia (code/update.f90:1387-1605) for one
+        operator: subs = [(bra, ket, matrix)] in the truncation's enlarged
+        basis (enlarged-sector indices, host complex matrices); returns
+        [(kept bra, kept ket, U_bra^dag M U_ket)] for the pairs of kept sectors."""
+        import torch
+        lib = L.load()
+        bra = np.array([b for b, _, _ in subs], np.int32)
+        ket = np.array([k for _, k, _ in subs], np.int32)
+        mats = [np.asarray(m, np.complex128) for _, _, m in subs]
+        off = np.zeros(len(subs), np.int64)
+        o = 0
+        for i, m in enumerate(mats):
+            off[i] = o
+            o += m.size
+        arena = np.concatenate([m.reshape(-1, order="F") for m in mats]) if mats else np.zeros(1, np.complex128)
+        n, size = L.i32(), L.i64()
+        self._rc(lib.hh_renormalize(self.ptr, trunc.ptr, len(subs), L.ip(bra), L.ip(ket), L.ip(off), None,
+                                    C.byref(n), C.byref(size), None, None, None, None), "hh_renormalize")
+        ob, ok = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        oo = np.zeros(n.value, np.int64)
+        src = torch.from_numpy(arena).cuda()
+        dst = torch.zeros(max(size.value, 1), dtype=torch.complex128, device="cuda")
+        _sync()
+        self._rc(lib.hh_renormalize(self.ptr, trunc.ptr, len(subs), L.ip(bra), L.ip(ket), L.ip(off),
+                                    _dptr(src, arena.size), C.byref(n), C.byref(size), L.ip(ob), L.ip(ok), L.ip(oo),
+                                    _dptr(dst, size.value)), "hh_renormalize")
+        h = dst.cpu().numpy()
+        keep = dict(zip(*trunc.kept()))
+        sec = trunc.kept()[0]
+        out = []
+        for b_, k_, o_ in zip(ob, ok, oo):
+            r, c = keep[sec[b_]], keep[sec[k_]]
+            out.append((int(b_), int(k_), h[o_:o_ + r * c].reshape((r, c), order="F")))
+        return out
+
+    def conjugate_block(self, block):
+        """Conjugate a block's arena in place (environment blocks taken from
+        the reference's arrays, which hold the conjugated operators)."""
+        self._rc(L.load().hh_block_conjugate(self.ptr, block.ptr), "hh_block_conjugate")
+
     def wave_trans(self, direction, old_sb, psi_old, sys_t, env_t, new_sb, psi_new):
         """wave_trans_LR (direction 0) / wave_trans_RL (1), code/warm_up.f90:1855-2170."""
         _sync()

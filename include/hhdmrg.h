@@ -231,6 +231,27 @@ int hh_trunc_kept(const hh_trunc* t, int32_t* nsec, int32_t* sector, int32_t* ke
 int hh_trunc_layout(const hh_trunc* t, int64_t* size, int64_t* u_off, int32_t* u_ld);
 int hh_trunc_download(hh_ctx* ctx, const hh_trunc* t, double* dst);
 
+/* update_trun_dia / update_trun_ndia (code/update.f90:1387-1605) for one
+ * operator: eff = U^dag O U per sub-block with the kept eigenvectors U of the
+ * truncation t.  The operator is given in the truncation's enlarged basis:
+ * nsub sub-blocks (bra, ket enlarged-sector indices, column-major dim_bra x
+ * dim_ket at element offset off of op_dev).  The result holds the
+ * sub-blocks whose two sectors are both kept, in input order, indexed by
+ * kept sector (the new block's sector order), keep_bra x keep_ket each,
+ * packed at out_off of out_dev.  With out_dev null only *out_nsub,
+ * *out_size (complex elements) and the out arrays are filled. */
+int hh_renormalize(hh_ctx* ctx, const hh_trunc* t, int nsub, const int32_t* bra, const int32_t* ket,
+                   const int64_t* off, const double* op_dev, int32_t* out_nsub, int64_t* out_size, int32_t* out_bra,
+                   int32_t* out_ket, int64_t* out_off, double* out_dev);
+
+/* Complex-conjugate a block's whole arena (H and every operator) in place.
+ * The reference stores environment-side operators conjugated and applies
+ * them as psi . B^dag (block_transfer_trans, code/space_info.f90:32-63;
+ * code/mat_vec.f90:3563, :3590); this library stores the physical operator
+ * on both sides and applies psi . B^T, so an environment block uploaded from
+ * the reference's arrays is conjugated once with this call. */
+int hh_block_conjugate(hh_ctx* ctx, hh_block* b);
+
 /* Block growth + renormalisation: replaces Get_Hamiltonian_Sys/Env
  * (code/get_ham.f90:4-659) and update_trun_dia/ndia (code/update.f90:1387-1605):
  * the side's block of sb grown by its centre site and truncated with t. */

@@ -251,3 +251,64 @@ def test_wave_trans_matches_oracle_on_truncated_steps(ctx, kind, Lx, Ly, flux, N
             drv.reset_transform()
             drv.sweep(1, ls + 2, ls + 1, 1e-9, driver.TRUN_ERR, kept, phase=1)
     assert checked == 2
+
+
+def test_renormalize_one_operator(ctx):
+    """hh_renormalize (update_trun_dia / update_trun_ndia, code/update.f90:
+    1387-1605, for one operator) against U_bra^dag M U_ket computed in numpy
+    with the truncation's own kept eigenvectors, on random sub-blocks of the
+    enlarged basis (kept and discarded sectors mixed); sub-blocks touching a
+    fully discarded sector are dropped."""
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="tj", Lx=4, Ly=4, J=0.4, t=1.0, flux=0.125, n_elec=14)
+    ctx.set_model(m)
+    drv = Driver(ctx, kept_max=30, tol=1e-9, seed=3)
+    drv.warmup()
+    n = m.nsite
+    t = drv.trunc(0, n // 2)
+    kept_a, kept_k = t.kept()
+    U = dict(t.matrices())
+    sb = ctx.bind(drv.block(0, n // 2 - 1), drv.block(1, n - n // 2 - 1), m.n_elec)
+    _, _, dims = sb.enlarged(0)
+    rng = np.random.default_rng(4)
+    subs = []
+    for b in range(len(dims)):
+        for k in range(len(dims)):
+            if (b + 2 * k) % 3 == 0:
+                subs.append((b, k, rng.normal(size=(dims[b], dims[k])) + 1j * rng.normal(size=(dims[b], dims[k]))))
+    out = ctx.renormalize(t, subs)
+    want = [(int(np.where(kept_a == b)[0][0]), int(np.where(kept_a == k)[0][0]), U[b].conj().T @ M @ U[k])
+            for b, k, M in subs if b in U and k in U]
+    assert len(out) == len(want) > 0
+    assert any(b not in U for b, _, _ in subs), "the case must discard a sector"
+    for (ob, ok, om), (wb, wk, wm) in zip(out, want):
+        assert (ob, ok) == (wb, wk)
+        assert np.abs(om - wm).max() <= 1e-12 * max(1.0, np.abs(wm).max())
+
+
+def test_block_conjugate(ctx):
+    """hh_block_conjugate conjugates the whole arena exactly; twice is the
+    identity, and H.psi with the conjugated-twice env block is unchanged."""
+    import torch
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    ctx.set_model(m)
+    run = driver.Run(m, driver.Schedule(kept_max=20, n_sweeps=0, tol=1e-9))
+    run.warm_up()
+    n = m.nsite
+    sysb = upload_block(ctx, run.sys_blocks[n // 2 - 1], 0, 1)
+    envb = upload_block(ctx, run.env_blocks[n - n // 2 - 1], 1, 1)
+    a0 = envb.download()
+    sb0 = ctx.bind(sysb, envb, m.n_elec)
+    psi = dev(dmrg.start_vector(sb0.dim, 8))
+    y0 = torch.empty_like(psi)
+    ctx.model_wave(sb0, psi, y0)
+    del sb0
+    ctx.conjugate_block(envb)
+    assert np.array_equal(envb.download(), np.conj(a0))
+    assert np.abs(np.imag(a0)).max() > 0.0, "the case must hold complex elements (flux)"
+    ctx.conjugate_block(envb)
+    assert np.array_equal(envb.download(), a0)
+    sb2 = ctx.bind(sysb, envb, m.n_elec)
+    y2 = torch.empty_like(psi)
+    ctx.model_wave(sb2, psi, y2)
+    assert torch.equal(y0, y2)

@@ -208,3 +208,88 @@ def test_truncation_file_in_the_reference_layout(ctx, tmp_path):
         with pytest.raises(Exception):
             ctx.trunc_from_disk(drv.block(side, l - 2), side, path)
     assert truncated, "the case must truncate"
+
+
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+
+READER = """program read_wave
+  implicit none
+  integer :: sl, el, num, dim, i, a, b, c, d, r, cc
+  complex(8), allocatable :: v(:, :)
+  complex(8) :: s
+  real(8) :: s2
+  open(11, file='%s', form='unformatted', status='old')
+  read(11) sl
+  read(11) el
+  read(11) num
+  read(11) dim
+  s = (0.0d0, 0.0d0)
+  s2 = 0.0d0
+  do i = 1, num
+    read(11)
+    read(11) a, b
+    read(11) c, d
+    read(11) r, cc
+    allocate(v(r, cc))
+    read(11) v
+    s = s + sum(v)
+    s2 = s2 + sum(abs(v) ** 2)
+    deallocate(v)
+  end do
+  close(11)
+  print '(I0, 1X, I0, 1X, I0, 1X, I0)', sl, el, num, dim
+  print '(3ES26.17)', real(s), aimag(s), s2
+end program read_wave
+"""
+
+
+def test_files_written_by_fortran_are_read_and_fortran_reads_ours(ctx, tmp_path):
+    """tests/golden/*_fortran.dat were written by a Fortran program compiled
+    with flang (tests/golden/make_fortran_files.py: the records of
+    wave_to_disk and block_to_disk, code/space_info.f90:1406-1436, :1470-1492):
+    hh_wave_from_disk and hh_trunc_from_disk read them to the last bit.  The
+    other way round, a Fortran reader with the records of wave_from_disk
+    (:1440-1468), compiled here with flang, reads a file hh_wave_to_disk
+    wrote and sees the same sector count, dimension and data."""
+    import os
+    import shutil
+    import subprocess
+    import torch
+    from oracle import dmrg
+    from .gpuhelp import dev, host
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    ctx.set_model(m)
+    d = Driver(ctx, kept_max=2000, tol=1e-9, kept_max_warm=2000, seed=3)
+    d.warmup()
+    sb = ctx.bind(d.block(0, 3), d.block(1, 3), m.n_elec)
+    psi = torch.zeros(sb.dim, dtype=torch.complex128, device="cuda")
+    ctx.wave_from_disk(sb, os.path.join(GOLD, "wave_hubbard4x2_fortran.dat"), psi)
+    k = np.arange(sb.dim, dtype=np.float64)
+    assert np.array_equal(host(psi), 1.0e-3 * k - 1j * (2.0e-3 * k))
+    t, length = ctx.trunc_from_disk(d.block(0, 2), 0, os.path.join(GOLD, "trun_hubbard4x2_fortran.dat"))
+    assert length == 3
+    mats = t.matrices()
+    assert len(mats) >= 2
+    for s, (_, U) in enumerate(mats):
+        i = np.arange(1, U.shape[0] + 1)[:, None]
+        j = np.arange(1, U.shape[1] + 1)[None, :]
+        assert np.array_equal(U, (0.1 * s + 1.0e-3 * i) - 1j * (2.0e-3 * j) + 0 * U)
+    # the other direction: a Fortran reader on a file this library wrote
+    flang = shutil.which("flang") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(flang):
+        pytest.skip("no Fortran compiler")
+    x = dmrg.start_vector(sb.dim, 21)
+    path = tmp_path / "ours.dat"
+    ctx.wave_to_disk(sb, dev(x), path)
+    (tmp_path / "r.f90").write_text(READER % str(path))
+    exe = tmp_path / "r"
+    rc = subprocess.run([flang, "-o", str(exe), str(tmp_path / "r.f90")], capture_output=True, text=True)
+    assert rc.returncode == 0, rc.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    l1, l2 = out.stdout.strip().splitlines()
+    assert [int(v) for v in l1.split()] == [4, 4, len(sb.pairs()[0]), sb.dim]
+    re_, im_, s2 = (float(v) for v in l2.split())
+    assert abs(re_ - x.sum().real) <= 1e-12 * abs(x).sum() and abs(im_ - x.sum().imag) <= 1e-12 * abs(x).sum()
+    assert abs(s2 - np.sum(np.abs(x) ** 2)) <= 1e-12 * np.sum(np.abs(x) ** 2)
