@@ -51,11 +51,14 @@ def run_leg(d, start, args, n):
     the left end, and right again to `start`: 2 (Nsite - 3) steps)."""
     if not args.sweep:
         return d.sweep(0, start, start + args.steps, args.tol, TRUN_ERR, args.m, fresh=(args.warmup == 0), phase=2)
-    recs = d.sweep(0, start, n - 2, args.tol, TRUN_ERR, args.m, fresh=(args.warmup == 0), phase=2)
-    d.reset_transform()
-    recs += d.sweep(1, n - 2, 1, args.tol, TRUN_ERR, args.m, phase=2)
-    d.reset_transform()
-    recs += d.sweep(0, 1, start, args.tol, TRUN_ERR, args.m, phase=2)
+    t0 = time.time()
+    recs = []
+    for k, (dr, a, b) in enumerate(((0, start, n - 2), (1, n - 2, 1), (0, 1, start))):
+        if k:
+            d.reset_transform()
+        recs += d.sweep(dr, a, b, args.tol, TRUN_ERR, args.m, fresh=(k == 0 and args.warmup == 0), phase=2)
+        print("# sweep leg %d -> %d done: %d steps so far, %.1f s" % (a, b, len(recs), time.time() - t0),
+              file=sys.stderr, flush=True)
     return recs
 
 
@@ -191,19 +194,54 @@ def run_c1(args):
     drv.reset_transform()
     recs += drv.sweep(1, n - 2, 1, wl.tol * 15, TRUN_ERR * 15, wl.kept_max, phase=1)
     t_half = time.time() - t1
-    sweep_s = []
-    for k in range(wl.n_sweeps):
-        t1 = time.time()
-        drv.reset_transform()
-        recs += drv.sweep(0, 1, n - 2, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
-        drv.reset_transform()
-        recs += drv.sweep(1, n - 2, 1, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
-        torch.cuda.synchronize()
-        sweep_s.append(round(time.time() - t1, 3))
-        print("# c1 sweep %d: %.3f s, lowest E %.10f" % (k + 1, sweep_s[-1],
-              min(r.energy for r in recs if r.phase == k + 2)), file=sys.stderr, flush=True)
+    forks = {}
+    if not args.no_chosen:
+        # the chosen forms start from device copies of this state; their
+        # preconditioner is trained on triples from one untimed Davidson sweep
+        # of a copy
+        tr = drv.fork()
+        tr.capture(16)
+        tr.set_solver(2)
+        tr.set_precond(1)
+        tr.reset_transform()
+        tr.sweep(0, 1, n - 2, wl.tol, TRUN_ERR, wl.kept_max, phase=1)
+        tr.train_precond(epochs=100, lr=1e-3)
+        for name, mode in (("chosen", 0), ("chosen_no_network", 1)):
+            f = drv.fork()
+            f.set_solver(2)
+            f.set_precond(mode)
+            forks[name] = f
+        forks["chosen"].copy_precond(tr)
+        del tr
+
+    def sweeps(d, tag, out):
+        times = []
+        for k in range(wl.n_sweeps):
+            t1 = time.time()
+            d.reset_transform()
+            out += d.sweep(0, 1, n - 2, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
+            d.reset_transform()
+            out += d.sweep(1, n - 2, 1, wl.tol, TRUN_ERR, wl.kept_max, phase=k + 2)
+            torch.cuda.synchronize()
+            times.append(round(time.time() - t1, 3))
+            print("# c1 %s sweep %d: %.3f s, lowest E %.10f" % (tag, k + 1, times[-1],
+                  min(r.energy for r in out if r.phase == k + 2)), file=sys.stderr, flush=True)
+        return times
+
+    sweep_s = sweeps(drv, "plain", recs)
     torch.cuda.synchronize()
     total = time.time() - t_all
+    forms = {}
+    for name, f in forks.items():
+        rf = []
+        ts = sweeps(f, name, rf)
+        last_f = [r for r in rf if r.phase == wl.n_sweeps + 1]
+        forms[name] = {"sweep_seconds": ts, "speedup_vs_plain": round(sum(sweep_s) / max(sum(ts), 1e-9), 3),
+                       "energy": min(r.energy for r in last_f),
+                       "hpsi_fp64": sum(r.hpsi_fp64 for r in rf), "hpsi_fp32": sum(r.hpsi_fp32 for r in rf),
+                       "fallbacks": sum(1 for r in rf if r.fallback),
+                       "steps_not_converged": sum(1 for r in rf if not r.converged)}
+    forks.clear()
     mv_time = sum(r.matvec_seconds for r in recs)
     mv_flops = sum(r.matvec_flops for r in recs)
     last = [r for r in recs if r.phase == wl.n_sweeps + 1]
@@ -224,7 +262,9 @@ def run_c1(args):
                                                                   for k in range(wl.n_sweeps)],
                    "steps_at_iteration_cap": sum(1 for r in recs if not r.converged),
                    "max_trunc_err_last_sweep": max(r.trunc_err for r in last),
-                   "matvec_share": round(mv_time / max(total, 1e-30), 3)},
+                   "hpsi_fp64": sum(r.hpsi_fp64 for r in recs if r.phase >= 2),
+                   "matvec_share": round(mv_time / max(total, 1e-30), 3),
+                   "forms": forms},
         "roofline": None,
         "cpu_baseline": cpu,
     }

@@ -18,7 +18,8 @@ P = C.POINTER
 
 class ModelParams(C.Structure):
     _fields_ = [("kind", i32), ("Lx", i32), ("Ly", i32), ("lattice", i32), ("periodic_y", i32),
-                ("n_elec", i32), ("t", f64), ("U", f64), ("J", f64), ("flux", f64)]
+                ("n_elec", i32), ("t", f64), ("U", f64), ("J", f64), ("flux", f64), ("gauge", i32),
+                ("reserved", i32), ("theta", f64)]
 
 
 class Schedule(C.Structure):
@@ -104,6 +105,8 @@ SIGNATURES = [
     ("hh_driver_capture", i32, [vp, i32]),
     ("hh_driver_train_precond", i32, [vp, i32, f64, P(f64), P(i32)]),
     ("hh_driver_set_precond", i32, [vp, i32]),
+    ("hh_driver_copy_precond", i32, [vp, vp]),
+    ("hh_driver_set_min_dim", i32, [vp, i64]),
     ("hh_driver_set_fallback", i32, [vp, f64]),
     ("hh_precond_eval", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), P(f64), P(f64), P(f64), f64,
                               P(C.c_float), P(C.c_float), P(C.c_float), P(f64), P(f64), P(C.c_float)]),
@@ -182,7 +185,11 @@ def ip(a):
 
 
 def model_params(m):
-    """ModelParams from an object with kind/Lx/Ly/lattice/periodic_y/n_elec/t/U/J/flux."""
+    """ModelParams from an object with kind/Lx/Ly/lattice/periodic_y/n_elec/t/U/J/flux (and optionally
+    gauge "landau" | "reference" and theta)."""
     kind = {"hubbard": 0, "tj": 1}.get(m.kind, m.kind)
     lattice = {"square": 0, "triangular": 1}.get(m.lattice, m.lattice)
-    return ModelParams(kind, m.Lx, m.Ly, lattice, int(m.periodic_y), m.n_elec, m.t, m.U, m.J, m.flux)
+    g = getattr(m, "gauge", "landau")
+    gauge = {"landau": 0, "reference": 1}.get(g, g)
+    return ModelParams(kind, m.Lx, m.Ly, lattice, int(m.periodic_y), m.n_elec, m.t, m.U, m.J, m.flux, gauge, 0,
+                       float(getattr(m, "theta", 0.0)))

@@ -63,7 +63,8 @@ double x12(int b2p, int b2, int s2p, int s2, int J2, int k2) {
 }
 
 // ============================================================================
-// lattice (Get_Lattice / dmrg_setup / get_phase restated, Landau gauge)
+// lattice (Get_Lattice / dmrg_setup / get_phase restated: the Landau gauge,
+// or get_phase's own gauge-1 pattern)
 // ============================================================================
 namespace {
 void lattice_bonds(const ModelParams& m, int ncol, std::map<std::pair<int, int>, Coupling>& out) {
@@ -76,6 +77,26 @@ void lattice_bonds(const ModelParams& m, int ncol, std::map<std::pair<int, int>,
     Coupling c{amp, tj ? m.J : 0.0, tj ? -0.25 * m.J : 0.0};
     out[{u, v}] = c;
   };
+  if (m.gauge == GAUGE_REFERENCE) {
+    // get_phase with gauge = 1 (code/quantum.f90:299-326): the factor of the
+    // bond from (x, y) to (x, y+1) is -(-1)^x, to (x+1, y) exp(i theta), to
+    // (x+1, y-1) (-1)^x; the neighbours are those of dmrg_setup's triangular
+    // branch (code/warm_up.f90:469-505)
+    for (int x = 1; x <= ncol; ++x) {
+      const double alt = (x % 2 == 0) ? 1.0 : -1.0;   // (-1)^x
+      for (int y = 1; y <= Ly; ++y) {
+        const int u = y + (x - 1) * Ly;
+        if (y < Ly) put(u, u + 1, cplx(-m.t * -alt, 0.0));
+        else if (Ly > 2 && m.periodic_y) put(u, 1 + (x - 1) * Ly, cplx(-m.t * -alt, 0.0));
+        if (x < ncol) {
+          put(u, u + Ly, -m.t * std::exp(cplx(0.0, m.theta)));
+          if (y > 1) put(u, u + Ly - 1, cplx(-m.t * alt, 0.0));
+          else if (Ly > 2 && m.periodic_y) put(u, u + 2 * Ly - 1, cplx(-m.t * alt, 0.0));
+        }
+      }
+    }
+    return;
+  }
   for (int x = 1; x <= ncol; ++x) {
     const cplx hy = -m.t * std::exp(cplx(0.0, tw * x));
     for (int y = 1; y <= Ly; ++y) {

@@ -32,6 +32,9 @@ double x12(int b2p, int b2, int s2p, int s2, int J2, int k2);
 // ----------------------------------------------------------------------------
 enum ModelKind { HUBBARD = 0, TJ = 1 };
 enum LatticeKind { SQUARE = 0, TRIANGULAR = 1 };
+// bond phases: the Landau gauge of `flux`, or get_phase's pattern with
+// gauge = 1 and x-bond angle `theta` (triangular lattice)
+enum GaugeKind { GAUGE_LANDAU = 0, GAUGE_REFERENCE = 1 };
 
 struct ModelParams {
   int kind = HUBBARD;
@@ -40,6 +43,8 @@ struct ModelParams {
   int periodic_y = 1;
   double t = 1.0, U = 8.0, J = 0.4, flux = 0.0;
   int n_elec = 8;
+  int gauge = GAUGE_LANDAU;
+  double theta = 0.0;
   int nsite() const { return Lx * Ly; }
 };
 

@@ -140,18 +140,18 @@ uint64_t mix64(uint64_t x) {
 
 }  // namespace
 
-void PcNet::init(hipStream_t s, uint64_t seed, double sigma0) {
+void PcNet::init(hipStream_t s, uint64_t seed) {
   const int n = MlpLayout::SIZE;
   std::vector<float> h(n, 0.f);
   auto uni = [&](uint64_t i) { return (double)(mix64(seed * 0x100000001B3ull + i) >> 11) * (1.0 / 9007199254740992.0); };
-  // Glorot-uniform hidden layers; a small output layer so the untrained
-  // network starts as the fixed diagonal preconditioner (sigma = sigma0 *
-  // scale, alpha = 1)
+  // Glorot-uniform hidden layers; a small output layer and zero output bias,
+  // so the untrained network starts at the fixed diagonal preconditioner
+  // (o = 0: sigma = PC_SIGMA0 * scale, alpha = 1)
   const double a1 = std::sqrt(6.0 / (PC_NF + PC_H)), a2 = std::sqrt(6.0 / (PC_H + PC_H)), a3 = 0.01;
   for (int i = 0; i < PC_H * PC_NF; ++i) h[MlpLayout::W1 + i] = (float)(a1 * (2.0 * uni(i) - 1.0));
   for (int i = 0; i < PC_H * PC_H; ++i) h[MlpLayout::W2 + i] = (float)(a2 * (2.0 * uni(100000 + i) - 1.0));
   for (int i = 0; i < PC_NO * PC_H; ++i) h[MlpLayout::W3 + i] = (float)(a3 * (2.0 * uni(200000 + i) - 1.0));
-  h[MlpLayout::B3] = (float)std::log(std::expm1(sigma0));   // softplus^-1(sigma0)
+  h[MlpLayout::B3] = 0.f;
   h[MlpLayout::B3 + 1] = 0.f;
   for (DevMem* b : {&w, &g, &m, &v}) b->alloc(sizeof(float) * n);
   loss.alloc(sizeof(double) * 4);

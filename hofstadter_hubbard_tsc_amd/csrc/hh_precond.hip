@@ -201,14 +201,13 @@ __global__ __launch_bounds__(PB) void k_dense(const float* __restrict__ X, const
   Y[g] = act ? tanhf(acc) : acc;
 }
 
-// sigma_p = scale * softplus(o_0), alpha_p = exp(clamp(o_1, -8, 8))
+// sigma_p = PC_SIGMA0 scale exp(PC_BOUND tanh(o_0)), alpha_p = exp(PC_BOUND tanh(o_1))
 __global__ void k_head(int np, const float* __restrict__ out, const double* __restrict__ glob, double* __restrict__ par) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double o0 = out[2 * p], o1 = out[2 * p + 1];
-  const double sp = o0 > 20.0 ? o0 : log1p(exp(o0));
-  par[2 * p] = glob[0] * sp;
-  par[2 * p + 1] = exp(fmin(fmax(o1, -8.0), 8.0));
+  par[2 * p] = PC_SIGMA0 * glob[0] * exp(PC_BOUND * tanh(o0));
+  par[2 * p + 1] = exp(PC_BOUND * tanh(o1));
 }
 
 __global__ void k_fixed(int np, double sigma0, const double* __restrict__ glob, double* __restrict__ par) {
@@ -273,7 +272,7 @@ __global__ __launch_bounds__(PB) void k_loss_sums(const int32_t* __restrict__ ch
 // (B |delta|^2), whatever the scale of t), and dL/d(out) per sector
 __global__ __launch_bounds__(PB) void k_loss_head(int np, const double* __restrict__ gout,
                                                   const double* __restrict__ par, const float* __restrict__ out,
-                                                  const double* __restrict__ glob, double dn,
+                                                  double dn,
                                                   float* __restrict__ dout, double* __restrict__ loss) {
   __shared__ double sh[PB];
   double A = 0.0, B = 0.0;
@@ -287,13 +286,13 @@ __global__ __launch_bounds__(PB) void k_loss_head(int np, const double* __restri
   if (threadIdx.x == 0) loss[0] = 1.0 - A * A / (B * dn);
   const double ka = 2.0 * A / (B * dn), kb = 2.0 * A * A / (B * B * dn);
   for (int p = threadIdx.x; p < np; p += PB) {
-    const double al = par[2 * p + 1];
+    const double sg = par[2 * p], al = par[2 * p + 1];
     const double gs = ka * gout[4 * p + 0] - kb * gout[4 * p + 1];
     const double ga = (-ka * gout[4 * p + 2] + kb * gout[4 * p + 3]) / al;
-    const double o0 = out[2 * p], o1 = out[2 * p + 1];
-    const double sgm = 1.0 / (1.0 + exp(-o0));
-    dout[2 * p] = (float)(gs * glob[0] * sgm);
-    dout[2 * p + 1] = (float)((o1 > -8.0 && o1 < 8.0) ? ga * al : 0.0);
+    const double t0 = tanh((double)out[2 * p]), t1 = tanh((double)out[2 * p + 1]);
+    // d sigma / d o_0 = sigma PC_BOUND (1 - tanh^2), likewise alpha
+    dout[2 * p] = (float)(gs * sg * PC_BOUND * (1.0 - t0 * t0));
+    dout[2 * p + 1] = (float)(ga * al * PC_BOUND * (1.0 - t1 * t1));
   }
 }
 
@@ -468,7 +467,7 @@ void pc_backward(PcDev& P, const double2* r, const double2* delta, double theta,
   hipLaunchKernelGGL(k_loss_sums, dim3(P.nchunk), dim3(PB), 0, s, P.ch_pair, P.ch_off, P.ch_cnt, P.d, P.par, r, delta,
                      theta, P.cstats);
   hipLaunchKernelGGL(k_chunk_reduce, dim3(nblk(B)), dim3(PB), 0, s, B, P.cfirst, P.cstats, P.gout, 0);
-  hipLaunchKernelGGL(k_loss_head, dim3(1), dim3(PB), 0, s, B, P.gout, P.par, P.out, P.glob, dn, P.dout, loss_dev);
+  hipLaunchKernelGGL(k_loss_head, dim3(1), dim3(PB), 0, s, B, P.gout, P.par, P.out, dn, P.dout, loss_dev);
   // layer 3 (linear): dW3, db3; dH2 through tanh
   hipLaunchKernelGGL(k_dense_back_w, dim3(nblk((int64_t)PC_NO * PC_H)), dim3(PB), 0, s, P.dout, P.h2,
                      g + MlpLayout::W3, g + MlpLayout::B3, B, PC_H, PC_NO);

@@ -13,7 +13,12 @@
 // (sigma_p, alpha_p) comes from a small MLP evaluated on features of block p
 // (diagonal statistics, residual share, quantum numbers):
 //     8 -> 256 (tanh) -> 256 (tanh) -> 2,
-//     sigma_p = scale * softplus(o_0),  alpha_p = exp(clamp(o_1, -8, 8)).
+//     sigma_p = PC_SIGMA0 * scale * exp(PC_BOUND * tanh(o_0)),
+//     alpha_p = exp(PC_BOUND * tanh(o_1)),
+// a bounded correction (a factor within [1/4, 4]) of the fixed diagonal form
+// sigma = PC_SIGMA0 * scale, alpha = 1, which the network gives at o = 0: a
+// sector unlike the training data cannot push the preconditioner further
+// than that from the fixed form.  scale is the spread of d.
 // The network is trained on (diagonal, residual, exact correction) triples:
 // the loss of one triple is 1 - Re<t, delta>^2 / (|t|^2 |delta|^2), the part
 // of the exact correction delta that one Davidson step along t misses (it
@@ -30,6 +35,10 @@ constexpr int PC_NF = 8;      // features per sector
 constexpr int PC_H = 256;     // hidden width (two hidden layers)
 constexpr int PC_NO = 2;      // outputs per sector: sigma, alpha (pre-activation)
 constexpr int PC_CHUNK = 32768;   // elements per reduction workgroup
+// shift of the fixed diagonal form, in units of the spread of d
+constexpr double PC_SIGMA0 = 0.1;
+// largest log-factor by which the network moves sigma_p and alpha_p (ln 4)
+constexpr double PC_BOUND = 1.3862943611198906;
 
 // parameter offsets in the flat FP32 parameter vector
 struct MlpLayout {

@@ -831,12 +831,14 @@ struct Driver {
   int capture_left = 0;
   int pc_mode = PC_LEARNED;
   double fall_factor = DAV_FALL_FACTOR;
+  int64_t min_dim = DAV_MIN_DIM;
   DevMem cap_u, cap_r;
 
   // the step's ground-state solve: the plain form (Lanczos with FP64 H.psi),
   // the mixed-precision Lanczos rounds, or the chosen Davidson form
   SolveOut solve(hh_superblock* sb, double2* p, double lanzero) {
-    if (S.solver == 2) {
+    const int solver = sb->plan.sb.dim < min_dim ? 0 : S.solver;
+    if (solver == 2) {
       bool fb = false;
       const bool have_net = net && net->trained;
       const int mode = (pc_mode == PC_LEARNED && !have_net) ? PC_DIAGONAL : pc_mode;
@@ -878,7 +880,7 @@ struct Driver {
       }
       return so;
     }
-    if (S.solver == 1) return mixed_lanczos(ctx, sb, p, lanzero, S.max_iter, 4);
+    if (solver == 1) return mixed_lanczos(ctx, sb, p, lanzero, S.max_iter, 4);
     if (capture_left <= 0) return lanczos(ctx, sb, p, lanzero, S.max_iter);
     // plain solve that also records a training triple of the preconditioner:
     // the unit start vector, its FP64 residual and Rayleigh quotient
@@ -2452,6 +2454,33 @@ int hh_boundary_sites(hh_ctx* ctx, int length, int32_t* n, int32_t* sites) {
   });
 }
 
+static ModelParams to_model(const hh_model_params* p) {
+  ModelParams m;
+  m.kind = p->kind;
+  m.Lx = p->Lx;
+  m.Ly = p->Ly;
+  m.lattice = p->lattice;
+  m.periodic_y = p->periodic_y;
+  m.n_elec = p->n_elec;
+  m.t = p->t;
+  m.U = p->U;
+  m.J = p->J;
+  m.flux = p->flux;
+  m.gauge = p->gauge;
+  m.theta = p->theta;
+  return m;
+}
+
+static void check_gauge(const hh_model_params* p) {
+  need(p->gauge == GAUGE_LANDAU || p->gauge == GAUGE_REFERENCE, "gauge must be 0 (Landau) or 1 (get_phase)");
+  need(p->reserved == 0, "reserved field of hh_model_params must be 0");
+  if (p->gauge == GAUGE_REFERENCE) {
+    need(p->lattice == TRIANGULAR, "gauge 1 (get_phase) is defined on the triangular lattice only");
+    need(p->flux == 0.0, "gauge 1 takes its flux from theta; flux must be 0");
+    need(std::isfinite(p->theta), "theta must be finite");
+  }
+}
+
 int hh_set_model(hh_ctx* ctx, const hh_model_params* p) {
   return guard(ctx, [&] {
     need(p != nullptr, "null model");
@@ -2459,18 +2488,8 @@ int hh_set_model(hh_ctx* ctx, const hh_model_params* p) {
     need(p->Lx >= 2 && p->Ly >= 1, "lattice must have Lx >= 2, Ly >= 1");
     need(p->n_elec >= 0 && p->n_elec <= 2 * p->Lx * p->Ly && p->n_elec % 2 == 0, "n_elec must be even (S = 0)");
     need(p->lattice == SQUARE || p->lattice == TRIANGULAR, "lattice must be 0 (square) or 1 (triangular)");
-    ModelParams m;
-    m.kind = p->kind;
-    m.Lx = p->Lx;
-    m.Ly = p->Ly;
-    m.lattice = p->lattice;
-    m.periodic_y = p->periodic_y;
-    m.n_elec = p->n_elec;
-    m.t = p->t;
-    m.U = p->U;
-    m.J = p->J;
-    m.flux = p->flux;
-    ctx->model = m;
+    check_gauge(p);
+    ctx->model = to_model(p);
     ctx->model_set = true;
   });
 }
@@ -3149,7 +3168,7 @@ int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_
 // ----------------------------------------------------------------------------
 namespace {
 constexpr char CK_MAGIC[8] = {'H', 'H', 'D', 'M', 'R', 'G', 'C', 'K'};
-constexpr uint32_t CK_VERSION = 1;
+constexpr uint32_t CK_VERSION = 2;
 
 struct CkOut {
   FILE* f;
@@ -3219,6 +3238,8 @@ void put_model(CkOut& o, const ModelParams& m) {
   o.put<double>(m.U);
   o.put<double>(m.J);
   o.put<double>(m.flux);
+  o.put<int32_t>(m.gauge);
+  o.put<double>(m.theta);
 }
 
 void save_driver(Driver& d, const char* path) {
@@ -3282,8 +3303,11 @@ void load_driver(Driver& d, const char* path) {
   const int kind = in.get<int32_t>(), Lx = in.get<int32_t>(), Ly = in.get<int32_t>(), lattice = in.get<int32_t>();
   const int periodic = in.get<int32_t>(), n_elec = in.get<int32_t>();
   const double t = in.get<double>(), U = in.get<double>(), J = in.get<double>(), flux = in.get<double>();
+  const int gauge = in.get<int32_t>();
+  const double theta = in.get<double>();
   need(kind == m.kind && Lx == m.Lx && Ly == m.Ly && lattice == m.lattice && periodic == (m.periodic_y ? 1 : 0) &&
-           n_elec == m.n_elec && t == m.t && U == m.U && J == m.J && flux == m.flux,
+           n_elec == m.n_elec && t == m.t && U == m.U && J == m.J && flux == m.flux && gauge == m.gauge &&
+           theta == m.theta,
        "the checkpoint was written for another model than the context's");
   Driver fresh;   // build the whole state aside; the driver is replaced only on success
   fresh.ctx = ctx;
@@ -3428,7 +3452,7 @@ int hh_driver_train_precond(hh_driver* h, int epochs, double lr, double* losses,
     need(std::isfinite(lr) && lr > 0.0, "learning rate must be positive");
     if (!d.net) {
       d.net.reset(new PcNet());
-      d.net->init(d.ctx->stream, d.S.seed, PC_SIGMA0);
+      d.net->init(d.ctx->stream, d.S.seed);
     }
     std::vector<double> ls;
     pc_train(d.ctx, *d.net, d.samples, epochs, lr, &ls);
@@ -3451,6 +3475,39 @@ int hh_driver_set_fallback(hh_driver* h, double factor) {
   return guard(h->d.ctx, [&] {
     need(std::isfinite(factor) && factor > 0.0, "the fallback factor must be positive");
     h->d.fall_factor = factor;
+  });
+}
+
+// dst's network becomes a device copy of src's (parameters and Adam state)
+static void copy_net(hh_ctx* ctx, Driver& dst, const Driver& src) {
+  dst.net.reset(new PcNet());
+  for (DevMem* m : {&dst.net->w, &dst.net->g, &dst.net->m, &dst.net->v}) m->alloc(sizeof(float) * MlpLayout::SIZE);
+  dst.net->loss.alloc(sizeof(double) * 4);
+  for (int k = 0; k < 3; ++k) {
+    const DevMem& f = k == 0 ? src.net->w : k == 1 ? src.net->m : src.net->v;
+    DevMem& t = k == 0 ? dst.net->w : k == 1 ? dst.net->m : dst.net->v;
+    check_hip(hipMemcpyAsync(t.p, f.p, f.bytes, hipMemcpyDeviceToDevice, ctx->stream), "d2d");
+  }
+  dst.net->adam_t = src.net->adam_t;
+  dst.net->trained = src.net->trained;
+}
+
+int hh_driver_set_min_dim(hh_driver* h, int64_t min_dim) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(min_dim >= 0, "min_dim must be >= 0");
+    h->d.min_dim = min_dim;
+  });
+}
+
+int hh_driver_copy_precond(hh_driver* dst, const hh_driver* src) {
+  if (!dst || !src) return 1;
+  return guard(dst->d.ctx, [&] {
+    need(dst->d.ctx == src->d.ctx, "both drivers must belong to one context");
+    need(src->d.net != nullptr, "the source driver has no network (train it first)");
+    if (dst == src) return;
+    copy_net(dst->d.ctx, dst->d, src->d);
+    check_hip(hipStreamSynchronize(dst->d.ctx->stream), "copy_precond");
   });
 }
 
@@ -3498,16 +3555,8 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out) {
     copy_dev(d.psi, s.psi);
     d.pc_mode = s.pc_mode;
     d.fall_factor = s.fall_factor;
-    if (s.net) {
-      d.net.reset(new PcNet());
-      for (DevMem* m : {&d.net->w, &d.net->g, &d.net->m, &d.net->v}) m->alloc(sizeof(float) * MlpLayout::SIZE);
-      d.net->loss.alloc(sizeof(double) * 4);
-      copy_dev(d.net->w, s.net->w);
-      copy_dev(d.net->m, s.net->m);
-      copy_dev(d.net->v, s.net->v);
-      d.net->adam_t = s.net->adam_t;
-      d.net->trained = s.net->trained;
-    }
+    d.min_dim = s.min_dim;
+    if (s.net) copy_net(ctx, d, s);
     if (s.prev_sb) {
       // the previous step's superblock, bound again to the copied blocks
       d.prev_sb.reset(bind(ctx, d.sysb.at(s.prev_sb->sys->d.length).get(), d.envb.at(s.prev_sb->env->d.length).get(),
@@ -3736,20 +3785,6 @@ struct hh_plan {
 };
 
 namespace {
-ModelParams to_model(const hh_model_params* p) {
-  ModelParams m;
-  m.kind = p->kind;
-  m.Lx = p->Lx;
-  m.Ly = p->Ly;
-  m.lattice = p->lattice;
-  m.periodic_y = p->periodic_y;
-  m.n_elec = p->n_elec;
-  m.t = p->t;
-  m.U = p->U;
-  m.J = p->J;
-  m.flux = p->flux;
-  return m;
-}
 BlockDesc make_desc(const ModelParams& m, int side, int len, int nsec, const int32_t* N, const int32_t* S2,
                     const int32_t* dim, int nsites, const int32_t* sites) {
   BlockDesc d;
@@ -3769,6 +3804,8 @@ int hh_plan_matvec(const hh_model_params* mp, int sys_len, int sys_nsec, const i
                    const int32_t* env_N, const int32_t* env_S2, const int32_t* env_dim, int env_nsites,
                    const int32_t* env_sites, int N, hh_plan** out) {
   try {
+    need(mp != nullptr, "null model");
+    check_gauge(mp);
     ModelParams m = to_model(mp);
     auto* p = new hh_plan();
     p->blk[0] = make_desc(m, 0, sys_len, sys_nsec, sys_N, sys_S2, sys_dim, sys_nsites, sys_sites);
@@ -3788,6 +3825,8 @@ int hh_plan_grow(const hh_model_params* mp, int side, int length, int nsec, cons
                  const int32_t* kept_a, const int32_t* kept_n, const int64_t* u_off, const int32_t* u_ld,
                  hh_plan** out) {
   try {
+    need(mp != nullptr, "null model");
+    check_gauge(mp);
     ModelParams m = to_model(mp);
     auto* p = new hh_plan();
     p->is_grow = true;

@@ -19,7 +19,7 @@ struct PcNet {
   DevMem w, g, m, v, loss;
   int adam_t = 0;
   bool trained = false;
-  void init(hipStream_t s, uint64_t seed, double sigma0);
+  void init(hipStream_t s, uint64_t seed);
 };
 
 // One training triple: a superblock's sector table and diagonal estimate,
@@ -32,10 +32,6 @@ struct PcSample {
   double theta = 0.0, dn = 0.0;
   int n_mid = 0;
 };
-
-// Shift of the untrained diagonal preconditioner and the network's starting
-// point, in units of the spread of the diagonal estimate.
-constexpr double PC_SIGMA0 = 0.1;
 
 // The diagonal estimate and sector table of a bound superblock on the device.
 void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P);
@@ -54,6 +50,11 @@ constexpr int DAV_KMAX = 32;
 // fall_factor: the FP64 residual must fall below fall_factor times its value
 // at the previous restart, or the mixed Lanczos takes over.
 constexpr double DAV_FALL_FACTOR = 0.9;
+// Superblocks below this dimension are solved by the plain Lanczos under
+// solvers 1 and 2: there the H.psi is a small part of the step and the
+// Davidson's extra host round trips per expansion cost more than its fewer
+// H.psi save.
+constexpr int64_t DAV_MIN_DIM = 262144;
 SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_hpsi, int mode,
                   const float* net, bool* fell_back, const AnchorHook& on_anchor = AnchorHook(),
                   double fall_factor = DAV_FALL_FACTOR);

@@ -604,6 +604,15 @@ class Driver:
         """0 the trained network, 1 the fixed diagonal form, 2 none (t = r)."""
         self.ctx._rc(L.load().hh_driver_set_precond(self.ptr, int(mode)), "hh_driver_set_precond")
 
+    def set_min_dim(self, min_dim):
+        """Solvers 1 and 2 hand superblocks below min_dim to the plain Lanczos
+        (default 262144; 0 = never)."""
+        self.ctx._rc(L.load().hh_driver_set_min_dim(self.ptr, int(min_dim)), "hh_driver_set_min_dim")
+
+    def copy_precond(self, src):
+        """Take a device copy of src's trained network (same context)."""
+        self.ctx._rc(L.load().hh_driver_copy_precond(self.ptr, src.ptr), "hh_driver_copy_precond")
+
     def set_fallback(self, factor):
         """Solver 2 hands the solve to the mixed Lanczos unless the FP64
         residual falls below factor x its value at the previous restart."""

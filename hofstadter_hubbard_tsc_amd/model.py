@@ -3,7 +3,9 @@
 `Model` carries what `hh_set_model` takes (include/hhdmrg.h
 `hh_model_params`): the couplings and lattice of program Quantum
 (code/quantum.f90:93-160) and dmrg_setup (code/warm_up.f90:347-666), with
-the flux in the Landau gauge (x bonds real, y bonds t exp(i 2 pi flux x)).
+the flux in the Landau gauge (x bonds real, y bonds t exp(i 2 pi flux x)) or,
+on the triangular lattice, in get_phase's own gauge-1 pattern
+(code/quantum.f90:299-326) with x-bond angle theta.
 
 `WORKLOADS` names the configurations the benchmark and the parity tests run,
 each with its sweep schedule (Get_Ground_State, code/quantum.f90:179-288):
@@ -32,6 +34,8 @@ class Model:
     J: float = 0.4
     flux: float = 0.0            # flux quanta per plaquette (Landau gauge)
     n_elec: int = 8
+    gauge: str = "landau"        # "landau" | "reference" (get_phase, gauge 1; triangular)
+    theta: float = 0.0           # "reference": phase of the x bonds (the reference runs pi/2)
 
     @property
     def nsite(self):

@@ -33,8 +33,15 @@ typedef struct hh_batch hh_batch;
 
 /* Model and lattice: replaces the hard-coded couplings and lattice setup of
  * program Quantum (code/quantum.f90:93-160), dmrg_setup
- * (code/warm_up.f90:347-666) and get_phase (code/quantum.f90:293-438; here the
- * Landau gauge x bonds real, y bonds t*exp(i 2 pi flux x)). */
+ * (code/warm_up.f90:347-666) and get_phase (code/quantum.f90:293-438).
+ * Bond phases: gauge 0 is the Landau gauge (x bonds real, y bonds
+ * t*exp(i 2 pi flux x), the triangular diagonal its straight-line phase);
+ * gauge 1 is get_phase's own pattern with gauge = 1 (code/quantum.f90:299-326)
+ * on the triangular lattice: for the bond from (x, y) to its neighbour j the
+ * factor p multiplying t in the amplitude of c^dag_(x,y) c_j is
+ * -(-1)^x on (x, y + 1), exp(i theta) on (x + 1, y) and (-1)^x on
+ * (x + 1, y - 1) (y wraps when periodic); the reference runs theta = pi/2,
+ * -pi/2 of flux in every triangle.  flux must be 0 with gauge 1. */
 typedef struct {
   int32_t kind;        /* 0 Hubbard, 1 t-J */
   int32_t Lx, Ly;
@@ -42,6 +49,9 @@ typedef struct {
   int32_t periodic_y;
   int32_t n_elec;
   double t, U, J, flux;
+  int32_t gauge;       /* 0 Landau, 1 get_phase gauge 1 (triangular only) */
+  int32_t reserved;    /* must be 0 */
+  double theta;        /* gauge 1: phase of the x bonds */
 } hh_model_params;
 
 /* Sweep schedule of Get_Ground_State (code/quantum.f90:179-288). */
@@ -310,7 +320,9 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out);
  * (element i) is t_i = alpha_p r_i / (max(d_i - theta, 0) + sigma_p), with r
  * the residual, theta the Ritz value and d the diagonal of the sys and env
  * block Hamiltonians in the enlarged bases; (sigma_p, alpha_p) come from an
- * MLP (8 -> 256 -> 256 -> 2, tanh, FP32) on features of block p.  Training
+ * MLP (8 -> 256 -> 256 -> 2, tanh, FP32) on features of block p, as a
+ * correction of the fixed diagonal form (sigma = 0.1 x the spread of d,
+ * alpha = 1) by factors within [1/4, 4].  Training
  * triples (diagonal, residual of the step's start vector, exact correction
  * toward the step's ground state) are captured from plain solves.
  *
@@ -320,19 +332,28 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out);
  * solved by solver 2 up to three of its FP64 restart points.
  * hh_driver_train_precond: epochs passes of Adam (learning rate lr) over the
  * captured triples with the library's own HIP forward and backward kernels;
- * losses (epochs doubles, may be null) receives the mean loss |t - delta|^2 /
- * |delta|^2 per epoch, *nsamples the number of triples.
+ * losses (epochs doubles, may be null) receives the mean loss
+ * 1 - Re<t, delta>^2 / (|t|^2 |delta|^2) per epoch, *nsamples the number of
+ * triples.
+ * hh_driver_copy_precond: dst takes a copy of src's network (parameters and
+ * Adam state; both drivers on one context), so one trained network serves
+ * several drivers.
  * hh_driver_set_precond: 0 the trained network (the fixed diagonal form while
  * none is trained), 1 the fixed diagonal form (sigma = 0.1 x the spread of
  * d, alpha = 1), 2 no preconditioner (t = r). */
 int hh_driver_capture(hh_driver* d, int max_samples);
 int hh_driver_train_precond(hh_driver* d, int epochs, double lr, double* losses, int32_t* nsamples);
 int hh_driver_set_precond(hh_driver* d, int mode);
+int hh_driver_copy_precond(hh_driver* dst, const hh_driver* src);
 /* Fallback rule of solver 2: at every FP64 restart the exact residual must
  * be below factor times its value at the previous restart (default 0.9), or
  * the mixed-precision Lanczos (solver 1) finishes the solve from the current
  * vector; the step record's fallback flag is then set. */
 int hh_driver_set_fallback(hh_driver* d, double factor);
+/* Solvers 1 and 2 hand superblocks of dimension below min_dim to the plain
+ * Lanczos (solver 0); default 262144, 0 = never.  Below that size the H.psi
+ * is a small part of the step and the plain form is the faster one. */
+int hh_driver_set_min_dim(hh_driver* d, int64_t min_dim);
 /* The preconditioner's whole training pass on caller data, for checking the
  * kernels: np sector blocks (rows x cols, 2S and N of the sys sector), the
  * diagonal d(a, b) = hs[a] + he[b] per block (hs, he concatenated over the

@@ -17,7 +17,8 @@ c_(x,y+1) (x is the 1-based column); on the triangular lattice the diagonal
 from (x+1,y-1) to (x,y) carries the same gauge's straight-line phase,
 t*exp(-i 2 pi Phi (x + 1/2)) for c^dag_(x,y) c_(x+1,y-1), so every triangle
 holds Phi/2.  The reference hard-codes its own phase pattern on the
-triangular lattice instead (``get_phase``, code/quantum.f90:293-438); its
+triangular lattice instead (``get_phase``, code/quantum.f90:293-438), which
+gauge="reference" restates (``_reference_gauge_bonds``); its
 couplings are
 jt = -sqrt(2) t, jd = -sqrt(3) J, jn = -J/4 (code/quantum.f90:93-122), which
 are the reduced-tensor forms of the physical terms below.
@@ -40,6 +41,8 @@ class Model:
     J: float = 0.4
     flux: float = 0.0              # Phi per plaquette (Landau gauge)
     n_elec: int = 8
+    gauge: str = "landau"          # "landau" | "reference" (get_phase, gauge = 1; triangular)
+    theta: float = 0.0             # "reference": angle of the x bonds (the reference runs pi/2)
 
     @property
     def nsite(self):
@@ -60,6 +63,8 @@ def lattice_bonds(model, ncol):
     Ly = model.Ly
     out = []
     tw = 2.0 * math.pi * model.flux
+    if getattr(model, "gauge", "landau") == "reference":
+        return _reference_gauge_bonds(model, ncol)
 
     def hop_y(x):
         return -model.t * cmath.exp(1j * tw * x)
@@ -88,6 +93,41 @@ def lattice_bonds(model, ncol):
                     elif Ly > 2 and model.periodic_y:
                         out.append((u, u + 2 * Ly - 1, "diag", hop_d(x)))
     return [(min(a, b), max(a, b), k, amp if a < b else amp.conjugate())
+            for (a, b, k, amp) in out]
+
+
+def _reference_gauge_bonds(model, ncol):
+    """get_phase with gauge = 1 (code/quantum.f90:299-326) on the triangular
+    lattice of dmrg_setup (code/warm_up.f90:469-505).  For site i = (x, y)
+    (label y + (x-1) Ly) the reference sets pint(i, j) for its neighbours
+    k = 1: j = (x, y+1) (y wraps), pint = phasey1 (-1)^x with phasey1 = -1;
+    k = 3: j = (x+1, y), pint = exp(i theta);
+    k = 5: j = (x+1, y-1) (y wraps), pint = phasey1 (-1)^(x+1);
+    and pint(j, i) = conj(pint(i, j)).  The hopping amplitude of
+    c^dag_i c_j is -t pint(i, j): the reference's term coefficient is
+    jt pint(is, js) with jt = -sqrt(2) t the reduced form of -t
+    (code/mat_vec.f90:3524, code/get_ham.f90:62)."""
+    if model.lattice != "triangular":
+        raise ValueError("the reference gauge is defined on the triangular lattice")
+    Ly = model.Ly
+    out = []
+    for x in range(1, ncol + 1):
+        for y in range(1, Ly + 1):
+            i = y + (x - 1) * Ly
+            p_y = -1.0 * (-1) ** x
+            p_d = -1.0 * (-1) ** (x + 1)
+            p_x = cmath.exp(1j * model.theta)
+            if y < Ly:
+                out.append((i, i + 1, "y", -model.t * p_y))
+            elif Ly > 2 and model.periodic_y:
+                out.append((i, 1 + (x - 1) * Ly, "ywrap", -model.t * p_y))
+            if x < ncol:
+                out.append((i, i + Ly, "x", -model.t * p_x))
+                if y > 1:
+                    out.append((i, i + Ly - 1, "diag", -model.t * p_d))
+                elif Ly > 2 and model.periodic_y:
+                    out.append((i, i + 2 * Ly - 1, "diag", -model.t * p_d))
+    return [(min(a, b), max(a, b), k, complex(amp) if a < b else complex(amp).conjugate())
             for (a, b, k, amp) in out]
 
 

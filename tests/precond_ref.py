@@ -1,7 +1,8 @@
 """numpy (float64) restatement of the learned per-sector preconditioner's
 training pass (test helper): features of each sector block, the MLP
-8 -> 256 (tanh) -> 256 (tanh) -> 2, the head sigma = scale softplus(o0),
-alpha = exp(clamp(o1, -8, 8)), the preconditioner
+8 -> 256 (tanh) -> 256 (tanh) -> 2, the head sigma = SIGMA0 scale
+exp(BOUND tanh(o0)), alpha = exp(BOUND tanh(o1)) (a bounded correction of the
+fixed diagonal form sigma = SIGMA0 scale, alpha = 1), the preconditioner
 t = alpha r / (max(d - theta, 0) + sigma), the loss
 1 - Re<t, delta>^2 / (|t|^2 |delta|^2) and its gradient with respect to every network parameter.  The GPU kernels
 (hh_precond.hip) are checked against it; it is checked against central
@@ -9,6 +10,8 @@ finite differences (tests/test_precond_ref.py)."""
 import numpy as np
 
 NF, H, NO = 8, 256, 2
+SIGMA0 = 0.1
+BOUND = np.log(4.0)
 W1 = 0
 B1 = W1 + H * NF
 W2 = B1 + H
@@ -66,9 +69,7 @@ def forward(F, w):
 
 
 def head(out, sd):
-    o0, o1 = out[:, 0], out[:, 1]
-    sp = np.where(o0 > 20.0, o0, np.log1p(np.exp(np.minimum(o0, 20.0))))
-    return sd * sp, np.exp(np.clip(o1, -8.0, 8.0))
+    return SIGMA0 * sd * np.exp(BOUND * np.tanh(out[:, 0])), np.exp(BOUND * np.tanh(out[:, 1]))
 
 
 def loss_and_grad(rows, cols, s2, nq, n_mid, hs, he, r, delta, theta, w, bug=None):
@@ -100,9 +101,8 @@ def loss_and_grad(rows, cols, s2, nq, n_mid, hs, he, r, delta, theta, w, bug=Non
         dB_da = 2.0 * np.sum(np.abs(tp) ** 2) / al[p]
         gs = -2.0 * A * dA_ds / (B * dn) + A * A * dB_ds / (B * B * dn)
         ga = -2.0 * A * dA_da / (B * dn) + A * A * dB_da / (B * B * dn)
-        sgm = 1.0 / (1.0 + np.exp(-out[p, 0]))
-        dout[p, 0] = gs * sd * sgm
-        dout[p, 1] = ga * al[p] if -8.0 < out[p, 1] < 8.0 else 0.0
+        dout[p, 0] = gs * sig[p] * BOUND * (1.0 - np.tanh(out[p, 0]) ** 2)
+        dout[p, 1] = ga * al[p] * BOUND * (1.0 - np.tanh(out[p, 1]) ** 2)
     g = np.zeros(SIZE)
     gw3 = dout.T @ h2
     gb3 = dout.sum(0)

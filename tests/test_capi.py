@@ -42,6 +42,25 @@ def test_plan_rejects_nothing_silently():
     lib.hh_plan_destroy(p)
 
 
+@pytest.mark.parametrize("fields,msg", [
+    (dict(lattice=0, gauge=1, theta=1.0), b"triangular"),           # gauge 1 on the square lattice
+    (dict(lattice=1, gauge=1, theta=1.0, flux=0.1), b"flux"),        # gauge 1 with a Landau flux
+    (dict(lattice=1, gauge=2), b"gauge"),
+    (dict(lattice=1, gauge=0, reserved=3), b"reserved"),
+])
+def test_plan_refuses_bad_gauges(fields, msg):
+    lib = L.load()
+    mp = L.ModelParams(0, 2, 2, 0, 0, 2, 1.0, 8.0, 0.0, 0.0)
+    for k, v in fields.items():
+        setattr(mp, k, v)
+    one = (C.c_int32 * 1)(1)
+    z = (C.c_int32 * 1)(0)
+    p = C.c_void_p()
+    rc = lib.hh_plan_matvec(C.byref(mp), 1, 1, z, z, one, 1, one, 1, 1, z, z, one, 1, one, 0, C.byref(p))
+    assert rc != 0
+    assert msg in lib.hh_last_error(None)
+
+
 def test_ctx_create_without_device_reports_error():
     import torch
     if torch.cuda.is_available():

@@ -171,10 +171,17 @@ def test_wave_transformation_predicts_ground_state(ctx):
             assert abs(x.iters - y.iters) <= 2, (x.sys_len, x.iters, y.iters)
 
 
-def test_untruncated_triangular_flux_run_equals_ed(ctx):
-    """Triangular 3 x 3 cylinder with flux through the whole GPU driver: every
+@pytest.mark.parametrize("gauge", ["landau", "reference"])
+def test_untruncated_triangular_flux_run_equals_ed(ctx, gauge):
+    """Triangular 3 x 3 cylinder with flux through the whole GPU driver (the
+    Landau gauge, and get_phase's own gauge-1 pattern at theta = pi/2): every
     final-sweep energy equals the exact ground state."""
-    m = lat.Model(kind="hubbard", Lx=3, Ly=3, lattice="triangular", U=8.0, t=1.0, flux=1.0 / 6, n_elec=6)
+    import math
+    if gauge == "landau":
+        m = lat.Model(kind="hubbard", Lx=3, Ly=3, lattice="triangular", U=8.0, t=1.0, flux=1.0 / 6, n_elec=6)
+    else:
+        m = lat.Model(kind="hubbard", Lx=3, Ly=3, lattice="triangular", U=8.0, t=1.0, gauge="reference",
+                      theta=math.pi / 2, n_elec=6)
     ctx.set_model(m)
     e0 = _ed_gs(m)
     recs = ctx.dmrg_run(kept_max=4000, n_sweeps=1, tol=1e-12)

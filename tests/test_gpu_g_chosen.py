@@ -15,6 +15,9 @@
   (1e-12 relative, checked with hh_model_wave in FP64).
 * The fallback to the mixed Lanczos, forced from the host by an unreachable
   residual-reduction factor: flagged in the records, energies unchanged.
+* A network copied between drivers acts exactly like its source.
+* Below the hand-over size (hh_driver_set_min_dim) solver 2 is the plain
+  Lanczos, bit for bit.
 * Training: the loss falls; what the trained network gains over the method
   without it (total H.psi over the same steps) is printed and bounded.
 """
@@ -49,6 +52,7 @@ def _prepared(ctx, m, kept, capture=0):
     ctx.set_model(m)
     n = m.nsite
     d = Driver(ctx, kept_max=kept, tol=1e-11, kept_max_warm=kept, seed=17)
+    d.set_min_dim(0)   # these superblocks are far below the default hand-over size
     d.warmup()
     bg = n // 2 - 1
     if capture:
@@ -159,6 +163,43 @@ def test_forms_reach_the_plain_energy_and_report_exact_energies(ctx):
         assert abs(e - first["plain"]) <= 1e-9 * abs(first["plain"]), (name, e, first["plain"])
 
 
+def test_copied_network_acts_like_its_source(ctx):
+    """A driver forked before training takes the trained network by
+    hh_driver_copy_precond and then sweeps exactly like a fork made after
+    training (same energies, same H.psi counts)."""
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.05, n_elec=10)
+    d, start = _prepared(ctx, m, 32, capture=8)
+    early = d.fork()
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    with pytest.raises(HHError):
+        early.copy_precond(early)      # no network yet
+    d.train_precond(epochs=20, lr=1e-3)
+    late = d.fork()
+    early.copy_precond(d)
+    out = []
+    for f in (early, late):
+        f.set_solver(2)
+        f.set_precond(0)
+        out.append(f.sweep(0, start, start + 2, 1e-11, TRUN, 32, phase=2))
+    for a, b in zip(*out):
+        assert a.energy == b.energy and a.hpsi_fp64 == b.hpsi_fp64 and a.hpsi_fp32 == b.hpsi_fp32
+
+
+def test_small_superblocks_go_to_the_plain_solver(ctx):
+    """Below the hand-over size solver 2 runs the plain Lanczos: the same
+    energies, iteration counts and H.psi counts as solver 0, bit for bit."""
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.05, n_elec=10)
+    d, start = _prepared(ctx, m, 32)
+    a, b = d.fork(), d.fork()
+    a.set_solver(0)
+    b.set_solver(2)
+    b.set_min_dim(10 ** 9)
+    ra = a.sweep(0, start, start + 2, 1e-11, TRUN, 32, phase=2)
+    rb = b.sweep(0, start, start + 2, 1e-11, TRUN, 32, phase=2)
+    for x, y in zip(ra, rb):
+        assert x.energy == y.energy and x.iters == y.iters and x.hpsi_fp64 == y.hpsi_fp64 and y.hpsi_fp32 == 0
+
+
 def test_forced_fallback_is_flagged_and_correct(ctx):
     m = lat.Model(kind="tj", Lx=4, Ly=4, J=0.4, t=1.0, flux=0.125, n_elec=14)
     d, start = _prepared(ctx, m, 40)
@@ -185,6 +226,7 @@ def test_trained_network_gain_over_the_method_without_it(ctx):
     n = m.nsite
     kept = 120
     d = Driver(ctx, kept_max=kept, tol=1e-9, kept_max_warm=kept, seed=5)
+    d.set_min_dim(0)
     d.warmup()
     bg = n // 2 - 1
     d.capture(16)

@@ -78,3 +78,15 @@ def test_matvec_tasks_match_oracle_triangular_flux():
     run = _run_oracle(model, 40)
     for ns in range(1, model.nsite - 2):
         _check_step(model, run, ns)
+
+
+def test_matvec_tasks_match_oracle_reference_gauge():
+    """The same on a triangular cylinder in get_phase's own gauge-1 pattern
+    (theta = pi/2, code/quantum.f90:299-326): every bond's amplitude from the
+    product's lattice generator enters the tasks as the oracle's does."""
+    import math
+    model = lat.Model(kind="tj", Lx=3, Ly=4, lattice="triangular", J=0.4, t=1.0, gauge="reference",
+                      theta=math.pi / 2, n_elec=10)
+    run = _run_oracle(model, 40)
+    for ns in range(1, model.nsite - 2):
+        _check_step(model, run, ns)

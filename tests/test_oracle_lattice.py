@@ -1,9 +1,13 @@
 """The oracle's lattice: bond counts and the Landau-gauge flux through every
 elementary plaquette (square) and triangle (triangular), read off the hopping
-amplitudes as the phase an electron collects around the loop."""
+amplitudes as the phase an electron collects around the loop; the
+reference's own gauge-1 phase pattern on the triangular lattice by its
+triangle phases and, through exact diagonalisation, as a gauge of the Landau
+field."""
 import cmath
 import math
 
+import numpy as np
 import pytest
 
 from oracle import lattice as lat
@@ -72,3 +76,49 @@ def test_triangle_flux_is_half_the_plaquette(flux):
             upper = [_site(x + 1, y, m.Ly), _site(x + 1, y + 1, m.Ly), _site(x, y + 1, m.Ly)]
             assert abs(_loop_phase(bonds, lower) - want) < 1e-12, (x, y)
             assert abs(_loop_phase(bonds, upper) - want) < 1e-12, (x, y)
+
+
+@pytest.mark.parametrize("theta", [math.pi / 2, 0.3, 1.1])
+def test_reference_gauge_triangle_phases(theta):
+    """get_phase's gauge-1 pattern (code/quantum.f90:299-326): counterclockwise,
+    the upper triangle of every plaquette holds theta and the lower one
+    pi - theta, periodic wrap included; at the reference's theta = pi/2 the
+    field is uniform.  Every bond keeps |amp| = t, the bonds are those of the
+    triangular lattice, and the loop around the even cylinder collects no
+    phase."""
+    m = lat.Model(Lx=5, Ly=4, lattice="triangular", gauge="reference", theta=theta)
+    bonds = lat.lattice_bonds(m, m.Lx)
+    landau = lat.lattice_bonds(lat.Model(Lx=5, Ly=4, lattice="triangular", flux=0.2), m.Lx)
+    assert sorted((u, v) for u, v, _, _ in bonds) == sorted((u, v) for u, v, _, _ in landau)
+    assert all(abs(abs(a) - m.t) < 1e-14 for _, _, _, a in bonds)
+    for x in range(1, m.Lx):
+        for y in range(1, m.Ly + 1):
+            lower = [_site(x, y, m.Ly), _site(x + 1, y, m.Ly), _site(x, y + 1, m.Ly)]
+            upper = [_site(x + 1, y, m.Ly), _site(x + 1, y + 1, m.Ly), _site(x, y + 1, m.Ly)]
+            assert abs(cmath.exp(1j * _loop_phase(bonds, upper)) - cmath.exp(1j * theta)) < 1e-12, (x, y)
+            assert abs(cmath.exp(1j * _loop_phase(bonds, lower)) - cmath.exp(1j * (math.pi - theta))) < 1e-12
+    for x in range(1, m.Lx + 1):
+        ring = [_site(x, y, m.Ly) for y in range(1, m.Ly + 1)]
+        assert abs(cmath.exp(1j * _loop_phase(bonds, ring)) - 1.0) < 1e-12
+
+
+def _lowest_singlets(m, k=3):
+    from oracle import dmrg, ed
+    sg = dmrg.StepGeometry(m, m.nsite // 2 - 1, m.nsite - m.nsite // 2 - 1)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    return ed.singlet_spectrum(m, list(range(1, m.nsite + 1)), bonds, m.n_elec, k=k)
+
+
+def test_reference_gauge_is_a_gauge_of_the_landau_field():
+    """At theta = pi/2 every triangle holds pi/2 and the cylinder loop
+    nothing, as in the Landau gauge at Phi = -1/2 (the triangles hold
+    -pi Phi): the two Hamiltonians are related by a gauge transformation, so
+    their singlet spectra agree (exact diagonalisation of the triangular
+    Hubbard cylinder 3 x 4); a different flux (Phi = 1/4) does not."""
+    ref = lat.Model(kind="hubbard", Lx=3, Ly=4, lattice="triangular", gauge="reference", theta=math.pi / 2,
+                    U=8.0, t=1.0, n_elec=6)
+    lan = lat.Model(kind="hubbard", Lx=3, Ly=4, lattice="triangular", flux=-0.5, U=8.0, t=1.0, n_elec=6)
+    other = lat.Model(kind="hubbard", Lx=3, Ly=4, lattice="triangular", flux=0.25, U=8.0, t=1.0, n_elec=6)
+    a, b, c = _lowest_singlets(ref), _lowest_singlets(lan), _lowest_singlets(other)
+    assert np.allclose(a, b, rtol=0, atol=1e-9), (a, b)
+    assert np.abs(a - c).max() > 1e-3
