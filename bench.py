@@ -301,7 +301,9 @@ def c1_cpu_baseline(ctx, drv, model, budget_s):
 
 def run_c3(args):
     """Batched H.psi on 16 synthetic superblocks (one grouped launch per stage)
-    against the same 16 H.psi launched one superblock at a time."""
+    against the same 16 H.psi launched one superblock at a time; then the
+    members' ground states by the batched Lanczos against one Lanczos per
+    member."""
     import torch
     from hofstadter_hubbard_tsc_amd import Context, synth
     ctx = Context(0)
@@ -346,6 +348,48 @@ def run_c3(args):
             "avg_launch_ms": round(st["stage2_ms"] / max(st["matvecs"], 1), 4), "launches": int(st["matvecs"]),
             "stage1_ms_per_launch": st["stage1_ms"] / max(st["matvecs"], 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(st["matvecs"], 1)}
+    # the members' ground states: batched Lanczos (one grouped H.psi per
+    # iteration, graph replay and kernel-by-kernel launches) against each
+    # member's own Lanczos, from the same start vectors
+    lz = None
+    if not args.no_lanczos:
+        import numpy as np
+        tol, cap = args.tol, 100
+        starts = [m[3].clone() for m in members]
+        times = {}
+        res = {}
+        for graph in (True, False):
+            batch.set_graph(graph)
+            psis = [p.clone() for p in starts]
+            torch.cuda.synchronize()
+            t1 = time.time()
+            res[graph] = ctx.batch_lanczos(batch, psis, tol, cap)
+            torch.cuda.synchronize()
+            times[graph] = time.time() - t1
+            print("# c3 batched Lanczos (graph %s): %.3f s, iterations %s" % (graph, times[graph],
+                  res[graph][1].tolist()), file=sys.stderr, flush=True)
+        single_e, single_it = [], []
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for m, p0 in zip(members, starts):
+            p = p0.clone()
+            e, it = ctx.lanczos(m[2], p, tol, cap)
+            single_e.append(e)
+            single_it.append(it)
+        torch.cuda.synchronize()
+        t_single = time.time() - t1
+        e_b = res[True][0]
+        lz = {"tol": tol, "max_iter": cap,
+              "batched_seconds": round(times[True], 4), "batched_no_graph_seconds": round(times[False], 4),
+              "one_at_a_time_seconds": round(t_single, 4),
+              "speedup_vs_one_at_a_time": round(t_single / times[True], 3),
+              "graph_gain": round(times[False] / times[True], 4),
+              "iterations_batched": res[True][1].tolist(), "iterations_alone": single_it,
+              "converged": int(res[True][2].sum()),
+              "max_rel_energy_diff_vs_alone": float(max(abs(a - b) / abs(b) for a, b in zip(e_b, single_e))),
+              "graph_equals_launches": bool(np.array_equal(res[True][0], res[False][0]))}
+        print("# c3 one Lanczos at a time: %.3f s; batched %.2fx" % (t_single, t_single / times[True]),
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         from tests.gpuhelp import oracle_superblock
@@ -380,7 +424,8 @@ def run_c3(args):
                    "flops_per_call": info["flops_alg"],
                    "one_at_a_time_ms": round(1e3 * t_seq, 3),
                    "one_at_a_time_gflops": round(info["flops_alg"] / t_seq / 1e9, 2),
-                   "batched_speedup": round(t_seq / t_batch, 3)},
+                   "batched_speedup": round(t_seq / t_batch, 3),
+                   "lanczos": lz},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -404,6 +449,7 @@ def main():
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
                          "schedule at m = 256; c3: batched H.psi micro-benchmark")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
+    ap.add_argument("--no-lanczos", action="store_true", help="c3: time the batched H.psi only")
     ap.add_argument("--measure", action="store_true",
                     help="c2: after the timed steps, time the measurement entry points on the last superblock "
                          "(reported under config.measure, outside the timed region)")

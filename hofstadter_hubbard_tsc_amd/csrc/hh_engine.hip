@@ -336,6 +336,10 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   if (wi >= nwork) return;
   const WorkItem w = work[wi];
   const Tile tile = tiles[w.tile];
+  if constexpr (std::is_same<Bufs, BufTable>::value) {
+    // a finished batch member's tiles are skipped (wave-uniform)
+    if (bufs.active && bufs.active[tile.c_buf >> 3] == 0) return;
+  }
   const int r0 = w.r0, c0 = w.c0;
   const int Mv = min(16 * WA, tile.M - r0), Nv = min(16 * WB, tile.N - c0);
   const int li = lane & 15, lk = lane >> 4;
@@ -565,6 +569,9 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
                                                 Bufs bufs, const double2* __restrict__ partial) {
   const ReduceItem ri = red[blockIdx.x];
   const Tile tile = tiles[ri.tile];
+  if constexpr (std::is_same<Bufs, BufTable>::value) {
+    if (bufs.active && bufs.active[tile.c_buf >> 3] == 0) return;
+  }
   const int Mv = min(MTR, tile.M - ri.r0), Nv = min(MTC, tile.N - ri.c0);
   typename StoreOf<Bufs>::type* __restrict__ C = bufs.p[tile.c_buf];
   for (int e = threadIdx.x; e < MTR * MTC; e += 256) {
@@ -1006,6 +1013,209 @@ void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStre
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scale_copy, dim3(vec_blocks(n)), dim3(VB), 0, s, d, src, a, n);
 }
+// ----------------------------------------------------------------------------
+// segmented vector kernels: several independent vectors stored one after the
+// other, each starting at a multiple of VB * EPT elements, so that every
+// block lies inside one segment.  bseg[b] is the segment of block b, send[s]
+// the end (exclusive) of segment s's elements; padding past it is never read
+// or written.  Within a segment, threads, blocks and the reduction order are
+// those of the unsegmented kernels above, so a segment's sums equal those of
+// the same vector alone bit for bit.
+// ----------------------------------------------------------------------------
+int vec_span() { return VB * EPT; }
+
+__global__ __launch_bounds__(VB) void k_smdot_part(const double2* __restrict__ V, int64_t stride, int nv,
+                                                   const double2* __restrict__ w, const int32_t* __restrict__ bseg,
+                                                   const int64_t* __restrict__ send,
+                                                   const int32_t* __restrict__ active, double* __restrict__ partial) {
+  __shared__ double red[MDOT_J][VB / 64];
+  if (active && active[bseg[blockIdx.x]] == 0) {   // block-uniform
+    for (int j = threadIdx.x; j < nv; j += VB) partial[(int64_t)j * gridDim.x + blockIdx.x] = 0.0;
+    return;
+  }
+  const int64_t n = send[bseg[blockIdx.x]];
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double2 x[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    x[q] = i < n ? w[i] : make_double2(0.0, 0.0);
+  }
+  for (int j0 = 0; j0 < nv; j0 += MDOT_J) {
+    const int jn = min(MDOT_J, nv - j0);
+    for (int jj = 0; jj < jn; ++jj) {
+      const double2* __restrict__ v = V + (int64_t)(j0 + jj) * stride;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int64_t i = base + (int64_t)VB * q;
+        if (i < n) {
+          const double2 a = v[i];
+          acc += a.x * x[q].x + a.y * x[q].y;
+        }
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) red[jj][wv] = acc;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < jn) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < VB / 64; ++k) t += red[threadIdx.x][k];
+      partial[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// out[s * nv + j] = sum over the blocks of segment s of partial[j * nb + b],
+// in the order of k_block_sum (one workgroup per (s, j))
+__global__ __launch_bounds__(VB) void k_seg_sum(const double* __restrict__ partial, int nb, int nv,
+                                                const int32_t* __restrict__ sfirst, double* __restrict__ out) {
+  __shared__ double red[VB / 64];
+  const int s = blockIdx.x / nv, j = blockIdx.x % nv;
+  const int b0 = sfirst[s], nblk = sfirst[s + 1] - b0;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += VB) acc += partial[(int64_t)j * nb + b0 + i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < VB / 64; ++k) t += red[k];
+    out[(int64_t)s * nv + j] = t;
+  }
+}
+
+// w = w - a_s v1 - b_s v2 (v2 may be null); partial |w|^2 per block
+__global__ __launch_bounds__(VB) void k_slupd_norm(double2* __restrict__ w, const double2* __restrict__ v1,
+                                                   const double* __restrict__ a, const double2* __restrict__ v2,
+                                                   const double* __restrict__ b, const int32_t* __restrict__ bseg,
+                                                   const int64_t* __restrict__ send,
+                                                   const int32_t* __restrict__ active, double* __restrict__ partial) {
+  const int sg = bseg[blockIdx.x];
+  if (active && active[sg] == 0) {   // block-uniform
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const int64_t n = send[sg];
+  const double as = a[sg], bs = v2 ? b[sg] : 0.0;
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      double2 x = w[i];
+      const double2 y = v1[i];
+      x.x -= as * y.x;
+      x.y -= as * y.y;
+      if (v2) {
+        const double2 z = v2[i];
+        x.x -= bs * z.x;
+        x.y -= bs * z.y;
+      }
+      w[i] = x;
+      acc += x.x * x.x + x.y * x.y;
+    }
+  }
+  block_norm(acc, partial);
+}
+
+// w = sw_s w + cs_s sum_j c[s * ldc + j] V_j; partial |w|^2 per block
+__global__ __launch_bounds__(VB) void k_scombine_norm(double2* __restrict__ w, const double* __restrict__ sw,
+                                                      const double2* __restrict__ V, int64_t stride, int nv,
+                                                      const double* __restrict__ c, int ldc,
+                                                      const double* __restrict__ cs,
+                                                      const int32_t* __restrict__ bseg,
+                                                      const int64_t* __restrict__ send, double* __restrict__ partial) {
+  const int sg = bseg[blockIdx.x];
+  const int64_t n = send[sg];
+  const double sws = sw[sg], css = cs[sg];
+  const double* __restrict__ cc = c + (int64_t)sg * ldc;
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+  double2 x[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    x[q] = make_double2(0.0, 0.0);
+    if (i < n && sws != 0.0) {
+      const double2 y = w[i];
+      x[q] = make_double2(sws * y.x, sws * y.y);
+    }
+  }
+  for (int j = 0; j < nv; ++j) {
+    const double cj = css * cc[j];
+    if (cj == 0.0) continue;   // block-uniform; 0 * v adds nothing (and leaves unused vectors unread)
+    const double2* __restrict__ v = V + (int64_t)j * stride;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int64_t i = base + (int64_t)VB * q;
+      if (i < n) {
+        const double2 y = v[i];
+        x[q].x += cj * y.x;
+        x[q].y += cj * y.y;
+      }
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      w[i] = x[q];
+      acc += x[q].x * x[q].x + x[q].y * x[q].y;
+    }
+  }
+  block_norm(acc, partial);
+}
+
+// d1 = a_s src and, when d2 is set, d2 = the same
+__global__ __launch_bounds__(VB) void k_sscale2(double2* __restrict__ d1, double2* __restrict__ d2,
+                                                const double2* __restrict__ src, const double* __restrict__ a,
+                                                const int32_t* __restrict__ bseg, const int64_t* __restrict__ send) {
+  const int sg = bseg[blockIdx.x];
+  const int64_t n = send[sg];
+  const double as = a[sg];
+  const int64_t base = (int64_t)blockIdx.x * VB * EPT + threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t i = base + (int64_t)VB * q;
+    if (i < n) {
+      const double2 y = src[i];
+      const double2 r = make_double2(as * y.x, as * y.y);
+      d1[i] = r;
+      if (d2) d2[i] = r;
+    }
+  }
+}
+
+void dev_seg_mdots(const double2* V, int64_t stride, int nv, const double2* w, const SegDev& g, double* out_dev,
+                   double* partial_dev, hipStream_t s) {
+  if (nv <= 0) return;
+  hipLaunchKernelGGL(k_smdot_part, dim3(g.nb), dim3(VB), 0, s, V, stride, nv, w, g.bseg, g.send, g.active,
+                     partial_dev);
+  hipLaunchKernelGGL(k_seg_sum, dim3(g.nseg * nv), dim3(VB), 0, s, partial_dev, g.nb, nv, g.sfirst, out_dev);
+}
+void dev_seg_lupd_norm(double2* w, const double2* v1, const double* a_dev, const double2* v2, const double* b_dev,
+                       const SegDev& g, double* out_dev, double* partial_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_slupd_norm, dim3(g.nb), dim3(VB), 0, s, w, v1, a_dev, v2, b_dev, g.bseg, g.send, g.active,
+                     partial_dev);
+  hipLaunchKernelGGL(k_seg_sum, dim3(g.nseg), dim3(VB), 0, s, partial_dev, g.nb, 1, g.sfirst, out_dev);
+}
+void dev_seg_combine_norm(double2* w, const double* sw_dev, const double2* V, int64_t stride, int nv,
+                          const double* c_dev, int ldc, const double* cs_dev, const SegDev& g, double* out_dev,
+                          double* partial_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_scombine_norm, dim3(g.nb), dim3(VB), 0, s, w, sw_dev, V, stride, nv, c_dev, ldc, cs_dev,
+                     g.bseg, g.send, partial_dev);
+  hipLaunchKernelGGL(k_seg_sum, dim3(g.nseg), dim3(VB), 0, s, partial_dev, g.nb, 1, g.sfirst, out_dev);
+}
+void dev_seg_scale2(double2* d1, double2* d2, const double2* src, const double* a_dev, const SegDev& g,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_sscale2, dim3(g.nb), dim3(VB), 0, s, d1, d2, src, a_dev, g.bseg, g.send);
+}
+
 // d = (ar + i ai) src
 __global__ __launch_bounds__(VB) void k_cscale_copy(double2* __restrict__ d, const double2* __restrict__ src,
                                                     double ar, double ai, int64_t n) {

@@ -14,8 +14,12 @@ namespace hh {
 
 struct BufPtrs { double2* p[8]; };
 // Buffer pointers held in device memory (up to 256 slots): lets one launch
-// address the buffers of many superblocks.
-struct BufTable { double2* const* p; };
+// address the buffers of many superblocks.  With `active` set, the tiles of
+// member k (buffer slots 8k .. 8k+7) run only while active[k] != 0.
+struct BufTable {
+  double2* const* p;
+  const int32_t* active = nullptr;
+};
 // complex64 buffer slots: the FP32-storage form of a task list (operands,
 // intermediates and outputs all single precision, FP32 MFMA)
 struct BufPtrs32 { float2* p[8]; };
@@ -120,6 +124,35 @@ void dev_combine2_norm(double2* w, const double2* V, const double2* W, int64_t s
                        const double* cw_dev, int64_t n, double* out_dev, double* partial_dev, hipStream_t s);
 // d = a * src
 void dev_scale_copy(double2* d, const double2* src, double a, int64_t n, hipStream_t s);
+
+// Segmented vectors: nseg vectors stored one after the other, segment s
+// starting at a multiple of vec_span() elements and holding send[s] - start
+// elements; every block of the launch lies inside one segment (bseg), and
+// segment s owns blocks sfirst[s] .. sfirst[s + 1] - 1.  Per-segment sums
+// equal the unsegmented kernels' sums of the same vector bit for bit.
+int vec_span();
+struct SegDev {
+  int nseg = 0, nb = 0;
+  const int32_t* bseg = nullptr;
+  const int64_t* send = nullptr;
+  const int32_t* sfirst = nullptr;
+  // with `active` set, the multi-dot and the three-term update skip the
+  // blocks of segments whose flag is 0 (their sums come out as 0)
+  const int32_t* active = nullptr;
+};
+// out[s * nv + j] = Re<V_j, w> over segment s
+void dev_seg_mdots(const double2* V, int64_t stride, int nv, const double2* w, const SegDev& g, double* out_dev,
+                   double* partial_dev, hipStream_t s);
+// w -= a_s v1 + b_s v2 per segment (v2 may be null); out[s] = |w_s|^2
+void dev_seg_lupd_norm(double2* w, const double2* v1, const double* a_dev, const double2* v2, const double* b_dev,
+                       const SegDev& g, double* out_dev, double* partial_dev, hipStream_t s);
+// w = sw_s w + cs_s sum_j c[s * ldc + j] V_j per segment; out[s] = |w_s|^2
+void dev_seg_combine_norm(double2* w, const double* sw_dev, const double2* V, int64_t stride, int nv,
+                          const double* c_dev, int ldc, const double* cs_dev, const SegDev& g, double* out_dev,
+                          double* partial_dev, hipStream_t s);
+// d1 = a_s src per segment, and d2 = the same when d2 is not null
+void dev_seg_scale2(double2* d1, double2* d2, const double2* src, const double* a_dev, const SegDev& g,
+                    hipStream_t s);
 // d = (ar + i ai) src
 void dev_cscale_copy(double2* d, const double2* src, double ar, double ai, int64_t n, hipStream_t s);
 

@@ -35,7 +35,7 @@
 
 using namespace hh;
 
-namespace {
+namespace hh {
 
 // ----------------------------------------------------------------------------
 // symmetric tridiagonal eigenproblem (DSTEV stand-in): implicit-shift QL
@@ -109,10 +109,6 @@ void tridiag_ql(std::vector<double>& d, std::vector<double> e, std::vector<doubl
   d = ds;
 }
 
-}  // namespace
-
-namespace {
-
 BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* env, double2* X, double2* mrg,
                 double2* aux, double2* aux2) {
   BufPtrs b;
@@ -126,6 +122,10 @@ BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* 
   b.p[B_AUX2] = aux2;
   return b;
 }
+
+}  // namespace hh
+
+namespace {
 
 cplx site_val(int model_kind, int kind, int sp, int s) {
   const double r2 = std::sqrt(2.0);
@@ -2330,20 +2330,6 @@ std::string& create_error() {
   return s;
 }
 
-// C-ABI guard: exceptions -> status codes
-template <typename F>
-int guard(hh_ctx* ctx, F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const HHError& e) {
-    if (ctx) ctx->err = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    if (ctx) ctx->err = e.what();
-    return -1;
-  }
-}
 
 }  // namespace
 
@@ -2866,108 +2852,6 @@ int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in, double* 
     need(psi_in && psi_out && psi_in != psi_out, "model_wave needs distinct input and output vectors");
     matvec(ctx, sb, (const double2*)psi_in, (double2*)psi_out);
     check_hip(hipStreamSynchronize(ctx->stream), "model_wave");
-  });
-}
-
-namespace {
-// member i's task list appended to `dst` with its buffer slots moved to 8*i+
-void append_tasks(TaskList& dst, const TaskList& src, int member) {
-  const int t0 = (int)dst.terms.size();
-  for (Term tm : src.terms) {
-    tm.a_buf = (uint8_t)(8 * member + tm.a_buf);
-    tm.b_buf = (uint8_t)(8 * member + tm.b_buf);
-    dst.terms.push_back(tm);
-  }
-  for (Tile tl : src.tiles) {
-    tl.t0 += t0;
-    tl.t1 += t0;
-    tl.c_buf = 8 * member + tl.c_buf;
-    dst.tiles.push_back(tl);
-  }
-  dst.flops_alg += src.flops_alg;
-  dst.flops_exec += src.flops_exec;
-}
-}  // namespace
-
-int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** out) {
-  if (!out) return 1;
-  *out = nullptr;
-  return guard(ctx, [&] {
-    need(nsb >= 1 && nsb <= 31, "a batch holds 1 to 31 superblocks");
-    need(sbs != nullptr, "null superblock list");
-    auto b = std::make_unique<hh_batch>();
-    TaskList s1, s2;
-    for (int i = 0; i < nsb; ++i) {
-      need(sbs[i] != nullptr, "null superblock in batch");
-      b->sbs.push_back(sbs[i]);
-      append_tasks(s1, sbs[i]->plan.stage1, i);
-      append_tasks(s2, sbs[i]->plan.stage2, i);
-      b->total_dim += sbs[i]->plan.sb.dim;
-    }
-    b->st1.upload(s1, ctx->stream);
-    b->st2.upload(s2, ctx->stream);
-    b->stage1_flops = s1.flops_alg;
-    b->stage2_flops = s2.flops_alg;
-    b->flops_alg = s1.flops_alg + s2.flops_alg;
-    b->flops_exec = s1.flops_exec + s2.flops_exec;
-    b->stage1_exec = s1.flops_exec;
-    b->stage2_exec = s2.flops_exec;
-    b->table.alloc(sizeof(double2*) * 8 * nsb);
-    b->host_table.assign(8 * nsb, nullptr);
-    *out = b.release();
-  });
-}
-
-int hh_batch_destroy(hh_batch* b) {
-  delete b;
-  return 0;
-}
-
-int hh_batch_info(const hh_batch* b, int64_t* total_dim, double* flops_alg) {
-  if (!b) return 1;
-  if (total_dim) *total_dim = b->total_dim;
-  if (flops_alg) *flops_alg = b->flops_alg;
-  return 0;
-}
-
-int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in, double* const* psi_out) {
-  return guard(ctx, [&] {
-    need(b && psi_in && psi_out, "null argument");
-    need(ctx->precision != PREC_FP32_STORE, "the batched H.psi runs in precision 0 or 1");
-    const int n = (int)b->sbs.size();
-    for (int i = 0; i < n; ++i) {
-      need(psi_in[i] && psi_out[i] && psi_in[i] != psi_out[i], "batch needs distinct input and output vectors");
-      const BufPtrs p = bufs_of((double2*)psi_in[i], (double2*)psi_out[i], b->sbs[i]->sys, b->sbs[i]->env,
-                                b->sbs[i]->X.c(), b->sbs[i]->merged.c(), nullptr, nullptr);
-      for (int k = 0; k < 8; ++k) b->host_table[8 * i + k] = p.p[k];
-    }
-    check_hip(hipMemcpyAsync(b->table.p, b->host_table.data(), sizeof(double2*) * b->host_table.size(),
-                             hipMemcpyHostToDevice, ctx->stream),
-              "h2d table");
-    const BufTable t{(double2* const*)b->table.p};
-    const double t0 = ctx->profile ? wall() : 0.0;
-    if (ctx->profile) check_hip(hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    launch_tiles(b->st1, t, ctx->stream, ctx->precision, 1, ctx->profile ? ctx->ev[1] : nullptr);
-    if (ctx->profile) check_hip(hipEventRecord(ctx->ev[2], ctx->stream), "event");
-    launch_tiles(b->st2, t, ctx->stream, ctx->precision, 2, ctx->profile ? ctx->ev[3] : nullptr);
-    if (ctx->profile) {
-      check_hip(hipEventRecord(ctx->ev[4], ctx->stream), "event");
-      check_hip(hipEventSynchronize(ctx->ev[4]), "event sync");
-      float ms[4] = {0.f, 0.f, 0.f, 0.f};
-      for (int i = 0; i < 4; ++i) check_hip(hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]), "elapsed");
-      ctx->stats.stage1_ms += ms[0];
-      ctx->stats.reduce_ms += ms[1] + ms[3];
-      ctx->stats.stage2_ms += ms[2];
-      ctx->stats.matvec_ms += (wall() - t0) * 1e3;
-    }
-    ctx->stats.matvecs += 1;
-    ctx->stats.flops_alg += b->flops_alg;
-    ctx->stats.stage1_flops_alg += b->stage1_flops;
-    ctx->stats.stage2_flops_alg += b->stage2_flops;
-    ctx->stats.flops_exec += b->flops_exec;
-    ctx->stats.stage1_flops_exec += b->stage1_exec;
-    ctx->stats.stage2_flops_exec += b->stage2_exec;
-    check_hip(hipStreamSynchronize(ctx->stream), "batch model_wave");
   });
 }
 

@@ -167,6 +167,25 @@ struct hh_batch {
   double flops_alg = 0.0, stage1_flops = 0.0, stage2_flops = 0.0, flops_exec = 0.0;
   double stage1_exec = 0.0, stage2_exec = 0.0;
   int64_t total_dim = 0;
+
+  // batched Lanczos: every member's vectors as one segmented vector (member
+  // i at off[i], a multiple of hh::vec_span(), npad elements in all), the
+  // Lanczos vectors V (stride npad), the H.psi input Q and output W at fixed
+  // addresses, the segment tables, and the grouped H.psi Q -> W captured as
+  // a HIP graph (its buffer table lz_table holds Q, W and the members'
+  // blocks and intermediates)
+  std::vector<int64_t> off;
+  int64_t npad = 0;
+  hh::DevMem V, Q, W, lz_table, seg_bseg, seg_send, seg_sfirst, scal, partial;
+  hh::DevMem active;   // int32 per member: 0 once the member's solve has stopped
+  hh::SegDev seg;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  ~hh_batch() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
 };
 
 struct hh_trunc {
@@ -181,6 +200,29 @@ struct hh_trunc {
 
 
 namespace hh {
+
+// C-ABI guard: exceptions -> status codes, message kept in the context
+template <typename F>
+int guard(hh_ctx* ctx, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const HHError& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return -1;
+  }
+}
+
+// symmetric tridiagonal eigenproblem (implicit-shift QL): ascending
+// eigenvalues in d (off-diagonal e), and the eigenvector of the lowest in *z0
+void tridiag_ql(std::vector<double>& d, std::vector<double> e, std::vector<double>* z0);
+
+// buffer-slot pointers of one superblock's H.psi
+BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* env, double2* X, double2* mrg,
+                double2* aux, double2* aux2);
 
 // H.psi of a bound superblock in the context's precision
 void matvec(hh_ctx* ctx, hh_superblock* sb, const double2* in, double2* out);

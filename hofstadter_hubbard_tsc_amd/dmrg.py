@@ -98,6 +98,11 @@ class Batch:
         L.load().hh_batch_info(self.ptr, C.byref(d), C.byref(f))
         return {"dim": d.value, "flops_alg": f.value}
 
+    def set_graph(self, on):
+        """Replay the batched Lanczos's grouped H.psi from a HIP graph (True)
+        or launch its kernels one by one (False)."""
+        L.load().hh_batch_set_graph(self.ptr, 1 if on else 0)
+
 
 class Block:
     """A device-resident block (Total_Model, code/pubdata.f90:167-181)."""
@@ -360,7 +365,7 @@ class Context:
         self._rc(L.load().hh_model_wave(self.ptr, sb.ptr, _dptr(psi_in, sb.dim), _dptr(psi_out, sb.dim)), "hh_model_wave")
 
     def batch(self, sbs):
-        """Group bound superblocks for one-launch-per-stage H.psi (at most 31)."""
+        """Group bound superblocks for one-launch-per-stage H.psi (at most 32)."""
         arr = (C.c_void_p * len(sbs))(*[sb.ptr for sb in sbs])
         p = C.c_void_p()
         self._rc(L.load().hh_batch_create(self.ptr, len(sbs), arr, C.byref(p)), "hh_batch_create")
@@ -375,6 +380,22 @@ class Context:
         a = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psi_in, batch._keep)])
         b = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psi_out, batch._keep)])
         self._rc(L.load().hh_batch_model_wave(self.ptr, batch.ptr, a, b), "hh_batch_model_wave")
+
+    def batch_lanczos(self, batch, psis, lanzero, max_iter=100):
+        """lanczos_memo_new of every member in lock step (one grouped H.psi per
+        iteration); psis[i] updated in place.  Returns (e0, iters, converged)
+        as numpy arrays."""
+        _sync()
+        n = len(batch._keep)
+        if len(psis) != n:
+            raise HHError("one start vector per batch member")
+        a = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psis, batch._keep)])
+        e0 = np.zeros(n)
+        it = np.zeros(n, np.int32)
+        cv = np.zeros(n, np.int32)
+        self._rc(L.load().hh_batch_lanczos(self.ptr, batch.ptr, a, float(lanzero), int(max_iter), L.ip(e0), L.ip(it),
+                                           L.ip(cv)), "hh_batch_lanczos")
+        return e0, it, cv.astype(bool)
 
     def lanczos(self, sb, psi, lanzero, max_iter=100):
         """lanczos_memo_new (code/lanczos.f90:4-176), levels=1; psi updated in place."""

@@ -14,7 +14,9 @@ counter-based stream keyed by ``seed_base + b``:
   clipped to [1, 1024], the residual put on the largest sector;
 * every arena element (block H and boundary-site operators) complex Gaussian
   N(0,1) + i N(0,1), scaled by 0.1, drawn on the device from a generator
-  seeded with the same key;
+  seeded with the same key; each sector's block H is then replaced by its
+  Hermitian part (H + H^dag) / 2, so that the superblock H is Hermitian (the
+  hopping terms carry their own h.c.) and has a ground state;
 * psi complex Gaussian, unit norm.
 
 The 6j recoupling coefficients and flux phases come from the library's own
@@ -62,6 +64,10 @@ def make_member(ctx, b, seed_base=0x5EED0000, m_lo=500, m_hi=4000, device="cuda"
         blk = ctx.create_block(side, BLOCK_LEN, secs, sites)
         size = blk.info()[0]
         a = torch.randn(size, dtype=torch.complex128, device=device, generator=gen) * 0.1
+        _, _, dims, hoff = blk.sectors()
+        for d, o in zip(dims.tolist(), hoff.tolist()):
+            h = a[o:o + d * d].view(d, d)
+            h.copy_(0.5 * (h + h.conj().transpose(0, 1)))
         blk.upload(a)
         del a
         blocks.append(blk)

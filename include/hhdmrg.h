@@ -181,7 +181,7 @@ int hh_model_wave(hh_ctx* ctx, hh_superblock* sb, const double* psi_in_dev, doub
 /* Batched H.psi over independent superblocks (different flux, doping or
  * geometry): one grouped launch per H.psi stage for all of them.  No
  * counterpart in the reference, which solves one superblock at a time; it is
- * model_wave (code/mat_vec.f90:3476) applied to each member.  At most 31
+ * model_wave (code/mat_vec.f90:3476) applied to each member.  At most 32
  * superblocks; they must stay bound while the batch exists. */
 int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** out);
 int hh_batch_destroy(hh_batch* b);
@@ -189,6 +189,21 @@ int hh_batch_destroy(hh_batch* b);
 int hh_batch_info(const hh_batch* b, int64_t* total_dim, double* flops_alg);
 /* psi_out_dev[i] = H_i psi_in_dev[i] for every member i */
 int hh_batch_model_wave(hh_ctx* ctx, hh_batch* b, const double* const* psi_in_dev, double* const* psi_out_dev);
+/* lanczos_memo_new (code/lanczos.f90:4-176, levels = 1) of every member at
+ * once, in lock step: one grouped FP64 H.psi per iteration for all members
+ * (replayed from a HIP graph captured on the batch's first call unless
+ * hh_batch_set_graph(b, 0)), the members' vector work in segmented launches,
+ * one host round trip for all members' scalars.  Each member follows the
+ * reference's stopping rule on its own and stops there; its later products
+ * are discarded.  psi_dev[i] holds member i's start vector on entry and its
+ * ground state on exit; e0[i], iters[i] and converged[i] (may be null) as
+ * hh_lanczos and hh_lanczos_status report them.  No counterpart in the
+ * reference, which solves one superblock at a time. */
+int hh_batch_lanczos(hh_ctx* ctx, hh_batch* b, double* const* psi_dev, double lanzero, int max_iter, double* e0,
+                     int32_t* iters, int32_t* converged);
+/* 1: hh_batch_lanczos replays its grouped H.psi from a HIP graph (default);
+ * 0: it launches the same kernels one by one */
+int hh_batch_set_graph(hh_batch* b, int on);
 
 /* Lanczos: replaces lanczos_memo_new (code/lanczos.f90:4-176) with levels=1:
  * psi_dev holds the start vector on entry and the ground state on exit. */
