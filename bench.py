@@ -348,6 +348,25 @@ def run_c3(args):
             "avg_launch_ms": round(st["stage2_ms"] / max(st["matvecs"], 1), 4), "launches": int(st["matvecs"]),
             "stage1_ms_per_launch": st["stage1_ms"] / max(st["matvecs"], 1),
             "reduce_ms_per_matvec": st["reduce_ms"] / max(st["matvecs"], 1)}
+    # the same batch with its launch cut autotuned (timed on the batch's own
+    # buffers, cached by the shape histogram of each stage's task list)
+    ctx.set_autotune(True)
+    tuned = ctx.batch([m[2] for m in members])
+    ctx.set_autotune(False)
+    for _ in range(max(args.warmup, 1)):
+        ctx.batch_model_wave(tuned, ins, outs)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(reps):
+        ctx.batch_model_wave(tuned, ins, outs)
+    torch.cuda.synchronize()
+    t_tuned = (time.time() - t0) / reps
+    tune = dict(ctx.autotune_stats())
+    tune.update({"tuned_ms": round(1e3 * t_tuned, 3), "fixed_ms": round(1e3 * t_batch, 3),
+                 "gain": round(t_batch / t_tuned, 3)})
+    print("# c3 autotuned batched H.psi: %.3f ms (fixed cut %.3f ms), %s" % (1e3 * t_tuned, 1e3 * t_batch, tune),
+          file=sys.stderr, flush=True)
+    del tuned
     # the members' ground states: batched Lanczos (one grouped H.psi per
     # iteration, graph replay and kernel-by-kernel launches) against each
     # member's own Lanczos, from the same start vectors
@@ -425,6 +444,7 @@ def run_c3(args):
                    "one_at_a_time_ms": round(1e3 * t_seq, 3),
                    "one_at_a_time_gflops": round(info["flops_alg"] / t_seq / 1e9, 2),
                    "batched_speedup": round(t_seq / t_batch, 3),
+                   "autotune": tune,
                    "lanczos": lz},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -450,6 +470,8 @@ def main():
                          "schedule at m = 256; c3: batched H.psi micro-benchmark")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--no-lanczos", action="store_true", help="c3: time the batched H.psi only")
+    ap.add_argument("--autotune", action="store_true",
+                    help="c2: autotune the H.psi launch cut of every superblock (cached by shape statistics)")
     ap.add_argument("--measure", action="store_true",
                     help="c2: after the timed steps, time the measurement entry points on the last superblock "
                          "(reported under config.measure, outside the timed region)")
@@ -489,6 +511,8 @@ def main():
     n = model.nsite
     ctx = Context(0)
     ctx.set_model(model)
+    if args.autotune:
+        ctx.set_autotune(True)
     drv = Driver(ctx, kept_max=args.m, tol=args.tol, kept_max_warm=args.m, seed=20240601)
     t_prep = time.time()
     warm = drv.warmup()
@@ -662,6 +686,7 @@ def main():
                                    "mixed-precision Lanczos; same steps from a device copy of the same state"
                                    if forms else None),
                    "precond_training": train,
+                   "autotune": ctx.autotune_stats() if args.autotune else None,
                    "forms": forms},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -172,6 +172,7 @@ int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** 
     b->stage2_exec = s2.flops_exec;
     b->table.alloc(sizeof(double2*) * 8 * nsb);
     b->host_table.assign(8 * nsb, nullptr);
+    if (ctx->autotune) autotune_batch(ctx, b.get(), s1, s2);
     *out = b.release();
   });
 }

@@ -45,16 +45,11 @@ static constexpr int WAVES = 4;        // independent waves per workgroup
 #ifndef HH_CPLX3
 #define HH_CPLX3 1
 #endif
-// Term lists are cut so that the launch holds about this many equal work
-// items: enough that longest-first dispatch leaves a short tail.
-#ifndef HH_TARGET_ITEMS
-#define HH_TARGET_ITEMS 65536
-#endif
-static constexpr int TARGET_ITEMS = HH_TARGET_ITEMS;
-// Macro tiles are listed in bands of SWZ macro rows, column by column, and
-// runs of XCHUNK consecutive workgroups are kept on one XCD: neighbouring
-// items then share operand panels in that XCD's L2.
-static constexpr int SWZ = 4;
+// Term lists are cut so that the launch holds about LaunchConfig::target_items
+// equal work items (enough that longest-first dispatch leaves a short tail);
+// macro tiles are listed in bands of LaunchConfig::band macro rows, column by
+// column, and runs of XCHUNK consecutive workgroups are kept on one XCD:
+// neighbouring items then share operand panels in that XCD's L2.
 static constexpr int XCHUNK = 8;
 
 void DevTasks::release() {
@@ -74,8 +69,11 @@ void DevTasks::release() {
   ntiles = nterms = nwork = nred = nslots = 0;
 }
 
-void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap) {
+void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap, const LaunchConfig& config) {
   release();
+  cfg = config;
+  if (cfg.target_items < 1 || cfg.max_groups < 1 || cfg.band < 1 || cfg.edge < -1 || cfg.edge > 1)
+    throw std::runtime_error("invalid launch configuration");
   if (place && (sizeof(Tile) * tl.tiles.size() + sizeof(Term) * tl.terms.size() > cap))
     throw HHCapacity("task list tables exceed the fixed scratch");
   ntiles = (int)tl.tiles.size();
@@ -106,7 +104,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     tw[i] = c;
     total += c * (double)((t.M + MTR - 1) / MTR) * (double)((t.N + MTC - 1) / MTC);
   }
-  const double target = std::max(64.0, total / TARGET_ITEMS);
+  const double target = std::max(64.0, total / cfg.target_items);
   std::vector<std::pair<double, WorkItem>> items;
   std::vector<ReduceItem> reds;
   int slot = 0;
@@ -114,7 +112,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     const Tile& t = tl.tiles[i];
     // contiguous term groups of roughly equal work
     std::vector<int> cuts{t.t0};
-    const int groups = std::max(1, std::min(32, (int)std::ceil(tw[i] / target)));
+    const int groups = std::max(1, std::min(cfg.max_groups, (int)std::ceil(tw[i] / target)));
     if (groups > 1) {
       double acc = 0.0;
       const double per = tw[i] / groups;
@@ -125,9 +123,9 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     }
     cuts.push_back(t.t1);
     const int g = (int)cuts.size() - 1;
-    // macro tiles in bands of SWZ macro rows, column by column inside a band
+    // macro tiles in bands of cfg.band macro rows, column by column inside a band
     const int nr = (t.M + MTR - 1) / MTR, nc = (t.N + MTC - 1) / MTC;
-    const int band = SWZ;
+    const int band = cfg.band;
     for (int rb = 0; rb < nr; rb += band)
       for (int ci = 0; ci < nc; ++ci)
         for (int ri = rb; ri < std::min(nr, rb + band); ++ri) {
@@ -155,7 +153,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
       all += it.first * full;
       empty += it.first * (full - used);
     }
-    edge_skip = all > 0.0 && empty > HH_EDGE_SHARE * all;
+    edge_skip = cfg.edge < 0 ? (all > 0.0 && empty > HH_EDGE_SHARE * all) : cfg.edge == 1;
   }
   // longest work items first (LPT) inside each shape class
   auto cls = [&](const WorkItem& it) {

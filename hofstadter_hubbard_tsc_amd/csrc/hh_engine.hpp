@@ -30,6 +30,21 @@ struct WorkItem { int32_t tile, r0, c0, t0, t1, slot; };
 // Deterministic reduction of split macro tiles: C = sum of nslot partials.
 struct ReduceItem { int32_t tile, r0, c0, slot0, nslot, pad; };
 
+// How a task list is cut into work items: about target_items work items of
+// equal K-work per launch, at most max_groups term groups per macro tile,
+// shape classes for edge items (edge 1 always, 0 never, -1 when more than a
+// fifth of the MFMA work would fall on empty 16 x 16 sub-tiles), macro tiles
+// listed in bands of `band` macro rows.
+struct LaunchConfig {
+  int target_items = 65536;
+  int max_groups = 32;
+  int edge = -1;
+  int band = 4;
+  bool operator==(const LaunchConfig& o) const {
+    return target_items == o.target_items && max_groups == o.max_groups && edge == o.edge && band == o.band;
+  }
+};
+
 // Tile list resident on the device, launched as one grouped kernel.
 struct DevTasks {
   Tile* tiles = nullptr;
@@ -48,9 +63,11 @@ struct DevTasks {
   bool edge_skip = false;
   // borrowed: the arrays live in caller-provided device memory (never freed here)
   bool borrowed = false;
+  LaunchConfig cfg;   // the configuration of the last upload
   // Device arrays from hipMalloc, or -- when `place` is given -- carved out of
   // the `cap` bytes at `place` (throws HHCapacity when they do not fit).
-  void upload(const TaskList& tl, hipStream_t s, void* place = nullptr, size_t cap = 0);
+  void upload(const TaskList& tl, hipStream_t s, void* place = nullptr, size_t cap = 0,
+              const LaunchConfig& config = LaunchConfig());
   void release();
   ~DevTasks() { release(); }
 };

@@ -758,6 +758,14 @@ hh_superblock* bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N
   launch_tiles(s->merge, bufs_of(nullptr, nullptr, sys, env, nullptr, s->merged.c(), nullptr, nullptr), ctx->stream,
                PREC_FP64);
   check_hip(hipStreamSynchronize(ctx->stream), "bind");
+  if (ctx->autotune) {
+    try {
+      autotune_superblock(ctx, s);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+  }
   return s;
 }
 

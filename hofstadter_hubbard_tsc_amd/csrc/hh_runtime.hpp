@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <functional>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -132,6 +133,12 @@ struct hh_ctx {
   // outcome of the last ground-state solve (hh_lanczos_status)
   int last_converged = 0;
   double last_ritz_change = 0.0, last_residual = 0.0;
+  // autotuning of the H.psi launches (hh_tune.cpp): on/off, the chosen
+  // configurations by task-list shape key, counters
+  int autotune = 0;
+  std::map<uint64_t, hh::LaunchConfig> tune_cache;
+  int tune_runs = 0, tune_hits = 0;
+  double tune_seconds = 0.0;
 };
 
 struct hh_block {
@@ -236,5 +243,13 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
                  const LanczosOpts& opt = LanczosOpts());
 // mixed-precision Lanczos rounds (FP32-storage H.psi, FP64 restarts)
 SolveOut mixed_lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_iter, int max_rounds);
+
+// Autotuning of the grouped H.psi launches (hh_tune.cpp): the shape key of a
+// task list (its flop-weighted (M, N, K) histogram in factors of two), and
+// the re-cut of a bound superblock's or a batch's stage lists by the
+// fastest of a set of launch configurations, cached by shape key.
+uint64_t shape_key(const TaskList& tl);
+void autotune_superblock(hh_ctx* ctx, hh_superblock* sb);
+void autotune_batch(hh_ctx* ctx, hh_batch* b, const TaskList& s1, const TaskList& s2);
 
 }  // namespace hh

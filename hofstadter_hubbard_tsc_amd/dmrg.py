@@ -381,6 +381,17 @@ class Context:
         b = (C.c_void_p * n)(*[_dptr(t, m.dim).value for t, m in zip(psi_out, batch._keep)])
         self._rc(L.load().hh_batch_model_wave(self.ptr, batch.ptr, a, b), "hh_batch_model_wave")
 
+    def set_autotune(self, on):
+        """Tune the H.psi launch cut of every superblock bound and batch
+        created from now on (cached by task-list shape statistics)."""
+        self._rc(L.load().hh_set_autotune(self.ptr, 1 if on else 0), "hh_set_autotune")
+
+    def autotune_stats(self):
+        k, t, h, sec = L.i32(), L.i32(), L.i32(), L.f64()
+        self._rc(L.load().hh_autotune_stats(self.ptr, C.byref(k), C.byref(t), C.byref(h), C.byref(sec)),
+                 "hh_autotune_stats")
+        return {"keys": k.value, "tunes": t.value, "hits": h.value, "seconds": sec.value}
+
     def batch_lanczos(self, batch, psis, lanzero, max_iter=100):
         """lanczos_memo_new of every member in lock step (one grouped H.psi per
         iteration); psis[i] updated in place.  Returns (e0, iters, converged)

@@ -205,6 +205,21 @@ int hh_batch_lanczos(hh_ctx* ctx, hh_batch* b, double* const* psi_dev, double la
  * 0: it launches the same kernels one by one */
 int hh_batch_set_graph(hh_batch* b, int on);
 
+/* Autotuning of the H.psi launches; no counterpart in the reference.  With
+ * mode 1, hh_superblock_bind and hh_batch_create time each H.psi stage's task
+ * list under a few ways of cutting it into work items (work items per launch,
+ * term groups per output tile, shape classes for edge tiles, tile order) on
+ * the superblock's own buffers, and keep the fastest; the choice is cached
+ * under the flop-weighted histogram of the list's (M, N, K) shapes in factors
+ * of two, so later superblocks of the same shape statistics reuse it without
+ * timing.  Mode 0 (default) keeps the fixed cut.  Every cut gives the same
+ * H.psi up to the order of the split partial sums (1e-13 relative); a tuned
+ * run stays deterministic for a given choice, but close timings can choose
+ * differently between runs. */
+int hh_set_autotune(hh_ctx* ctx, int mode);
+/* shape keys tuned so far, timing runs, cache hits, seconds spent timing */
+int hh_autotune_stats(const hh_ctx* ctx, int32_t* keys, int32_t* tunes, int32_t* hits, double* seconds);
+
 /* Lanczos: replaces lanczos_memo_new (code/lanczos.f90:4-176) with levels=1:
  * psi_dev holds the start vector on entry and the ground state on exit. */
 int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi_dev, double lanzero, int max_iter, double* e0,

@@ -190,3 +190,40 @@ def test_batch_holds_32_members(ctx):
         p = members[i][3].clone()
         e, _ = ctx.lanczos(members[i][2], p, 1e-9, 60)
         assert abs(e0[i] - e) <= 1e-10 * abs(e), (i, e0[i], e)
+
+
+def test_autotuned_launches_give_the_same_hpsi(ctx):
+    """With autotuning on, binding a superblock times its two H.psi stages
+    under the candidate launch cuts and keeps the fastest; the tuned H.psi
+    equals the fixed cut's (1e-13, only the order of split partial sums can
+    differ); binding a superblock of the same shape statistics again is a
+    cache hit without timing; a tuned batch's grouped H.psi equals each
+    member's own."""
+    import torch
+    from hofstadter_hubbard_tsc_amd import synth
+    sysb, envb, sb0, psi = synth.make_member(ctx, 3, m_lo=300, m_hi=420)
+    y0 = torch.zeros_like(psi)
+    ctx.model_wave(sb0, psi, y0)
+    before = ctx.autotune_stats()
+    ctx.set_autotune(True)
+    try:
+        sb1 = ctx.bind(sysb, envb, synth.N_TARGET)
+        st = ctx.autotune_stats()
+        assert st["tunes"] - before["tunes"] + st["hits"] - before["hits"] == 2
+        y1 = torch.zeros_like(psi)
+        ctx.model_wave(sb1, psi, y1)
+        assert (y1 - y0).abs().max().item() <= 1e-13 * y0.abs().max().item()
+        sb2 = ctx.bind(sysb, envb, synth.N_TARGET)
+        st2 = ctx.autotune_stats()
+        assert st2["tunes"] == st["tunes"] and st2["hits"] == st["hits"] + 2
+        members = [synth.make_member(ctx, b, m_lo=200, m_hi=300) for b in range(4)]
+        batch = ctx.batch([m[2] for m in members])
+        outs = [torch.zeros_like(m[3]) for m in members]
+        ctx.batch_model_wave(batch, [m[3] for m in members], outs)
+        for (s_, e_, sb, p), yb in zip(members, outs):
+            y = torch.zeros_like(p)
+            ctx.model_wave(sb, p, y)
+            assert (yb - y).abs().max().item() <= 1e-13 * y.abs().max().item()
+        print("autotune:", ctx.autotune_stats())
+    finally:
+        ctx.set_autotune(False)
