@@ -14,8 +14,8 @@ Timed, from identical device copies of that state: `--steps` further
 left-to-right finite-sweep steps (or with --sweep one whole sweep) of
   * the plain FP64 form (Lanczos with FP64 H.psi; the headline), then
   * the chosen form (Davidson with FP32-storage H.psi, FP64 restarts, the
-    learned per-sector preconditioner, fallback to mixed Lanczos), and
-  * the chosen form without the network (fixed diagonal preconditioner).
+    fixed per-sector diagonal preconditioner, fallback to mixed Lanczos), and
+  * the same with the learned per-sector preconditioner ("chosen_learned").
 Each step is wave transformation, ground-state solve, density matrix,
 eigendecomposition, truncation, growth + renormalisation, all on the GPU with
 the blocks resident in HBM.  One step = one DMRG step; a full sweep is
@@ -206,12 +206,12 @@ def run_c1(args):
         tr.reset_transform()
         tr.sweep(0, 1, n - 2, wl.tol, TRUN_ERR, wl.kept_max, phase=1)
         tr.train_precond(epochs=100, lr=1e-3)
-        for name, mode in (("chosen", 0), ("chosen_no_network", 1)):
+        for name, mode in (("chosen", 1), ("chosen_learned", 0)):
             f = drv.fork()
             f.set_solver(2)
             f.set_precond(mode)
             forks[name] = f
-        forks["chosen"].copy_precond(tr)
+        forks["chosen_learned"].copy_precond(tr)
         del tr
 
     def sweeps(d, tag, out):
@@ -545,7 +545,7 @@ def main():
     forks = {}
     if chosen:
         # identical starting states for the other forms (device copies)
-        for name, mode in (("chosen", 0), ("chosen_no_network", 1)):
+        for name, mode in (("chosen", 1), ("chosen_learned", 0)):
             f = drv.fork()
             f.set_solver(2)
             f.set_precond(mode)
@@ -682,8 +682,9 @@ def main():
                    "max_residual": max((r.residual for r in recs), default=None),
                    "oracle_check": check,
                    "chosen_form": ("Davidson with FP32-storage H.psi (FP32 MFMA), FP64 H.psi restarts (exact Rayleigh "
-                                   "quotient and residual), the learned per-sector preconditioner with fallback to "
-                                   "mixed-precision Lanczos; same steps from a device copy of the same state"
+                                   "quotient and residual), the fixed per-sector diagonal preconditioner, fallback "
+                                   "to mixed-precision Lanczos; chosen_learned: the same with the learned per-sector "
+                                   "preconditioner; same steps from a device copy of the same state"
                                    if forms else None),
                    "precond_training": train,
                    "autotune": ctx.autotune_stats() if args.autotune else None,

@@ -315,6 +315,10 @@ SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, 
     }
     const bool out_of_budget = used >= max_hpsi;
     if (stuck || dtheta <= lanzero || k == kmax || out_of_budget) {
+      // the round's last Ritz value (FP32 products) and whether the round
+      // ended by the stopping rule itself
+      const double theta_round = theta;
+      const bool stagnated = !stuck && dtheta <= lanzero;
       // restart from the Ritz vector with an FP64 product
       if (k > 1) {
         put(cv, c);
@@ -327,7 +331,12 @@ SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, 
       r_ready = true;
       const double de = std::fabs(theta - anchor_e);
       out.ritz_change = de;
-      if (de <= lanzero || rho <= lanzero) {
+      // converged: the exact Rayleigh quotient moved by at most lanzero since
+      // the previous restart, or the round ended by the stopping rule and the
+      // exact Rayleigh quotient of its Ritz vector confirms the round's last
+      // Ritz value to lanzero (the FP32 products did not move it)
+      const bool confirmed = stagnated && std::fabs(theta - theta_round) <= lanzero;
+      if (de <= lanzero || rho <= lanzero || confirmed) {
         out.converged = true;
         break;
       }

@@ -37,9 +37,13 @@ struct PcSample {
 void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P);
 
 // Davidson ground state of sb from the start vector psi (overwritten with
-// the result).  max_hpsi bounds the H.psi applications of both precisions;
-// the stopping rule is the reference's (code/lanczos.f90:77-80) applied to
-// successive FP64 Rayleigh quotients at the restarts.  net is the trained
+// the result).  max_hpsi bounds the H.psi applications of both precisions.
+// A round of FP32 expansions ends by the reference's stopping rule
+// (code/lanczos.f90:77-80: the lowest Ritz value moved by at most lanzero)
+// or at the subspace limit, and restarts from its Ritz vector with an FP64
+// H.psi; the solve has converged when that exact Rayleigh quotient moved by
+// at most lanzero since the previous restart, or when it confirms to lanzero
+// the Ritz value of a round that ended by the stopping rule.  net is the trained
 // network (PC_LEARNED) or null.  *fell_back is set when the FP64 residual
 // stopped falling and the mixed-precision Lanczos finished the solve.
 // Called at every FP64 restart with the unit vector u, its exact residual

@@ -73,7 +73,10 @@ typedef struct {
                                 2 the chosen form: Davidson with FP32-storage H.psi, FP64
                                   restarts and the learned per-sector preconditioner (see
                                   hh_driver_train_precond), falling back to solver 1 when the
-                                  FP64 residual stops falling */
+                                  FP64 residual stops falling; converged when the FP64 Rayleigh
+                                  quotient moved by at most the tolerance since the previous
+                                  restart, or confirms to the tolerance the last Ritz value of a
+                                  round of FP32 expansions that ended by the stopping rule */
 } hh_schedule;
 
 /* One DMRG step (a line of Energy_All.dat + Num_State.dat + Entropy.dat,
