@@ -457,7 +457,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--m", type=int, default=2000)
+    ap.add_argument("--m", type=int, default=None, help="kept multiplets (default: the workload's, 2000 for c2)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -465,9 +465,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="c2: time one whole sweep per form (2 (Nsite - 3) steps) instead of --steps steps")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c2",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c2",
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
-                         "schedule at m = 256; c3: batched H.psi micro-benchmark")
+                         "schedule at m = 256; c3: batched H.psi micro-benchmark; c4: DMRG steps of the flux "
+                         "Hubbard 24x6 cylinder at m = 4000 (as c2)")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--no-lanczos", action="store_true", help="c3: time the batched H.psi only")
     ap.add_argument("--autotune", action="store_true",
@@ -506,7 +507,9 @@ def main():
     os.environ.setdefault("HHDMRG_VERBOSE", "1")   # progress line per DMRG step on stderr
     from hofstadter_hubbard_tsc_amd import Context, Driver
 
-    wl = workload("c2")
+    wl = workload(args.workload)
+    if args.m is None:
+        args.m = wl.kept_max
     model = wl.model
     n = model.nsite
     ctx = Context(0)
@@ -582,6 +585,8 @@ def main():
     bytes_alg = st["bytes_alg"] / max(s2_launches, 1)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hpsi_stage2_tj_cyl32x4_m%d_traffic.json" % args.m)
+    if args.workload != "c2":
+        tpath = ""
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roof = {"bound": "mfma", "achieved": round(s2_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -614,6 +619,8 @@ def main():
         measure = measure_at_scale(ctx, drv, model, recs[-1].sys_len + 1)
     forms = {}
     t32 = os.path.join(ROOT, "profiles", "hpsi_stage2_fp32_tj_cyl32x4_m%d_traffic.json" % args.m)
+    if args.workload != "c2":
+        t32 = ""
     traffic32 = json.load(open(t32)).get("hbm_bytes_per_launch") if os.path.exists(t32) else None
     for name, f in forks.items():
         ctx.reset_stats()
@@ -668,8 +675,9 @@ def main():
         "vs_baseline": None,
         "dtype": "c128 (FP64 MFMA)",
         "data": "synthetic: deterministic counter-based start vectors, no external data",
-        "config": {"workload": "flux t-J cylinder Ly=4 x Lx=32, N=112, S=0, flux 1/8 (Landau), t=1, J=0.4, "
-                               "m=%d SU(2) multiplets, plain FP64 form, finite-sweep steps from the centre" % args.m,
+        "config": {"workload": (wl.description.split(", m ")[0] if args.workload == "c2" else
+                                wl.description.split(", m=")[0]) +
+                               ", m=%d SU(2) multiplets, plain FP64 form, finite-sweep steps from the centre" % args.m,
                    "m": args.m, "lanczos_tol": args.tol, "form": "plain FP64",
                    "sweep_seconds_est": round(ms_per_step * 2 * (n - 3) / 1e3, 1),
                    "sweep_seconds": round(wall, 2) if args.sweep else None,

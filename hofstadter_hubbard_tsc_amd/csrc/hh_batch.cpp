@@ -162,8 +162,9 @@ int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** 
       append_tasks(s2, sbs[i]->plan.stage2, i);
       b->total_dim += sbs[i]->plan.sb.dim;
     }
-    b->st1.upload(s1, ctx->stream);
-    b->st2.upload(s2, ctx->stream);
+    const LaunchConfig cfg = ctx->forced_cfg ? ctx->forced : LaunchConfig();
+    b->st1.upload(s1, ctx->stream, nullptr, 0, cfg);
+    b->st2.upload(s2, ctx->stream, nullptr, 0, cfg);
     b->stage1_flops = s1.flops_alg;
     b->stage2_flops = s2.flops_alg;
     b->flops_alg = s1.flops_alg + s2.flops_alg;
@@ -172,7 +173,7 @@ int hh_batch_create(hh_ctx* ctx, int nsb, hh_superblock* const* sbs, hh_batch** 
     b->stage2_exec = s2.flops_exec;
     b->table.alloc(sizeof(double2*) * 8 * nsb);
     b->host_table.assign(8 * nsb, nullptr);
-    if (ctx->autotune) autotune_batch(ctx, b.get(), s1, s2);
+    if (ctx->autotune && !ctx->forced_cfg) autotune_batch(ctx, b.get(), s1, s2);
     *out = b.release();
   });
 }

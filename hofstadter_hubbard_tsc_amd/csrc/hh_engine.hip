@@ -38,6 +38,15 @@ void check_hip(hipError_t e, const char* what) {
 static constexpr int WA = 2, WB = 2;   // 16x16 MFMA sub-tiles per wave: rows, columns
 static constexpr int MTR = 16 * WA;     // macro tile rows (one wave)
 static constexpr int MTC = 16 * WB;     // macro tile columns
+// A wide macro tile (LaunchConfig::wide) has WBW sub-tile columns: with
+// wide 1 a region whose last 32-column chunk would leave at most 16 columns
+// over takes them into one 32 x 48 tile, so its rows' operand panels are not
+// read again for a nearly empty tile; with wide 2 every chunk is 48 wide
+// (each operand panel of the rows is read for 48 output columns, not 32).
+static constexpr int WBW = 3;
+static constexpr int MTCW = 16 * WBW;
+// partial-sum slot of a split macro tile (the widest tile, column-major)
+static constexpr int SLOT = MTR * MTCW;
 static constexpr int WAVES = 4;        // independent waves per workgroup
 // FP64 complex products by the 3M scheme (3 real MFMAs per complex product)
 // in every shape class; HH_CPLX3 = 0 builds the four-product form instead.
@@ -72,7 +81,8 @@ void DevTasks::release() {
 void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap, const LaunchConfig& config) {
   release();
   cfg = config;
-  if (cfg.target_items < 1 || cfg.max_groups < 1 || cfg.band < 1 || cfg.edge < -1 || cfg.edge > 1)
+  if (cfg.target_items < 1 || cfg.max_groups < 1 || cfg.band < 1 || cfg.edge < -1 || cfg.edge > 1 || cfg.wide < 0 ||
+      cfg.wide > 2)
     throw std::runtime_error("invalid launch configuration");
   if (place && (sizeof(Tile) * tl.tiles.size() + sizeof(Term) * tl.terms.size() > cap))
     throw HHCapacity("task list tables exceed the fixed scratch");
@@ -94,6 +104,19 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
       if (a_end >= (int64_t(1) << 32) || b_end >= (int64_t(1) << 32))
         throw std::runtime_error("operand sub-block exceeds 2^32 elements");
     }
+  // column chunks of a region: 32 wide, the last one up to 48 wide when
+  // wide tiles are on and it would otherwise leave at most 16 columns over
+  auto chunks = [&](int N) {
+    std::vector<std::pair<int, int>> c;   // (first column, width)
+    const int step = cfg.wide == 2 ? MTCW : MTC;
+    for (int c0 = 0; c0 < N; c0 += step) c.push_back({c0, std::min(step, N - c0)});
+    if (cfg.wide == 1 && c.size() >= 2 && c.back().second <= 16) {
+      const int w = c.back().second;
+      c.pop_back();
+      c.back().second += w;
+    }
+    return c;
+  };
   // work per macro tile = sum of max(K, 1) over its terms
   double total = 0.0;
   std::vector<double> tw(ntiles, 0.0);
@@ -102,10 +125,17 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     double c = 0.0;
     for (int k = t.t0; k < t.t1; ++k) c += (double)std::max(tl.terms[k].K, 1);
     tw[i] = c;
-    total += c * (double)((t.M + MTR - 1) / MTR) * (double)((t.N + MTC - 1) / MTC);
+    total += c * (double)((t.M + MTR - 1) / MTR) * (double)chunks(t.N).size();
   }
   const double target = std::max(64.0, total / cfg.target_items);
-  std::vector<std::pair<double, WorkItem>> items;
+  // a work item with its K-work and the width of its column chunk (32, or
+  // up to 48 for a wide macro tile)
+  struct Item {
+    double work;
+    WorkItem it;
+    int width;
+  };
+  std::vector<Item> items;
   std::vector<ReduceItem> reds;
   int slot = 0;
   for (int i = 0; i < ntiles; ++i) {
@@ -124,21 +154,23 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     cuts.push_back(t.t1);
     const int g = (int)cuts.size() - 1;
     // macro tiles in bands of cfg.band macro rows, column by column inside a band
-    const int nr = (t.M + MTR - 1) / MTR, nc = (t.N + MTC - 1) / MTC;
+    const int nr = (t.M + MTR - 1) / MTR;
+    const std::vector<std::pair<int, int>> cols = chunks(t.N);
+    const int nc = (int)cols.size();
     const int band = cfg.band;
     for (int rb = 0; rb < nr; rb += band)
       for (int ci = 0; ci < nc; ++ci)
         for (int ri = rb; ri < std::min(nr, rb + band); ++ri) {
-          const int r = ri * MTR, c = ci * MTC;
+          const int r = ri * MTR, c = cols[ci].first;
           if (g == 1) {
-            items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}});
+            items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}, cols[ci].second});
           } else {
             for (int q = 0; q < g; ++q) {
               double wq = 0.0;
               for (int k = cuts[q]; k < cuts[q + 1]; ++k) wq += (double)std::max(tl.terms[k].K, 1);
-              items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}});
+              items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}, cols[ci].second});
             }
-            reds.push_back(ReduceItem{i, r, c, slot, g, 0});
+            reds.push_back(ReduceItem{i, r, c, slot, g, cols[ci].second > MTC ? MTCW : MTC});
             slot += g;
           }
         }
@@ -147,34 +179,39 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
   {
     double all = 0.0, empty = 0.0;
     for (auto& it : items) {
-      const Tile& t = tl.tiles[it.second.tile];
-      const int mv = std::min(MTR, t.M - it.second.r0), nv = std::min(MTC, t.N - it.second.c0);
-      const int full = (MTR / 16) * (MTC / 16), used = ((mv + 15) / 16) * ((nv + 15) / 16);
-      all += it.first * full;
-      empty += it.first * (full - used);
+      const Tile& t = tl.tiles[it.it.tile];
+      const int width = it.width > MTC ? MTCW : MTC;
+      const int mv = std::min(MTR, t.M - it.it.r0), nv = it.width;
+      const int full = (MTR / 16) * (width / 16), used = ((mv + 15) / 16) * ((nv + 15) / 16);
+      all += it.work * full;
+      empty += it.work * (full - used);
     }
     edge_skip = cfg.edge < 0 ? (all > 0.0 && empty > HH_EDGE_SHARE * all) : cfg.edge == 1;
   }
   // longest work items first (LPT) inside each shape class
-  auto cls = [&](const WorkItem& it) {
+  // classes: 0 full 32 x 32, 1 at most 16 rows, 2 at most 16 columns, 3
+  // both (the last three only with edge_skip); 4 wide 32 x 48, 5 wide with at
+  // most 16 rows (with edge_skip)
+  auto cls = [&](const Item& x) {
+    const Tile& t = tl.tiles[x.it.tile];
+    if (x.width > MTC) return 4 + ((edge_skip && t.M - x.it.r0 <= 16) ? 1 : 0);
     if (!edge_skip) return 0;
-    const Tile& t = tl.tiles[it.tile];
-    return (t.M - it.r0 <= 16 ? 1 : 0) | (t.N - it.c0 <= 16 ? 2 : 0);
+    return (t.M - x.it.r0 <= 16 ? 1 : 0) | (x.width <= 16 ? 2 : 0);
   };
   std::stable_sort(items.begin(), items.end(),
-                   [&](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
-                     const int ca = cls(a.second), cb = cls(b.second);
+                   [&](const Item& a, const Item& b) {
+                     const int ca = cls(a), cb = cls(b);
                      if (ca != cb) return ca < cb;
-                     return a.first > b.first;
+                     return a.work > b.work;
                    });
   std::vector<WorkItem> w;
   w.reserve(items.size());
-  for (int c = 0; c < 5; ++c) cls_off[c] = 0;
+  for (int c = 0; c < NCLASS + 1; ++c) cls_off[c] = 0;
   for (auto& it : items) {
-    w.push_back(it.second);
-    cls_off[cls(it.second) + 1] += 1;
+    w.push_back(it.it);
+    cls_off[cls(it) + 1] += 1;
   }
-  for (int c = 1; c < 5; ++c) cls_off[c] += cls_off[c - 1];
+  for (int c = 1; c < NCLASS + 1; ++c) cls_off[c] += cls_off[c - 1];
   nwork = (int)w.size();
   nred = (int)reds.size();
   nslots = slot;
@@ -186,7 +223,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     const size_t o_work = o_terms + up(sizeof(Term) * nterms);
     const size_t o_red = o_work + up(sizeof(WorkItem) * nwork);
     const size_t o_part = o_red + up(sizeof(ReduceItem) * nred);
-    const size_t end = o_part + sizeof(double2) * (size_t)nslots * MTR * MTC;
+    const size_t end = o_part + sizeof(double2) * (size_t)nslots * SLOT;
     if (end > cap) {
       nwork = nred = nslots = ntiles = nterms = 0;
       throw HHCapacity("task list needs " + std::to_string(end) + " bytes of device tables and partial sums, the "
@@ -225,7 +262,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
   if (nred) {
     check_hip(hipMalloc(&red, sizeof(ReduceItem) * nred), "hipMalloc red");
     check_hip(hipMemcpyAsync(red, reds.data(), sizeof(ReduceItem) * nred, hipMemcpyHostToDevice, s), "copy red");
-    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * MTR * MTC), "hipMalloc partial");
+    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * SLOT), "hipMalloc partial");
   }
   check_hip(hipStreamSynchronize(s), "task upload");
 }
@@ -312,7 +349,7 @@ __device__ __forceinline__ double2 ld_op(const S* __restrict__ p, int64_t idx, b
 // pipeline, so no wave carries the code or registers of another shape.
 // CPLX3 selects three real MFMAs per complex product (FP64) over four.
 // ----------------------------------------------------------------------------
-template <typename T, int TAG, typename Bufs, int NA, int NB, bool CPLX3>
+template <typename T, int TAG, typename Bufs, int NA, int NB, bool CPLX3, int WBT>
 __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles, const Term* __restrict__ terms,
                                                const WorkItem* __restrict__ work, int nwork, Bufs bufs,
                                                double2* __restrict__ partial) {
@@ -339,14 +376,14 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     if (bufs.active && bufs.active[tile.c_buf >> 3] == 0) return;
   }
   const int r0 = w.r0, c0 = w.c0;
-  const int Mv = min(16 * WA, tile.M - r0), Nv = min(16 * WB, tile.N - c0);
+  const int Mv = min(16 * WA, tile.M - r0), Nv = min(16 * WBT, tile.N - c0);
   const int li = lane & 15, lk = lane >> 4;
 
-  V accR[WA][WB], accI[WA][WB];
+  V accR[WA][WBT], accI[WA][WBT];
 #pragma unroll
   for (int a = 0; a < WA; ++a)
 #pragma unroll
-    for (int b = 0; b < WB; ++b) {
+    for (int b = 0; b < WBT; ++b) {
       accR[a][b] = V{0, 0, 0, 0};
       accI[a][b] = V{0, 0, 0, 0};
     }
@@ -361,7 +398,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
 #pragma unroll
       for (int a = 0; a < WA; ++a)
 #pragma unroll
-        for (int b = 0; b < WB; ++b)
+        for (int b = 0; b < WBT; ++b)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
@@ -378,12 +415,12 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   // During the GEMM terms accR holds P1, accP holds P2 and accI holds P3, all
   // starting from zero.
   constexpr bool M3 = CPLX3;
-  V accP[M3 ? WA : 1][M3 ? WB : 1];
+  V accP[M3 ? WA : 1][M3 ? WBT : 1];
   if constexpr (M3) {
 #pragma unroll
     for (int a = 0; a < WA; ++a)
 #pragma unroll
-      for (int b = 0; b < WB; ++b) accP[a][b] = V{0, 0, 0, 0};
+      for (int b = 0; b < WBT; ++b) accP[a][b] = V{0, 0, 0, 0};
   } else {
     elementwise();
   }
@@ -397,7 +434,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
   struct OpAddr {
     const S* Ab;
     const S* Bb;
-    uint32_t pa[WA], pb[WB];   // k-independent part of each lane's element offset
+    uint32_t pa[WA], pb[WBT];   // k-independent part of each lane's element offset
     int a_ks, b_ks, K;
     double sa, sb, cr, ci;
   };
@@ -413,7 +450,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
 #pragma unroll
     for (int a = 0; a < WA; ++a) o.pa[a] = (uint32_t)(min(r0 + 16 * a + li, r0 + Mv - 1) * a_rs);
 #pragma unroll
-    for (int b = 0; b < WB; ++b) o.pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
+    for (int b = 0; b < WBT; ++b) o.pb[b] = (uint32_t)(min(c0 + 16 * b + li, c0 + Nv - 1) * b_cs);
     o.K = tm.K;
     o.sa = (tm.a_op == OP_C || tm.a_op == OP_R) ? -1.0 : 1.0;
     o.sb = (tm.b_op == OP_C || tm.b_op == OP_R) ? -1.0 : 1.0;
@@ -421,10 +458,10 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     o.ci = tm.ci;
     return o;
   };
-  // NA x NB of the WA x WB sub-tiles hold data (the kernels of the narrow
+  // NA x NB of the WA x WBT sub-tiles hold data (the kernels of the narrow
   // shape classes run work items with Mv <= 16 and / or Nv <= 16: fewer loads
   // and MFMAs, and one pipeline per kernel)
-  auto load = [&](S (&xa)[WA], S (&xb)[WB], const OpAddr& o, int k0) {
+  auto load = [&](S (&xa)[WA], S (&xb)[WBT], const OpAddr& o, int k0) {
     const uint32_t kc = (uint32_t)min(k0 + lk, o.K - 1);
 #pragma unroll
     for (int a = 0; a < NA; ++a) xa[a] = o.Ab[o.pa[a] + kc * (uint32_t)o.a_ks];
@@ -441,15 +478,15 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
     OpAddr c = make_op(terms[ti]);
     int tn = next_gemm(ti + 1);
     Term dn = terms[tn < w.t1 ? tn : ti];   // descriptor of the following GEMM term
-    S xa0[WA], xb0[WB], xa1[WA], xb1[WB];
+    S xa0[WA], xb0[WBT], xa1[WA], xb1[WBT];
     load(xa0, xb0, c, 0);
     int kb = 0;
     bool done = false;
-    auto step = [&](const S (&xa)[WA], const S (&xb)[WB], S (&ya)[WA], S (&yb)[WB]) {
+    auto step = [&](const S (&xa)[WA], const S (&xb)[WBT], S (&ya)[WA], S (&yb)[WBT]) {
       // only the last k-step of a term can hold k >= K
       const T kf = (kb + 4 <= c.K || kb + lk < c.K) ? T(1) : T(0);
       const T cr = (T)c.cr, ci = (T)c.ci, sa = (T)c.sa, sb = (T)c.sb;
-      T ar[WA], ai[WA], br[WB], bi[WB];
+      T ar[WA], ai[WA], br[WBT], bi[WBT];
 #pragma unroll
       for (int a = 0; a < NA; ++a) {
         const T xr = kf * (T)xa[a].x, xi = kf * sa * (T)xa[a].y;
@@ -467,7 +504,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       else load(ya, yb, c, kb + 4);
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (M3) {
-        T pa[WA], pb[WB];
+        T pa[WA], pb[WBT];
 #pragma unroll
         for (int a = 0; a < NA; ++a) pa[a] = ar[a] + ai[a];
 #pragma unroll
@@ -523,13 +560,13 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
       }
     }
   };
-  static_assert(NA >= 1 && NA <= WA && NB >= 1 && NB <= WB, "sub-tile counts of a shape class");
+  static_assert(NA >= 1 && NA <= WA && NB >= 1 && NB <= WBT, "sub-tile counts of a shape class");
   run();
   if constexpr (M3) {
 #pragma unroll
     for (int a = 0; a < WA; ++a)
 #pragma unroll
-      for (int b = 0; b < WB; ++b) {
+      for (int b = 0; b < WBT; ++b) {
         accI[a][b] = accI[a][b] - accR[a][b] - accP[a][b];
         accR[a][b] = accR[a][b] - accP[a][b];
       }
@@ -540,14 +577,14 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
 #pragma unroll
   for (int a = 0; a < WA; ++a)
 #pragma unroll
-    for (int b = 0; b < WB; ++b)
+    for (int b = 0; b < WBT; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = a * 16 + Acc<T>::row(lk, v), c = b * 16 + li;
         if (r < Mv && c < Nv) {
           double re = (double)accR[a][b][v], im = (double)accI[a][b][v];
           if (w.slot >= 0) {
-            partial[(int64_t)w.slot * MTR * MTC + r + c * MTR] = make_double2(re, im);
+            partial[(int64_t)w.slot * SLOT + r + c * MTR] = make_double2(re, im);
           } else {
             const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
             if (tile.accumulate) {
@@ -570,14 +607,15 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   if constexpr (std::is_same<Bufs, BufTable>::value) {
     if (bufs.active && bufs.active[tile.c_buf >> 3] == 0) return;
   }
-  const int Mv = min(MTR, tile.M - ri.r0), Nv = min(MTC, tile.N - ri.c0);
+  const int width = ri.width;   // 32, or 48 for a wide macro tile
+  const int Mv = min(MTR, tile.M - ri.r0), Nv = min(width, tile.N - ri.c0);
   typename StoreOf<Bufs>::type* __restrict__ C = bufs.p[tile.c_buf];
-  for (int e = threadIdx.x; e < MTR * MTC; e += 256) {
+  for (int e = threadIdx.x; e < MTR * width; e += 256) {
     const int r = e % MTR, c = e / MTR;
     if (r >= Mv || c >= Nv) continue;
     double re = 0.0, im = 0.0;
     for (int q = 0; q < ri.nslot; ++q) {
-      const double2 x = partial[(int64_t)(ri.slot0 + q) * MTR * MTC + e];
+      const double2 x = partial[(int64_t)(ri.slot0 + q) * SLOT + e];
       re += x.x;
       im += x.y;
     }
@@ -591,14 +629,14 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
   }
 }
 
-template <typename T, typename Bufs, int NA, int NB, bool CPLX3>
+template <typename T, typename Bufs, int NA, int NB, bool CPLX3, int WBT>
 static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, int c) {
   const int n = t.cls_off[c + 1] - t.cls_off[c];
   if (n <= 0) return;
   const WorkItem* w = t.work + t.cls_off[c];
   const dim3 g((n + WAVES - 1) / WAVES), blk(64 * WAVES);
 #define HH_LAUNCH(TG) \
-  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, NA, NB, CPLX3>), g, blk, 0, s, t.tiles, t.terms, w, n, b, t.partial)
+  hipLaunchKernelGGL((k_tiles<T, TG, Bufs, NA, NB, CPLX3, WBT>), g, blk, 0, s, t.tiles, t.terms, w, n, b, t.partial)
   if (tag == 1) HH_LAUNCH(1);
   else if (tag == 2) HH_LAUNCH(2);
   else if (tag == 3) HH_LAUNCH(3);
@@ -609,10 +647,14 @@ static void launch_ts(const DevTasks& t, const Bufs& b, hipStream_t s, int tag, 
 template <typename T, typename Bufs>
 static void launch_t(const DevTasks& t, const Bufs& b, hipStream_t s, int tag) {
   constexpr bool m3 = HH_CPLX3 != 0;
-  launch_ts<T, Bufs, 2, 2, m3>(t, b, s, tag, 0);
-  launch_ts<T, Bufs, 1, 2, m3>(t, b, s, tag, 1);
-  launch_ts<T, Bufs, 2, 1, m3>(t, b, s, tag, 2);
-  launch_ts<T, Bufs, 1, 1, m3>(t, b, s, tag, 3);
+  launch_ts<T, Bufs, 2, 2, m3, WB>(t, b, s, tag, 0);
+  launch_ts<T, Bufs, 1, 2, m3, WB>(t, b, s, tag, 1);
+  launch_ts<T, Bufs, 2, 1, m3, WB>(t, b, s, tag, 2);
+  launch_ts<T, Bufs, 1, 1, m3, WB>(t, b, s, tag, 3);
+  if (t.cls_off[NCLASS] > t.cls_off[4]) {   // wide macro tiles
+    launch_ts<T, Bufs, 2, WBW, m3, WBW>(t, b, s, tag, 4);
+    launch_ts<T, Bufs, 1, WBW, m3, WBW>(t, b, s, tag, 5);
+  }
 }
 
 template <typename Bufs>

@@ -28,22 +28,28 @@ struct BufPtrs32 { float2* p[8]; };
 // partial buffer (tiles whose term list is split across several waves).
 struct WorkItem { int32_t tile, r0, c0, t0, t1, slot; };
 // Deterministic reduction of split macro tiles: C = sum of nslot partials.
-struct ReduceItem { int32_t tile, r0, c0, slot0, nslot, pad; };
+struct ReduceItem { int32_t tile, r0, c0, slot0, nslot, width; };
 
 // How a task list is cut into work items: about target_items work items of
 // equal K-work per launch, at most max_groups term groups per macro tile,
 // shape classes for edge items (edge 1 always, 0 never, -1 when more than a
 // fifth of the MFMA work would fall on empty 16 x 16 sub-tiles), macro tiles
-// listed in bands of `band` macro rows.
+// listed in bands of `band` macro rows, and the column chunks: wide 0 all
+// 32 wide, 1 a region's last chunk of at most 16 columns merged into the one
+// before it (a 32 x 48 tile), 2 all 48 wide.
 struct LaunchConfig {
   int target_items = 65536;
   int max_groups = 32;
   int edge = -1;
   int band = 4;
+  int wide = 0;
   bool operator==(const LaunchConfig& o) const {
-    return target_items == o.target_items && max_groups == o.max_groups && edge == o.edge && band == o.band;
+    return target_items == o.target_items && max_groups == o.max_groups && edge == o.edge && band == o.band &&
+           wide == o.wide;
   }
 };
+// shape classes of the work items (each its own kernel instantiation)
+constexpr int NCLASS = 6;
 
 // Tile list resident on the device, launched as one grouped kernel.
 struct DevTasks {
@@ -55,11 +61,12 @@ struct DevTasks {
   int ntiles = 0, nterms = 0, nwork = 0, nred = 0, nslots = 0;
   double flops_alg = 0.0, flops_exec = 0.0;
   // work items grouped by shape class, each class run by its own kernel:
-  // 0 full 32 x 32, 1 at most 16 rows, 2 at most 16 columns, 3 both; items
+  // 0 full 32 x 32, 1 at most 16 rows, 2 at most 16 columns, 3 both, 4 wide
+  // 32 x 48, 5 wide with at most 16 rows; items
   // of class c are work[cls_off[c] .. cls_off[c + 1]).  Edge items are
   // classed only when more than a set share of the MFMA work would fall on
   // empty 16 x 16 sub-tiles; otherwise every item is class 0.
-  int cls_off[5] = {0, 0, 0, 0, 0};
+  int cls_off[NCLASS + 1] = {0, 0, 0, 0, 0, 0, 0};
   bool edge_skip = false;
   // borrowed: the arrays live in caller-provided device memory (never freed here)
   bool borrowed = false;

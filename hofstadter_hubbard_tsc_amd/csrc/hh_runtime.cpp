@@ -753,12 +753,13 @@ hh_superblock* bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N
   s->merged.alloc(sizeof(double2) * (size_t)std::max<int64_t>(s->plan.merged_size, 1));
   s->X.alloc(sizeof(double2) * (size_t)std::max<int64_t>(s->plan.x_size, 1));
   s->merge.upload(s->plan.merge_tasks, ctx->stream);
-  s->st1.upload(s->plan.stage1, ctx->stream);
-  s->st2.upload(s->plan.stage2, ctx->stream);
+  const LaunchConfig cfg = ctx->forced_cfg ? ctx->forced : LaunchConfig();
+  s->st1.upload(s->plan.stage1, ctx->stream, nullptr, 0, cfg);
+  s->st2.upload(s->plan.stage2, ctx->stream, nullptr, 0, cfg);
   launch_tiles(s->merge, bufs_of(nullptr, nullptr, sys, env, nullptr, s->merged.c(), nullptr, nullptr), ctx->stream,
                PREC_FP64);
   check_hip(hipStreamSynchronize(ctx->stream), "bind");
-  if (ctx->autotune) {
+  if (ctx->autotune && !ctx->forced_cfg) {
     try {
       autotune_superblock(ctx, s);
     } catch (...) {
@@ -3874,7 +3875,7 @@ int hh_run_tasks(hh_ctx* ctx, const void* tiles, int32_t ntiles, const void* ter
   return guard(ctx, [&] {
     const TaskList tl = checked_tasks(tiles, ntiles, terms, nterms, bufs_dev, precision);
     DevTasks dt;
-    dt.upload(tl, ctx->stream);
+    dt.upload(tl, ctx->stream, nullptr, 0, ctx->forced_cfg ? ctx->forced : LaunchConfig());
     BufPtrs b;
     for (int i = 0; i < 8; ++i) b.p[i] = (double2*)bufs_dev[i];
     launch_tiles(dt, b, ctx->stream, precision);
@@ -3895,7 +3896,7 @@ int hh_run_tasks_noalloc(hh_ctx* ctx, const void* tiles, int32_t ntiles, const v
     need(scr != nullptr, "no fixed scratch on this device");
     DevTasks dt;
     try {
-      dt.upload(tl, ctx->stream, scr, NOALLOC_SCRATCH_BYTES);
+      dt.upload(tl, ctx->stream, scr, NOALLOC_SCRATCH_BYTES, ctx->forced_cfg ? ctx->forced : LaunchConfig());
     } catch (const HHCapacity& e) {
       throw HHError(1, e.what());
     }

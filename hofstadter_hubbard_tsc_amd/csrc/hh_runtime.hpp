@@ -136,6 +136,10 @@ struct hh_ctx {
   // autotuning of the H.psi launches (hh_tune.cpp): on/off, the chosen
   // configurations by task-list shape key, counters
   int autotune = 0;
+  // a launch cut forced on every task list uploaded from now on (tests,
+  // experiments); overrides the autotuner when set
+  bool forced_cfg = false;
+  hh::LaunchConfig forced;
   std::map<uint64_t, hh::LaunchConfig> tune_cache;
   int tune_runs = 0, tune_hits = 0;
   double tune_seconds = 0.0;

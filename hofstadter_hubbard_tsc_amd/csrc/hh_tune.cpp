@@ -53,6 +53,16 @@ std::vector<LaunchConfig> candidates() {
     x.band = band;
     c.push_back(x);
   }
+  // wide 32 x 48 tiles for regions that end in a short column chunk
+  for (int wide : {1, 2})
+    for (int edge : {0, 1})
+      for (int ti : {32768, 65536}) {
+        LaunchConfig x;
+        x.wide = wide;
+        x.edge = edge;
+        x.target_items = ti;
+        c.push_back(x);
+      }
   return c;
 }
 
@@ -104,7 +114,8 @@ LaunchConfig choose(hh_ctx* ctx, const TaskList& tl, const Bufs& bufs, int tag, 
     if (best_ms > 0.98 * base) best = LaunchConfig();
     if (verbose())
       std::fprintf(stderr, "[hhdmrg] autotune stage %d: %.3f ms fixed, %.3f ms best (items %d, groups %d, edge %d, "
-                   "band %d)\n", tag, base, best_ms, best.target_items, best.max_groups, best.edge, best.band);
+                   "band %d, wide %d)\n", tag, base, best_ms, best.target_items, best.max_groups, best.edge, best.band,
+                   best.wide);
   } catch (...) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -197,6 +208,27 @@ int hh_set_autotune(hh_ctx* ctx, int mode) {
   return guard(ctx, [&] {
     need(mode == 0 || mode == 1, "autotune mode must be 0 or 1");
     ctx->autotune = mode;
+  });
+}
+
+int hh_set_launch_config(hh_ctx* ctx, int target_items, int max_groups, int edge, int band, int wide) {
+  if (!ctx) return 1;
+  return guard(ctx, [&] {
+    if (target_items == 0) {
+      ctx->forced_cfg = false;
+      return;
+    }
+    LaunchConfig c;
+    c.target_items = target_items;
+    c.max_groups = max_groups;
+    c.edge = edge;
+    c.band = band;
+    c.wide = wide;
+    need(target_items >= 1 && max_groups >= 1 && max_groups <= 1024 && band >= 1 && edge >= -1 && edge <= 1 &&
+             wide >= 0 && wide <= 2,
+         "launch configuration out of range");
+    ctx->forced = c;
+    ctx->forced_cfg = true;
   });
 }
 

@@ -386,6 +386,12 @@ class Context:
         created from now on (cached by task-list shape statistics)."""
         self._rc(L.load().hh_set_autotune(self.ptr, 1 if on else 0), "hh_set_autotune")
 
+    def set_launch_config(self, target_items=0, max_groups=32, edge=-1, band=4, wide=0):
+        """Force one H.psi launch cut on every superblock bound and batch
+        created from now on; target_items=0 clears it."""
+        self._rc(L.load().hh_set_launch_config(self.ptr, int(target_items), int(max_groups), int(edge), int(band),
+                                               int(wide)), "hh_set_launch_config")
+
     def autotune_stats(self):
         k, t, h, sec = L.i32(), L.i32(), L.i32(), L.f64()
         self._rc(L.load().hh_autotune_stats(self.ptr, C.byref(k), C.byref(t), C.byref(h), C.byref(sec)),

@@ -222,6 +222,15 @@ int hh_batch_set_graph(hh_batch* b, int on);
 int hh_set_autotune(hh_ctx* ctx, int mode);
 /* shape keys tuned so far, timing runs, cache hits, seconds spent timing */
 int hh_autotune_stats(const hh_ctx* ctx, int32_t* keys, int32_t* tunes, int32_t* hits, double* seconds);
+/* Force one cut on every H.psi task list uploaded from now on (superblocks
+ * bound, batches created), in place of the fixed cut and of the autotuner:
+ * about target_items work items per launch, at most max_groups term groups
+ * per output tile, edge shape classes (1 on, 0 off, -1 by the 20 % rule),
+ * tiles ordered in bands of `band` tile rows, wide 0 (32-column tiles), 1 (a
+ * last chunk of at most 16 columns merged into a 48-column tile) or 2 (all
+ * 48-column tiles).  target_items = 0 clears it.  No counterpart in the
+ * reference; for tests and experiments. */
+int hh_set_launch_config(hh_ctx* ctx, int target_items, int max_groups, int edge, int band, int wide);
 
 /* Lanczos: replaces lanczos_memo_new (code/lanczos.f90:4-176) with levels=1:
  * psi_dev holds the start vector on entry and the ground state on exit. */

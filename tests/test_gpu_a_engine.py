@@ -27,10 +27,32 @@ def _rand(rng, n):
     return rng.normal(size=n) + 1j * rng.normal(size=n)
 
 
+CUTS = {
+    "fixed": None,
+    "wide-last": dict(target_items=65536, max_groups=32, edge=1, band=4, wide=1),
+    "wide-all": dict(target_items=65536, max_groups=32, edge=0, band=4, wide=2),
+    "wide-all-split": dict(target_items=64, max_groups=4, edge=1, band=2, wide=2),
+    "split-band8": dict(target_items=64, max_groups=64, edge=0, band=8, wide=0),
+}
+
+
+@pytest.mark.parametrize("cut", list(CUTS))
 @pytest.mark.parametrize("precision,tol", [(0, 1e-13), (1, 2e-5)])
-def test_engine_random_tiles(ctx, precision, tol):
+def test_engine_random_tiles(ctx, precision, tol, cut):
     """Every operand op (N, T, C, conj), ragged M/N/K, K=0 elementwise terms,
-    several terms per tile, the accumulate flag; FP64 and FP32 MFMA."""
+    several terms per tile, the accumulate flag; FP64 and FP32 MFMA; every
+    way of cutting the list into work items (32- and 48-column tiles, term
+    lists split over many waves, shape classes on and off)."""
+    from hofstadter_hubbard_tsc_amd import _lib as L
+    if CUTS[cut]:
+        ctx.set_launch_config(**CUTS[cut])
+    try:
+        _random_tiles(ctx, precision, tol)
+    finally:
+        ctx.set_launch_config(0)
+
+
+def _random_tiles(ctx, precision, tol):
     from hofstadter_hubbard_tsc_amd import _lib as L
     rng = np.random.default_rng(11)
     A = _rand(rng, 40000)

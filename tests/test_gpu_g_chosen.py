@@ -65,23 +65,33 @@ def _prepared(ctx, m, kept, capture=0):
     return d, bg + 2
 
 
-def test_fp32_storage_matvec_matches_oracle(ctx):
+@pytest.mark.parametrize("wide", [0, 2])
+def test_fp32_storage_matvec_matches_oracle(ctx, wide):
+    """FP32-storage H.psi against the oracle, with 32-column tiles and with
+    every tile 48 columns wide (and FP64 H.psi of the wide cut to 1e-13)."""
     import torch
     m = lat.Model(kind="tj", Lx=4, Ly=4, J=0.4, t=1.0, flux=0.125, n_elec=14)
     d, _ = _prepared(ctx, m, 40)
     ls, le, psi = d.wave()
     sysb, envb = d.block(0, ls), d.block(1, le)
-    sb = ctx.bind(sysb, envb, m.n_elec)
+    if wide:
+        ctx.set_launch_config(target_items=4096, max_groups=8, edge=1, band=4, wide=wide)
+    try:
+        sb = ctx.bind(sysb, envb, m.n_elec)
+    finally:
+        ctx.set_launch_config(0)
     H = oracle_superblock(ctx, sysb, envb, m, m.n_elec)
     ref = H.matvec(host(psi))
     y = torch.empty_like(psi)
+    ctx.model_wave(sb, psi, y)
+    assert np.abs(host(y) - ref).max() <= 1e-13 * np.abs(ref).max()
     ctx.set_precision(2)
     try:
         ctx.model_wave(sb, psi, y)
     finally:
         ctx.set_precision(0)
     err = np.abs(host(y) - ref).max() / np.abs(ref).max()
-    print("FP32-storage H.psi relative error %.2e" % err)
+    print("FP32-storage H.psi relative error %.2e (wide %d)" % (err, wide))
     assert err <= 1e-5
 
 
