@@ -5,6 +5,7 @@ Usage: python -m hofstadter_hubbard_tsc_amd.build   (or build() from Python)
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -29,7 +30,7 @@ def _run(cmd):
 def build(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
@@ -39,8 +40,13 @@ def build(verbose=False):
             cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # one compiler process per source, at most `jobs` at a time
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for _ in pool.map(_run, cmds):
+            pass
     cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", "-o", LIB + ".tmp"] + objs + [
         "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     _run(cmd)

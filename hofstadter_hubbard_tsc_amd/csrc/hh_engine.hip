@@ -45,8 +45,10 @@ static constexpr int MTC = 16 * WB;     // macro tile columns
 // (each operand panel of the rows is read for 48 output columns, not 32).
 static constexpr int WBW = 3;
 static constexpr int MTCW = 16 * WBW;
-// partial-sum slot of a split macro tile (the widest tile, column-major)
-static constexpr int SLOT = MTR * MTCW;
+// partial sums of a split macro tile are stored column-major in strips of
+// MTR x 16 elements: a 32-wide tile's partial takes two strips, a 48-wide
+// one three, so plain tiles need no room for the wide shape
+static constexpr int STRIP = MTR * 16;
 static constexpr int WAVES = 4;        // independent waves per workgroup
 // FP64 complex products by the 3M scheme (3 real MFMAs per complex product)
 // in every shape class; HH_CPLX3 = 0 builds the four-product form instead.
@@ -162,16 +164,17 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
       for (int ci = 0; ci < nc; ++ci)
         for (int ri = rb; ri < std::min(nr, rb + band); ++ri) {
           const int r = ri * MTR, c = cols[ci].first;
+          const int width = cols[ci].second > MTC ? MTCW : MTC, strips = width / 16;
           if (g == 1) {
             items.push_back({tw[i], WorkItem{i, r, c, t.t0, t.t1, -1}, cols[ci].second});
           } else {
             for (int q = 0; q < g; ++q) {
               double wq = 0.0;
               for (int k = cuts[q]; k < cuts[q + 1]; ++k) wq += (double)std::max(tl.terms[k].K, 1);
-              items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q}, cols[ci].second});
+              items.push_back({wq, WorkItem{i, r, c, cuts[q], cuts[q + 1], slot + q * strips}, cols[ci].second});
             }
-            reds.push_back(ReduceItem{i, r, c, slot, g, cols[ci].second > MTC ? MTCW : MTC});
-            slot += g;
+            reds.push_back(ReduceItem{i, r, c, slot, g, width});
+            slot += g * strips;
           }
         }
   }
@@ -223,7 +226,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
     const size_t o_work = o_terms + up(sizeof(Term) * nterms);
     const size_t o_red = o_work + up(sizeof(WorkItem) * nwork);
     const size_t o_part = o_red + up(sizeof(ReduceItem) * nred);
-    const size_t end = o_part + sizeof(double2) * (size_t)nslots * SLOT;
+    const size_t end = o_part + sizeof(double2) * (size_t)nslots * STRIP;
     if (end > cap) {
       nwork = nred = nslots = ntiles = nterms = 0;
       throw HHCapacity("task list needs " + std::to_string(end) + " bytes of device tables and partial sums, the "
@@ -262,7 +265,7 @@ void DevTasks::upload(const TaskList& tl, hipStream_t s, void* place, size_t cap
   if (nred) {
     check_hip(hipMalloc(&red, sizeof(ReduceItem) * nred), "hipMalloc red");
     check_hip(hipMemcpyAsync(red, reds.data(), sizeof(ReduceItem) * nred, hipMemcpyHostToDevice, s), "copy red");
-    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * SLOT), "hipMalloc partial");
+    check_hip(hipMalloc(&partial, sizeof(double2) * (size_t)nslots * STRIP), "hipMalloc partial");
   }
   check_hip(hipStreamSynchronize(s), "task upload");
 }
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(256, 2) void k_tiles(const Tile* __restrict__ tiles
         if (r < Mv && c < Nv) {
           double re = (double)accR[a][b][v], im = (double)accI[a][b][v];
           if (w.slot >= 0) {
-            partial[(int64_t)w.slot * SLOT + r + c * MTR] = make_double2(re, im);
+            partial[(int64_t)w.slot * STRIP + r + c * MTR] = make_double2(re, im);
           } else {
             const int64_t o = tile.c_off + (int64_t)(r0 + r) + (int64_t)(c0 + c) * tile.c_ld;
             if (tile.accumulate) {
@@ -608,6 +611,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
     if (bufs.active && bufs.active[tile.c_buf >> 3] == 0) return;
   }
   const int width = ri.width;   // 32, or 48 for a wide macro tile
+  const int strips = width / 16; // partial-sum strips per slot
   const int Mv = min(MTR, tile.M - ri.r0), Nv = min(width, tile.N - ri.c0);
   typename StoreOf<Bufs>::type* __restrict__ C = bufs.p[tile.c_buf];
   for (int e = threadIdx.x; e < MTR * width; e += 256) {
@@ -615,7 +619,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ tiles, 
     if (r >= Mv || c >= Nv) continue;
     double re = 0.0, im = 0.0;
     for (int q = 0; q < ri.nslot; ++q) {
-      const double2 x = partial[(int64_t)(ri.slot0 + q) * SLOT + e];
+      const double2 x = partial[(int64_t)(ri.slot0 + q * strips) * STRIP + e];
       re += x.x;
       im += x.y;
     }

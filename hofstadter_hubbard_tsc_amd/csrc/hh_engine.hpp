@@ -25,9 +25,11 @@ struct BufTable {
 struct BufPtrs32 { float2* p[8]; };
 // One wave's share of the work: a 32x32 macro tile of output region `tile`
 // and the term range [t0, t1); slot >= 0 sends the partial sum to the
-// partial buffer (tiles whose term list is split across several waves).
+// partial buffer (tiles whose term list is split across several waves),
+// starting at strip `slot` (a strip is 32 x 16 complex elements).
 struct WorkItem { int32_t tile, r0, c0, t0, t1, slot; };
-// Deterministic reduction of split macro tiles: C = sum of nslot partials.
+// Deterministic reduction of split macro tiles: C = sum of nslot partials,
+// the q-th starting at strip slot0 + q * width / 16.
 struct ReduceItem { int32_t tile, r0, c0, slot0, nslot, width; };
 
 // How a task list is cut into work items: about target_items work items of
