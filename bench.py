@@ -546,14 +546,19 @@ def main():
             print("# preconditioner trained on %d triples in %.2f s: loss %.4f -> %.4f"
                   % (ntrip, train["seconds"], losses[0], losses[-1]), file=sys.stderr, flush=True)
     forks = {}
+    start = bg + args.warmup
+    free_b, total_b = torch.cuda.mem_get_info()
+    state_gb = (total_b - free_b) / 1e9
+    print("# device memory in use after the preparation: %.1f GB" % state_gb, file=sys.stderr, flush=True)
     if chosen:
-        # identical starting states for the other forms (device copies)
+        # identical starting states for the other forms (device copies); for
+        # --steps, only the blocks and truncations those steps touch
+        window = None if args.sweep else ((start - 1, start + 1), (n - start - args.steps - 2, n - start))
         for name, mode in (("chosen", 1), ("chosen_learned", 0)):
-            f = drv.fork()
+            f = drv.fork(window)
             f.set_solver(2)
             f.set_precond(mode)
             forks[name] = f
-    start = bg + args.warmup
     ctx.reset_stats()
     ctx.set_profiling(True)
     torch.cuda.synchronize()
@@ -683,6 +688,7 @@ def main():
                    "sweep_seconds": round(wall, 2) if args.sweep else None,
                    "matvec_share": round(mv_time / max(wall, 1e-30), 3),
                    "superblock_dim": int(recs[-1].dim) if recs else 0,
+                   "device_gb_after_preparation": round(state_gb, 1),
                    "energy": recs[-1].energy if recs else None,
                    "lanczos_iters": [r.iters for r in recs],
                    "steps_at_iteration_cap": sum(1 for r in recs if not r.converged),

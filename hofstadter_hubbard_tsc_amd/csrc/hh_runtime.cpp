@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <functional>
 #include <initializer_list>
@@ -818,6 +819,17 @@ struct Driver {
 
   static constexpr double TRUN_ERR = 1.0000000180025095e-18;   // float32(1e-18) widened
 
+  // the stored block or truncation of `len` sites (a windowed fork holds
+  // only some of them)
+  template <typename T>
+  static T* held(const std::map<int, std::unique_ptr<T>>& m, int len, const char* what) {
+    auto it = m.find(len);
+    if (it == m.end())
+      throw HHError(1, std::string("this driver holds no ") + what + " of " + std::to_string(len) +
+                           " sites (outside the window of a windowed fork, or not built yet)");
+    return it->second.get();
+  }
+
   int kmin(int kmax) const { return std::min(3 * kmax / 4, 3600); }
 
   // wall time per phase of the current step: bind, wave, lanczos, trunc, grow
@@ -1009,8 +1021,8 @@ struct Driver {
       const double t0 = wall();
       const int64_t mv0 = ctx->stats.matvecs;
       const double fl0 = ctx->stats.flops_alg;
-      hh_block* sb_sys = sysb.at(label).get();
-      hh_block* sb_env = envb.at(ns_ - label - 2).get();
+      hh_block* sb_sys = held(sysb, label, "sys block");
+      hh_block* sb_env = held(envb, ns_ - label - 2, "env block");
       lap_start();
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
       lap(0);
@@ -1025,7 +1037,8 @@ struct Driver {
         }
       } else {
         double2* np = psi_buf(n, psi2);
-        wave_trans(ctx, 0, prev_sb.get(), psi.c(), syst.at(label).get(), envt.at(sb_env->d.length + 1).get(),
+        wave_trans(ctx, 0, prev_sb.get(), psi.c(), held(syst, label, "sys truncation"),
+                   held(envt, sb_env->d.length + 1, "env truncation"),
                    sb.get(), np);
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
@@ -1059,8 +1072,8 @@ struct Driver {
       const double t0 = wall();
       const int64_t mv0 = ctx->stats.matvecs;
       const double fl0 = ctx->stats.flops_alg;
-      hh_block* sb_sys = sysb.at(label - 1).get();
-      hh_block* sb_env = envb.at(ns_ - label - 1).get();
+      hh_block* sb_sys = held(sysb, label - 1, "sys block");
+      hh_block* sb_env = held(envb, ns_ - label - 1, "env block");
       lap_start();
       std::unique_ptr<hh_superblock> sb(bind(ctx, sb_sys, sb_env, m.n_elec));
       lap(0);
@@ -1075,8 +1088,8 @@ struct Driver {
         }
       } else {
         double2* np = psi_buf(n, psi2);
-        wave_trans(ctx, 1, prev_sb.get(), psi.c(), syst.at(sb_sys->d.length + 1).get(),
-                   envt.at(ns_ - label - 1).get(), sb.get(), np);
+        wave_trans(ctx, 1, prev_sb.get(), psi.c(), held(syst, sb_sys->d.length + 1, "sys truncation"),
+                   held(envt, ns_ - label - 1, "env truncation"), sb.get(), np);
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
         p = psi.c();
@@ -3404,10 +3417,19 @@ int hh_driver_copy_precond(hh_driver* dst, const hh_driver* src) {
   });
 }
 
-int hh_driver_fork(const hh_driver* src, hh_driver** out) {
+namespace {
+// a copy of the driver holding the blocks and truncations whose length lies in
+// [lo[side], hi[side]] on each side (all of them with the full windows)
+int fork_driver(const hh_driver* src, const int lo[2], const int hi[2], hh_driver** out) {
   if (!src || !out) return 1;
   hh_ctx* ctx = src->d.ctx;
   return guard(ctx, [&] {
+    for (int side = 0; side < 2; ++side) need(lo[side] <= hi[side], "empty length window");
+    if (src->d.prev_sb) {
+      const int ls = src->d.prev_sb->sys->d.length, le = src->d.prev_sb->env->d.length;
+      need(ls >= lo[0] && ls <= hi[0] && le >= lo[1] && le <= hi[1],
+           "the length window leaves out a block of the driver's current superblock");
+    }
     std::unique_ptr<hh_driver> h(new hh_driver());
     Driver& d = h->d;
     const Driver& s = src->d;
@@ -3424,6 +3446,7 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out) {
       const auto& sb = side == 0 ? s.sysb : s.envb;
       auto& db = side == 0 ? d.sysb : d.envb;
       for (const auto& kv : sb) {
+        if (kv.first < lo[side] || kv.first > hi[side]) continue;
         std::unique_ptr<hh_block> b(new hh_block());
         b->d = kv.second->d;
         copy_dev(b->arena, kv.second->arena);
@@ -3432,6 +3455,7 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out) {
       const auto& st = side == 0 ? s.syst : s.envt;
       auto& dt = side == 0 ? d.syst : d.envt;
       for (const auto& kv : st) {
+        if (kv.first < lo[side] || kv.first > hi[side]) continue;
         std::unique_ptr<hh_trunc> t(new hh_trunc());
         const hh_trunc& o = *kv.second;
         t->side = o.side;
@@ -3458,6 +3482,17 @@ int hh_driver_fork(const hh_driver* src, hh_driver** out) {
     check_hip(hipStreamSynchronize(ctx->stream), "fork");
     *out = h.release();
   });
+}
+}  // namespace
+
+int hh_driver_fork(const hh_driver* src, hh_driver** out) {
+  const int lo[2] = {INT_MIN, INT_MIN}, hi[2] = {INT_MAX, INT_MAX};
+  return fork_driver(src, lo, hi, out);
+}
+
+int hh_driver_fork_window(const hh_driver* src, int sys_lo, int sys_hi, int env_lo, int env_hi, hh_driver** out) {
+  const int lo[2] = {sys_lo, env_lo}, hi[2] = {sys_hi, env_hi};
+  return fork_driver(src, lo, hi, out);
 }
 
 int hh_driver_set_solver(hh_driver* h, int solver) {

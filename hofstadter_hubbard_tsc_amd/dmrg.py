@@ -614,10 +614,17 @@ class Driver:
             self._dims[key] = self.ctx.bind(sysb, envb, self.ctx.model.n_elec).dim
         return _dptr(psi, self._dims[key])
 
-    def fork(self):
-        """An independent copy of this driver's state (made on the device)."""
+    def fork(self, window=None):
+        """An independent copy of this driver's state (made on the device).
+        window = ((sys_lo, sys_hi), (env_lo, env_hi)) copies only the blocks
+        and truncations of those lengths (hh_driver_fork_window)."""
         p = C.c_void_p()
-        self.ctx._rc(L.load().hh_driver_fork(self.ptr, C.byref(p)), "hh_driver_fork")
+        if window is None:
+            self.ctx._rc(L.load().hh_driver_fork(self.ptr, C.byref(p)), "hh_driver_fork")
+        else:
+            (slo, shi), (elo, ehi) = window
+            self.ctx._rc(L.load().hh_driver_fork_window(self.ptr, int(slo), int(shi), int(elo), int(ehi), C.byref(p)),
+                         "hh_driver_fork_window")
         d = Driver.__new__(Driver)
         d.ctx = self.ctx
         d.ptr = p

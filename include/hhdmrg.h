@@ -355,6 +355,13 @@ int hh_driver_set_solver(hh_driver* d, int solver);
  * device; both continue on their own.  No counterpart in the reference: it
  * lets one state be swept by two solvers. */
 int hh_driver_fork(const hh_driver* src, hh_driver** out);
+/* hh_driver_fork holding only the blocks and truncations of sys_lo..sys_hi
+ * sites on the sys side and env_lo..env_hi on the env side (the current
+ * superblock's blocks must lie inside): the copy for a few steps that touch
+ * only those, at a fraction of the memory.  A step that needs a block
+ * outside the window fails with status 1.  No counterpart in the reference
+ * (whose blocks live on disk, model_to_disk, code/space_info.f90:1523). */
+int hh_driver_fork_window(const hh_driver* src, int sys_lo, int sys_hi, int env_lo, int env_hi, hh_driver** out);
 
 /* ---- learned preconditioner of the chosen form (solver 2) ----------------
  * No counterpart in the reference, whose steps all use lanczos_memo_new

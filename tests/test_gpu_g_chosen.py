@@ -16,6 +16,8 @@
 * The fallback to the mixed Lanczos, forced from the host by an unreachable
   residual-reduction factor: flagged in the records, energies unchanged.
 * A network copied between drivers acts exactly like its source.
+* A windowed fork (hh_driver_fork_window) sweeps the steps inside its window
+  exactly like a full fork and refuses a step outside it.
 * Below the hand-over size (hh_driver_set_min_dim) solver 2 is the plain
   Lanczos, bit for bit.
 * Training: the loss falls; what the trained network gains over the method
@@ -208,6 +210,33 @@ def test_small_superblocks_go_to_the_plain_solver(ctx):
     rb = b.sweep(0, start, start + 2, 1e-11, TRUN, 32, phase=2)
     for x, y in zip(ra, rb):
         assert x.energy == y.energy and x.iters == y.iters and x.hpsi_fp64 == y.hpsi_fp64 and y.hpsi_fp32 == 0
+
+
+def test_windowed_fork_sweeps_like_a_full_fork(ctx):
+    """hh_driver_fork_window: a copy holding only the blocks and truncations
+    that the next two left-to-right steps touch sweeps them exactly like a
+    full copy (energies, iteration counts, kept counts); a third step, whose
+    env block lies outside the window, fails with status 1; a window that
+    leaves out the current superblock's blocks is refused."""
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.05, n_elec=10)
+    d, start = _prepared(ctx, m, 32)
+    n, steps = m.nsite, 2
+    # a step at sys length l reads sys block l, env block n - l - 2, the sys
+    # truncation of l and the env truncation of n - l - 1; the current
+    # superblock holds sys block start - 1 and env block n - start - 1
+    win = ((start - 1, start + 1), (n - start - steps - 1, n - start - 1))
+    full, part = d.fork(), d.fork(win)
+    ra = full.sweep(0, start, start + steps, 1e-11, TRUN, 32, phase=2)
+    rb = part.sweep(0, start, start + steps, 1e-11, TRUN, 32, phase=2)
+    assert len(ra) == len(rb) == steps
+    for x, y in zip(ra, rb):
+        assert x.energy == y.energy and x.iters == y.iters and x.num_state == y.num_state, (x, y)
+    with pytest.raises(HHError) as e:
+        part.sweep(0, start + steps, start + steps + 1, 1e-11, TRUN, 32, phase=2)
+    assert "window" in str(e.value)
+    with pytest.raises(HHError):
+        d.fork(((start + 2, start + 3), (1, n)))
 
 
 def test_forced_fallback_is_flagged_and_correct(ctx):
