@@ -107,6 +107,9 @@ SIGNATURES = [
     ("hh_driver_wave", i32, [vp, P(i32), P(i32), P(i64), vp]),
     ("hh_driver_set_solver", i32, [vp, i32]),
     ("hh_driver_fork", i32, [vp, P(vp)]),
+    ("hh_eigh_sectors", i32, [vp, i32, P(i32), P(i32), vp, vp, i32]),
+    ("hh_set_eigensolver", i32, [vp, i32]),
+    ("hh_eigensolver_status", i32, [vp, P(i32), P(i64), P(i32), P(f64)]),
     ("hh_driver_fork_window", i32, [vp, i32, i32, i32, i32, P(vp)]),
     ("hh_driver_capture", i32, [vp, i32]),
     ("hh_driver_train_precond", i32, [vp, i32, f64, P(f64), P(i32)]),

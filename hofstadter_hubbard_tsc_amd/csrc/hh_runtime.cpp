@@ -545,7 +545,11 @@ void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, 
   }
 }
 
-// Reduced density matrix + Get_truncation_operator (code/get_den.f90:184-428)
+// Reduced density matrix + Get_truncation_operator (code/get_den.f90:184-428).
+// The eigendecomposition (ZHEEV per sector, get_den.f90:242) runs in the
+// hand-written eigensolver (hh_heev.hip: every eigenvalue, then the
+// eigenvectors of the kept ones only) or, with eig_solver 1 or for a
+// truncation the hand-written solver hands back, rocSOLVER zheevd per sector.
 hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi, int kept_min, int kept_max,
                    double trun_err) {
   const SuperBasis& S = sb->plan.sb;
@@ -555,7 +559,7 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   TaskList tl = compile_density(S, side, rsec, roff, rdim);
   int64_t total = 0;
   for (int d : rdim) total += (int64_t)d * d;
-  auto* t = new hh_trunc();
+  std::unique_ptr<hh_trunc> t(new hh_trunc());
   t->side = side;
   t->eb = side == 0 ? sb->plan.sbs : sb->plan.ebs;
   t->evec.alloc(sizeof(double2) * (size_t)std::max<int64_t>(total, 1));
@@ -569,40 +573,35 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
     nev += d;
     dmax = std::max(dmax, d);
   }
+  const bool hand = ctx->eig_solver == 0 && dmax <= HEEV_MAXN;
+  if (ctx->eig_solver == 0 && !hand) ++ctx->eig_handbacks;
   DevMem evals, work, info;
   evals.alloc(sizeof(double) * (size_t)std::max<int64_t>(nev, 1));
-  work.alloc(sizeof(double) * (size_t)dmax * (hh_ctx::NSIDE + 1));
-  info.alloc(sizeof(rocblas_int) * rdim.size() + 16);
   double t1 = 0.0;
   if (verbose()) {
     check_hip(hipStreamSynchronize(ctx->stream), "density matrix");
     t1 = wall();
   }
-  heev_sectors(ctx, t->evec.c(), roff, rdim, (double*)evals.p, (double*)work.p, dmax, (rocblas_int*)info.p,
-               [&]() { delete t; });
-  std::vector<double> ev(nev);
-  std::vector<rocblas_int> inf(rdim.size());
-  check_hip(hipMemcpyAsync(ev.data(), evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-  check_hip(hipMemcpyAsync(inf.data(), info.p, sizeof(rocblas_int) * rdim.size(), hipMemcpyDeviceToHost, ctx->stream),
-            "d2h");
-  check_hip(hipStreamSynchronize(ctx->stream), "eig");
-  if (verbose()) {
-    int64_t cols = 0;
-    for (int d : rdim) cols += d;
-    std::fprintf(stderr, "# truncation side=%d: density matrix %.4f s, zheevd %.4f s (%zu sectors, largest %d, %lld "
-                 "columns)\n", side, t1 - t0, wall() - t1, rdim.size(), dmax, (long long)cols);
-    std::fflush(stderr);
+  std::vector<rocblas_int> inf;
+  if (hand) {
+    ctx->heev.timing = verbose();
+    ctx->heev.values(ctx->stream, t->evec.c(), roff, rdim, (double*)evals.p);
+  } else {
+    work.alloc(sizeof(double) * (size_t)dmax * (hh_ctx::NSIDE + 1));
+    info.alloc(sizeof(rocblas_int) * rdim.size() + 16);
+    heev_sectors(ctx, t->evec.c(), roff, rdim, (double*)evals.p, (double*)work.p, dmax, (rocblas_int*)info.p, []() {});
+    inf.resize(rdim.size());
+    check_hip(hipMemcpyAsync(inf.data(), info.p, sizeof(rocblas_int) * rdim.size(), hipMemcpyDeviceToHost,
+                             ctx->stream), "d2h");
   }
+  std::vector<double> ev(nev);
+  check_hip(hipMemcpyAsync(ev.data(), evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+  check_hip(hipStreamSynchronize(ctx->stream), "eig");
+  const double t2 = verbose() ? wall() : 0.0;
   for (auto x : inf)
-    if (x != 0) {
-      delete t;
-      throw HHError(-2, "rocsolver_zheevd did not converge");
-    }
+    if (x != 0) throw HHError(-2, "rocsolver_zheevd did not converge");
   for (double x : ev)
-    if (!std::isfinite(x)) {
-      delete t;
-      throw HHError(-2, "rocsolver_zheevd returned a non-finite eigenvalue");
-    }
+    if (!std::isfinite(x)) throw HHError(-2, "the density-matrix eigensolver returned a non-finite eigenvalue");
   // global descending sort, ties by position in the concatenated ascending
   // per-sector spectra (deterministic stand-in for QuickSort_Double)
   std::vector<int64_t> order(nev);
@@ -644,6 +643,40 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
       for (int k = 0; k < rdim[i]; ++k) owner[o++] = (int)i;
     for (int64_t i = 0; i < num; ++i) keep[owner[order[i]]]++;
   }
+  if (hand && !ctx->heev.vectors(ctx->stream, t->evec.c(), keep, (double*)evals.p)) {
+    // the kept vectors could not be orthonormalised: redo this truncation
+    // (density matrix included) with zheevd
+    ++ctx->eig_handbacks;
+    if (verbose()) {
+      std::fprintf(stderr, "# truncation side=%d: the hand-written eigensolver hands back (%s)\n", side,
+                   ctx->heev.fail_info.c_str());
+      std::fflush(stderr);
+    }
+    const int saved = ctx->eig_solver;
+    ctx->eig_solver = 1;
+    try {
+      hh_trunc* r = truncate(ctx, sb, side, psi, kept_min, kept_max, trun_err);
+      ctx->eig_solver = saved;
+      return r;
+    } catch (...) {
+      ctx->eig_solver = saved;
+      throw;
+    }
+  }
+  if (verbose()) {
+    if (hand) check_hip(hipStreamSynchronize(ctx->stream), "eigenvectors");
+    std::fprintf(stderr, "# truncation side=%d: density matrix %.4f s, %s %.4f s + vectors %.4f s (%zu sectors, "
+                 "largest %d, %lld columns, %lld kept%s)\n", side, t1 - t0, hand ? "eigenvalues" : "zheevd", t2 - t1,
+                 wall() - t2, rdim.size(), dmax, (long long)nev, (long long)num,
+                 hand ? (", " + std::to_string(ctx->heev.ns_iterations) + " orthonormalisation steps, " +
+                         std::to_string(ctx->heev.unresolved) + " completed, largest cluster " +
+                         std::to_string(ctx->heev.max_cluster) + "; ms: reduction " +
+                         std::to_string(ctx->heev.ms[0]) + " bisection " + std::to_string(ctx->heev.ms[1]) +
+                         " inverse iteration " + std::to_string(ctx->heev.ms[2]) + " orthonormalisation " +
+                         std::to_string(ctx->heev.ms[3]) + " back-transformation " +
+                         std::to_string(ctx->heev.ms[4])).c_str() : "");
+    std::fflush(stderr);
+  }
   for (size_t i = 0; i < rdim.size(); ++i) {
     if (keep[i] == 0) continue;
     KeptSector ks;
@@ -657,7 +690,7 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   t->num_state = (int)num;
   t->err = err;
   t->entropy = ent;
-  return t;
+  return t.release();
 }
 
 hh_trunc* trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side) {
@@ -3662,6 +3695,62 @@ int hh_driver_pair_block(hh_driver* h, int sys_len, const double* psi, int nquad
     }
     pair_same_block(h->d.ctx, h->d, sys_len, (const double2*)psi, q, (cplx*)p_out);
   });
+}
+
+int hh_eigh_sectors(hh_ctx* ctx, int nsec, const int32_t* dim, const int32_t* keep, double* blocks_dev,
+                    double* evals_dev, int solver) {
+  return guard(ctx, [&] {
+    need(nsec >= 1 && dim && keep && blocks_dev && evals_dev, "null or empty argument");
+    need(solver == 0 || solver == 1, "solver must be 0 (hand-written) or 1 (zheevd)");
+    std::vector<int64_t> off(nsec);
+    std::vector<int> d(nsec), kp(nsec);
+    int64_t o = 0;
+    int dmax = 1;
+    for (int i = 0; i < nsec; ++i) {
+      need(dim[i] >= 1 && keep[i] >= 0 && keep[i] <= dim[i], "bad block dimension or kept count");
+      off[i] = o;
+      d[i] = dim[i];
+      kp[i] = keep[i];
+      o += (int64_t)dim[i] * dim[i];
+      dmax = std::max(dmax, dim[i]);
+    }
+    double2* A = (double2*)blocks_dev;
+    if (solver == 0) {
+      if (dmax > HEEV_MAXN) throw HHError(3, "a block is larger than the hand-written eigensolver takes");
+      ctx->heev.values(ctx->stream, A, off, d, evals_dev);
+      if (!ctx->heev.vectors(ctx->stream, A, kp, evals_dev))
+        throw HHError(3, "the hand-written eigensolver handed the problem back: " + ctx->heev.fail_info);
+    } else {
+      DevMem work, info;
+      work.alloc(sizeof(double) * (size_t)dmax * (hh_ctx::NSIDE + 1));
+      info.alloc(sizeof(rocblas_int) * nsec + 16);
+      heev_sectors(ctx, A, off, d, evals_dev, (double*)work.p, dmax, (rocblas_int*)info.p, []() {});
+      std::vector<rocblas_int> inf(nsec);
+      check_hip(hipMemcpyAsync(inf.data(), info.p, sizeof(rocblas_int) * nsec, hipMemcpyDeviceToHost, ctx->stream),
+                "d2h");
+      check_hip(hipStreamSynchronize(ctx->stream), "eig");
+      for (auto x : inf)
+        if (x != 0) throw HHError(-2, "rocsolver_zheevd did not converge");
+    }
+    check_hip(hipStreamSynchronize(ctx->stream), "eig");
+  });
+}
+
+int hh_set_eigensolver(hh_ctx* ctx, int solver) {
+  return guard(ctx, [&] {
+    need(solver == 0 || solver == 1, "solver must be 0 (hand-written) or 1 (zheevd)");
+    ctx->eig_solver = solver;
+  });
+}
+
+int hh_eigensolver_status(const hh_ctx* ctx, int32_t* solver, int64_t* handbacks, int32_t* ortho_steps,
+                          double* ortho_error) {
+  if (!ctx) return 1;
+  if (solver) *solver = ctx->eig_solver;
+  if (handbacks) *handbacks = ctx->eig_handbacks;
+  if (ortho_steps) *ortho_steps = ctx->heev.ns_iterations;
+  if (ortho_error) *ortho_error = ctx->heev.ortho_error;
+  return 0;
 }
 
 int hh_set_precision(hh_ctx* ctx, int precision) {

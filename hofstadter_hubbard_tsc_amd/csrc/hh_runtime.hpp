@@ -17,6 +17,7 @@
 #include "../../include/hhdmrg.h"
 #include "hh_core.hpp"
 #include "hh_engine.hpp"
+#include "hh_heev.hpp"
 
 namespace hh {
 
@@ -143,6 +144,13 @@ struct hh_ctx {
   std::map<uint64_t, hh::LaunchConfig> tune_cache;
   int tune_runs = 0, tune_hits = 0;
   double tune_seconds = 0.0;
+  // eigensolver of the truncation: 0 the hand-written one (hh_heev.hip), 1
+  // rocSOLVER zheevd per sector; truncations that the hand-written solver
+  // hands back (a sector above HEEV_MAXN, vectors it cannot orthonormalise)
+  // are counted and redone with zheevd
+  int eig_solver = 0;
+  hh::HeevWork heev;
+  int64_t eig_handbacks = 0;
 };
 
 struct hh_block {

@@ -562,6 +562,35 @@ class Context:
     def set_profiling(self, on):
         L.load().hh_set_profiling(self.ptr, 1 if on else 0)
 
+    def eigh_sectors(self, blocks, dims, keep, solver=0):
+        """The truncation's eigensolver on given Hermitian blocks
+        (hh_eigh_sectors): blocks is a complex128 device tensor holding the
+        dims[i] x dims[i] column-major blocks one after another (overwritten:
+        block i ends with the eigenvectors of its top keep[i] eigenvalues as
+        its last keep[i] columns); returns the eigenvalues (device tensor,
+        ascending per block)."""
+        import torch
+        dims = np.ascontiguousarray(dims, dtype=np.int32)
+        keep = np.ascontiguousarray(keep, dtype=np.int32)
+        total = int(np.sum(dims.astype(np.int64) ** 2))
+        ptr = _dptr(blocks, total)
+        w = torch.empty(int(dims.sum()), dtype=torch.float64, device=blocks.device)
+        _sync()
+        self._rc(L.load().hh_eigh_sectors(self.ptr, len(dims), L.ip(dims), L.ip(keep), ptr, C.c_void_p(w.data_ptr()),
+                                          int(solver)), "hh_eigh_sectors")
+        return w
+
+    def set_eigensolver(self, solver):
+        """0: hand-written density-matrix eigensolver (default); 1: rocSOLVER zheevd."""
+        self._rc(L.load().hh_set_eigensolver(self.ptr, int(solver)), "hh_set_eigensolver")
+
+    def eigensolver_status(self):
+        """(solver, truncations handed back to zheevd, orthonormalisation
+        steps and max |U^dag U - I| of the last hand-written call)."""
+        a, b, c, d = L.i32(), L.i64(), L.i32(), L.f64()
+        L.load().hh_eigensolver_status(self.ptr, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
     def set_precision(self, precision):
         """0: FP64 MFMA H.psi (plain form); 1: FP32 MFMA H.psi, FP64 vectors."""
         self._rc(L.load().hh_set_precision(self.ptr, int(precision)), "hh_set_precision")

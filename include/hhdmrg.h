@@ -259,6 +259,27 @@ int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* ps
  * (code/get_den.f90:4-99, :184-428): side 0 sys (psi psi^dag), 1 env. */
 int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, int kept_min, int kept_max,
                 double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy);
+/* The eigendecomposition inside Get_truncation_operator on its own (ZHEEV
+ * per sector, code/get_den.f90:242): nsec Hermitian blocks of dim[i] x dim[i]
+ * complex128, column-major, packed one after another at blocks_dev (the upper
+ * triangle is read).  evals_dev receives every eigenvalue, ascending per
+ * block, blocks concatenated.  On return block i holds the eigenvectors of
+ * its top keep[i] eigenvalues as its last keep[i] columns, in ascending
+ * order; with solver 0 (the hand-written eigensolver) every other entry of
+ * the block is zero, with solver 1 (rocSOLVER zheevd) the other columns hold
+ * the other eigenvectors.  Status 3: the hand-written solver handed the
+ * problem back (a block above 3072, or kept vectors it could not
+ * orthonormalise). */
+int hh_eigh_sectors(hh_ctx* ctx, int nsec, const int32_t* dim, const int32_t* keep, double* blocks_dev,
+                    double* evals_dev, int solver);
+/* Eigensolver of hh_truncate and the drivers: 0 hand-written (default), 1
+ * rocSOLVER zheevd per sector.  hh_eigensolver_status reports it, how many
+ * truncations the hand-written solver handed back to zheevd so far, and the
+ * orthonormalisation steps and max |U^dag U - I| of its last call.  No
+ * counterpart in the reference (which calls ZHEEV). */
+int hh_set_eigensolver(hh_ctx* ctx, int solver);
+int hh_eigensolver_status(const hh_ctx* ctx, int32_t* solver, int64_t* handbacks, int32_t* ortho_steps,
+                          double* ortho_error);
 /* truns_to_disk / truns_from_disk (code/space_info.f90:1621-1661, the
  * block_to_disk / block_from_disk records of :1470-1521) of the truncation
  * operator of Get_truncation_operator (code/get_den.f90:362-421): len, num,
