@@ -39,7 +39,7 @@ HeevBuf::~HeevBuf() {
 namespace {
 
 constexpr int HT = 256;   // threads per workgroup
-constexpr int CW = 16;    // trailing columns per workgroup in a Householder step
+constexpr int CW = 8;     // trailing columns per workgroup in a Householder step
 constexpr int GT = 32;    // Gram / Newton-Schulz tile
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -103,67 +103,95 @@ __global__ __launch_bounds__(HT) void k_herm_fix(const HeevSec* __restrict__ sec
 // written by none in this step) to build reflector k in LDS, then updates
 // its own columns once and forms their entries of p_k and their share of
 // p_k^H v_k.  The four waves split the rows (64-row chunks dealt round
-// robin) and each takes all of the workgroup's columns, so a lane walks few
-// rows per step; the column sums meet in LDS in a fixed order.  v, p and the
-// share sums alternate between two halves of their buffers (step k reads
-// half (k - 1) & 1, writes half k & 1).  The workgroup holding column k
-// writes d_k, e_k = beta_k, tau_k and v_k, and stores reflector k - 1 into
-// the strictly lower part of column k - 1.
-__global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevSec* __restrict__ secs, const int2* __restrict__ wgs,
-                                                   double2* __restrict__ A, double* __restrict__ dd,
-                                                   double* __restrict__ ee, double2* __restrict__ tau,
-                                                   double2* __restrict__ vb, double2* __restrict__ pb,
-                                                   double2* __restrict__ part, int64_t ncol, int64_t npart) {
-  const int2 g = wgs[blockIdx.x];   // (sector, chunk)
-  const HeevSec S = secs[g.x];
-  const int n = S.n;
+// robin, RCH chunks per pass so that their loads are in flight together)
+// and each takes all of the workgroup's columns; the column sums meet in LDS
+// in a fixed order.  The step is a chain of dependent memory round trips, so
+// every phase issues its loads at once: the workgroup record carries the
+// sector's fields, the column phase is unrolled over at most HEEV_MAXN / HT
+// rows per thread.  v, p and the share sums alternate between two halves of
+// their buffers (step k reads half (k - 1) & 1, writes half k & 1).  The
+// workgroup holding column k writes d_k, e_k = beta_k, tau_k and v_k, and
+// stores reflector k - 1 into the strictly lower part of column k - 1.
+constexpr int RCH = 3;
+constexpr int CROWS = HEEV_MAXN / HT;   // column-phase rows per thread
+__global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevWg* __restrict__ wgs, double2* __restrict__ A,
+                                                   double* __restrict__ dd, double* __restrict__ ee,
+                                                   double2* __restrict__ tau, double2* __restrict__ vb,
+                                                   double2* __restrict__ pb, double2* __restrict__ part, int64_t ncol,
+                                                   int64_t npart) {
+  const HeevWg g = wgs[blockIdx.x];
+  const int n = g.n;
   if (k >= n) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int c_lo = g.y * CW, c_hi = min(n, c_lo + CW);
+  const int c_lo = g.q * CW, c_hi = min(n, c_lo + CW);
   const int cur = k & 1, prv = cur ^ 1;
-  double2* part_cur = part + (int64_t)cur * npart + S.ch0;
+  double2* part_cur = part + (int64_t)cur * npart + g.ch0;
   if (c_hi <= k) {   // every column of this chunk is finished
-    if (tid == 0) part_cur[g.y] = make_double2(0.0, 0.0);
+    if (tid == 0) part_cur[g.q] = make_double2(0.0, 0.0);
     return;
   }
   __shared__ double2 vn[HEEV_MAXN];
   __shared__ double red[4];
   __shared__ double2 redc[4];
   __shared__ double2 redp[4][CW];
-  double2* M = A + S.a;
+  double2* M = A + g.a;
   const int64_t ld = n;
-  const double2* vo = vb + (int64_t)prv * ncol + S.c;
-  const double2* po = pb + (int64_t)prv * ncol + S.c;
+  const double2* vo = vb + (int64_t)prv * ncol + g.c;
+  const double2* po = pb + (int64_t)prv * ncol + g.c;
   const bool upd = k >= 1;
+  // loads of the step issued together: the chunks' shares, column k, and
+  // v, p of reflector k - 1 over the column's rows
+  double2 sh = make_double2(0.0, 0.0);
+  double2 ca[CROWS], cv[CROWS], cp[CROWS];
+  double2 t_prev = make_double2(0.0, 0.0), vk = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
+  if (upd) {
+    const double2* pp = part + (int64_t)prv * npart + g.ch0;
+    for (int q = tid; q < g.nch; q += HT) {
+      const double2 x = pp[q];
+      sh.x += x.x;
+      sh.y += x.y;
+    }
+    t_prev = tau[g.c + k - 1];
+    vk = vo[k];
+    pk = po[k];
+  }
+#pragma unroll
+  for (int i = 0; i < CROWS; ++i) {
+    const int r = k + tid + HT * i;
+    ca[i] = cv[i] = cp[i] = make_double2(0.0, 0.0);
+    if (r < n) {
+      ca[i] = M[r + (int64_t)k * ld];
+      if (upd) {
+        cv[i] = vo[r];
+        cp[i] = po[r];
+      }
+    }
+  }
   double2 alpha = make_double2(0.0, 0.0);   // w = p + alpha v
   if (upd) {
-    // s = sum of the chunks' shares (at most 192 of them, one per thread)
-    const double2* pp = part + (int64_t)prv * npart + S.ch0;
-    const double2 x = tid < S.nch ? pp[tid] : make_double2(0.0, 0.0);
-    const double sr = wsum(x.x), si = wsum(x.y);
+    const double sr = wsum(sh.x), si = wsum(sh.y);
     if (lane == 0) redc[wv] = make_double2(sr, si);
     __syncthreads();
     const double2 sm = make_double2((redc[0].x + redc[1].x) + (redc[2].x + redc[3].x),
                                     (redc[0].y + redc[1].y) + (redc[2].y + redc[3].y));
-    const double2 ts = cmul(tau[S.c + k - 1], sm);
+    const double2 ts = cmul(t_prev, sm);
     alpha = make_double2(-0.5 * ts.x, -0.5 * ts.y);
   }
   // column k after reflector k - 1, rows k .. n - 1
-  double2 vk = make_double2(0.0, 0.0), wk = make_double2(0.0, 0.0);
-  if (upd) {
-    vk = vo[k];
-    wk = cadd(po[k], cmul(alpha, vk));
-  }
+  const double2 wk = cadd(pk, cmul(alpha, vk));
   double xs = 0.0;
-  for (int r = k + tid; r < n; r += HT) {
-    double2 a = M[r + (int64_t)k * ld];
-    if (upd) {
-      const double2 v = vo[r];
-      const double2 w = cadd(po[r], cmul(alpha, v));
-      a = csub(csub(a, cmulc(v, wk)), cmulc(w, vk));
+#pragma unroll
+  for (int i = 0; i < CROWS; ++i) {
+    const int r = k + tid + HT * i;
+    if (r < n) {
+      double2 a = ca[i];
+      if (upd) {
+        const double2 w = cadd(cp[i], cmul(alpha, cv[i]));
+        a = csub(csub(a, cmulc(cv[i], wk)), cmulc(w, vk));
+      }
+      vn[r] = a;
+      if (r >= k + 2) xs += a.x * a.x + a.y * a.y;
     }
-    vn[r] = a;
-    if (r >= k + 2) xs += a.x * a.x + a.y * a.y;
   }
   xs = wsum(xs);
   if (lane == 0) red[wv] = xs;
@@ -190,23 +218,23 @@ __global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevSec* __restr
   if (refl)
     for (int r = k + 1 + tid; r < n; r += HT) vn[r] = r == k + 1 ? make_double2(1.0, 0.0) : cmul(vn[r], sc);
   __syncthreads();
-  if (g.y == k / CW) {
+  if (g.q == k / CW) {
     if (tid == 0) {
-      dd[S.c + k] = akk.x;
+      dd[g.c + k] = akk.x;
       if (refl) {
-        ee[S.c + k] = beta;
-        tau[S.c + k] = tk;
+        ee[g.c + k] = beta;
+        tau[g.c + k] = tk;
       }
     }
     if (refl) {
-      double2* vc = vb + (int64_t)cur * ncol + S.c;
+      double2* vc = vb + (int64_t)cur * ncol + g.c;
       for (int r = k + 1 + tid; r < n; r += HT) vc[r] = vn[r];
     }
     if (upd)
       for (int r = k + tid; r < n; r += HT) M[r + (int64_t)(k - 1) * ld] = vo[r];
   }
   if (!refl) {
-    if (tid == 0) part_cur[g.y] = make_double2(0.0, 0.0);
+    if (tid == 0) part_cur[g.q] = make_double2(0.0, 0.0);
     return;
   }
   // own trailing columns c0 .. c_hi - 1 (at most CW), rows k + 1 .. n - 1
@@ -222,25 +250,41 @@ __global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevSec* __restr
       wcl[j] = cadd(po[c0 + j], cmul(alpha, vcl[j]));
     }
   }
-  for (int r = k + 1 + 64 * wv + lane; r < n; r += HT) {
-    double2 v = make_double2(0.0, 0.0), w = make_double2(0.0, 0.0);
-    if (upd) {
-      v = vo[r];
-      w = cadd(po[r], cmul(alpha, v));
-    }
-    const double2 x = vn[r];
+  for (int rb = k + 1 + 64 * wv; rb < n; rb += HT * RCH) {
+    double2 a[RCH][CW], v[RCH], w[RCH];
 #pragma unroll
-    for (int j = 0; j < CW; ++j) {
-      if (j < nc) {
-        double2* ap = M + r + (int64_t)(c0 + j) * ld;
-        double2 a = *ap;
+    for (int h = 0; h < RCH; ++h) {
+      const int r = rb + HT * h + lane;
+      v[h] = w[h] = make_double2(0.0, 0.0);
+      if (r < n) {
         if (upd) {
-          a = csub(csub(a, cmulc(v, wcl[j])), cmulc(w, vcl[j]));
-          *ap = a;
+          v[h] = vo[r];
+          w[h] = po[r];
         }
-        const double2 t = cjmul(a, x);   // (A^H v)_c = sum_r conj(A(r, c)) v(r)
-        acc[j].x += t.x;
-        acc[j].y += t.y;
+#pragma unroll
+        for (int j = 0; j < CW; ++j)
+          if (j < nc) a[h][j] = M[r + (int64_t)(c0 + j) * ld];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < RCH; ++h) {
+      const int r = rb + HT * h + lane;
+      if (r < n) {
+        w[h] = cadd(w[h], cmul(alpha, v[h]));
+        const double2 x = vn[r];
+#pragma unroll
+        for (int j = 0; j < CW; ++j) {
+          if (j < nc) {
+            double2 b = a[h][j];
+            if (upd) {
+              b = csub(csub(b, cmulc(v[h], wcl[j])), cmulc(w[h], vcl[j]));
+              M[r + (int64_t)(c0 + j) * ld] = b;
+            }
+            const double2 t = cjmul(b, x);   // (A^H v)_c = sum_r conj(A(r, c)) v(r)
+            acc[j].x += t.x;
+            acc[j].y += t.y;
+          }
+        }
       }
     }
   }
@@ -250,9 +294,8 @@ __global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevSec* __restr
     if (lane == 0) redp[wv][j] = make_double2(sr, si);
   }
   __syncthreads();
-  double2 contrib = make_double2(0.0, 0.0);
   if (tid < 64) {
-    double2* pc = pb + (int64_t)cur * ncol + S.c;
+    double2* pc = pb + (int64_t)cur * ncol + g.c;
     double2 t = make_double2(0.0, 0.0);
     if (tid < nc) {
       const double2 sm = make_double2((redp[0][tid].x + redp[1][tid].x) + (redp[2][tid].x + redp[3][tid].x),
@@ -261,10 +304,8 @@ __global__ __launch_bounds__(HT) void k_hetrd_step(int k, const HeevSec* __restr
       pc[c0 + tid] = p;
       t = cjmul(p, vn[c0 + tid]);
     }
-    // fixed-order sum over the (at most CW) columns in lanes 0 .. CW - 1
-    contrib.x = wsum(t.x);
-    contrib.y = wsum(t.y);
-    if (tid == 0) part_cur[g.y] = contrib;
+    const double cx = wsum(t.x), cy = wsum(t.y);
+    if (tid == 0) part_cur[g.q] = make_double2(cx, cy);
   }
 }
 
@@ -679,119 +720,207 @@ __global__ __launch_bounds__(HT) void k_ns(const HeevSec* __restrict__ secs, con
   }
 }
 
-// Mutual products of the reflectors of each block of four, c(a, b) =
-// v_{4m+a}^H v_{4m+b} for a < b (pairs in the order (0,1) (0,2) (0,3) (1,2)
-// (1,3) (2,3)), one wave per (sector, block).
-__global__ __launch_bounds__(64) void k_refl_gram(const HeevSec* __restrict__ secs, const int2* __restrict__ list,
-                                                  const double2* __restrict__ A, double2* __restrict__ C) {
-  const int2 o = list[blockIdx.x];   // (sector, block)
-  const HeevSec S = secs[o.x];
-  const int n = S.n, lane = threadIdx.x, j0 = 4 * o.y;
-  const double2* M = A + S.a;
-  double2 acc[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) acc[q] = make_double2(0.0, 0.0);
-  for (int r = j0 + 1 + lane; r < n; r += 64) {
-    double2 v[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int jq = j0 + q;
-      v[q] = (jq <= n - 2 && r > jq) ? M[r + (int64_t)jq * n] : make_double2(0.0, 0.0);
+// Back-transformation U = Q Z = H_0 H_1 ... H_{n-2} Z in blocks of NB
+// reflectors: the block j0 .. j0 + nb - 1 is I - V T V^H (V the block's
+// reflectors, T upper triangular: T(i, i) = tau_i, T(0:i, i) = -tau_i
+// T(0:i, 0:i) V(:, 0:i)^H v_i), applied from the last block to the first as
+// U -= V (T (V^H U)).
+constexpr int NB = 32;
+
+// reflector j of a sector (zero at rows <= j; 1 at row j + 1)
+__device__ __forceinline__ double2 refl_at(const double2* M, int n, int j, int r) {
+  return (j <= n - 2 && r > j) ? M[r + (int64_t)j * n] : make_double2(0.0, 0.0);
+}
+
+// U = Z (real, [row][vector]) as complex, column-major n x keep
+__global__ __launch_bounds__(HT) void k_z2u(const HeevSec* __restrict__ secs, int nsec, int64_t total,
+                                            const int64_t* __restrict__ zoff, const double* __restrict__ Z,
+                                            double2* __restrict__ U) {
+  for (int64_t x = (int64_t)blockIdx.x * HT + threadIdx.x; x < total; x += (int64_t)gridDim.x * HT) {
+    int lo = 0, hi = nsec - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (zoff[mid] <= x) lo = mid;
+      else hi = mid - 1;
     }
-    int x = 0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = a + 1; b < 4; ++b) {
-        const double2 t = cjmul(v[a], v[b]);
-        acc[x].x += t.x;
-        acc[x].y += t.y;
-        ++x;
-      }
-  }
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double sr = wsum(acc[q].x), si = wsum(acc[q].y);
-    if (lane == 0) C[S.cb + 6 * (int64_t)o.y + q] = make_double2(sr, si);
+    const HeevSec S = secs[lo];
+    const int64_t e = x - zoff[lo];
+    const int q = (int)(e / S.n), r = (int)(e - (int64_t)q * S.n);
+    U[S.z + e] = make_double2(Z[S.z + (int64_t)r * S.keep + q], 0.0);
   }
 }
 
-// U = Q Z: one wave per kept vector applies H_{n-2}, ..., H_0 (the
-// reflectors in the strictly lower part of the sector's block) to its
-// column held in registers (row r in lane r % 64, slot r / 64), four at a
-// time: one pass takes the four products d_a = v_a^H u, the coefficients
-// follow from the highest reflector down (s_a = d_a - sum_{b > a} tau_b s_b
-// c(a, b)), and a second pass applies u -= sum_a tau_a s_a v_a.
-template <int NSL>
-__global__ __launch_bounds__(HT) void k_backtr(const HeevSec* __restrict__ secs, const int2* __restrict__ list,
-                                               int count, const double2* __restrict__ A,
-                                               const double2* __restrict__ tau, const double2* __restrict__ C,
-                                               const double* __restrict__ Z, double2* __restrict__ U) {
-  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (wid >= count) return;
-  const int2 o = list[wid];
+// T of every block of every sector: one workgroup per (sector, block)
+__global__ __launch_bounds__(HT) void k_wy_t(const HeevSec* __restrict__ secs, const int2* __restrict__ list,
+                                             const double2* __restrict__ A, const double2* __restrict__ tau,
+                                             double2* __restrict__ T) {
+  const int2 o = list[blockIdx.x];   // (sector, block)
   const HeevSec S = secs[o.x];
-  const int n = S.n, kp = S.keep, q = o.y;
-  double2 u[NSL];
-#pragma unroll
-  for (int t = 0; t < NSL; ++t) {
-    const int r = t * 64 + lane;
-    u[t] = make_double2(r < n ? Z[S.z + (int64_t)r * kp + q] : 0.0, 0.0);
-  }
+  const int n = S.n, j0 = NB * o.y, nb = min(NB, n - 1 - j0);
   const double2* M = A + S.a;
-  for (int m = n >= 2 ? (n - 2) / 4 : -1; m >= 0; --m) {
-    const int j0 = 4 * m, nq = min(4, n - 1 - j0);   // reflectors j0 .. j0 + nq - 1
-    double dr[4] = {0.0, 0.0, 0.0, 0.0}, di[4] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ double2 sv[NB][NB + 1];   // V^H V
+  __shared__ double2 st[NB][NB + 1];   // T
+  __shared__ double2 vs[NB][NB + 1];   // (column, row) chunk of V
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // S(a = tx, b = ty + 8 j)
+  double2 acc[4];
 #pragma unroll
-    for (int t = 0; t < NSL; ++t) {
-      if (t * 64 + 63 > j0) {
-        const int r = t * 64 + lane;
-        if (r > j0 && r < n) {
+  for (int j = 0; j < 4; ++j) acc[j] = make_double2(0.0, 0.0);
+  for (int r0 = j0 + 1; r0 < n; r0 += NB) {
+    for (int cc = ty; cc < NB; cc += 8) {
+      const int r = r0 + tx;
+      vs[cc][tx] = r < n ? refl_at(M, n, j0 + cc, r) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < NB; ++rr) {
+      const double2 va = vs[tx][rr];
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
-            if (a < nq && r > j0 + a) {
-              const double2 c = cjmul(M[r + (int64_t)(j0 + a) * n], u[t]);
-              dr[a] += c.x;
-              di[a] += c.y;
-            }
-        }
+      for (int j = 0; j < 4; ++j) {
+        const double2 t = cjmul(va, vs[ty + 8 * j][rr]);
+        acc[j].x += t.x;
+        acc[j].y += t.y;
       }
     }
-    double2 ts[4];   // tau_a s_a
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      dr[a] = wsum(dr[a]);
-      di[a] = wsum(di[a]);
-    }
-    const double2* cm = C + S.cb + 6 * (int64_t)m;
-#pragma unroll
-    for (int a = 3; a >= 0; --a) {
-      ts[a] = make_double2(0.0, 0.0);
-      if (a < nq) {
-        double2 sa = make_double2(dr[a], di[a]);
-#pragma unroll
-        for (int b = a + 1; b < 4; ++b)
-          if (b < nq) sa = csub(sa, cmul(ts[b], cm[a == 0 ? b - 1 : (a == 1 ? b + 1 : 5)]));
-        ts[a] = cmul(tau[S.c + j0 + a], sa);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NSL; ++t) {
-      if (t * 64 + 63 > j0) {
-        const int r = t * 64 + lane;
-        if (r > j0 && r < n) {
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-            if (a < nq && r > j0 + a) u[t] = csub(u[t], cmul(ts[a], M[r + (int64_t)(j0 + a) * n]));
-        }
-      }
-    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int t = 0; t < NSL; ++t) {
-    const int r = t * 64 + lane;
-    if (r < n) U[S.z + (int64_t)q * n + r] = u[t];
+  for (int j = 0; j < 4; ++j) {
+    sv[tx][ty + 8 * j] = acc[j];
+    st[tx][ty + 8 * j] = make_double2(0.0, 0.0);
   }
+  __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    // T(0:i, i) = -tau_i T(0:i, 0:i) S(0:i, i); one thread per row a < i
+    const double2 ti = tau[S.c + j0 + i];
+    double2 val = make_double2(0.0, 0.0);
+    if (threadIdx.x < i) {
+      const int a = threadIdx.x;
+      for (int b = a; b < i; ++b) {
+        const double2 t = cmul(st[a][b], sv[b][i]);
+        val.x += t.x;
+        val.y += t.y;
+      }
+      val = cmul(make_double2(-ti.x, -ti.y), val);
+    }
+    __syncthreads();
+    if (threadIdx.x < i) st[threadIdx.x][i] = val;
+    if (threadIdx.x == 0) st[i][i] = ti;
+    __syncthreads();
+  }
+  double2* To = T + S.cb + (int64_t)o.y * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += HT) To[e] = st[e % NB][e / NB];   // column-major
+}
+
+// W = V^H U for block m in 256-row pieces: one workgroup per (sector,
+// 32-column tile of U, row block); the pieces are summed in row-block order
+// by k_wy_u (a row block with no row below the block's reflectors writes
+// zeros)
+__global__ __launch_bounds__(HT) void k_wy_w(int m, const HeevSec* __restrict__ secs, const int4* __restrict__ list,
+                                             const double2* __restrict__ A, const double2* __restrict__ U,
+                                             double2* __restrict__ W) {
+  const int4 o = list[blockIdx.x];   // (sector, column tile, row block, -)
+  const HeevSec S = secs[o.x];
+  const int n = S.n, kp = S.keep, j0 = NB * m;
+  if (j0 > n - 2) return;
+  const int rlo = max(j0 + 1, o.z * HT), rhi = min(n, (o.z + 1) * HT);
+  const double2* M = A + S.a;
+  const double2* Us = U + S.z;
+  __shared__ double2 vs[NB][NB + 1], us[NB][NB + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // W(a = tx, col = ty + 8 j)
+  double2 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = make_double2(0.0, 0.0);
+  for (int r0 = rlo; r0 < rhi; r0 += NB) {
+    for (int cc = ty; cc < NB; cc += 8) {
+      const int r = r0 + tx, col = o.y * NB + cc;
+      vs[cc][tx] = r < rhi ? refl_at(M, n, j0 + cc, r) : make_double2(0.0, 0.0);
+      us[cc][tx] = (r < rhi && col < kp) ? Us[r + (int64_t)col * n] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < NB; ++rr) {
+      const double2 v = vs[tx][rr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 t = cjmul(v, us[ty + 8 * j][rr]);
+        acc[j].x += t.x;
+        acc[j].y += t.y;
+      }
+    }
+    __syncthreads();
+  }
+  double2* Ws = W + S.wb;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = o.y * NB + ty + 8 * j;
+    if (col < kp) Ws[((int64_t)o.z * kp + col) * NB + tx] = acc[j];
+  }
+}
+
+// U(rows, cols) -= V (T W) for block m: one workgroup per (sector, column
+// tile, 256-row block); T W of the tile formed in LDS
+__global__ __launch_bounds__(HT) void k_wy_u(int m, const HeevSec* __restrict__ secs, const int4* __restrict__ list,
+                                             const double2* __restrict__ A, const double2* __restrict__ T,
+                                             const double2* __restrict__ W, double2* __restrict__ U) {
+  const int4 o = list[blockIdx.x];   // (sector, column tile, row block, -)
+  const HeevSec S = secs[o.x];
+  const int n = S.n, kp = S.keep, j0 = NB * m;
+  if (j0 > n - 2) return;
+  const int rlo = max(j0 + 1, o.z * HT), rhi = min(n, (o.z + 1) * HT);
+  if (rlo >= rhi) return;
+  const double2* M = A + S.a;
+  const double2* Tm = T + S.cb + (int64_t)m * NB * NB;
+  const double2* Ws = W + S.wb;
+  __shared__ double2 tw[NB][NB + 1];   // (T W)(a, col)
+  __shared__ double2 ws[NB][NB + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int nrb = (n + HT - 1) / HT;
+  for (int e = threadIdx.x; e < NB * NB; e += HT) {
+    const int a = e % NB, c = e / NB, col = o.y * NB + c;
+    double2 sm = make_double2(0.0, 0.0);
+    if (col < kp)
+      for (int rb = 0; rb < nrb; ++rb) {
+        const double2 x = Ws[((int64_t)rb * kp + col) * NB + a];
+        sm.x += x.x;
+        sm.y += x.y;
+      }
+    ws[a][c] = sm;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB * NB; e += HT) {
+    const int a = e % NB, c = e / NB;
+    double2 sm = make_double2(0.0, 0.0);
+    for (int b = a; b < NB; ++b) {
+      const double2 t = cmul(Tm[a + (int64_t)b * NB], ws[b][c]);
+      sm.x += t.x;
+      sm.y += t.y;
+    }
+    tw[a][c] = sm;
+  }
+  __syncthreads();
+  // thread: row r = rlo + threadIdx.x, all 32 columns of the tile
+  const int r = rlo + threadIdx.x;
+  if (r < rhi) {
+    double2 v[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) v[a] = refl_at(M, n, j0 + a, r);
+    double2* Us = U + S.z;
+    for (int c = 0; c < NB; ++c) {
+      const int col = o.y * NB + c;
+      if (col >= kp) break;
+      double2 sm = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        const double2 t = cmul(v[a], tw[a][c]);
+        sm.x += t.x;
+        sm.y += t.y;
+      }
+      double2* up = Us + r + (int64_t)col * n;
+      *up = csub(*up, sm);
+    }
+  }
+  (void)tx;
+  (void)ty;
 }
 
 // every block: the kept vectors as its last keep columns, zero elsewhere
@@ -834,7 +963,7 @@ void HeevWork::values(hipStream_t s, double2* A, const std::vector<int64_t>& off
   if (ns == 0) return;
   off_ = off;
   dim_ = dim;
-  h_wg_.clear();
+  h_wgr_.clear();
   h_tile_.clear();
   h_owner_.clear();
   int64_t col = 0, blk = 0, cbo = 0;
@@ -851,10 +980,7 @@ void HeevWork::values(hipStream_t s, double2* A, const std::vector<int64_t>& off
     S.n = n;
     S.ch0 = ch;
     S.nch = (n + CW - 1) / CW;
-    for (int q = 0; q < S.nch; ++q) {
-      h_wg_.push_back(i);
-      h_wg_.push_back(q);
-    }
+    for (int q = 0; q < S.nch; ++q) h_wgr_.push_back(HeevWg{S.a, S.c, n, S.ch0, S.nch, q});
     const int T = (n + 31) / 32;
     for (int bi = 0; bi < T; ++bi)
       for (int bj = 0; bj <= bi; ++bj) {
@@ -864,13 +990,13 @@ void HeevWork::values(hipStream_t s, double2* A, const std::vector<int64_t>& off
         h_tile_.push_back(0);
       }
     for (int j = 0; j < n; ++j) h_owner_.push_back(i);
-    S.cb = cbo;
-    const int nbk = n >= 2 ? (n - 2) / 4 + 1 : 0;
+    S.cb = cbo;   // T factors of the sector's blocks of NB reflectors
+    const int nbk = n >= 2 ? (n - 2) / NB + 1 : 0;
     for (int m = 0; m < nbk; ++m) {
       h_rg_.push_back(i);
       h_rg_.push_back(m);
     }
-    cbo += 6 * (int64_t)nbk;
+    cbo += (int64_t)nbk * NB * NB;
     col += n;
     ch += S.nch;
     blk += (int64_t)n * n;
@@ -879,10 +1005,10 @@ void HeevWork::values(hipStream_t s, double2* A, const std::vector<int64_t>& off
   ncol_ = col;
   nblk_ = blk;
   npart_ = ch;
-  nwg_ = (int)(h_wg_.size() / 2);
+  nwg_ = (int)h_wgr_.size();
   ntile_ = (int)(h_tile_.size() / 4);
   upload(secs_, sec_, s);
-  upload(wgs_, h_wg_, s);
+  upload(wgs_, h_wgr_, s);
   upload(tiles_, h_tile_, s);
   upload(owner_, h_owner_, s);
   upload(rg_, h_rg_, s);
@@ -901,12 +1027,12 @@ void HeevWork::values(hipStream_t s, double2* A, const std::vector<int64_t>& off
   }
   hipLaunchKernelGGL(k_herm_fix, dim3(ntile_), dim3(HT), 0, s, S, tiles_.as<int4>(), A);
   for (int k = 0; k < nmax_; ++k)
-    hipLaunchKernelGGL(k_hetrd_step, dim3(nwg_), dim3(HT), 0, s, k, S, wgs_.as<int2>(), A, d_.as<double>(),
+    hipLaunchKernelGGL(k_hetrd_step, dim3(nwg_), dim3(HT), 0, s, k, wgs_.as<HeevWg>(), A, d_.as<double>(),
                        e_.as<double>(), tau_.as<double2>(), vb_.as<double2>(), pb_.as<double2>(),
                        part_.as<double2>(), ncol_, (int64_t)npart_);
   if (!h_rg_.empty())
-    hipLaunchKernelGGL(k_refl_gram, dim3((unsigned)(h_rg_.size() / 2)), dim3(64), 0, s, S, rg_.as<int2>(),
-                       (const double2*)A, cg_.as<double2>());
+    hipLaunchKernelGGL(k_wy_t, dim3((unsigned)(h_rg_.size() / 2)), dim3(HT), 0, s, S, rg_.as<int2>(),
+                       (const double2*)A, (const double2*)tau_.as<double2>(), cg_.as<double2>());
   if (timing) {
     check_hip(hipStreamSynchronize(s), "sync");
     ms[0] = now_ms() - t0;
@@ -930,8 +1056,11 @@ bool HeevWork::vectors(hipStream_t s, double2* A, const std::vector<int>& keep, 
   unresolved = 0;
   fail_info.clear();
   h_ztile_.clear();
-  std::vector<int32_t> bt[3];
-  int64_t z = 0, w = 0, g = 0;
+  h_wyw_.clear();
+  h_wyu_.clear();
+  h_zoff_.assign(ns, 0);
+  int64_t z = 0, w = 0, g = 0, wb = 0;
+  int mblk = 0;
   for (int i = 0; i < ns; ++i) {
     HeevSec& S = sec_[i];
     const int n = S.n, kp = keep[i];
@@ -940,15 +1069,29 @@ bool HeevWork::vectors(hipStream_t s, double2* A, const std::vector<int>& keep, 
     S.z = z;
     S.w = w;
     S.g = g;
+    S.wb = wb;   // NB x keep partial products per 256-row block
+    h_zoff_[i] = z;
     z += (int64_t)n * kp;
     w += 5 * (int64_t)n * kp;
     g += (int64_t)kp * kp;
-    const int cls = n <= 1024 ? 0 : (n <= 2048 ? 1 : 2);
+    wb += (int64_t)NB * kp * ((n + HT - 1) / HT);
     for (int q = 0; q < kp; ++q) {
       h_vown_.push_back(i);
       h_vown_.push_back(q);
-      bt[cls].push_back(i);
-      bt[cls].push_back(q);
+    }
+    if (kp > 0 && n >= 2) {
+      mblk = std::max(mblk, (n - 2) / NB + 1);
+      const int Tc = (kp + NB - 1) / NB, Tr = (n + HT - 1) / HT;
+      for (int tc = 0; tc < Tc; ++tc) {
+        h_wyw_.push_back(i);
+        h_wyw_.push_back(tc);
+        for (int tr = 0; tr < Tr; ++tr) {
+          h_wyu_.push_back(i);
+          h_wyu_.push_back(tc);
+          h_wyu_.push_back(tr);
+          h_wyu_.push_back(0);
+        }
+      }
     }
     const int Tk = (kp + GT - 1) / GT, Tn = (n + GT - 1) / GT;
     for (int ta = 0; ta < Tk; ++ta)
@@ -1026,12 +1169,7 @@ bool HeevWork::vectors(hipStream_t s, double2* A, const std::vector<int>& keep, 
   }
   const int nclus = (int)(h_clus_.size() / 4);
   const int ngt = (int)(h_gtile_.size() / 4), nzt = (int)(h_ztile_.size() / 4);
-  h_bt_.clear();
-  int bt_off[4] = {0, 0, 0, 0};
-  for (int c = 0; c < 3; ++c) {
-    h_bt_.insert(h_bt_.end(), bt[c].begin(), bt[c].end());
-    bt_off[c + 1] = (int)(h_bt_.size() / 2);
-  }
+
   upload(secs_, sec_, s);
   upload(vown_, h_clus_, s);
   upload(nu_, h_nu_, s);
@@ -1041,7 +1179,9 @@ bool HeevWork::vectors(hipStream_t s, double2* A, const std::vector<int>& keep, 
   const int nft = (int)(h_ftile_.size() / 4), nxt = (int)(h_xtile_.size() / 4);
   upload(gtile_, h_gtile_, s);
   upload(ztile_, h_ztile_, s);
-  upload(bt_, h_bt_, s);
+  upload(bt_, h_wyw_, s);
+  upload(wyu_, h_wyu_, s);
+  upload(zoff_, h_zoff_, s);
   const HeevSec* S = secs_.as<HeevSec>();
   U_.ensure(sizeof(double2) * std::max<int64_t>(z, 1));
   ns_iterations = 0;
@@ -1137,20 +1277,18 @@ bool HeevWork::vectors(hipStream_t s, double2* A, const std::vector<int>& keep, 
       ms[3] = now_ms() - t0;
       t0_bt = now_ms();
     }
-    const int2* L = bt_.as<int2>();
-    const int c0 = bt_off[1] - bt_off[0], c1 = bt_off[2] - bt_off[1], c2 = bt_off[3] - bt_off[2];
-    if (c0)
-      hipLaunchKernelGGL(k_backtr<16>, dim3((c0 + 3) / 4), dim3(HT), 0, s, S, L + bt_off[0], c0, (const double2*)A,
-                         (const double2*)tau_.as<double2>(), (const double2*)cg_.as<double2>(),
-                         (const double*)zc->as<double>(), U_.as<double2>());
-    if (c1)
-      hipLaunchKernelGGL(k_backtr<32>, dim3((c1 + 3) / 4), dim3(HT), 0, s, S, L + bt_off[1], c1, (const double2*)A,
-                         (const double2*)tau_.as<double2>(), (const double2*)cg_.as<double2>(),
-                         (const double*)zc->as<double>(), U_.as<double2>());
-    if (c2)
-      hipLaunchKernelGGL(k_backtr<48>, dim3((c2 + 3) / 4), dim3(HT), 0, s, S, L + bt_off[2], c2, (const double2*)A,
-                         (const double2*)tau_.as<double2>(), (const double2*)cg_.as<double2>(),
-                         (const double*)zc->as<double>(), U_.as<double2>());
+    // U = Q Z by blocks of NB reflectors, last block first
+    Wb_.ensure(sizeof(double2) * (size_t)std::max<int64_t>(wb, 1));
+    const unsigned zg = (unsigned)std::min<int64_t>((z + HT - 1) / HT, 16384);
+    hipLaunchKernelGGL(k_z2u, dim3(std::max(zg, 1u)), dim3(HT), 0, s, S, ns, z, (const int64_t*)zoff_.as<int64_t>(),
+                       (const double*)zc->as<double>(), U_.as<double2>());
+    const int nww = (int)(h_wyw_.size() / 2), nwu = (int)(h_wyu_.size() / 4);
+    for (int m = mblk - 1; m >= 0 && nww > 0; --m) {
+      hipLaunchKernelGGL(k_wy_w, dim3(nwu), dim3(HT), 0, s, m, S, wyu_.as<int4>(), (const double2*)A,
+                         (const double2*)U_.as<double2>(), Wb_.as<double2>());
+      hipLaunchKernelGGL(k_wy_u, dim3(nwu), dim3(HT), 0, s, m, S, wyu_.as<int4>(), (const double2*)A,
+                         (const double2*)cg_.as<double2>(), (const double2*)Wb_.as<double2>(), U_.as<double2>());
+    }
   }
   const unsigned grid = (unsigned)std::min<int64_t>((nblk_ + HT - 1) / HT, 16384);
   hipLaunchKernelGGL(k_place, dim3(std::max(grid, 1u)), dim3(HT), 0, s, S, ns, nblk_, A,

@@ -15,8 +15,8 @@
 //              orthonormal basis of the complement of the others), Newton-Schulz
 //              orthonormalisation of each sector's kept vectors
 //              (Z <- Z (3I - Z^T Z) / 2 until |Z^T Z - I| <= 1e-12), and
-//              the back-transformation U = Q Z (reflectors applied four at
-//              a time with their precomputed mutual products)
+//              the back-transformation U = Q Z in blocks of 32 reflectors
+//              (U -= V T V^H U)
 //              (one wave per vector, all reflectors in one launch).  U is
 //              written as the last keep[i] columns of the sector's block in
 //              ascending eigenvalue order (the layout ZHEEV leaves), every
@@ -42,8 +42,16 @@ struct HeevSec {
   int64_t z;      // offset of its n x keep real blocks (Z, Z2) and U (n x keep complex)
   int64_t w;      // offset of its inverse-iteration scratch (5 n keep doubles)
   int64_t g;      // offset of its keep x keep Gram matrix
-  int64_t cb;     // offset of its reflector-block Gram entries (6 per block of 4 reflectors)
+  int64_t cb;     // offset of the T factors of its blocks of reflectors (NB x NB each)
+  int64_t wb;     // offset of its NB x keep product V^H U of the back-transformation
   int32_t n, ch0, nch, keep;
+};
+
+// Workgroup record of a Householder step: the sector's block and vector
+// offsets, size and share range, and the workgroup's column chunk q.
+struct HeevWg {
+  int64_t a, c;
+  int32_t n, ch0, nch, q;
 };
 
 // grow-only device buffer
@@ -86,10 +94,13 @@ struct HeevWork {
   int64_t ncol_ = 0, nblk_ = 0;
   int nmax_ = 0, nwg_ = 0, ntile_ = 0;
   // host staging of the tables (alive until the copies are done)
-  std::vector<int32_t> h_wg_, h_tile_, h_owner_, h_vown_, h_clus_, h_gtile_, h_ztile_, h_bt_, h_nu_, h_ftile_,
-      h_xtile_, h_ucol_, h_rg_;
+  std::vector<HeevWg> h_wgr_;
+  std::vector<int32_t> h_tile_, h_owner_, h_vown_, h_clus_, h_gtile_, h_ztile_, h_bt_, h_nu_, h_ftile_,
+      h_xtile_, h_ucol_, h_rg_, h_wyw_, h_wyu_;
+  std::vector<int64_t> h_zoff_;
   HeevBuf secs_, wgs_, tiles_, owner_, d_, e_, tau_, vb_, pb_, part_;
-  HeevBuf vown_, scratch_, Z_, Z2_, G_, gtile_, gerr_, ztile_, U_, bt_, scl_, nu_, ftile_, xtile_, ucol_, rg_, cg_;
+  HeevBuf vown_, scratch_, Z_, Z2_, G_, gtile_, gerr_, ztile_, U_, bt_, scl_, nu_, ftile_, xtile_, ucol_, rg_, cg_,
+      wyu_, zoff_, Wb_;
   int npart_ = 0;
 };
 
