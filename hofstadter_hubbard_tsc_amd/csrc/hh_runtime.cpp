@@ -484,9 +484,46 @@ namespace {
 // (with info = 0) for every padded member once the size reaches the
 // divide-and-conquer range, whatever the pad block, because the padded
 // matrix is block diagonal.
+// rocBLAS / rocSOLVER set up kernels and workspaces lazily on first use.
+// The library eigensolver runs on several handles from several host
+// threads, so that first use happens here, before its first call on the
+// context, once per handle, on one thread: every handle solves small
+// diagonal problems of the sizes that select each of zheevd's code paths.
+void warm_eigensolver(hh_ctx* c) {
+  rocblas_initialize();
+  const int sizes[3] = {8, 100, 600};
+  const int nmax = 600;
+  DevMem a, w, e, inf;
+  a.alloc(sizeof(double2) * nmax * nmax);
+  w.alloc(sizeof(double) * nmax);
+  e.alloc(sizeof(double) * nmax);
+  inf.alloc(sizeof(rocblas_int));
+  std::vector<double2> h((size_t)nmax * nmax);
+  rocblas_handle hs[hh_ctx::NSIDE + 1] = {c->rb};
+  hipStream_t ss[hh_ctx::NSIDE + 1] = {c->stream};
+  for (int k = 0; k < hh_ctx::NSIDE; ++k) {
+    hs[k + 1] = c->rbs[k];
+    ss[k + 1] = c->side[k];
+  }
+  for (int k = 0; k <= hh_ctx::NSIDE; ++k)
+    for (int n : sizes) {
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) h[i + (size_t)j * n] = make_double2(i == j ? 1.0 + i : 0.0, 0.0);
+      check_hip(hipMemcpyAsync(a.p, h.data(), sizeof(double2) * n * n, hipMemcpyHostToDevice, ss[k]), "h2d");
+      if (rocsolver_zheevd(hs[k], rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex*)a.p, n,
+                           (double*)w.p, (double*)e.p, (rocblas_int*)inf.p) != rocblas_status_success)
+        throw HHError(-2, "rocsolver_zheevd warm-up failed");
+      check_hip(hipStreamSynchronize(ss[k]), "eigensolver warm-up");
+    }
+}
+
 template <typename OnFail>
 void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, const std::vector<int>& rdim,
                   double* evals, double* work, int64_t work_stride, rocblas_int* info, OnFail on_fail) {
+  if (!ctx->eig_warm) {   // the library's lazy set-up, once, before any side thread uses a handle
+    warm_eigensolver(ctx);
+    ctx->eig_warm = true;
+  }
   const size_t ns = rdim.size();
   std::vector<int64_t> eo(ns, 0);
   for (size_t i = 1; i < ns; ++i) eo[i] = eo[i - 1] + rdim[i - 1];
@@ -2393,38 +2430,6 @@ std::string& create_error() {
 // ============================================================================
 extern "C" {
 
-// rocBLAS / rocSOLVER set up kernels and workspaces lazily on first use.
-// The eigensolver later runs on several handles from several host threads,
-// so that first use happens here, once per handle, on one thread: every
-// handle solves small diagonal problems of the sizes that select each of
-// zheevd's code paths.
-static void warm_eigensolver(hh_ctx* c) {
-  rocblas_initialize();
-  const int sizes[3] = {8, 100, 600};
-  const int nmax = 600;
-  DevMem a, w, e, inf;
-  a.alloc(sizeof(double2) * nmax * nmax);
-  w.alloc(sizeof(double) * nmax);
-  e.alloc(sizeof(double) * nmax);
-  inf.alloc(sizeof(rocblas_int));
-  std::vector<double2> h((size_t)nmax * nmax);
-  rocblas_handle hs[hh_ctx::NSIDE + 1] = {c->rb};
-  hipStream_t ss[hh_ctx::NSIDE + 1] = {c->stream};
-  for (int k = 0; k < hh_ctx::NSIDE; ++k) {
-    hs[k + 1] = c->rbs[k];
-    ss[k + 1] = c->side[k];
-  }
-  for (int k = 0; k <= hh_ctx::NSIDE; ++k)
-    for (int n : sizes) {
-      for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) h[i + (size_t)j * n] = make_double2(i == j ? 1.0 + i : 0.0, 0.0);
-      check_hip(hipMemcpyAsync(a.p, h.data(), sizeof(double2) * n * n, hipMemcpyHostToDevice, ss[k]), "h2d");
-      if (rocsolver_zheevd(hs[k], rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex*)a.p, n,
-                           (double*)w.p, (double*)e.p, (rocblas_int*)inf.p) != rocblas_status_success)
-        throw HHError(-2, "rocsolver_zheevd warm-up failed");
-      check_hip(hipStreamSynchronize(ss[k]), "eigensolver warm-up");
-    }
-}
 
 int hh_ctx_create(int device, hh_ctx** out) {
   if (!out) return 1;
@@ -2449,7 +2454,6 @@ int hh_ctx_create(int device, hh_ctx** out) {
       rocblas_set_stream(c->rbs[k], c->side[k]);
       check_hip(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming), "hipEventCreate");
     }
-    warm_eigensolver(c);
   });
   if (rc != 0) {
     create_error() = c->err;

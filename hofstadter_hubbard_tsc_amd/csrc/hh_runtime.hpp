@@ -149,6 +149,7 @@ struct hh_ctx {
   // hands back (a sector above HEEV_MAXN, vectors it cannot orthonormalise)
   // are counted and redone with zheevd
   int eig_solver = 0;
+  bool eig_warm = false;   // the library eigensolver's set-up has run
   hh::HeevWork heev;
   int64_t eig_handbacks = 0;
 };
