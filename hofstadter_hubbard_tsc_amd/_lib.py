@@ -116,6 +116,7 @@ SIGNATURES = [
     ("hh_driver_set_precond", i32, [vp, i32]),
     ("hh_driver_copy_precond", i32, [vp, vp]),
     ("hh_driver_set_min_dim", i32, [vp, i64]),
+    ("hh_driver_set_subspace", i32, [vp, i32]),
     ("hh_driver_set_fallback", i32, [vp, f64]),
     ("hh_precond_eval", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), P(f64), P(f64), P(f64), f64,
                               P(C.c_float), P(C.c_float), P(C.c_float), P(f64), P(f64), P(C.c_float)]),
@@ -142,6 +143,8 @@ SIGNATURES = [
     ("hh_wave_from_disk", i32, [vp, vp, C.c_char_p, vp]),
     ("hh_trunc_to_disk", i32, [vp, vp, i32, C.c_char_p]),
     ("hh_trunc_from_disk", i32, [vp, vp, i32, C.c_char_p, P(i32), P(vp)]),
+    ("hh_basis_to_disk", i32, [vp, vp, C.c_char_p]),
+    ("hh_basis_from_disk", i32, [vp, vp, C.c_char_p, P(i32)]),
     ("hh_driver_spin_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_hopping_all", i32, [vp, i32, vp, P(f64)]),
     ("hh_driver_trunc", i32, [vp, i32, i32, P(vp)]),

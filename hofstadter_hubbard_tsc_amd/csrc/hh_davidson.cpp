@@ -172,13 +172,15 @@ void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P) {
 }
 
 SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_hpsi, int mode,
-                  const float* net, bool* fell_back, const AnchorHook& on_anchor, double fall_factor) {
+                  const float* net, bool* fell_back, const AnchorHook& on_anchor, double fall_factor,
+                  int kmax_limit) {
   const int64_t n = sb->plan.sb.dim;
   need(n > 0, "empty superblock");
   need(max_hpsi >= 2, "max_iter must be >= 2");
   need(mode != PC_LEARNED || net != nullptr, "the learned preconditioner has no trained network");
   if (fell_back) *fell_back = false;
-  const int kmax = std::min<int>(DAV_KMAX, std::max<int>(2, max_hpsi));
+  need(kmax_limit >= 2 && kmax_limit <= DAV_KMAX_LIMIT, "Davidson subspace size out of range");
+  const int kmax = std::min<int>(kmax_limit, std::max<int>(2, max_hpsi));
   ensure_ws(ctx, n, 2 * kmax + 1);   // kmax basis vectors, kmax products, residual, expansion
   double2* base = ctx->V.c();
   auto Vs = [&](int j) { return base + (size_t)j * n; };
@@ -331,6 +333,11 @@ SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, 
       r_ready = true;
       const double de = std::fabs(theta - anchor_e);
       out.ritz_change = de;
+      if (verbose())
+        std::fprintf(stderr, "#   davidson restart: %d FP32 products so far, round ended by %s; round Ritz %.12f, "
+                     "exact %.12f (moved %.2e since the last restart), residual %.3e (was %.3e)\n",
+                     out.hpsi_fp32, stuck ? "stagnation of t" : (stagnated ? "the stopping rule" : "a full subspace or the budget"),
+                     theta_round, theta, de, rho, anchor_rho);
       // converged: the exact Rayleigh quotient moved by at most lanzero since
       // the previous restart, or the round ended by the stopping rule and the
       // exact Rayleigh quotient of its Ritz vector confirms the round's last

@@ -50,9 +50,9 @@ static constexpr int MTCW = 16 * WBW;
 // one three, so plain tiles need no room for the wide shape
 static constexpr int STRIP = MTR * 16;
 static constexpr int WAVES = 4;        // independent waves per workgroup
-// FP64 complex products by the 3M scheme (3 real MFMAs per complex product)
-// in every shape class; HH_CPLX3 = 0 builds the four-product form instead.
-// FP32 always uses four products.
+// Complex products by the 3M scheme (3 real MFMAs per complex product) in
+// every shape class, FP64 and FP32; HH_CPLX3 = 0 builds the four-product form
+// instead.
 #ifndef HH_CPLX3
 #define HH_CPLX3 1
 #endif

@@ -923,6 +923,7 @@ struct Driver {
   int pc_mode = PC_LEARNED;
   double fall_factor = DAV_FALL_FACTOR;
   int64_t min_dim = DAV_MIN_DIM;
+  int dav_kmax = DAV_KMAX;
   DevMem cap_u, cap_r;
 
   // the step's ground-state solve: the plain form (Lanczos with FP64 H.psi),
@@ -935,7 +936,7 @@ struct Driver {
       const int mode = (pc_mode == PC_LEARNED && !have_net) ? PC_DIAGONAL : pc_mode;
       const float* w = mode == PC_LEARNED ? (const float*)net->w.p : nullptr;
       if (capture_left <= 0) {
-        SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, AnchorHook(), fall_factor);
+        SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, AnchorHook(), fall_factor, dav_kmax);
         so.fallback = fb;
         return so;
       }
@@ -958,7 +959,7 @@ struct Driver {
         a.theta = th;
         ++count;
       };
-      SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, hook, fall_factor);
+      SolveOut so = davidson(ctx, sb, p, lanzero, S.max_iter, mode, w, &fb, hook, fall_factor, dav_kmax);
       so.fallback = fb;
       // the last restart of a converged solve is the returned vector itself
       // (its correction is zero): the ones before it are the triples
@@ -2894,6 +2895,142 @@ int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* 
   });
 }
 
+// basis_to_disk / basis_from_disk (code/space_info.f90:1665-1762): Get_basis
+// (code/update.f90:11-112) of a block and one site as formatted text, with
+// the reference's edit descriptors: (A4,I6) for len / num / dim / idx / pos,
+// (A11,I4) new_num_up, (A13,I4) new_num_down, (A10,I4,2X,A12,I4) for the
+// block and site quantum numbers of a subsector, and an empty list-directed
+// record before each sector.  num_up / num_down are N and 2S; idx ends equal
+// to the subsector count and pos is the 0-based offset in the sector.
+namespace {
+struct BasisLine {   // one formatted record: a label and one or two integers
+  std::string label, label2;
+  int width = 6;
+  int64_t a = 0, b = 0;
+  bool two = false;
+};
+std::vector<BasisLine> basis_lines(const EnlBasis& eb, int length) {
+  std::vector<BasisLine> L;
+  auto one = [&](const char* lab, int64_t v, int w) {
+    BasisLine x;
+    x.label = lab;
+    x.a = v;
+    x.width = w;
+    L.push_back(x);
+  };
+  auto two = [&](const char* l1, int64_t v1, const char* l2, int64_t v2) {
+    BasisLine x;
+    x.label = l1;
+    x.label2 = l2;
+    x.a = v1;
+    x.b = v2;
+    x.width = 4;
+    x.two = true;
+    L.push_back(x);
+  };
+  one("len=", length, 6);
+  one("num=", (int64_t)eb.sec.size(), 6);
+  one("dim=", eb.total_dim(), 6);
+  for (size_t x = 0; x < eb.sec.size(); ++x) {
+    L.push_back(BasisLine());   // the empty record
+    one("idx=", (int64_t)eb.subs[x].size(), 6);
+    one("num=", (int64_t)eb.subs[x].size(), 6);
+    one("dim=", eb.sec[x].dim, 6);
+    one("new_num_up=", eb.sec[x].N, 4);
+    one("new_num_down=", eb.sec[x].S2, 4);
+    for (const SubSector& q : eb.subs[x]) {
+      one("pos=", q.pos, 6);
+      one("dim=", q.dim, 6);
+      two("bl_num_up=", eb.bl[q.b].N, "bl_num_down=", eb.bl[q.b].S2);
+      two("st_num_up=", eb.st[q.s].N, "st_num_down=", eb.st[q.s].S2);
+    }
+  }
+  return L;
+}
+// Fortran Iw output: right-justified in w columns, or w asterisks when the
+// value does not fit
+std::string fortran_int(int64_t v, int w) {
+  std::string t = std::to_string(v);
+  if ((int)t.size() > w) return std::string(w, '*');
+  return std::string(w - t.size(), ' ') + t;
+}
+}  // namespace
+
+int hh_basis_to_disk(hh_ctx* ctx, const hh_block* blk, const char* path) {
+  return guard(ctx, [&] {
+    need(blk && path, "null argument");
+    const EnlBasis eb = enlarge_basis(blk->d.sec, site_sectors(ctx->model.kind));
+    const std::vector<BasisLine> L = basis_lines(eb, blk->d.length + 1);
+    std::string out;
+    for (const BasisLine& x : L) {
+      if (x.label.empty()) {
+        out += "\n";
+        continue;
+      }
+      const std::string v = fortran_int(x.a, x.width);
+      need(v[0] != '*', "basis file: " + x.label + " " + std::to_string(x.a) +
+                            " does not fit the reference's I" + std::to_string(x.width) + " field");
+      out += x.label + v;
+      if (x.two) {
+        const std::string v2 = fortran_int(x.b, x.width);
+        need(v2[0] != '*', "basis file: " + x.label2 + " does not fit the reference's I4 field");
+        out += "  " + x.label2 + v2;
+      }
+      out += "\n";
+    }
+    FileCloser fc{std::fopen(path, "wb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    need(std::fwrite(out.data(), 1, out.size(), fc.f) == out.size() && std::fflush(fc.f) == 0,
+         "cannot write the basis file");
+  });
+}
+
+int hh_basis_from_disk(hh_ctx* ctx, const hh_block* blk, const char* path, int32_t* length) {
+  return guard(ctx, [&] {
+    need(blk && path, "null argument");
+    FileCloser fc{std::fopen(path, "rb")};
+    need(fc.f != nullptr, std::string("cannot open ") + path);
+    const EnlBasis eb = enlarge_basis(blk->d.sec, site_sectors(ctx->model.kind));
+    const std::vector<BasisLine> L = basis_lines(eb, blk->d.length + 1);
+    int lineno = 0;
+    auto next_line = [&]() {
+      std::string t;
+      int c;
+      while ((c = std::fgetc(fc.f)) != EOF && c != '\n') t.push_back((char)c);
+      need(c != EOF || !t.empty(), "basis file: ends early (line " + std::to_string(lineno + 1) + ")");
+      if (!t.empty() && t.back() == '\r') t.pop_back();
+      ++lineno;
+      return t;
+    };
+    // an Iw input field: blanks are ignored, an empty field reads as 0
+    auto field = [&](const std::string& t, size_t at, int w) {
+      std::string f = at < t.size() ? t.substr(at, (size_t)w) : std::string();
+      std::string d;
+      for (char ch : f)
+        if (ch != ' ') d.push_back(ch);
+      if (d.empty()) return (int64_t)0;
+      need(d.find_first_not_of("+-0123456789") == std::string::npos,
+           "basis file: line " + std::to_string(lineno) + " holds no integer where one is expected");
+      return (int64_t)std::stoll(d);
+    };
+    for (size_t i = 0; i < L.size(); ++i) {
+      const BasisLine& x = L[i];
+      const std::string t = next_line();
+      if (x.label.empty()) continue;   // list-directed read of nothing: skips the record
+      const size_t l1 = x.label.size();
+      int64_t a = field(t, l1, x.width), b = 0;
+      if (x.two) b = field(t, l1 + (size_t)x.width + 2 + x.label2.size(), x.width);
+      if (i == 0) {
+        if (length) *length = (int32_t)a;
+        need(a == x.a, "basis file: len differs from the enlarged length of this block");
+        continue;
+      }
+      need(a == x.a && b == x.b, "basis file: line " + std::to_string(lineno) + " (" + x.label +
+                                     ") differs from Get_basis of this block and one site");
+    }
+  });
+}
+
 int hh_superblock_enlarged(const hh_superblock* sb, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim) {
   if (!sb) return 1;
   const EnlBasis& E = side == 0 ? sb->plan.sbs : sb->plan.ebs;
@@ -3443,6 +3580,14 @@ int hh_driver_set_min_dim(hh_driver* h, int64_t min_dim) {
   });
 }
 
+int hh_driver_set_subspace(hh_driver* h, int kmax) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(kmax >= 2 && kmax <= DAV_KMAX_LIMIT, "the Davidson subspace size must be in [2, 128]");
+    h->d.dav_kmax = kmax;
+  });
+}
+
 int hh_driver_copy_precond(hh_driver* dst, const hh_driver* src) {
   if (!dst || !src) return 1;
   return guard(dst->d.ctx, [&] {
@@ -3510,6 +3655,7 @@ int fork_driver(const hh_driver* src, const int lo[2], const int hi[2], hh_drive
     d.pc_mode = s.pc_mode;
     d.fall_factor = s.fall_factor;
     d.min_dim = s.min_dim;
+    d.dav_kmax = s.dav_kmax;
     if (s.net) copy_net(ctx, d, s);
     if (s.prev_sb) {
       // the previous step's superblock, bound again to the copied blocks

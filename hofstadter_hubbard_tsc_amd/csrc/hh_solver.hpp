@@ -49,8 +49,10 @@ void precond_setup(hh_ctx* ctx, const hh_superblock* sb, PcDev& P);
 // Called at every FP64 restart with the unit vector u, its exact residual
 // H u - theta u and theta (device vectors valid during the call).
 typedef std::function<void(const double2*, const double2*, double)> AnchorHook;
-// Largest Davidson subspace before a restart.
+// Largest Davidson subspace before a restart (default; hh_driver_set_subspace
+// sets it per driver, up to DAV_KMAX_LIMIT).
 constexpr int DAV_KMAX = 32;
+constexpr int DAV_KMAX_LIMIT = 128;
 // fall_factor: the FP64 residual must fall below fall_factor times its value
 // at the previous restart, or the mixed Lanczos takes over.
 constexpr double DAV_FALL_FACTOR = 0.9;
@@ -61,7 +63,7 @@ constexpr double DAV_FALL_FACTOR = 0.9;
 constexpr int64_t DAV_MIN_DIM = 262144;
 SolveOut davidson(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, int max_hpsi, int mode,
                   const float* net, bool* fell_back, const AnchorHook& on_anchor = AnchorHook(),
-                  double fall_factor = DAV_FALL_FACTOR);
+                  double fall_factor = DAV_FALL_FACTOR, int kmax_limit = DAV_KMAX);
 
 // Record one training triple from a finished plain solve: u0 is the unit
 // start vector, r0 = H u0 - theta0 u0 its FP64 residual, psi the ground state.

@@ -463,6 +463,19 @@ class Context:
                  "hh_trunc_from_disk")
         return Trunc(self, p), n.value
 
+    def basis_to_disk(self, block, path):
+        """basis_to_disk (code/space_info.f90:1665-1706): Get_basis of `block`
+        and one site as the reference's formatted text file."""
+        self._rc(L.load().hh_basis_to_disk(self.ptr, block.ptr, os.fsencode(path)), "hh_basis_to_disk")
+
+    def basis_from_disk(self, block, path):
+        """basis_from_disk (code/space_info.f90:1710-1762): read the file and
+        check it against Get_basis of `block`; returns the file's len."""
+        n = L.i32()
+        self._rc(L.load().hh_basis_from_disk(self.ptr, block.ptr, os.fsencode(path), C.byref(n)),
+                 "hh_basis_from_disk")
+        return n.value
+
     def wave_to_disk(self, sb, psi, path):
         """wave_to_disk (code/space_info.f90:1406-1436): psi in the reference's
         Fortran unformatted wavefunction file."""
@@ -682,6 +695,11 @@ class Driver:
         """Solvers 1 and 2 hand superblocks below min_dim to the plain Lanczos
         (default 262144; 0 = never)."""
         self.ctx._rc(L.load().hh_driver_set_min_dim(self.ptr, int(min_dim)), "hh_driver_set_min_dim")
+
+    def set_subspace(self, kmax):
+        """Largest Davidson subspace of solver 2 before a restart (default 32,
+        at most 128)."""
+        self.ctx._rc(L.load().hh_driver_set_subspace(self.ptr, int(kmax)), "hh_driver_set_subspace")
 
     def copy_precond(self, src):
         """Take a device copy of src's trained network (same context)."""

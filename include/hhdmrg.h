@@ -293,6 +293,16 @@ int hh_eigensolver_status(const hh_ctx* ctx, int32_t* solver, int64_t* handbacks
 int hh_trunc_to_disk(hh_ctx* ctx, const hh_trunc* t, int length, const char* path);
 int hh_trunc_from_disk(hh_ctx* ctx, const hh_block* prev, int side, const char* path, int32_t* length,
                        hh_trunc** out);
+/* basis_to_disk / basis_from_disk (code/space_info.f90:1665-1762): the
+ * enlarged basis of blk and one site (Get_basis, code/update.f90:11-112) as
+ * the reference's formatted text file (len, num, dim; per sector an empty
+ * record, idx, num, dim, new_num_up = N, new_num_down = 2S; per subsector
+ * pos, dim and the (N, 2S) of its block and site multiplets).  Writing
+ * refuses a value wider than the reference's I6 / I4 fields.  Reading checks
+ * every field against Get_basis of blk (status 1 names the first line that
+ * differs); *length receives the file's len (blk's length + 1). */
+int hh_basis_to_disk(hh_ctx* ctx, const hh_block* blk, const char* path);
+int hh_basis_from_disk(hh_ctx* ctx, const hh_block* blk, const char* path, int32_t* length);
 /* identity truncation of the warm-up (code/warm_up.f90:1421-1432) */
 int hh_trunc_identity(hh_ctx* ctx, hh_superblock* sb, int side, hh_trunc** out);
 int hh_trunc_destroy(hh_trunc* t);
@@ -424,6 +434,11 @@ int hh_driver_set_fallback(hh_driver* d, double factor);
  * Lanczos (solver 0); default 262144, 0 = never.  Below that size the H.psi
  * is a small part of the step and the plain form is the faster one. */
 int hh_driver_set_min_dim(hh_driver* d, int64_t min_dim);
+/* Largest Davidson subspace of solver 2 before it restarts from its Ritz
+ * vector (default 32, at most 128); the workspace holds twice that many
+ * vectors.  No counterpart in the reference, whose lanczos_memo_new keeps up
+ * to loop_in = 100 Lanczos vectors (code/lanczos.f90:14, :22). */
+int hh_driver_set_subspace(hh_driver* d, int kmax);
 /* The preconditioner's whole training pass on caller data, for checking the
  * kernels: np sector blocks (rows x cols, 2S and N of the sys sector), the
  * diagonal d(a, b) = hs[a] + he[b] per block (hs, he concatenated over the

@@ -22,6 +22,12 @@ flang and runs it, producing
                                 max(1, dim // 2) columns, element (i, j) of
                                 kept sector s is (0.1 s + 0.001 (i + 1),
                                 -0.002 (j + 1))
+  basis_hubbard4x2_fortran.txt  Get_basis of the sys block of 3 sites and one
+                                site in the formatted layout of
+                                basis_to_disk (code/space_info.f90:1665-1706:
+                                its write statements and edit descriptors
+                                (A4,I6), (A11,I4), (A13,I4), (A10,I4,2X,A12,I4)
+                                and the empty list-directed record)
 
 The sector tables come from the oracle (test infrastructure).  Run from the
 repository root:  python tests/golden/make_fortran_files.py
@@ -61,10 +67,30 @@ def tables():
     for a, (q, d) in enumerate(zip(eb.sectors, eb.dims)):
         if a % 2 == 0:
             trun.append((q, d, max(1, d // 2)))
-    return wave, off, trun
+    basis = dmrg.enlarge_basis(sysb.sectors, sysb.dims, dmrg.site_sectors("hubbard"))
+    return wave, off, trun, basis
 
 
-def fortran_source(wave, dim, trun):
+def basis_source(basis, length):
+    """The write statements of basis_to_disk for one enlarged basis."""
+    L = ["  open(13, file='basis_hubbard4x2_fortran.txt', status='replace')",
+         "  write(13,101) \"len=\", %d" % length, "  write(13,101) \"num=\", %d" % len(basis.sectors),
+         "  write(13,101) \"dim=\", %d" % sum(basis.dims)]
+    for (N, S2), d, subs in zip(basis.sectors, basis.dims, basis.subs):
+        L += ["  write(13,*)", "  write(13,101) \"idx=\", %d" % len(subs), "  write(13,101) \"num=\", %d" % len(subs),
+              "  write(13,101) \"dim=\", %d" % d, "  write(13,102) \"new_num_up=\", %d" % N,
+              "  write(13,103) \"new_num_down=\", %d" % S2]
+        for (b, st, pos, sdim) in subs:
+            bN, bS = basis.bl_sectors[b]
+            sN, sS = basis.st_sectors[st]
+            L += ["  write(13,101) \"pos=\", %d" % pos, "  write(13,101) \"dim=\", %d" % sdim,
+                  "  write(13,104) \"bl_num_up=\", %d, \"bl_num_down=\", %d" % (bN, bS),
+                  "  write(13,104) \"st_num_up=\", %d, \"st_num_down=\", %d" % (sN, sS)]
+    L += ["  close(13)"]
+    return L
+
+
+def fortran_source(wave, dim, trun, basis):
     L = ["program write_fixtures", "  implicit none", "  integer :: i, j, k", "  complex(8), allocatable :: v(:, :)",
          "  open(11, file='wave_hubbard4x2_fortran.dat', form='unformatted', status='replace')",
          "  write(11) 4", "  write(11) 4", "  write(11) %d" % len(wave), "  write(11) %d" % dim]
@@ -85,20 +111,24 @@ def fortran_source(wave, dim, trun):
               "  do j = 1, %d" % keep, "    do i = 1, %d" % d,
               "      v(i, j) = cmplx(0.1d0 * %d + 1.0d-3 * i, -2.0d-3 * j, kind=8)" % s, "    end do", "  end do",
               "  write(12) v", "  deallocate(v)"]
-    L += ["  close(12)", "end program write_fixtures"]
+    L += ["  close(12)"]
+    L += basis_source(basis, 4)
+    L += ["101 format(A4,I6)", "102 format(A11,I4)", "103 format(A13,I4)", "104 format(A10,I4,2X,A12,I4)",
+          "end program write_fixtures"]
     return "\n".join(L) + "\n"
 
 
 def main():
     flang = shutil.which("flang") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
-    wave, dim, trun = tables()
+    wave, dim, trun, basis = tables()
     with tempfile.TemporaryDirectory() as tmp:
         src = os.path.join(tmp, "w.f90")
-        open(src, "w").write(fortran_source(wave, dim, trun))
+        open(src, "w").write(fortran_source(wave, dim, trun, basis))
         exe = os.path.join(tmp, "w")
         subprocess.run([flang, "-o", exe, src], check=True)
         subprocess.run([exe], check=True, cwd=HERE)
-    print("wrote", os.path.join(HERE, "wave_hubbard4x2_fortran.dat"), os.path.join(HERE, "trun_hubbard4x2_fortran.dat"))
+    print("wrote", *(os.path.join(HERE, f) for f in ("wave_hubbard4x2_fortran.dat", "trun_hubbard4x2_fortran.dat",
+                                                      "basis_hubbard4x2_fortran.txt")))
 
 
 if __name__ == "__main__":

@@ -293,3 +293,37 @@ def test_files_written_by_fortran_are_read_and_fortran_reads_ours(ctx, tmp_path)
     re_, im_, s2 = (float(v) for v in l2.split())
     assert abs(re_ - x.sum().real) <= 1e-12 * abs(x).sum() and abs(im_ - x.sum().imag) <= 1e-12 * abs(x).sum()
     assert abs(s2 - np.sum(np.abs(x) ** 2)) <= 1e-12 * np.sum(np.abs(x) ** 2)
+
+
+def test_basis_file_matches_the_fortran_written_one(ctx, tmp_path):
+    """hh_basis_to_disk writes, byte for byte, the file that the write
+    statements of basis_to_disk (code/space_info.f90:1665-1706) produce when
+    compiled with flang (tests/golden/basis_hubbard4x2_fortran.txt, from
+    tests/golden/make_fortran_files.py) for the same block;
+    hh_basis_from_disk reads it back and refuses a file whose basis differs
+    from Get_basis of the block it is given."""
+    import os
+    from hofstadter_hubbard_tsc_amd import Driver
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    ctx.set_model(m)
+    d = Driver(ctx, kept_max=2000, tol=1e-9, kept_max_warm=2000, seed=3)
+    d.warmup()
+    gold = os.path.join(GOLD, "basis_hubbard4x2_fortran.txt")
+    ours = tmp_path / "basis.txt"
+    ctx.basis_to_disk(d.block(0, 3), ours)
+    assert ours.read_bytes() == open(gold, "rb").read()
+    assert ctx.basis_from_disk(d.block(0, 3), gold) == 4
+    # the basis of another block, and a file with one dimension changed
+    with pytest.raises(HHError):
+        ctx.basis_from_disk(d.block(0, 2), gold)
+    text = open(gold).read().splitlines(True)
+    i = next(k for k, t in enumerate(text) if t.startswith("pos=") and k > 12)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("".join(text[:i + 1] + ["dim=%6d\n" % 7] + text[i + 2:]))
+    with pytest.raises(HHError):
+        ctx.basis_from_disk(d.block(0, 3), bad)
+    short = tmp_path / "short.txt"
+    short.write_text("".join(text[:40]))
+    with pytest.raises(HHError):
+        ctx.basis_from_disk(d.block(0, 3), short)

@@ -151,11 +151,18 @@ def test_forms_reach_the_plain_energy_and_report_exact_energies(ctx):
     losses, ns = d.train_precond(epochs=40, lr=1e-3)
     assert ns >= 2 and losses[-1] <= losses[0]
     forms = {}
-    for name, solver, mode in (("plain", 0, 0), ("mixed", 1, 0), ("learned", 2, 0), ("diagonal", 2, 1),
-                               ("none", 2, 2)):
+    from hofstadter_hubbard_tsc_amd.dmrg import HHError
+    for name, solver, mode, kmax in (("plain", 0, 0, 0), ("mixed", 1, 0, 0), ("learned", 2, 0, 0),
+                                     ("diagonal", 2, 1, 0), ("none", 2, 2, 0), ("diagonal_k8", 2, 1, 8)):
         f = d.fork()
         f.set_solver(solver)
         f.set_precond(mode)
+        if kmax:
+            # a subspace of 8 restarts the Davidson every seven expansions
+            for bad in (1, 129):
+                with pytest.raises(HHError):
+                    f.set_subspace(bad)
+            f.set_subspace(kmax)
         forms[name] = f
     first = {}
     for name, f in forms.items():
