@@ -89,6 +89,9 @@ SIGNATURES = [
     ("hh_autotune_stats", i32, [vp, P(i32), P(i32), P(i32), P(f64)]),
     ("hh_set_launch_config", i32, [vp, i32, i32, i32, i32, i32]),
     ("hh_lanczos", i32, [vp, vp, vp, f64, i32, P(f64), P(i32)]),
+    ("hh_lanczos_levels", i32, [vp, vp, P(vp), i32, f64, i32, P(f64), P(i32)]),
+    ("hh_density_matrix_levels", i32, [vp, vp, i32, vp, vp, vp]),
+    ("hh_truncate_levels", i32, [vp, vp, i32, vp, vp, i32, i32, f64, P(vp), P(i32), P(f64), P(f64)]),
     ("hh_lanczos_status", i32, [vp, P(i32), P(f64), P(f64)]),
     ("hh_truncate", i32, [vp, vp, i32, vp, i32, i32, f64, P(vp), P(i32), P(f64), P(f64)]),
     ("hh_trunc_identity", i32, [vp, vp, i32, P(vp)]),

@@ -308,6 +308,19 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
   const double nrm0 = std::sqrt(fetch(1)[0]);
   need(std::isfinite(nrm0) && nrm0 > 0.0, "zero or non-finite start vector");
   dev_scale_copy(vec(0), psi, 1.0 / nrm0, n, ctx->stream);
+  const int nd = opt.ndeflate;
+  need(nd == 0 || (opt.deflate && nd + 1 < max_iter + 2), "bad deflation set");
+  // x -= sum_j Re<d_j, x> d_j over the deflation set, |x|^2 -> *sq
+  auto deflate = [&](double2* x, double* sq) {
+    dev_mdots(opt.deflate, n, nd, x, n, dots + 1, part, ctx->stream);
+    dev_combine_norm(x, 1.0, opt.deflate, n, nd, dots + 1, -1.0, n, sq, part, ctx->stream);
+  };
+  if (nd > 0) {
+    deflate(vec(0), dots);
+    const double nv = std::sqrt(fetch(1)[0]);
+    need(std::isfinite(nv) && nv > 1e-12, "start vector lies in the span of the lower levels");
+    dev_scale(vec(0), 1.0 / nv, n, ctx->stream);
+  }
   hpsi(vec(0), w);
   // a non-finite H.psi (NaN or Inf in a block or in psi) stops the solve with an error
   auto finite = [](double x) {
@@ -325,6 +338,9 @@ SolveOut lanczos(hh_ctx* ctx, hh_superblock* sb, double2* psi, double lanzero, i
     dev_mdots(vec(j - 1), 0, 1, w, n, dots, part, ctx->stream);
     dev_lanczos_update_norm(w, vec(j - 1), 0.0, j >= 2 ? vec(j - 2) : nullptr, j >= 2 ? beta[j - 2] : 0.0, n, wsq,
                             part, ctx->stream, dots);
+    // levels > 1: Lanczos on the projected operator, beta the norm after
+    // the projection (alpha_j in dots[0] stays)
+    if (nd > 0) deflate(w, wsq);
     double h[2];
     check_hip(hipMemcpyAsync(&h[0], dots, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "d2h");
     check_hip(hipMemcpyAsync(&h[1], wsq, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "d2h");
@@ -582,13 +598,27 @@ void heev_sectors(hh_ctx* ctx, double2* evec, const std::vector<int64_t>& roff, 
   }
 }
 
+// Get_density_matrix_sys_new / _env_new with levels >= 2 (code/get_den.f90:
+// 103-177): rho = (0.1 rho(psi_1) + 1.9 rho(psi_2)) / 2, with rho(psi_1)
+// already in `rho` (`dt` is the density task list).  0.1 and 1.9 are
+// default-real literals there, hence the float constants.
+void mix_levels(hh_ctx* ctx, const DevTasks& dt, double2* rho, const double2* psi2, int64_t total) {
+  DevMem tmp;
+  tmp.alloc(sizeof(double2) * (size_t)std::max<int64_t>(total, 1));
+  launch_tiles(dt, bufs_of((double2*)psi2, tmp.c(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
+               ctx->stream, PREC_FP64);
+  dev_scale(rho, 0.5 * (double)0.1f, total, ctx->stream);
+  dev_lanczos_update(rho, tmp.c(), -0.5 * (double)1.9f, nullptr, 0.0, total, ctx->stream);
+  check_hip(hipStreamSynchronize(ctx->stream), "density matrix of two levels");
+}
+
 // Reduced density matrix + Get_truncation_operator (code/get_den.f90:184-428).
 // The eigendecomposition (ZHEEV per sector, get_den.f90:242) runs in the
 // hand-written eigensolver (hh_heev.hip: every eigenvalue, then the
 // eigenvectors of the kept ones only) or, with eig_solver 1 or for a
 // truncation the hand-written solver hands back, rocSOLVER zheevd per sector.
 hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi, int kept_min, int kept_max,
-                   double trun_err) {
+                   double trun_err, const double2* psi2 = nullptr) {
   const SuperBasis& S = sb->plan.sb;
   const double t0 = wall();
   std::vector<int> rsec, rdim;
@@ -604,6 +634,7 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
   dt.upload(tl, ctx->stream);
   launch_tiles(dt, bufs_of((double2*)psi, t->evec.c(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
                ctx->stream, PREC_FP64);
+  if (psi2) mix_levels(ctx, dt, t->evec.c(), psi2, total);
   int64_t nev = 0;
   int dmax = 1;
   for (int d : rdim) {
@@ -692,7 +723,7 @@ hh_trunc* truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double2* psi,
     const int saved = ctx->eig_solver;
     ctx->eig_solver = 1;
     try {
-      hh_trunc* r = truncate(ctx, sb, side, psi, kept_min, kept_max, trun_err);
+      hh_trunc* r = truncate(ctx, sb, side, psi, kept_min, kept_max, trun_err, psi2);
       ctx->eig_solver = saved;
       return r;
     } catch (...) {
@@ -3060,6 +3091,32 @@ int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi, double lanzero, int 
   });
 }
 
+int hh_lanczos_levels(hh_ctx* ctx, hh_superblock* sb, double* const* psi, int levels, double lanzero, int max_iter,
+                      double* e, int32_t* iters) {
+  return guard(ctx, [&] {
+    need(sb && psi && e && iters, "null argument");
+    need(levels >= 1 && 2 * (levels - 1) + 2 < max_iter + 2, "levels must be >= 1 and below max_iter / 2");
+    const int64_t n = sb->plan.sb.dim;
+    for (int l = 0; l < levels; ++l) need(psi[l] != nullptr, "null level vector");
+    DevMem defl;
+    if (levels > 1) defl.alloc(sizeof(double2) * (size_t)n * (size_t)(2 * (levels - 1)));
+    for (int l = 0; l < levels; ++l) {
+      LanczosOpts opt;
+      opt.deflate = defl.c();
+      opt.ndeflate = 2 * l;
+      const SolveOut so = lanczos(ctx, sb, (double2*)psi[l], lanzero, max_iter, opt);
+      e[l] = so.e0;
+      iters[l] = so.iters;
+      if (l + 1 < levels) {
+        // this level and i times it join the deflation set (psi[l] is unit)
+        dev_scale_copy(defl.c() + (size_t)(2 * l) * n, (const double2*)psi[l], 1.0, n, ctx->stream);
+        dev_cscale_copy(defl.c() + (size_t)(2 * l + 1) * n, (const double2*)psi[l], 0.0, 1.0, n, ctx->stream);
+      }
+    }
+    check_hip(hipStreamSynchronize(ctx->stream), "lanczos levels");
+  });
+}
+
 int hh_lanczos_status(const hh_ctx* ctx, int32_t* converged, double* ritz_change, double* residual) {
   if (!ctx) return 1;
   if (converged) *converged = ctx->last_converged;
@@ -3096,6 +3153,39 @@ int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* ps
     b.p[B_IN] = (double2*)psi;
     b.p[B_OUT] = (double2*)rho;
     launch_on(ctx, tl, b);
+  });
+}
+
+int hh_density_matrix_levels(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi1, const double* psi2,
+                             double* rho) {
+  return guard(ctx, [&] {
+    need(sb && psi1 && psi2 && rho, "null argument");
+    need(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
+    std::vector<int> rsec, rdim;
+    std::vector<int64_t> roff;
+    TaskList tl = compile_density(sb->plan.sb, side, rsec, roff, rdim);
+    int64_t total = 0;
+    for (int d : rdim) total += (int64_t)d * d;
+    DevTasks dt;
+    dt.upload(tl, ctx->stream);
+    launch_tiles(dt, bufs_of((double2*)psi1, (double2*)rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
+                 ctx->stream, PREC_FP64);
+    mix_levels(ctx, dt, (double2*)rho, (const double2*)psi2, total);
+  });
+}
+
+int hh_truncate_levels(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi1, const double* psi2,
+                       int kept_min, int kept_max, double trun_err, hh_trunc** out, int* num_state, double* err,
+                       double* entropy) {
+  return guard(ctx, [&] {
+    need(side == 0 || side == 1, "side must be 0 or 1");
+    need(kept_min >= 0 && kept_max >= 1, "bad kept range");
+    need(sb && psi1 && psi2 && out, "null argument");
+    hh_trunc* t = truncate(ctx, sb, side, (const double2*)psi1, kept_min, kept_max, trun_err, (const double2*)psi2);
+    *out = t;
+    if (num_state) *num_state = t->num_state;
+    if (err) *err = t->err;
+    if (entropy) *entropy = t->entropy;
   });
 }
 

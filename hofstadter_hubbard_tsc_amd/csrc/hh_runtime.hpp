@@ -92,6 +92,12 @@ struct LanczosOpts {
   // called once after the first H.psi with v_1 (the unit start vector),
   // H v_1 - alpha_1 v_1 and alpha_1 (device vectors valid during the call)
   std::function<void(const double2*, const double2*, double)> on_first;
+  // levels > 1 (Gram_Schmidt_Lanczos, code/space_info.f90:1222-1279): the
+  // start vector and every new Lanczos vector are projected off `ndeflate`
+  // unit vectors at stride n, orthonormal in the real inner product (each
+  // lower level u and i u, so that the projection is complex)
+  const double2* deflate = nullptr;
+  int ndeflate = 0;
 };
 
 }  // namespace hh

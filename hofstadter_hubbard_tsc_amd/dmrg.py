@@ -422,15 +422,30 @@ class Context:
                  "hh_lanczos")
         return e.value, it.value
 
+    def lanczos_levels(self, sb, psis, lanzero, max_iter=100):
+        """lanczos_memo_new with levels = len(psis) (code/lanczos.f90:29-44):
+        each device vector holds its level's start vector and receives its
+        unit eigenvector; returns (energies, Lanczos vector counts)."""
+        _sync()
+        k = len(psis)
+        ptrs = (C.c_void_p * k)(*[_dptr(p, sb.dim) for p in psis])
+        e = np.zeros(k, np.float64)
+        it = np.zeros(k, np.int32)
+        self._rc(L.load().hh_lanczos_levels(self.ptr, sb.ptr, ptrs, k, lanzero, max_iter,
+                                            e.ctypes.data_as(C.POINTER(C.c_double)),
+                                            it.ctypes.data_as(C.POINTER(C.c_int32))), "hh_lanczos_levels")
+        return e, it
+
     def lanczos_status(self):
         """(converged, last Ritz-value change, |H psi - E psi|) of the last lanczos call."""
         c, d, r = L.i32(), L.f64(), L.f64()
         self._rc(L.load().hh_lanczos_status(self.ptr, C.byref(c), C.byref(d), C.byref(r)), "hh_lanczos_status")
         return bool(c.value), d.value, r.value
 
-    def density_matrix(self, sb, side, psi):
+    def density_matrix(self, sb, side, psi, psi2=None):
         """Get_density_matrix_sys_new / _env_new (code/get_den.f90:106-177):
-        [(enlarged sector, rho)] in super-basis order, rho a host ndarray."""
+        [(enlarged sector, rho)] in super-basis order, rho a host ndarray; with
+        psi2 (a second level) the mixed (0.1 rho(psi) + 1.9 rho(psi2)) / 2."""
         import torch
         lib = L.load()
         nb, size = L.i32(), C.c_int64()
@@ -445,7 +460,12 @@ class Context:
                                               off.ctypes.data_as(C.POINTER(C.c_int64))), "hh_density_matrix_layout")
         rho = torch.zeros(max(size.value, 1), dtype=torch.complex128, device="cuda")
         _sync()
-        self._rc(lib.hh_density_matrix(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), _dptr(rho, size.value)), "hh_density_matrix")
+        if psi2 is None:
+            self._rc(lib.hh_density_matrix(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), _dptr(rho, size.value)),
+                     "hh_density_matrix")
+        else:
+            self._rc(lib.hh_density_matrix_levels(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), _dptr(psi2, sb.dim),
+                                                  _dptr(rho, size.value)), "hh_density_matrix_levels")
         h = rho.cpu().numpy()
         return [(int(sec[i]), h[off[i]:off[i] + dim[i] * dim[i]].reshape((dim[i], dim[i]), order="F"))
                 for i in range(nb.value)]
@@ -487,13 +507,19 @@ class Context:
         _sync()
         self._rc(L.load().hh_wave_from_disk(self.ptr, sb.ptr, os.fsencode(path), _dptr(psi, sb.dim)), "hh_wave_from_disk")
 
-    def truncate(self, sb, side, psi, kept_min, kept_max, trun_err):
-        """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90)."""
+    def truncate(self, sb, side, psi, kept_min, kept_max, trun_err, psi2=None):
+        """Get_density_matrix_* + Get_truncation_operator (code/get_den.f90);
+        with psi2 (a second level) on the mixed density matrix of two levels."""
         _sync()
         p = C.c_void_p()
         ns, er, en = L.i32(), L.f64(), L.f64()
-        self._rc(L.load().hh_truncate(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), kept_min, kept_max, trun_err, C.byref(p),
-                                      C.byref(ns), C.byref(er), C.byref(en)), "hh_truncate")
+        if psi2 is None:
+            self._rc(L.load().hh_truncate(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), kept_min, kept_max, trun_err,
+                                          C.byref(p), C.byref(ns), C.byref(er), C.byref(en)), "hh_truncate")
+        else:
+            self._rc(L.load().hh_truncate_levels(self.ptr, sb.ptr, side, _dptr(psi, sb.dim), _dptr(psi2, sb.dim),
+                                                 kept_min, kept_max, trun_err, C.byref(p), C.byref(ns), C.byref(er),
+                                                 C.byref(en)), "hh_truncate_levels")
         return Trunc(self, p, ns.value, er.value, en.value)
 
     def trunc_identity(self, sb, side):

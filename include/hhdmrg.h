@@ -236,6 +236,18 @@ int hh_set_launch_config(hh_ctx* ctx, int target_items, int max_groups, int edge
  * psi_dev holds the start vector on entry and the ground state on exit. */
 int hh_lanczos(hh_ctx* ctx, hh_superblock* sb, double* psi_dev, double lanzero, int max_iter, double* e0,
                int* iters);
+/* lanczos_memo_new with levels > 1 (code/lanczos.f90:29-44): the `levels`
+ * lowest states of the superblock one by one.  psi[l] (device, complex128,
+ * super-basis order) holds the start vector of level l and receives its unit
+ * eigenvector, e[l] its energy and iters[l] its Lanczos vector count.  Level
+ * l runs the Lanczos of levels = 1 with its start vector and every new Lanczos
+ * vector projected off the levels below (Gram_Schmidt_Lanczos,
+ * code/space_info.f90:1222-1279).  The projection removes u and i u of each
+ * lower level u (a complex projection); the reference's real-coefficient one
+ * keeps i u, which has u's energy in the real inner-product space, so there
+ * level 2 returns E_0 again.  Requires 2 (levels - 1) < max_iter. */
+int hh_lanczos_levels(hh_ctx* ctx, hh_superblock* sb, double* const* psi_dev, int levels, double lanzero,
+                      int max_iter, double* e, int32_t* iters);
 /* Outcome of the last hh_lanczos call on ctx: whether the stopping rule of
  * lanczos_memo_new (code/lanczos.f90:77-80, :102) held before max_iter, the
  * last change of the lowest Ritz value and |H psi - E psi| of the returned
@@ -257,6 +269,16 @@ int hh_density_matrix(hh_ctx* ctx, hh_superblock* sb, int side, const double* ps
 
 /* Reduced density matrix + Get_truncation_operator
  * (code/get_den.f90:4-99, :184-428): side 0 sys (psi psi^dag), 1 env. */
+/* Get_density_matrix_sys_new / _env_new with levels >= 2
+ * (code/get_den.f90:103-177): (0.1 rho(psi1) + 1.9 rho(psi2)) / 2 in the
+ * layout of hh_density_matrix, the weights being the reference's
+ * single-precision literals (trace 1 - 1.1e-8); hh_truncate_levels is
+ * hh_truncate on that mixed density matrix. */
+int hh_density_matrix_levels(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi1_dev,
+                             const double* psi2_dev, double* rho_dev);
+int hh_truncate_levels(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi1_dev, const double* psi2_dev,
+                       int kept_min, int kept_max, double trun_err, hh_trunc** out, int* num_state, double* err,
+                       double* entropy);
 int hh_truncate(hh_ctx* ctx, hh_superblock* sb, int side, const double* psi_dev, int kept_min, int kept_max,
                 double trun_err, hh_trunc** out, int* num_state, double* err, double* entropy);
 /* The eigendecomposition inside Get_truncation_operator on its own (ZHEEV

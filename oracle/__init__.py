@@ -26,6 +26,12 @@ Pins (how each part of the oracle is checked, see ``tests/test_oracle_*.py``):
                             rotation.
 * Lanczos                -- numpy ``eigh`` on explicit Hermitian matrices;
                             stopping rule tested as a specification.
+* ``lanczos_levels``     -- the lowest eigenvalues of ``eigh`` and the S=0
+                            levels of the ED on an untruncated superblock;
+                            the reference's real-only projection, planted as
+                            a mistake, returns E0 again for level 2.
+* mixed density matrix   -- (0.1 rho_1 + 1.9 rho_2)/2 block by block with the
+                            single-precision weights, trace 1 - 1.1e-8.
 * density / truncation   -- trace 1, numpy ``eigh``; kept-count rule on
                             synthetic spectra (ties, kept_min/kept_max edges).
 

@@ -572,10 +572,23 @@ def tridiag_eigh(alpha, beta):
     return np.linalg.eigh(T)
 
 
-def lanczos(matvec, v0, lanzero, max_iter=100):
-    """``lanczos_memo_new`` with levels=1 (code/lanczos.f90:4-176).
+def _deflate(w, lower):
+    """w minus its complex projection on the (orthonormal) lower levels: the
+    real-coefficient projection on u and on i u for every u."""
+    for u in lower:
+        w = w - _rdot(u, w) * u - _rdot(1j * u, w) * (1j * u)
+    return w
 
-    Returns (E0, psi, iterations)."""
+
+def lanczos(matvec, v0, lanzero, max_iter=100, lower=()):
+    """``lanczos_memo_new`` (code/lanczos.f90:4-176) for one level.
+
+    lower: unit eigenvectors of the levels below (levels > 1): the start
+    vector and every new Lanczos vector are projected off them
+    (Gram_Schmidt_Lanczos, code/space_info.f90:1222-1279), by complex
+    projection, and beta is the norm after the projection (Lanczos on the
+    projected operator).  Returns (E, psi, iterations)."""
+    v0 = _deflate(v0, lower)
     V = [v0 / math.sqrt(_rdot(v0, v0))]
     w = matvec(V[0])
     alpha = [_rdot(V[0], w)]
@@ -587,6 +600,7 @@ def lanczos(matvec, v0, lanzero, max_iter=100):
             w = w - alpha[last - 1] * V[last - 1]
         else:
             w = w - alpha[last - 1] * V[last - 1] - beta[last - 2] * V[last - 2]
+        w = _deflate(w, lower)
         b = math.sqrt(_rdot(w, w))
         beta.append(b)
         if abs(b) <= lanzero:
@@ -615,6 +629,24 @@ def lanczos(matvec, v0, lanzero, max_iter=100):
     return float(ev[0]), psi, last
 
 
+def lanczos_levels(matvec, starts, lanzero, max_iter=100):
+    """``lanczos_memo_new`` with levels = len(starts) (code/lanczos.f90:29-44):
+    the levels one by one, level k from starts[k] projected off levels < k.
+
+    The reference projects with real coefficients only (wave_product,
+    Gram_Schmidt_Lanczos), which leaves i u of every lower level u; in the
+    real inner-product space that is an eigenvector of energy E_u, so its
+    level 2 returns E_0 again.  Here the projection is complex (u and i u).
+    Returns (energies, vectors, iterations)."""
+    es, vs, its = [], [], []
+    for v0 in starts:
+        e, psi, it = lanczos(matvec, v0, lanzero, max_iter, lower=vs)
+        es.append(e)
+        vs.append(psi)
+        its.append(it)
+    return es, vs, its
+
+
 # ----------------------------------------------------------------------------
 # density matrix and truncation (get_den.f90)
 # ----------------------------------------------------------------------------
@@ -628,6 +660,21 @@ def density_matrix(sb, psi, side):
         else:
             out.append((y, m.T @ m.conj()))
     return out
+
+
+def density_matrix_levels(sb, psis, side):
+    """``Get_density_matrix_sys_new`` / ``_env_new`` (code/get_den.f90:103-177):
+    with one level the density matrix of psis[0]; with two or more,
+    (0.1 rho(psis[0]) + 1.9 rho(psis[1])) / 2 (the other levels do not
+    enter).  0.1 and 1.9 are default-real (single precision) literals in the
+    reference (get_den.f90:133, :171), so the weights are float32(0.1) / 2 and
+    float32(1.9) / 2 and the trace is 1 - 1.1e-8."""
+    r0 = density_matrix(sb, psis[0], side)
+    if len(psis) == 1:
+        return r0
+    r1 = density_matrix(sb, psis[1], side)
+    w0, w1 = float(np.float32(0.1)), float(np.float32(1.9))
+    return [(a, (w0 * m0 + w1 * m1) / 2.0) for (a, m0), (_, m1) in zip(r0, r1)]
 
 
 @dataclass
