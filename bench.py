@@ -43,6 +43,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 TRUN_ERR = 1.0000000180025095e-18
+# the flux t-J 48 x 6 cylinder: device memory after the warm-up at m = 2000,
+# and the largest m it is run at on one GPU
+C5_GB_AT_2000 = 98.9
+C5_M_LIMIT = 3000
 
 
 def run_leg(d, start, args, n):
@@ -465,10 +469,11 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="c2: time one whole sweep per form (2 (Nsite - 3) steps) instead of --steps steps")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c2",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
                          "schedule at m = 256; c3: batched H.psi micro-benchmark; c4: DMRG steps of the flux "
-                         "Hubbard 24x6 cylinder at m = 4000 (as c2)")
+                         "Hubbard 24x6 cylinder at m = 4000 (as c2); c5: DMRG steps of the flux t-J 48x6 "
+                         "cylinder at a reduced --m (its m = 8000 does not fit one GPU)")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--no-lanczos", action="store_true", help="c3: time the batched H.psi only")
     ap.add_argument("--autotune", action="store_true",
@@ -508,6 +513,11 @@ def main():
     from hofstadter_hubbard_tsc_amd import Context, Driver
 
     wl = workload(args.workload)
+    if args.workload == "c5" and (args.m is None or args.m > C5_M_LIMIT):
+        print("c5 at m = %d does not fit one MI355X: its blocks grow as m^2 from the measured %.0f GB at "
+              "m = 2000 to about 16 times that at m = 8000 (beyond 288 GB); run it with --m %d or less"
+              % (wl.kept_max, C5_GB_AT_2000, C5_M_LIMIT), file=sys.stderr)
+        sys.exit(2)
     if args.m is None:
         args.m = wl.kept_max
     model = wl.model
