@@ -175,6 +175,94 @@ def measure_at_scale(ctx, drv, model, sys_len):
     return out
 
 
+SCAN_FLUX = (0.0, 1.0 / 32, 1.0 / 24, 1.0 / 16, 1.0 / 12, 1.0 / 10, 1.0 / 8, 1.0 / 6)
+
+
+def run_scan(args):
+    """Batching on physical superblocks: the lattice of the headline workload
+    (flux t-J Ly = 4 cylinder x Lx = 32, N = 112, J = 0.4) at eight flux
+    values, each warmed up to its centre superblock at m (default 500); then
+    the eight ground states by one batched Lanczos (one grouped H.psi per
+    iteration for all members, replayed from a HIP graph) against one Lanczos
+    per member, from the same seeded start vectors."""
+    import numpy as np
+    import torch
+    from dataclasses import replace
+    from hofstadter_hubbard_tsc_amd import Context, Driver
+    wl = workload("c2")
+    m_kept = args.m or 500
+    ctx = Context(0)
+    drivers, sbs, dims = [], [], []
+    t_prep = time.time()
+    for k, flux in enumerate(SCAN_FLUX[:args.nsb]):
+        model = replace(wl.model, flux=flux)
+        ctx.set_model(model)
+        d = Driver(ctx, kept_max=m_kept, tol=args.tol, kept_max_warm=m_kept, seed=20240601 + k)
+        d.warmup()
+        # the superblock of the warm-up's last step (two half-lattice blocks
+        # around the two centre sites)
+        ls = le = model.nsite // 2 - 1
+        sb = ctx.bind(d.block(0, ls), d.block(1, le), model.n_elec)
+        drivers.append(d)
+        sbs.append(sb)
+        dims.append(sb.dim)
+        print("# scan member %d: flux %.5f, sys %d env %d, D = %d (%.1f s so far)" % (k, flux, ls, le, sb.dim,
+              time.time() - t_prep), file=sys.stderr, flush=True)
+    batch = ctx.batch(sbs)
+    g = torch.Generator().manual_seed(20240601)
+    starts = []
+    for n in dims:
+        x = torch.randn(n, generator=g, dtype=torch.float64) + 1j * torch.randn(n, generator=g, dtype=torch.float64)
+        starts.append((x / torch.linalg.vector_norm(x)).to(torch.complex128).cuda())
+    tol, cap = args.tol, 100
+    # untimed pass of each path (first-use set-up of graphs and workspaces)
+    ctx.batch_lanczos(batch, [p.clone() for p in starts], tol, 3)
+    ctx.lanczos(sbs[0], starts[0].clone(), tol, 3)
+    psis = [p.clone() for p in starts]
+    torch.cuda.synchronize()
+    t0 = time.time()
+    e_b, it_b, cv_b = ctx.batch_lanczos(batch, psis, tol, cap)
+    torch.cuda.synchronize()
+    t_batch = time.time() - t0
+    e_s, it_s = [], []
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for sb, p0 in zip(sbs, starts):
+        e, it = ctx.lanczos(sb, p0.clone(), tol, cap)
+        e_s.append(e)
+        it_s.append(it)
+    torch.cuda.synchronize()
+    t_single = time.time() - t0
+    print("# scan: batched Lanczos %.3f s (iterations %s), one at a time %.3f s (%s)"
+          % (t_batch, it_b.tolist(), t_single, it_s), file=sys.stderr, flush=True)
+    out = {
+        "metric": "batched Lanczos over a flux scan: ground states per second, 8 superblocks of the flux t-J "
+                  "32x4 lattice at once vs one at a time",
+        "value": round(len(sbs) / t_batch, 3),
+        "unit": "ground states/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * t_batch, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (FP64 MFMA)",
+        "data": "synthetic: seeded start vectors; blocks from each member's own warm-up",
+        "config": {"workload": "flux scan of the t-J cylinder Ly=4 x Lx=32, N=112, J=0.4: fluxes %s, centre "
+                               "superblock of each warm-up at m=%d" % ([round(f, 5) for f in SCAN_FLUX[:len(sbs)]],
+                                                                      m_kept),
+                   "superblock_dims": dims, "lanczos_tol": tol, "max_iter": cap,
+                   "batched_seconds": round(t_batch, 4), "one_at_a_time_seconds": round(t_single, 4),
+                   "speedup_vs_one_at_a_time": round(t_single / t_batch, 3),
+                   "iterations_batched": it_b.tolist(), "iterations_alone": it_s,
+                   "energies": [float(x) for x in e_b],
+                   "max_rel_energy_diff_vs_alone": float(max(abs(a - b) / abs(b) for a, b in zip(e_b, e_s))),
+                   "prepare_seconds": round(time.time() - t_prep, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_c1(args):
     """The C1 schedule (Hubbard 16 x 2 ladder, U = 8, N = 28, m = 256): warm-up,
     the two warm-up half sweeps at 15x tolerance, then 6 sweeps at 1e-10, all
@@ -469,11 +557,13 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="c2: time one whole sweep per form (2 (Nsite - 3) steps) instead of --steps steps")
     ap.add_argument("--dump-structure", default="", help="write the sector structure of the last step (JSON)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "scan"], default="c2",
                     help="c2: DMRG steps of the flux t-J cylinder at m (default); c1: the whole Hubbard 16x2 "
                          "schedule at m = 256; c3: batched H.psi micro-benchmark; c4: DMRG steps of the flux "
                          "Hubbard 24x6 cylinder at m = 4000 (as c2); c5: DMRG steps of the flux t-J 48x6 "
-                         "cylinder at a reduced --m (its m = 8000 does not fit one GPU)")
+                         "cylinder at a reduced --m (its m = 8000 does not fit one GPU); scan: the ground "
+                         "states of the headline lattice at 8 flux values by one batched Lanczos (m = 500 "
+                         "unless --m)")
     ap.add_argument("--nsb", type=int, default=16, help="superblocks of the c3 workload")
     ap.add_argument("--no-lanczos", action="store_true", help="c3: time the batched H.psi only")
     ap.add_argument("--autotune", action="store_true",
@@ -508,6 +598,10 @@ def main():
         return
     if args.workload == "c1":
         run_c1(args)
+        return
+    if args.workload == "scan":
+        args.nsb = min(args.nsb, len(SCAN_FLUX))
+        run_scan(args)
         return
     os.environ.setdefault("HHDMRG_VERBOSE", "1")   # progress line per DMRG step on stderr
     from hofstadter_hubbard_tsc_amd import Context, Driver
