@@ -575,6 +575,11 @@ def main():
                     help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
     args = ap.parse_args()
+    if args.workload == "c5" and (args.m is None or args.m > C5_M_LIMIT):
+        print("c5 at m = %d does not fit one MI355X: its blocks grow as m^2 from the measured %.0f GB at "
+              "m = 2000 to about 16 times that at m = 8000 (beyond 288 GB); run it with --m %d or less"
+              % (args.m or 8000, C5_GB_AT_2000, C5_M_LIMIT), file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the SU(2) DMRG inner loop is measured on a single MI355X "
               "(no multi-GPU path is built); use --gpus 1", file=sys.stderr)
@@ -607,11 +612,6 @@ def main():
     from hofstadter_hubbard_tsc_amd import Context, Driver
 
     wl = workload(args.workload)
-    if args.workload == "c5" and (args.m is None or args.m > C5_M_LIMIT):
-        print("c5 at m = %d does not fit one MI355X: its blocks grow as m^2 from the measured %.0f GB at "
-              "m = 2000 to about 16 times that at m = 8000 (beyond 288 GB); run it with --m %d or less"
-              % (wl.kept_max, C5_GB_AT_2000, C5_M_LIMIT), file=sys.stderr)
-        sys.exit(2)
     if args.m is None:
         args.m = wl.kept_max
     model = wl.model
