@@ -82,6 +82,30 @@ def test_c2_full_size_matvec_energy_and_residual(ctx):
     assert abs(r.energy - e_ref) <= 1e-8 * abs(e_ref)
     # the Lanczos estimate beta_k |z_k| is the true residual of the returned vector
     assert abs(r.residual - res) <= 0.05 * res + 1e-9
+    # the chosen form (Davidson, FP32-storage H.psi, FP64 restarts) on the next
+    # step from a device copy of the same state, beside the plain form on that
+    # same superblock: the energies agree, and the chosen form's energy is the
+    # exact FP64 Rayleigh quotient of its vector under the oracle's H
+    f = drv.fork()
+    f.set_solver(2)
+    f.set_precond(1)
+    c = f.sweep(0, bg + 2, bg + 3, 1e-10, TRUN_ERR, 2000, phase=1)[-1]
+    p = drv.sweep(0, bg + 2, bg + 3, 1e-10, TRUN_ERR, 2000, phase=1)[-1]
+    print("c2: next step dim=%d: plain E=%.12f (%d H.psi), chosen E=%.12f (FP64 %d + FP32 %d H.psi), "
+          "residual %.3e" % (p.dim, p.energy, p.hpsi_fp64, c.energy, c.hpsi_fp64, c.hpsi_fp32, c.residual),
+          flush=True)
+    assert c.dim == p.dim and c.solver == 2 and c.hpsi_fp32 > 0 and not c.fallback
+    assert abs(c.energy - p.energy) <= 1e-7 * abs(p.energy)
+    ls2, le2, psi2 = f.wave()
+    H2 = oracle_superblock(ctx, f.block(0, ls2), f.block(1, le2), _oracle_model(m), m.n_elec)
+    x2 = host(psi2)
+    ref2 = H2.matvec(x2)
+    e2 = np.vdot(x2, ref2).real / np.vdot(x2, x2).real
+    res2 = np.linalg.norm(ref2 - e2 * x2)
+    print("c2: chosen form E - <psi|H_oracle|psi> = %.3e, oracle residual %.3e, reported %.3e"
+          % (c.energy - e2, res2, c.residual), flush=True)
+    assert abs(c.energy - e2) <= 1e-10 * abs(e2)
+    assert abs(c.residual - res2) <= 0.05 * res2 + 1e-9
 
 
 def _c1_driver(ctx):
