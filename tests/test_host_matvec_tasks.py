@@ -54,6 +54,12 @@ def _check_step(model, run, ns):
         npexec.run(t2, m2, bufs)
         err = np.abs(out - ref).max() / max(np.abs(ref).max(), 1e-300)
         assert err < 1e-12, (ns, err)
+        # the strided executor (the CPU baseline's) runs the same lists alike
+        out2 = np.full(H.sb.dim, np.nan + 0j)
+        bufs2 = [psi, out2, sys_arena, env_arena, np.zeros_like(X), np.zeros_like(merged), None, None]
+        for tt, mm in ((t0, m0), (t1, m1), (t2, m2)):
+            npexec.run_strided(tt, mm, bufs2)
+        assert np.abs(out2 - out).max() <= 1e-13 * max(np.abs(out).max(), 1e-300), ns
     finally:
         L.load().hh_plan_destroy(plan)
 
