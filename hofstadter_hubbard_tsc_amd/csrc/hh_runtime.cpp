@@ -915,6 +915,15 @@ struct Driver {
   DevMem psi, psi2;
   int64_t psi_dim = 0;
   uint64_t step = 0;
+  // levels > 1 (lanczos_memo_new with levels, Get_density_matrix_*_new): the
+  // carried vectors of the levels above the first and their transform
+  // scratch, the current step's level energies and every record's
+  int levels = 1;
+  std::vector<std::unique_ptr<DevMem>> lpsi;
+  int64_t lpsi_dim = 0;
+  int solved_levels = 1;
+  std::vector<double> cur_e;
+  std::vector<std::vector<double>> lev_e;
   // previous step (for the wave transformation)
   std::unique_ptr<hh_superblock> prev_sb;
 
@@ -1023,6 +1032,59 @@ struct Driver {
     return so;
   }
 
+  // the start vectors of the levels above the first: seeded afresh at every
+  // step (wave_initialize of levels 2.. after the wave transformation,
+  // code/warm_up.f90:1616-1620), or the vectors already held when the step
+  // carries the state of the same superblock
+  void level_starts(int64_t n, bool carry) {
+    const bool held = carry && lpsi_dim == n && (int)lpsi.size() == levels - 1;
+    lpsi.resize(levels - 1);
+    if (held) return;
+    for (int l = 1; l < levels; ++l) {
+      if (!lpsi[l - 1]) lpsi[l - 1].reset(new DevMem());
+      dev_start_vector(psi_buf(n, *lpsi[l - 1]), n, S.seed + step + 7919ull * (uint64_t)l, ctx->stream);
+    }
+  }
+
+  // every level's solve (levels > 1): level 1 into p, level l into lpsi[l-1],
+  // each projected off the levels below; the first level's outcome is the
+  // step's, all energies go to cur_e
+  SolveOut solve_levels(hh_superblock* sb, double2* p, double lanzero) {
+    need(S.solver == 0, "several levels are solved by the plain Lanczos only (solver 0)");
+    const int64_t n = sb->plan.sb.dim;
+    // a level needs a start vector outside the 2 l real directions of the
+    // levels below it, so a superblock of dimension n holds at most n levels;
+    // the rest report NaN
+    const int nl = (int)std::min<int64_t>(levels, n);
+    DevMem defl;
+    defl.alloc(sizeof(double2) * (size_t)n * (size_t)(2 * (levels - 1)));
+    cur_e.assign(levels, std::nan(""));
+    SolveOut first;
+    for (int l = 0; l < nl; ++l) {
+      double2* v = l == 0 ? p : lpsi[l - 1]->c();
+      LanczosOpts opt;
+      opt.deflate = defl.c();
+      opt.ndeflate = 2 * l;
+      const SolveOut so = lanczos(ctx, sb, v, lanzero, S.max_iter, opt);
+      if (l == 0) first = so;
+      cur_e[l] = so.e0;
+      if (l + 1 < nl) {
+        // this level and i times it join the deflation set (v is unit)
+        dev_scale_copy(defl.c() + (size_t)(2 * l) * n, v, 1.0, n, ctx->stream);
+        dev_cscale_copy(defl.c() + (size_t)(2 * l + 1) * n, v, 0.0, 1.0, n, ctx->stream);
+      }
+    }
+    lpsi_dim = n;
+    solved_levels = nl;
+    return first;
+  }
+  SolveOut solve_step(hh_superblock* sb, double2* p, double lanzero) {
+    if (levels == 1) return solve(sb, p, lanzero);
+    return solve_levels(sb, p, lanzero);
+  }
+  // the second level's vector for the mixed density matrix (null with one level)
+  const double2* level2() { return levels > 1 && solved_levels > 1 ? lpsi[0]->c() : nullptr; }
+
   void record(int phase, hh_superblock* sb, const SolveOut& so, const hh_trunc* t, double secs, int64_t mv0,
               double fl0) {
     const double E = so.e0, mvs = so.mv_seconds;
@@ -1053,6 +1115,7 @@ struct Driver {
     r.grow_seconds = ph[4];
     (void)mv0;
     recs.push_back(r);
+    lev_e.push_back(levels > 1 ? cur_e : std::vector<double>{E});
     if (verbose()) {
       std::fprintf(stderr,
                    "# step phase=%d sys=%d env=%d dim=%lld iters=%d%s%s kept=%d E=%.12f res=%.1e %.2fs "
@@ -1090,15 +1153,16 @@ struct Driver {
       const int64_t n = sb->plan.sb.dim;
       double2* p = psi_buf(n, psi);
       dev_start_vector(p, n, S.seed + step, ctx->stream);
+      if (levels > 1) level_starts(n, false);
       ++step;
       lap(1);
-      const SolveOut so = solve(sb.get(), p, lanzero);
+      const SolveOut so = solve_step(sb.get(), p, lanzero);
       lap(2);
       std::unique_ptr<hh_trunc> ts, te;
       if (sb->plan.sbs.total_dim() <= kmax) ts.reset(trunc_identity(ctx, sb.get(), 0));
-      else ts.reset(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
+      else ts.reset(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr, level2()));
       if (sb->plan.ebs.total_dim() <= kmax) te.reset(trunc_identity(ctx, sb.get(), 1));
-      else te.reset(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      else te.reset(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr, level2()));
       lap(3);
       std::unique_ptr<hh_block> ns(grow(ctx, sb.get(), 0, ts.get()));
       std::unique_ptr<hh_block> ne(grow(ctx, sb.get(), 1, te.get()));
@@ -1131,27 +1195,30 @@ struct Driver {
       const int64_t n = sb->plan.sb.dim;
       double2* p;
       if (first && !prev_sb) {
-        if (fresh || psi_dim != n) {
+        const bool carry = !fresh && psi_dim == n;
+        if (!carry) {
           p = psi_buf(n, psi);
           dev_start_vector(p, n, S.seed + step, ctx->stream);
         } else {
           p = psi.c();
         }
+        if (levels > 1) level_starts(n, carry);
       } else {
+        const hh_trunc* ta = held(syst, label, "sys truncation");
+        const hh_trunc* tb = held(envt, sb_env->d.length + 1, "env truncation");
         double2* np = psi_buf(n, psi2);
-        wave_trans(ctx, 0, prev_sb.get(), psi.c(), held(syst, label, "sys truncation"),
-                   held(envt, sb_env->d.length + 1, "env truncation"),
-                   sb.get(), np);
+        wave_trans(ctx, 0, prev_sb.get(), psi.c(), ta, tb, sb.get(), np);
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
         p = psi.c();
+        if (levels > 1) level_starts(n, false);
       }
       ++step;
       lap(1);
-      const SolveOut so = solve(sb.get(), p, lanzero);
+      const SolveOut so = solve_step(sb.get(), p, lanzero);
       lap(2);
       psi_dim = n;
-      std::unique_ptr<hh_trunc> ts(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr));
+      std::unique_ptr<hh_trunc> ts(truncate(ctx, sb.get(), 0, p, kmin(kmax), kmax, terr, level2()));
       lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 0, ts.get()));
       lap(4);
@@ -1182,26 +1249,30 @@ struct Driver {
       const int64_t n = sb->plan.sb.dim;
       double2* p;
       if (first && !prev_sb) {
-        if (psi_dim != n) {
+        const bool carry = psi_dim == n;
+        if (!carry) {
           p = psi_buf(n, psi);
           dev_start_vector(p, n, S.seed + step, ctx->stream);
         } else {
           p = psi.c();
         }
+        if (levels > 1) level_starts(n, carry);
       } else {
+        const hh_trunc* ta = held(syst, sb_sys->d.length + 1, "sys truncation");
+        const hh_trunc* tb = held(envt, ns_ - label - 1, "env truncation");
         double2* np = psi_buf(n, psi2);
-        wave_trans(ctx, 1, prev_sb.get(), psi.c(), held(syst, sb_sys->d.length + 1, "sys truncation"),
-                   held(envt, ns_ - label - 1, "env truncation"), sb.get(), np);
+        wave_trans(ctx, 1, prev_sb.get(), psi.c(), ta, tb, sb.get(), np);
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
         p = psi.c();
+        if (levels > 1) level_starts(n, false);
       }
       ++step;
       lap(1);
-      const SolveOut so = solve(sb.get(), p, lanzero);
+      const SolveOut so = solve_step(sb.get(), p, lanzero);
       lap(2);
       psi_dim = n;
-      std::unique_ptr<hh_trunc> te(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr));
+      std::unique_ptr<hh_trunc> te(truncate(ctx, sb.get(), 1, p, kmin(kmax), kmax, terr, level2()));
       lap(3);
       std::unique_ptr<hh_block> nb(grow(ctx, sb.get(), 1, te.get()));
       lap(4);
@@ -3203,9 +3274,13 @@ void load_driver(Driver& d, const char* path) {
 }
 }  // namespace
 
+// the most levels a driver targets (the reference's runs use 1 or 2)
+constexpr int MAX_LEVELS = 8;
+
 struct hh_driver {
   Driver d;
   size_t taken = 0;
+  size_t last_first = 0;   // first record of the last warmup / sweep call
 };
 
 extern "C" {
@@ -3232,6 +3307,7 @@ static void take_records(hh_driver* h, hh_step_record* rec, int max_rec, int* nr
   const size_t c = std::min<size_t>(n, (size_t)std::max(max_rec, 0));
   if (rec && c) std::memcpy(rec, h->d.recs.data() + h->taken, sizeof(hh_step_record) * c);
   if (nrec) *nrec = (int)n;
+  h->last_first = h->taken;
   h->taken = h->d.recs.size();
 }
 
@@ -3331,6 +3407,31 @@ int hh_driver_set_subspace(hh_driver* h, int kmax) {
   });
 }
 
+int hh_driver_set_levels(hh_driver* h, int levels) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(levels >= 1 && levels <= MAX_LEVELS, "levels must be in [1, 8]");
+    need(2 * (levels - 1) < h->d.S.max_iter, "levels must stay below max_iter / 2");
+    need(levels == 1 || h->d.S.solver == 0, "several levels are solved by the plain Lanczos only (solver 0)");
+    h->d.levels = levels;
+  });
+}
+
+int hh_driver_level_energies(const hh_driver* h, int max_rec, double* e, int* nrec) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    const Driver& d = h->d;
+    const size_t first = h->last_first, n = h->taken - first;
+    const size_t c = std::min<size_t>(n, (size_t)std::max(max_rec, 0));
+    need(c == 0 || e != nullptr, "null argument");
+    for (size_t i = 0; i < c; ++i) {
+      const std::vector<double>& v = d.lev_e[first + i];
+      for (int l = 0; l < d.levels; ++l) e[i * d.levels + l] = l < (int)v.size() ? v[l] : std::nan("");
+    }
+    if (nrec) *nrec = (int)n;
+  });
+}
+
 int hh_driver_copy_precond(hh_driver* dst, const hh_driver* src) {
   if (!dst || !src) return 1;
   return guard(dst->d.ctx, [&] {
@@ -3399,6 +3500,13 @@ int fork_driver(const hh_driver* src, const int lo[2], const int hi[2], hh_drive
     d.fall_factor = s.fall_factor;
     d.min_dim = s.min_dim;
     d.dav_kmax = s.dav_kmax;
+    d.levels = s.levels;
+    for (const auto& v : s.lpsi) {
+      d.lpsi.emplace_back(new DevMem());
+      copy_dev(*d.lpsi.back(), *v);
+    }
+    d.lpsi_dim = s.lpsi_dim;
+    d.solved_levels = s.solved_levels;
     if (s.net) copy_net(ctx, d, s);
     if (s.prev_sb) {
       // the previous step's superblock, bound again to the copied blocks
@@ -3425,6 +3533,7 @@ int hh_driver_set_solver(hh_driver* h, int solver) {
   if (!h) return 1;
   return guard(h->d.ctx, [&] {
     need(solver >= 0 && solver <= 2, "solver must be 0, 1 or 2");
+    need(solver == 0 || h->d.levels == 1, "several levels are solved by the plain Lanczos only (solver 0)");
     h->d.S.solver = solver;
   });
 }

@@ -696,6 +696,7 @@ class Driver:
         d = Driver.__new__(Driver)
         d.ctx = self.ctx
         d.ptr = p
+        d._levels = self.levels
         return d
 
     def capture(self, max_samples):
@@ -726,6 +727,28 @@ class Driver:
         """Largest Davidson subspace of solver 2 before a restart (default 32,
         at most 128)."""
         self.ctx._rc(L.load().hh_driver_set_subspace(self.ptr, int(kmax)), "hh_driver_set_subspace")
+
+    @property
+    def levels(self):
+        return getattr(self, "_levels", 1)
+
+    def set_levels(self, levels):
+        """Levels every later step targets (default 1, at most 8; plain
+        Lanczos only): each step solves them one by one, projecting each off
+        the ones below, and truncates on the mixed density matrix of levels 1
+        and 2.  Records carry level 1's energy; level_energies() gives all."""
+        self.ctx._rc(L.load().hh_driver_set_levels(self.ptr, int(levels)), "hh_driver_set_levels")
+        self._levels = int(levels)
+
+    def level_energies(self, max_rec=100000):
+        """Level energies of the records the last warmup / sweep call
+        returned, shape (records, levels)."""
+        n = L.i32()
+        self.ctx._rc(L.load().hh_driver_level_energies(self.ptr, 0, None, C.byref(n)), "hh_driver_level_energies")
+        e = np.zeros((max(min(n.value, max_rec), 0), self.levels))
+        self.ctx._rc(L.load().hh_driver_level_energies(self.ptr, e.shape[0], e.ctypes.data_as(C.POINTER(C.c_double)),
+                                                       C.byref(n)), "hh_driver_level_energies")
+        return e
 
     def copy_precond(self, src):
         """Take a device copy of src's trained network (same context)."""

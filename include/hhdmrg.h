@@ -461,6 +461,26 @@ int hh_driver_set_min_dim(hh_driver* d, int64_t min_dim);
  * vectors.  No counterpart in the reference, whose lanczos_memo_new keeps up
  * to loop_in = 100 Lanczos vectors (code/lanczos.f90:14, :22). */
 int hh_driver_set_subspace(hh_driver* d, int kmax);
+/* Number of levels every later warm-up and sweep step targets (default 1, at
+ * most 8; the levels argument of warm_up_point / left_to_right /
+ * right_to_left, code/warm_up.f90:1284-1296, :1495-1508, carried from
+ * Get_Ground_State, code/quantum.f90:179-184).  With levels > 1 each step
+ * solves the levels as hh_lanczos_levels does (lanczos_memo_new with levels,
+ * code/lanczos.f90:29-44), starting level 1 from the wave-transformed state
+ * and levels 2.. from seeded vectors (wave_initialize after the wave
+ * transformation, code/warm_up.f90:1616-1620), and truncates on the mixed
+ * density matrix of levels 1 and 2 (Get_density_matrix_*_new,
+ * code/get_den.f90:106-177; hh_density_matrix_levels).  The step record's
+ * energy is level 1's; hh_driver_level_energies gives them all.  Several
+ * levels need solver 0; a checkpoint holds level 1 only, so after a load the
+ * first step seeds the others.  A superblock of dimension n solves at most n
+ * levels and reports NaN for the rest. */
+int hh_driver_set_levels(hh_driver* d, int levels);
+/* The level energies of the records the last warmup / sweep call returned,
+ * levels per record (record-major, the levels set at that call) into e,
+ * at most max_rec records; *nrec = that call's record count. (Save_result_to_disk
+ * writes Energys(1:levels), code/quantum.f90:794-805.) */
+int hh_driver_level_energies(const hh_driver* d, int max_rec, double* e, int* nrec);
 /* The preconditioner's whole training pass on caller data, for checking the
  * kernels: np sector blocks (rows x cols, 2S and N of the sys sector), the
  * diagonal d(a, b) = hs[a] + he[b] per block (hs, he concatenated over the

@@ -101,3 +101,84 @@ def test_mixed_density_matrix_and_truncation_match_the_oracle(ctx):
             kept_gpu = {int(a): int(k) for a, k in zip(*t.kept()) if k > 0}
             kept_ref = {a: c for (a, _), c in zip(ref, tr.kept_per_sector) if c > 0}
             assert kept_gpu == kept_ref
+
+
+def test_driver_levels_sweep_equals_ed_singlet_levels(ctx):
+    """The step-wise driver with levels = 2 (Get_Ground_State with levels,
+    code/quantum.f90:179-288): every step of the first full left-to-right
+    sweep over the untruncated Hubbard 4 x 2 lattice (flux 0.15, N = 6) reports the two lowest
+    S = 0 levels of the exact diagonalisation (1e-9), the record's energy is
+    level 1's, and the levels stay set on a fork; several levels refuse
+    solvers 1 and 2."""
+    from hofstadter_hubbard_tsc_amd import Driver, HHError
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    _, _, sg = _superblock(ctx, m, 2000, 3)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    spec = ed.singlet_spectrum(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    drv = Driver(ctx, kept_max=2000, tol=1e-12, kept_max_warm=2000, seed=11)
+    drv.set_levels(2)
+    w = drv.warmup()
+    assert drv.level_energies().shape == (len(w), 2)
+    n = m.nsite
+    # the warm-up's blocks hold the warm-up's own geometry: the half sweeps
+    # out from the centre and back rebuild every block in the lattice, after
+    # which each step's superblock is the whole untruncated lattice
+    drv.sweep(0, n // 2 - 1, n - 2, 1e-12, driver.TRUN_ERR, 2000, fresh=True, phase=1)
+    drv.reset_transform()
+    drv.sweep(1, n - 2, 1, 1e-12, driver.TRUN_ERR, 2000, phase=1)
+    drv.reset_transform()
+    recs = drv.sweep(0, 1, n - 2, 1e-12, driver.TRUN_ERR, 2000, phase=2)
+    E = drv.level_energies()
+    assert E.shape == (len(recs), 2) and len(recs) == n - 3
+    for r, (e1, e2) in zip(recs, E):
+        print("sys_len=%d dim=%d E1=%.12f E2=%.12f (ED %.12f %.12f)" % (r.sys_len, r.dim, e1, e2, spec[0], spec[1]))
+        assert r.energy == e1
+        assert abs(e1 - spec[0]) < 1e-9 * abs(spec[0])
+        assert abs(e2 - spec[1]) < 1e-9 * abs(spec[1])
+    f = drv.fork()
+    assert f.levels == 2
+    f.reset_transform()
+    f.sweep(1, n - 2, n - 3, 1e-12, driver.TRUN_ERR, 2000, phase=1)
+    assert abs(f.level_energies()[0, 1] - spec[1]) < 1e-9 * abs(spec[1])
+    with pytest.raises(HHError):
+        drv.set_solver(2)
+    drv.set_levels(1)
+    drv.set_solver(2)
+    with pytest.raises(HHError):
+        drv.set_levels(2)
+
+
+def test_driver_levels_truncate_on_the_mixed_density_matrix(ctx):
+    """A truncated t-J 4 x 4 cylinder sweep (flux 0.125, m = 40) with levels
+    = 2: each step's kept count, truncation error and entropy are those of the
+    mixed density matrix (0.1 rho_1 + 1.9 rho_2) / 2 of the step's superblock,
+    recomputed from that superblock's two lowest levels (kept count exact,
+    error 1e-4 relative, entropy 1e-6), and not those of rho_1 alone."""
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="tj", Lx=4, Ly=4, J=0.4, t=1.0, flux=0.125, n_elec=14)
+    ctx.set_model(m)
+    kmax = 40
+    drv = Driver(ctx, kept_max=kmax, tol=1e-12, kept_max_warm=kmax, seed=5)
+    drv.set_levels(2)
+    drv.warmup()
+    n = m.nsite
+    checked = 0
+    for ls in range(n // 2 - 1, n // 2 + 2):
+        r = drv.sweep(0, ls, ls + 1, 1e-12, driver.TRUN_ERR, kmax, fresh=(ls == n // 2 - 1), phase=1)[-1]
+        e1, e2 = drv.level_energies()[-1]
+        sl, el, psi = drv.wave()
+        sb = ctx.bind(drv.block(0, sl), drv.block(1, el), m.n_elec)
+        psis = [psi.clone(), dev(dmrg.start_vector(sb.dim, 40 + ls))]
+        es, _ = ctx.lanczos_levels(sb, psis, 1e-13, 100)
+        print("sys_len=%d dim=%d driver E=(%.12f, %.12f) again (%.12f, %.12f) kept=%d err=%.6e S=%.9f"
+              % (sl, r.dim, e1, e2, es[0], es[1], r.num_state, r.trunc_err, r.entropy))
+        assert abs(es[0] - e1) < 1e-9 * abs(e1) and abs(es[1] - e2) < 1e-9 * abs(e2)
+        kmin = min(3 * kmax // 4, 3600)
+        t = ctx.truncate(sb, 0, psis[0], kmin, kmax, driver.TRUN_ERR, psi2=psis[1])
+        t1 = ctx.truncate(sb, 0, psis[0], kmin, kmax, driver.TRUN_ERR)
+        assert t.num_state == r.num_state
+        assert abs(t.err - r.trunc_err) <= 1e-4 * r.trunc_err
+        assert abs(t.entropy - r.entropy) <= 1e-6
+        assert abs(t1.err - r.trunc_err) > 1e-2 * r.trunc_err
+        checked += 1
+    assert checked == 3
