@@ -269,7 +269,6 @@ void layout_block(BlockDesc& b, int nkinds) {
 // ============================================================================
 // site operators (Get_site_operator_su2, code/warm_up.f90:818-991)
 // ============================================================================
-namespace {
 // reduced element <s'||O||s> of a site operator; sector indices of site_sectors
 cplx site_val(int model_kind, int kind, int sp, int s) {
   const double r2 = std::sqrt(2.0);
@@ -293,6 +292,8 @@ cplx site_tilde(int model_kind, const std::vector<Sector>& st, int sp, int s) {
   if (c == 0.0) return 0.0;
   return -phase2(1 + st[s].S2 - st[sp].S2) * std::conj(c);
 }
+
+namespace {
 inline double nsign(int N) { return (N & 1) ? -1.0 : 1.0; }
 
 // (block sector, site sector) -> all enlarged (sector, pos, dim)

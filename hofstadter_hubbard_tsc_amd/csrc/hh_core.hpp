@@ -92,6 +92,13 @@ struct SuperBasis {
 SuperBasis super_basis(const EnlBasis& s, const EnlBasis& e, int N);
 
 std::vector<Sector> site_sectors(int kind);
+// reduced element <s'||O||s> of the site operator `kind` (OP_CDAG, OP_S,
+// OP_N) between sectors sp and s of site_sectors (Get_site_operator_su2,
+// code/warm_up.f90:818-991)
+cplx site_val(int model_kind, int kind, int sp, int s);
+// annihilator partner of the site c^dag over the site sectors st:
+// <j||a~||j'> = -(-1)^(1/2+j'-j) conj(<j'||c^dag||j>)
+cplx site_tilde(int model_kind, const std::vector<Sector>& st, int sp, int s);
 
 // ----------------------------------------------------------------------------
 // operator layouts (one per operator type on a block)

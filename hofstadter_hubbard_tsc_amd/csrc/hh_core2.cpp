@@ -12,27 +12,6 @@ namespace hh {
 namespace {
 inline double nsign(int N) { return (N & 1) ? -1.0 : 1.0; }
 
-cplx site_val(int model_kind, int kind, int sp, int s) {
-  const double r2 = std::sqrt(2.0);
-  if (kind == OP_CDAG) {
-    if (sp == 1 && s == 0) return -r2;
-    if (model_kind == HUBBARD && sp == 2 && s == 1) return r2;
-    return 0.0;
-  }
-  if (kind == OP_S) return (sp == 1 && s == 1) ? std::sqrt(1.5) : 0.0;
-  if (kind == OP_N) {
-    if (sp != s) return 0.0;
-    if (s == 1) return r2;
-    if (s == 2) return 2.0;
-  }
-  return 0.0;
-}
-cplx site_tilde(int model_kind, const std::vector<Sector>& st, int sp, int s) {
-  cplx c = site_val(model_kind, OP_CDAG, s, sp);
-  if (c == 0.0) return 0.0;
-  return -phase2(1 + st[s].S2 - st[sp].S2) * std::conj(c);
-}
-
 inline Term mk(cplx c, uint8_t abuf, int64_t aoff, int ald, uint8_t aop, uint8_t bbuf, int64_t boff, int bld,
                uint8_t bop, int K) {
   Term t;
