@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhhdmrg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["hh_core.cpp", "hh_core2.cpp", "hh_runtime.cpp", "hh_measure.cpp", "hh_files.cpp", "hh_davidson.cpp", "hh_batch.cpp",
+SOURCES = ["hh_core.cpp", "hh_core2.cpp", "hh_runtime.cpp", "hh_measure.cpp", "hh_checkpoint.cpp", "hh_plans.cpp",
+           "hh_files.cpp", "hh_davidson.cpp", "hh_batch.cpp",
            "hh_tune.cpp",
            "hh_engine.hip", "hh_precond.hip", "hh_heev.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(HERE, "..", "include"), "-I" + CSRC]

@@ -35,6 +35,16 @@ hh_block* site_block(hh_ctx* ctx, int side);
 // one FP64 task list on the context's stream, waited for
 void launch_on(hh_ctx* ctx, const TaskList& tl, const BufPtrs& b);
 
+// a block of `length` sites with sector table `sec` and its operator arena
+hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>& sec, const std::vector<int>& sites);
+// a sweep schedule the driver can run (from the caller or from a checkpoint)
+void check_schedule(const hh_schedule& s);
+
+struct Driver;
+// the driver's whole state to / from one binary file (hh_checkpoint.cpp)
+void save_driver(Driver& d, const char* path);
+void load_driver(Driver& d, const char* path);
+
 // ----------------------------------------------------------------------------
 // sweep driver: Get_Ground_State (code/quantum.f90:179-288)
 // ----------------------------------------------------------------------------

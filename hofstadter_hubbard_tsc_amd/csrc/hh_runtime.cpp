@@ -130,7 +130,7 @@ BufPtrs bufs_of(double2* in, double2* out, const hh_block* sys, const hh_block* 
 
 }  // namespace hh
 
-namespace {
+namespace hh {
 
 hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>& sec, const std::vector<int>& sites) {
   auto* b = new hh_block();
@@ -143,7 +143,7 @@ hh_block* new_block(hh_ctx* ctx, int side, int length, const std::vector<Sector>
   return b;
 }
 
-}  // namespace
+}  // namespace hh
 
 namespace hh {
 
@@ -860,6 +860,35 @@ hh_superblock* bind(hh_ctx* ctx, const hh_block* sys, const hh_block* env, int N
   return s;
 }
 
+// the C model struct as the task compilers take it
+ModelParams to_model(const hh_model_params* p) {
+  ModelParams m;
+  m.kind = p->kind;
+  m.Lx = p->Lx;
+  m.Ly = p->Ly;
+  m.lattice = p->lattice;
+  m.periodic_y = p->periodic_y;
+  m.n_elec = p->n_elec;
+  m.t = p->t;
+  m.U = p->U;
+  m.J = p->J;
+  m.flux = p->flux;
+  m.gauge = p->gauge;
+  m.theta = p->theta;
+  return m;
+}
+
+// the gauge fields of a C model struct checked
+void check_gauge(const hh_model_params* p) {
+  need(p->gauge == GAUGE_LANDAU || p->gauge == GAUGE_REFERENCE, "gauge must be 0 (Landau) or 1 (get_phase)");
+  need(p->reserved == 0, "reserved field of hh_model_params must be 0");
+  if (p->gauge == GAUGE_REFERENCE) {
+    need(p->lattice == TRIANGULAR, "gauge 1 (get_phase) is defined on the triangular lattice only");
+    need(p->flux == 0.0, "gauge 1 takes its flux from theta; flux must be 0");
+    need(std::isfinite(p->theta), "theta must be finite");
+  }
+}
+
 void launch_on(hh_ctx* ctx, const TaskList& tl, const BufPtrs& b) {
   DevTasks dt;
   dt.upload(tl, ctx->stream);
@@ -980,33 +1009,6 @@ int hh_boundary_sites(hh_ctx* ctx, int length, int32_t* n, int32_t* sites) {
     if (sites)
       for (size_t i = 0; i < b.size(); ++i) sites[i] = b[i];
   });
-}
-
-static ModelParams to_model(const hh_model_params* p) {
-  ModelParams m;
-  m.kind = p->kind;
-  m.Lx = p->Lx;
-  m.Ly = p->Ly;
-  m.lattice = p->lattice;
-  m.periodic_y = p->periodic_y;
-  m.n_elec = p->n_elec;
-  m.t = p->t;
-  m.U = p->U;
-  m.J = p->J;
-  m.flux = p->flux;
-  m.gauge = p->gauge;
-  m.theta = p->theta;
-  return m;
-}
-
-static void check_gauge(const hh_model_params* p) {
-  need(p->gauge == GAUGE_LANDAU || p->gauge == GAUGE_REFERENCE, "gauge must be 0 (Landau) or 1 (get_phase)");
-  need(p->reserved == 0, "reserved field of hh_model_params must be 0");
-  if (p->gauge == GAUGE_REFERENCE) {
-    need(p->lattice == TRIANGULAR, "gauge 1 (get_phase) is defined on the triangular lattice only");
-    need(p->flux == 0.0, "gauge 1 takes its flux from theta; flux must be 0");
-    need(std::isfinite(p->theta), "theta must be finite");
-  }
 }
 
 int hh_set_model(hh_ctx* ctx, const hh_model_params* p) {
@@ -1433,226 +1435,6 @@ int hh_dmrg_run(hh_ctx* ctx, const hh_schedule* s, hh_step_record* rec, int max_
 
 }  // extern "C"
 
-// ----------------------------------------------------------------------------
-// checkpoint / restart of a driver: what model_to_disk, truns_to_disk and
-// wave_to_disk (code/space_info.f90:1406-1762) keep for restart.dat's resume
-// (code/quantum.f90:43-47, :207-253) -- every stored block with its
-// operators, every truncation, the carried wave function and the step
-// counter -- in one binary file of this library's own layout.
-// ----------------------------------------------------------------------------
-namespace {
-constexpr char CK_MAGIC[8] = {'H', 'H', 'D', 'M', 'R', 'G', 'C', 'K'};
-constexpr uint32_t CK_VERSION = 2;
-
-struct CkOut {
-  FILE* f;
-  void raw(const void* p, size_t n) { need(std::fwrite(p, 1, n, f) == n, "checkpoint write failed"); }
-  template <typename V> void put(V v) { raw(&v, sizeof(V)); }
-  void sectors(const std::vector<Sector>& v) {
-    put<uint32_t>((uint32_t)v.size());
-    for (auto& q : v) {
-      put<int32_t>(q.N);
-      put<int32_t>(q.S2);
-      put<int32_t>(q.dim);
-    }
-  }
-  void device(hh_ctx* ctx, const void* dev, int64_t elems) {
-    put<int64_t>(elems);
-    if (elems <= 0) return;
-    std::vector<double2> h((size_t)elems);
-    check_hip(hipMemcpyAsync(h.data(), dev, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    check_hip(hipStreamSynchronize(ctx->stream), "d2h");
-    raw(h.data(), sizeof(double2) * h.size());
-  }
-};
-struct CkIn {
-  FILE* f;
-  void raw(void* p, size_t n) { need(std::fread(p, 1, n, f) == n, "checkpoint file is truncated"); }
-  template <typename V> V get() {
-    V v;
-    raw(&v, sizeof(V));
-    return v;
-  }
-  std::vector<Sector> sectors() {
-    const uint32_t n = get<uint32_t>();
-    need(n < (1u << 24), "checkpoint file is corrupt (sector count)");
-    std::vector<Sector> v(n);
-    for (auto& q : v) {
-      q.N = get<int32_t>();
-      q.S2 = get<int32_t>();
-      q.dim = get<int32_t>();
-      need(q.dim >= 0 && q.S2 >= 0, "checkpoint file is corrupt (sector)");
-    }
-    return v;
-  }
-  // elements into a device buffer of at least `expect` elements (expect < 0: any)
-  int64_t device(hh_ctx* ctx, DevMem& dst, int64_t expect) {
-    const int64_t n = get<int64_t>();
-    need(n >= 0 && (expect < 0 || n == expect), "checkpoint file is corrupt (array length)");
-    dst.alloc(sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
-    if (n > 0) {
-      std::vector<double2> h((size_t)n);
-      raw(h.data(), sizeof(double2) * h.size());
-      check_hip(hipMemcpyAsync(dst.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream),
-                "h2d");
-      check_hip(hipStreamSynchronize(ctx->stream), "h2d");
-    }
-    return n;
-  }
-};
-
-void put_model(CkOut& o, const ModelParams& m) {
-  o.put<int32_t>(m.kind);
-  o.put<int32_t>(m.Lx);
-  o.put<int32_t>(m.Ly);
-  o.put<int32_t>(m.lattice);
-  o.put<int32_t>(m.periodic_y ? 1 : 0);
-  o.put<int32_t>(m.n_elec);
-  o.put<double>(m.t);
-  o.put<double>(m.U);
-  o.put<double>(m.J);
-  o.put<double>(m.flux);
-  o.put<int32_t>(m.gauge);
-  o.put<double>(m.theta);
-}
-
-void save_driver(Driver& d, const char* path) {
-  hh_ctx* ctx = d.ctx;
-  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
-  need(f != nullptr, "cannot open the checkpoint file for writing");
-  CkOut o{f.get()};
-  o.raw(CK_MAGIC, sizeof(CK_MAGIC));
-  o.put<uint32_t>(CK_VERSION);
-  put_model(o, ctx->model);
-  o.put<hh_schedule>(d.S);
-  o.put<uint64_t>(d.step);
-  o.put<int64_t>(d.psi_dim);
-  o.device(ctx, d.psi.p, d.psi.p ? d.psi_dim : 0);
-  for (int side = 0; side < 2; ++side) {
-    const auto& blocks = side == 0 ? d.sysb : d.envb;
-    o.put<uint32_t>((uint32_t)blocks.size());
-    for (auto& kv : blocks) {
-      const BlockDesc& b = kv.second->d;
-      o.put<int32_t>(kv.first);
-      o.put<int32_t>(b.length);
-      o.sectors(b.sec);
-      o.put<uint32_t>((uint32_t)b.sites.size());
-      for (int k : b.sites) o.put<int32_t>(k);
-      o.device(ctx, kv.second->arena.p, b.size);
-    }
-    const auto& truns = side == 0 ? d.syst : d.envt;
-    o.put<uint32_t>((uint32_t)truns.size());
-    for (auto& kv : truns) {
-      const hh_trunc* t = kv.second.get();
-      o.put<int32_t>(kv.first);
-      o.sectors(t->eb.bl);
-      o.put<uint32_t>((uint32_t)t->kept.size());
-      for (auto& k : t->kept) {
-        o.put<int32_t>(k.a);
-        o.put<int32_t>(k.keep);
-        o.put<int64_t>(k.u_off);
-        o.put<int32_t>(k.u_ld);
-      }
-      o.put<uint32_t>((uint32_t)t->kept_per_rho.size());
-      for (int k : t->kept_per_rho) o.put<int32_t>(k);
-      o.put<int32_t>(t->num_state);
-      o.put<double>(t->err);
-      o.put<double>(t->entropy);
-      o.device(ctx, t->evec.p, t->evec.p ? (int64_t)(t->evec.bytes / sizeof(double2)) : 0);
-    }
-  }
-  need(std::fflush(f.get()) == 0, "checkpoint write failed");
-}
-
-void load_driver(Driver& d, const char* path) {
-  hh_ctx* ctx = d.ctx;
-  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "rb"), &std::fclose);
-  need(f != nullptr, "cannot open the checkpoint file");
-  CkIn in{f.get()};
-  char magic[8];
-  in.raw(magic, sizeof(magic));
-  need(std::memcmp(magic, CK_MAGIC, sizeof(magic)) == 0, "not a checkpoint of this library");
-  need(in.get<uint32_t>() == CK_VERSION, "checkpoint version not supported");
-  const ModelParams& m = ctx->model;
-  const int kind = in.get<int32_t>(), Lx = in.get<int32_t>(), Ly = in.get<int32_t>(), lattice = in.get<int32_t>();
-  const int periodic = in.get<int32_t>(), n_elec = in.get<int32_t>();
-  const double t = in.get<double>(), U = in.get<double>(), J = in.get<double>(), flux = in.get<double>();
-  const int gauge = in.get<int32_t>();
-  const double theta = in.get<double>();
-  need(kind == m.kind && Lx == m.Lx && Ly == m.Ly && lattice == m.lattice && periodic == (m.periodic_y ? 1 : 0) &&
-           n_elec == m.n_elec && t == m.t && U == m.U && J == m.J && flux == m.flux && gauge == m.gauge &&
-           theta == m.theta,
-       "the checkpoint was written for another model than the context's");
-  Driver fresh;   // build the whole state aside; the driver is replaced only on success
-  fresh.ctx = ctx;
-  fresh.S = in.get<hh_schedule>();
-  check_schedule(fresh.S);
-  fresh.step = in.get<uint64_t>();
-  fresh.psi_dim = in.get<int64_t>();
-  in.device(ctx, fresh.psi, fresh.psi_dim > 0 ? fresh.psi_dim : 0);
-  const auto st = site_sectors(m.kind);
-  for (int side = 0; side < 2; ++side) {
-    auto& blocks = side == 0 ? fresh.sysb : fresh.envb;
-    const uint32_t nb = in.get<uint32_t>();
-    for (uint32_t i = 0; i < nb; ++i) {
-      const int key = in.get<int32_t>(), length = in.get<int32_t>();
-      need(key == length && length >= 1 && length <= m.nsite(), "checkpoint file is corrupt (block length)");
-      std::vector<Sector> sec = in.sectors();
-      const uint32_t ns = in.get<uint32_t>();
-      need(ns <= (uint32_t)m.nsite(), "checkpoint file is corrupt (sites)");
-      std::vector<int> sites(ns);
-      for (auto& k : sites) k = in.get<int32_t>();
-      std::unique_ptr<hh_block> b(new_block(ctx, side, length, sec, sites));
-      in.device(ctx, b->arena, b->d.size);
-      blocks[key] = std::move(b);
-    }
-    auto& truns = side == 0 ? fresh.syst : fresh.envt;
-    const uint32_t nt = in.get<uint32_t>();
-    for (uint32_t i = 0; i < nt; ++i) {
-      const int key = in.get<int32_t>();
-      need(key >= 1 && key <= m.nsite(), "checkpoint file is corrupt (truncation length)");
-      std::unique_ptr<hh_trunc> tr(new hh_trunc());
-      tr->side = side;
-      tr->eb = enlarge_basis(in.sectors(), st);
-      const uint32_t nk = in.get<uint32_t>();
-      need(nk <= tr->eb.sec.size(), "checkpoint file is corrupt (kept sectors)");
-      tr->kept.resize(nk);
-      int64_t need_elems = 0;
-      for (auto& k : tr->kept) {
-        k.a = in.get<int32_t>();
-        k.keep = in.get<int32_t>();
-        k.u_off = in.get<int64_t>();
-        k.u_ld = in.get<int32_t>();
-        need(k.a >= 0 && k.a < (int)tr->eb.sec.size() && k.keep >= 0 && k.keep <= tr->eb.sec[k.a].dim &&
-                 k.u_ld >= tr->eb.sec[k.a].dim && k.u_off >= 0,
-             "checkpoint file is corrupt (kept sector)");
-        need_elems = std::max<int64_t>(need_elems, k.u_off + (int64_t)k.u_ld * k.keep);
-      }
-      const uint32_t nr = in.get<uint32_t>();
-      need(nr < (1u << 24), "checkpoint file is corrupt (rho blocks)");
-      tr->kept_per_rho.resize(nr);
-      for (auto& k : tr->kept_per_rho) k = in.get<int32_t>();
-      tr->num_state = in.get<int32_t>();
-      tr->err = in.get<double>();
-      tr->entropy = in.get<double>();
-      const int64_t n = in.device(ctx, tr->evec, -1);
-      need(n >= need_elems, "checkpoint file is corrupt (eigenvector arena)");
-      truns[key] = std::move(tr);
-    }
-  }
-  need(std::fgetc(f.get()) == EOF, "checkpoint file is corrupt (data after the last record)");
-  d.S = fresh.S;
-  d.step = fresh.step;
-  d.psi_dim = fresh.psi_dim;
-  std::swap(d.psi.p, fresh.psi.p);
-  std::swap(d.psi.bytes, fresh.psi.bytes);
-  d.sysb.swap(fresh.sysb);
-  d.envb.swap(fresh.envb);
-  d.syst.swap(fresh.syst);
-  d.envt.swap(fresh.envt);
-  d.prev_sb.reset();   // the next step starts from the carried wave function
-}
-}  // namespace
 
 
 extern "C" {
@@ -2055,184 +1837,6 @@ int hh_superblock_tasks(const hh_superblock* sb, int stage, const void** tiles, 
   if (terms) *terms = tl->terms.data();
   if (nterms) *nterms = (int32_t)tl->terms.size();
   if (scratch) *scratch = stage == 0 ? sb->plan.merged_size : sb->plan.x_size;
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
-// host-only plans
-// ---------------------------------------------------------------------------
-}  // extern "C"
-
-struct hh_plan {
-  BlockDesc blk[3];
-  MatvecPlan mv;
-  GrowPlan gp;
-  bool is_grow = false;
-  std::string err;
-};
-
-namespace {
-BlockDesc make_desc(const ModelParams& m, int side, int len, int nsec, const int32_t* N, const int32_t* S2,
-                    const int32_t* dim, int nsites, const int32_t* sites) {
-  BlockDesc d;
-  d.side = side;
-  d.length = len;
-  for (int i = 0; i < nsec; ++i) d.sec.push_back({N[i], S2[i], dim[i]});
-  for (int i = 0; i < nsites; ++i) d.sites.push_back(sites[i]);
-  layout_block(d, n_op_kinds(m.kind));
-  return d;
-}
-}  // namespace
-
-extern "C" {
-
-int hh_plan_matvec(const hh_model_params* mp, int sys_len, int sys_nsec, const int32_t* sys_N, const int32_t* sys_S2,
-                   const int32_t* sys_dim, int sys_nsites, const int32_t* sys_sites, int env_len, int env_nsec,
-                   const int32_t* env_N, const int32_t* env_S2, const int32_t* env_dim, int env_nsites,
-                   const int32_t* env_sites, int N, hh_plan** out) {
-  try {
-    need(mp != nullptr, "null model");
-    check_gauge(mp);
-    ModelParams m = to_model(mp);
-    auto* p = new hh_plan();
-    p->blk[0] = make_desc(m, 0, sys_len, sys_nsec, sys_N, sys_S2, sys_dim, sys_nsites, sys_sites);
-    p->blk[1] = make_desc(m, 1, env_len, env_nsec, env_N, env_S2, env_dim, env_nsites, env_sites);
-    Geometry g = make_geometry(m, sys_len, env_len);
-    p->mv = compile_matvec(m, g, p->blk[0], p->blk[1], N);
-    *out = p;
-    return 0;
-  } catch (const std::exception& e) {
-    create_error() = e.what();
-    return 1;
-  }
-}
-
-int hh_plan_grow(const hh_model_params* mp, int side, int length, int nsec, const int32_t* Nq, const int32_t* S2q,
-                 const int32_t* dimq, int nsites, const int32_t* sites, int ns, int ne, int nkept,
-                 const int32_t* kept_a, const int32_t* kept_n, const int64_t* u_off, const int32_t* u_ld,
-                 hh_plan** out) {
-  try {
-    need(mp != nullptr, "null model");
-    check_gauge(mp);
-    ModelParams m = to_model(mp);
-    auto* p = new hh_plan();
-    p->is_grow = true;
-    p->blk[side] = make_desc(m, side, length, nsec, Nq, S2q, dimq, nsites, sites);
-    Geometry g = make_geometry(m, ns, ne);
-    std::vector<KeptSector> kept;
-    for (int i = 0; i < nkept; ++i) kept.push_back({kept_a[i], kept_n[i], u_off[i], u_ld[i]});
-    p->gp = compile_grow(m, g, p->blk[side], kept);
-    p->blk[2] = p->gp.out;
-    *out = p;
-    return 0;
-  } catch (const std::exception& e) {
-    create_error() = e.what();
-    return 1;
-  }
-}
-
-int hh_plan_destroy(hh_plan* p) {
-  delete p;
-  return 0;
-}
-
-int hh_plan_tasks(const hh_plan* p, int stage, const void** tiles, int32_t* ntiles, const void** terms,
-                  int32_t* nterms, int64_t* scratch) {
-  if (!p || stage < 0 || stage > 2) return 1;
-  const TaskList* tl;
-  int64_t sc;
-  if (p->is_grow) {
-    tl = stage == 0 ? &p->gp.merge_tasks : stage == 1 ? &p->gp.stage1 : &p->gp.stage2;
-    sc = stage == 0 ? p->gp.merged_size : p->gp.x_size;
-  } else {
-    tl = stage == 0 ? &p->mv.merge_tasks : stage == 1 ? &p->mv.stage1 : &p->mv.stage2;
-    sc = stage == 0 ? p->mv.merged_size : p->mv.x_size;
-  }
-  if (tiles) *tiles = tl->tiles.data();
-  if (ntiles) *ntiles = (int32_t)tl->tiles.size();
-  if (terms) *terms = tl->terms.data();
-  if (nterms) *nterms = (int32_t)tl->terms.size();
-  if (scratch) *scratch = sc;
-  return 0;
-}
-
-int hh_plan_block(const hh_plan* p, int which, int64_t* arena, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim,
-                  int64_t* h_off, int32_t* nsites, int32_t* sites) {
-  if (!p || which < 0 || which > 2) return 1;
-  const BlockDesc& d = p->blk[which];
-  if (arena) *arena = d.size;
-  if (nsec) *nsec = (int32_t)d.sec.size();
-  for (size_t i = 0; i < d.sec.size(); ++i) {
-    if (N) N[i] = d.sec[i].N;
-    if (S2) S2[i] = d.sec[i].S2;
-    if (dim) dim[i] = d.sec[i].dim;
-    if (h_off) h_off[i] = d.h_off[i];
-  }
-  if (nsites) *nsites = (int32_t)d.sites.size();
-  if (sites)
-    for (size_t i = 0; i < d.sites.size(); ++i) sites[i] = d.sites[i];
-  return 0;
-}
-
-int hh_plan_layout(const hh_plan* p, int which, int kind, int32_t* nsub, int32_t* bra, int32_t* ket, int64_t* off) {
-  if (!p || which < 0 || which > 2 || kind < 0 || kind > 2) return 1;
-  const OpLayout& L = p->blk[which].lay[kind];
-  if (nsub) *nsub = (int32_t)L.subs.size();
-  for (size_t i = 0; i < L.subs.size(); ++i) {
-    if (bra) bra[i] = L.subs[i].bra;
-    if (ket) ket[i] = L.subs[i].ket;
-    if (off) off[i] = L.subs[i].off;
-  }
-  return 0;
-}
-
-int hh_plan_op_offset(const hh_plan* p, int which, int site_index, int kind, int64_t* off) {
-  if (!p || which < 0 || which > 2) return 1;
-  const BlockDesc& d = p->blk[which];
-  if (site_index < 0 || site_index >= (int)d.sites.size() || kind < 0 || kind > 2) return 1;
-  *off = d.op_off[site_index][kind];
-  return *off < 0 ? 1 : 0;
-}
-
-int hh_plan_enlarged(const hh_plan* p, int side, int32_t* nsec, int32_t* N, int32_t* S2, int32_t* dim, int32_t* nsub,
-                     int32_t* subs) {
-  if (!p || p->is_grow || (side != 0 && side != 1) || !nsec) return 1;
-  const EnlBasis& e = side == 0 ? p->mv.sbs : p->mv.ebs;
-  *nsec = (int32_t)e.sec.size();
-  int32_t k = 0;
-  for (size_t a = 0; a < e.sec.size(); ++a) {
-    if (N) N[a] = e.sec[a].N;
-    if (S2) S2[a] = e.sec[a].S2;
-    if (dim) dim[a] = e.sec[a].dim;
-    for (const SubSector& q : e.subs[a]) {
-      if (subs) {
-        int32_t* r = subs + 5 * (size_t)k;
-        r[0] = (int32_t)a;
-        r[1] = q.b;
-        r[2] = q.s;
-        r[3] = q.pos;
-        r[4] = q.dim;
-      }
-      ++k;
-    }
-  }
-  if (nsub) *nsub = k;
-  return 0;
-}
-
-int hh_plan_super(const hh_plan* p, int64_t* dim, int32_t* npairs, int32_t* sys_sec, int32_t* env_sec, int32_t* rows,
-                  int32_t* cols, int64_t* off) {
-  if (!p || p->is_grow) return 1;
-  const SuperBasis& S = p->mv.sb;
-  if (dim) *dim = S.dim;
-  if (npairs) *npairs = (int32_t)S.pairs.size();
-  for (size_t i = 0; i < S.pairs.size(); ++i) {
-    if (sys_sec) sys_sec[i] = S.pairs[i].first;
-    if (env_sec) env_sec[i] = S.pairs[i].second;
-    if (rows) rows[i] = S.rows[i];
-    if (cols) cols[i] = S.cols[i];
-    if (off) off[i] = S.off[i];
-  }
   return 0;
 }
 

@@ -271,4 +271,11 @@ uint64_t shape_key(const TaskList& tl);
 void autotune_superblock(hh_ctx* ctx, hh_superblock* sb);
 void autotune_batch(hh_ctx* ctx, hh_batch* b, const TaskList& s1, const TaskList& s2);
 
+// The C model struct as the task compilers take it, and the check of its
+// gauge fields (hh_runtime.cpp)
+ModelParams to_model(const hh_model_params* p);
+void check_gauge(const hh_model_params* p);
+// message of a failed create call (the object it would have made does not exist)
+std::string& create_error();
+
 }  // namespace hh
