@@ -594,6 +594,10 @@ def main():
     ap.add_argument("--measure", action="store_true",
                     help="c2: after the timed steps, time the measurement entry points on the last superblock "
                          "(reported under config.measure, outside the timed region)")
+    ap.add_argument("--levels", type=int, default=1,
+                    help="c2: also time the same steps targeting this many levels (plain Lanczos per level, "
+                         "truncation on the mixed density matrix of levels 1 and 2), reported under "
+                         "config.forms.levelsL; 1 = off")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -686,6 +690,14 @@ def main():
             f.set_solver(2)
             f.set_precond(mode)
             forks[name] = f
+    if args.levels > 1:
+        # the same steps with several levels (plain Lanczos per level, mixed
+        # density matrix), from another device copy of the same state
+        window = None if args.sweep else ((start - 1, start + 1), (n - start - args.steps - 2, n - start))
+        f = drv.fork(window)
+        f.set_solver(0)
+        f.set_levels(args.levels)
+        forks["levels%d" % args.levels] = f
     ctx.reset_stats()
     ctx.set_profiling(True)
     torch.cuda.synchronize()
@@ -764,10 +776,30 @@ def main():
         wall_f = time.time() - t1
         ctx.set_profiling(False)
         sf = ctx.stats()
-        for r in rc:
+        for r in ([] if name.startswith("levels") else rc):
             print("# %s step sys_len=%d dim=%d H.psi fp64 %d fp32 %d%s E=%.10f res=%.2e %.3fs (H.psi %.3fs)"
                   % (name, r.sys_len, r.dim, r.hpsi_fp64, r.hpsi_fp32, " fallback" if r.fallback else "", r.energy,
                      r.residual, r.seconds, r.matvec_seconds), file=sys.stderr, flush=True)
+        if name.startswith("levels"):
+            E = f.level_energies()
+            for r, e in zip(rc, E):
+                print("# %s step sys_len=%d dim=%d H.psi %d E=%s %.3fs (H.psi %.3fs)" % (name, r.sys_len, r.dim,
+                      r.hpsi_fp64, " ".join("%.10f" % x for x in e), r.seconds, r.matvec_seconds), file=sys.stderr,
+                      flush=True)
+            forms[name] = {
+                "sweep_seconds": round(wall_f, 2) if args.sweep else None,
+                "ms_per_step": round(1e3 * wall_f / max(len(rc), 1), 2),
+                "cost_vs_plain": round(1e3 * wall_f / max(len(rc), 1) / max(ms_per_step, 1e-9), 3),
+                "hpsi_per_step": [r.hpsi_fp64 for r in rc],
+                "level_energies": E.tolist(),
+                "level1_minus_plain": [float(e[0] - q.energy) for e, q in zip(E, recs)],
+                "kept": [r.num_state for r in rc],
+                "trunc_err": [r.trunc_err for r in rc],
+                "matvec_share": round(sum(r.matvec_seconds for r in rc) / max(wall_f, 1e-30), 3),
+                "hpsi_tflops": round(sf["stage2_flops_alg"] / max(sf["stage2_ms"], 1e-30) / 1e9, 3),
+            }
+            del f
+            continue
         s2f = sf["stage2_flops_alg_fp32"] / max(sf["stage2_ms_fp32"], 1e-30) / 1e9
         forms[name] = {
             "sweep_seconds": round(wall_f, 2) if args.sweep else None,

@@ -190,7 +190,8 @@ struct Driver {
 
   // every level's solve (levels > 1): level 1 into p, level l into lpsi[l-1],
   // each projected off the levels below; the first level's outcome is the
-  // step's, all energies go to cur_e
+  // step's (energy, iterations, convergence, residual) with the H.psi count
+  // and time of all levels, all energies go to cur_e
   SolveOut solve_levels(hh_superblock* sb, double2* p, double lanzero) {
     need(S.solver == 0, "several levels are solved by the plain Lanczos only (solver 0)");
     const int64_t n = sb->plan.sb.dim;
@@ -208,7 +209,14 @@ struct Driver {
       opt.deflate = defl.c();
       opt.ndeflate = 2 * l;
       const SolveOut so = lanczos(ctx, sb, v, lanzero, S.max_iter, opt);
-      if (l == 0) first = so;
+      if (l == 0) {
+        first = so;
+      } else {
+        // the step's H.psi count and time cover every level
+        first.mv_seconds += so.mv_seconds;
+        first.hpsi_fp64 += so.hpsi_fp64;
+        first.hpsi_fp32 += so.hpsi_fp32;
+      }
       cur_e[l] = so.e0;
       if (l + 1 < nl) {
         // this level and i times it join the deflation set (v is unit)

@@ -471,7 +471,9 @@ int hh_driver_set_subspace(hh_driver* d, int kmax);
  * transformation, code/warm_up.f90:1616-1620), and truncates on the mixed
  * density matrix of levels 1 and 2 (Get_density_matrix_*_new,
  * code/get_den.f90:106-177; hh_density_matrix_levels).  The step record's
- * energy is level 1's; hh_driver_level_energies gives them all.  Several
+ * energy, iterations, convergence and residual are level 1's, its H.psi
+ * count and time those of all levels; hh_driver_level_energies gives every
+ * level's energy.  Several
  * levels need solver 0; a checkpoint holds level 1 only, so after a load the
  * first step seeds the others.  A superblock of dimension n solves at most n
  * levels and reports NaN for the rest. */

@@ -133,6 +133,8 @@ def test_driver_levels_sweep_equals_ed_singlet_levels(ctx):
     for r, (e1, e2) in zip(recs, E):
         print("sys_len=%d dim=%d E1=%.12f E2=%.12f (ED %.12f %.12f)" % (r.sys_len, r.dim, e1, e2, spec[0], spec[1]))
         assert r.energy == e1
+        # the record's H.psi count covers both levels, its iterations level 1's
+        assert r.hpsi_fp64 > r.iters
         assert abs(e1 - spec[0]) < 1e-9 * abs(spec[0])
         assert abs(e2 - spec[1]) < 1e-9 * abs(spec[1])
     f = drv.fork()
