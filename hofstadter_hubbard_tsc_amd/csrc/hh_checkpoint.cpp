@@ -26,7 +26,9 @@ namespace hh {
 // ----------------------------------------------------------------------------
 namespace {
 constexpr char CK_MAGIC[8] = {'H', 'H', 'D', 'M', 'R', 'G', 'C', 'K'};
-constexpr uint32_t CK_VERSION = 2;
+// version 3 adds the levels record (the levels setting and the vectors of the
+// upper levels); version 2 files (one level) are still read
+constexpr uint32_t CK_VERSION = 3;
 
 struct CkOut {
   FILE* f;
@@ -114,6 +116,13 @@ void save_driver(Driver& d, const char* path) {
   o.put<uint64_t>(d.step);
   o.put<int64_t>(d.psi_dim);
   o.device(ctx, d.psi.p, d.psi.p ? d.psi_dim : 0);
+  // levels record: the setting, then the upper levels' vectors when they
+  // belong to the carried superblock
+  o.put<int32_t>(d.levels);
+  const bool lv = d.lpsi_dim == d.psi_dim && d.psi_dim > 0 && (int)d.lpsi.size() == d.levels - 1;
+  o.put<int32_t>(lv ? (int32_t)d.lpsi.size() : 0);
+  if (lv)
+    for (const auto& v : d.lpsi) o.device(ctx, v->p, d.psi_dim);
   for (int side = 0; side < 2; ++side) {
     const auto& blocks = side == 0 ? d.sysb : d.envb;
     o.put<uint32_t>((uint32_t)blocks.size());
@@ -158,7 +167,8 @@ void load_driver(Driver& d, const char* path) {
   char magic[8];
   in.raw(magic, sizeof(magic));
   need(std::memcmp(magic, CK_MAGIC, sizeof(magic)) == 0, "not a checkpoint of this library");
-  need(in.get<uint32_t>() == CK_VERSION, "checkpoint version not supported");
+  const uint32_t version = in.get<uint32_t>();
+  need(version == 2 || version == CK_VERSION, "checkpoint version not supported");
   const ModelParams& m = ctx->model;
   const int kind = in.get<int32_t>(), Lx = in.get<int32_t>(), Ly = in.get<int32_t>(), lattice = in.get<int32_t>();
   const int periodic = in.get<int32_t>(), n_elec = in.get<int32_t>();
@@ -176,6 +186,19 @@ void load_driver(Driver& d, const char* path) {
   fresh.step = in.get<uint64_t>();
   fresh.psi_dim = in.get<int64_t>();
   in.device(ctx, fresh.psi, fresh.psi_dim > 0 ? fresh.psi_dim : 0);
+  if (version >= 3) {
+    fresh.levels = in.get<int32_t>();
+    need(fresh.levels >= 1 && fresh.levels <= MAX_LEVELS && 2 * (fresh.levels - 1) < fresh.S.max_iter &&
+             (fresh.levels == 1 || fresh.S.solver == 0),
+         "checkpoint file is corrupt (levels)");
+    const int nl = in.get<int32_t>();
+    need(nl == 0 || (nl == fresh.levels - 1 && fresh.psi_dim > 0), "checkpoint file is corrupt (level vectors)");
+    for (int l = 0; l < nl; ++l) {
+      fresh.lpsi.emplace_back(new DevMem());
+      in.device(ctx, *fresh.lpsi.back(), fresh.psi_dim);
+    }
+    fresh.lpsi_dim = nl > 0 ? fresh.psi_dim : 0;
+  }
   const auto st = site_sectors(m.kind);
   for (int side = 0; side < 2; ++side) {
     auto& blocks = side == 0 ? fresh.sysb : fresh.envb;
@@ -237,5 +260,8 @@ void load_driver(Driver& d, const char* path) {
   d.syst.swap(fresh.syst);
   d.envt.swap(fresh.envt);
   d.prev_sb.reset();   // the next step starts from the carried wave function
+  d.levels = fresh.levels;
+  d.lpsi.swap(fresh.lpsi);
+  d.lpsi_dim = fresh.lpsi_dim;
 }
 }  // namespace hh

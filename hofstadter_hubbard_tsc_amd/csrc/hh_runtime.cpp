@@ -1571,6 +1571,12 @@ int hh_driver_set_levels(hh_driver* h, int levels) {
   });
 }
 
+int hh_driver_levels(const hh_driver* h, int32_t* levels) {
+  if (!h || !levels) return 1;
+  *levels = h->d.levels;
+  return 0;
+}
+
 int hh_driver_level_energies(const hh_driver* h, int max_rec, double* e, int* nrec) {
   if (!h) return 1;
   return guard(h->d.ctx, [&] {

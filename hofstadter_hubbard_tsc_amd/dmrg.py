@@ -696,7 +696,6 @@ class Driver:
         d = Driver.__new__(Driver)
         d.ctx = self.ctx
         d.ptr = p
-        d._levels = self.levels
         return d
 
     def capture(self, max_samples):
@@ -730,7 +729,10 @@ class Driver:
 
     @property
     def levels(self):
-        return getattr(self, "_levels", 1)
+        """Levels every step targets (1 unless set or restored by load)."""
+        v = L.i32()
+        self.ctx._rc(L.load().hh_driver_levels(self.ptr, C.byref(v)), "hh_driver_levels")
+        return v.value
 
     def set_levels(self, levels):
         """Levels every later step targets (default 1, at most 8; plain
@@ -738,7 +740,6 @@ class Driver:
         the ones below, and truncates on the mixed density matrix of levels 1
         and 2.  Records carry level 1's energy; level_energies() gives all."""
         self.ctx._rc(L.load().hh_driver_set_levels(self.ptr, int(levels)), "hh_driver_set_levels")
-        self._levels = int(levels)
 
     def level_energies(self, max_rec=100000):
         """Level energies of the records the last warmup / sweep call

@@ -473,11 +473,13 @@ int hh_driver_set_subspace(hh_driver* d, int kmax);
  * code/get_den.f90:106-177; hh_density_matrix_levels).  The step record's
  * energy, iterations, convergence and residual are level 1's, its H.psi
  * count and time those of all levels; hh_driver_level_energies gives every
- * level's energy.  Several
- * levels need solver 0; a checkpoint holds level 1 only, so after a load the
- * first step seeds the others.  A superblock of dimension n solves at most n
- * levels and reports NaN for the rest. */
+ * level's energy.  Several levels need solver 0.  hh_driver_save keeps the
+ * setting and the upper levels' vectors (the reference writes one wave file
+ * per level), hh_driver_load restores both.  A superblock of dimension n
+ * solves at most n levels and reports NaN for the rest. */
 int hh_driver_set_levels(hh_driver* d, int levels);
+/* The levels setting of d (1 unless set, or restored by hh_driver_load). */
+int hh_driver_levels(const hh_driver* d, int32_t* levels);
 /* The level energies of the records the last warmup / sweep call returned,
  * levels per record (record-major, the levels set at that call) into e,
  * at most max_rec records; *nrec = that call's record count. (Save_result_to_disk

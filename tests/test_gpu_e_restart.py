@@ -55,6 +55,42 @@ def test_resumed_sweep_repeats_the_run(ctx, tmp_path):
         assert x.energy == y.energy and x.trunc_err == y.trunc_err
 
 
+def test_resumed_two_level_sweep_repeats_the_run(ctx, tmp_path):
+    """With two levels the checkpoint also holds the levels setting and level
+    2's vector (the reference writes one wave file per level): a driver of
+    one level that loads it targets two levels again, and its sweep repeats
+    the uninterrupted run's records and level energies bit for bit."""
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="hubbard", Lx=6, Ly=2, U=8.0, t=1.0, flux=0.1, n_elec=10)
+    n = m.nsite
+    ctx.set_model(m)
+    a = Driver(ctx, kept_max=24, tol=1e-9, kept_max_warm=24, seed=3)
+    a.set_levels(2)
+    a.warmup()
+    a.sweep(0, n // 2 - 1, n - 2, 1e-9, TRUN, 24, fresh=True, phase=1)
+    a.reset_transform()
+    a.sweep(1, n - 2, 1, 1e-9, TRUN, 24, phase=1)
+    a.reset_transform()
+    path = str(tmp_path / "levels.ck")
+    a.save(path)
+    ra = a.sweep(0, 1, n - 2, 1e-9, TRUN, 24, phase=2)
+    ea = a.level_energies()
+    b = Driver(ctx, kept_max=24, tol=1e-9, kept_max_warm=24, seed=99)
+    assert b.levels == 1
+    b.load(path)
+    assert b.levels == 2
+    rb = b.sweep(0, 1, n - 2, 1e-9, TRUN, 24, phase=2)
+    eb = b.level_energies()
+    assert len(ra) == len(rb) > 0 and ea.shape == eb.shape == (len(ra), 2)
+    assert any(r.num_state < r.dim for r in ra), "the case must truncate"
+    for x, y in zip(ra, rb):
+        assert (x.sys_len, x.iters, x.num_state, x.dim, x.hpsi_fp64) == (y.sys_len, y.iters, y.num_state, y.dim,
+                                                                          y.hpsi_fp64)
+        assert x.energy == y.energy and x.trunc_err == y.trunc_err
+    assert np.array_equal(ea, eb)
+    assert np.all(ea[:, 1] > ea[:, 0])
+
+
 def test_checkpoint_of_another_model_or_damaged_is_refused(ctx, tmp_path):
     from hofstadter_hubbard_tsc_amd import Driver
     from hofstadter_hubbard_tsc_amd.dmrg import HHError
