@@ -598,6 +598,9 @@ def main():
                     help="c2: also time the same steps targeting this many levels (plain Lanczos per level, "
                          "truncation on the mixed density matrix of levels 1 and 2), reported under "
                          "config.forms.levelsL; 1 = off")
+    ap.add_argument("--level-carry", action="store_true",
+                    help="with --levels: start levels 2.. from the wave-transformed previous levels instead of "
+                         "seeded vectors (the reference's)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (out = in * 1.0, float64) five times, for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -697,7 +700,8 @@ def main():
         f = drv.fork(window)
         f.set_solver(0)
         f.set_levels(args.levels)
-        forks["levels%d" % args.levels] = f
+        f.set_level_start(args.level_carry)
+        forks["levels%d%s" % (args.levels, "_carried" if args.level_carry else "")] = f
     ctx.reset_stats()
     ctx.set_profiling(True)
     torch.cuda.synchronize()

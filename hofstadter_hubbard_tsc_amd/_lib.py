@@ -122,6 +122,7 @@ SIGNATURES = [
     ("hh_driver_set_subspace", i32, [vp, i32]),
     ("hh_driver_set_levels", i32, [vp, i32]),
     ("hh_driver_levels", i32, [vp, P(i32)]),
+    ("hh_driver_set_level_start", i32, [vp, i32]),
     ("hh_driver_level_energies", i32, [vp, i32, P(f64), P(i32)]),
     ("hh_driver_set_fallback", i32, [vp, f64]),
     ("hh_precond_eval", i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), P(f64), P(f64), P(f64), f64,

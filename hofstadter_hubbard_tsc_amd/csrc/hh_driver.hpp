@@ -61,6 +61,7 @@ struct Driver {
   // carried vectors of the levels above the first and their transform
   // scratch, the current step's level energies and every record's
   int levels = 1;
+  bool level_carry = false;
   std::vector<std::unique_ptr<DevMem>> lpsi;
   int64_t lpsi_dim = 0;
   int solved_levels = 1;
@@ -177,9 +178,22 @@ struct Driver {
   // the start vectors of the levels above the first: seeded afresh at every
   // step (wave_initialize of levels 2.. after the wave transformation,
   // code/warm_up.f90:1616-1620), or the vectors already held when the step
-  // carries the state of the same superblock
-  void level_starts(int64_t n, bool carry) {
+  // carries the state of the same superblock.  With level_carry the previous
+  // step's upper levels go through the same wave transformation as level 1
+  // (dir 0 LR / 1 RL with its truncations ta, tb) instead of being seeded.
+  void level_starts(int64_t n, bool carry, hh_superblock* sb = nullptr, int dir = 0, const hh_trunc* ta = nullptr,
+                    const hh_trunc* tb = nullptr) {
     const bool held = carry && lpsi_dim == n && (int)lpsi.size() == levels - 1;
+    if (level_carry && sb && ta && prev_sb && lpsi_dim == prev_sb->plan.sb.dim && (int)lpsi.size() == levels - 1) {
+      // psi2 is free here: it holds the previous step's level 1
+      for (auto& v : lpsi) {
+        wave_trans(ctx, dir, prev_sb.get(), v->c(), ta, tb, sb, psi_buf(n, psi2));
+        std::swap(v->p, psi2.p);
+        std::swap(v->bytes, psi2.bytes);
+      }
+      lpsi_dim = n;
+      return;
+    }
     lpsi.resize(levels - 1);
     if (held) return;
     for (int l = 1; l < levels; ++l) {
@@ -361,7 +375,7 @@ struct Driver {
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
         p = psi.c();
-        if (levels > 1) level_starts(n, false);
+        if (levels > 1) level_starts(n, false, sb.get(), 0, ta, tb);
       }
       ++step;
       lap(1);
@@ -415,7 +429,7 @@ struct Driver {
         std::swap(psi.p, psi2.p);
         std::swap(psi.bytes, psi2.bytes);
         p = psi.c();
-        if (levels > 1) level_starts(n, false);
+        if (levels > 1) level_starts(n, false, sb.get(), 1, ta, tb);
       }
       ++step;
       lap(1);

@@ -1571,6 +1571,14 @@ int hh_driver_set_levels(hh_driver* h, int levels) {
   });
 }
 
+int hh_driver_set_level_start(hh_driver* h, int carry) {
+  if (!h) return 1;
+  return guard(h->d.ctx, [&] {
+    need(carry == 0 || carry == 1, "carry must be 0 (seeded, the reference's) or 1 (wave-transformed)");
+    h->d.level_carry = carry != 0;
+  });
+}
+
 int hh_driver_levels(const hh_driver* h, int32_t* levels) {
   if (!h || !levels) return 1;
   *levels = h->d.levels;
@@ -1661,6 +1669,7 @@ int fork_driver(const hh_driver* src, const int lo[2], const int hi[2], hh_drive
     d.min_dim = s.min_dim;
     d.dav_kmax = s.dav_kmax;
     d.levels = s.levels;
+    d.level_carry = s.level_carry;
     for (const auto& v : s.lpsi) {
       d.lpsi.emplace_back(new DevMem());
       copy_dev(*d.lpsi.back(), *v);

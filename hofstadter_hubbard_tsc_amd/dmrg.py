@@ -741,6 +741,11 @@ class Driver:
         and 2.  Records carry level 1's energy; level_energies() gives all."""
         self.ctx._rc(L.load().hh_driver_set_levels(self.ptr, int(levels)), "hh_driver_set_levels")
 
+    def set_level_start(self, carry):
+        """Start vectors of levels 2.. after a wave transformation: False
+        seeded (the reference's), True the transformed previous levels."""
+        self.ctx._rc(L.load().hh_driver_set_level_start(self.ptr, 1 if carry else 0), "hh_driver_set_level_start")
+
     def level_energies(self, max_rec=100000):
         """Level energies of the records the last warmup / sweep call
         returned, shape (records, levels)."""

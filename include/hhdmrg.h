@@ -478,6 +478,13 @@ int hh_driver_set_subspace(hh_driver* d, int kmax);
  * per level), hh_driver_load restores both.  A superblock of dimension n
  * solves at most n levels and reports NaN for the rest. */
 int hh_driver_set_levels(hh_driver* d, int levels);
+/* Start vectors of levels 2.. at a step with a wave transformation: 0 seeded
+ * vectors (default, the reference's: it transforms every level,
+ * code/warm_up.f90:1606-1611, then re-initialises levels 2.. with
+ * wave_initialize, :1616-1620), 1 the transformed vectors themselves.  The
+ * converged levels are the same; carried starts need fewer H.psi.  No
+ * counterpart as a setting in the reference; not kept in a checkpoint. */
+int hh_driver_set_level_start(hh_driver* d, int carry);
 /* The levels setting of d (1 unless set, or restored by hh_driver_load). */
 int hh_driver_levels(const hh_driver* d, int32_t* levels);
 /* The level energies of the records the last warmup / sweep call returned,

@@ -150,6 +150,36 @@ def test_driver_levels_sweep_equals_ed_singlet_levels(ctx):
         drv.set_levels(2)
 
 
+def test_driver_carried_level_starts_give_the_same_levels_with_fewer_hpsi(ctx):
+    """Levels 2.. started from the wave-transformed previous levels
+    (set_level_start(True)) instead of seeded vectors: on the untruncated
+    Hubbard 4 x 2 schedule every step of the full sweep still reports the two
+    lowest ED singlet levels (1e-9), with fewer H.psi over the sweep."""
+    from hofstadter_hubbard_tsc_amd import Driver
+    m = lat.Model(kind="hubbard", Lx=4, Ly=2, U=8.0, t=1.0, flux=0.15, n_elec=6)
+    _, _, sg = _superblock(ctx, m, 2000, 3)
+    bonds = [(u, v) + sg.coupling(u, v) for (u, v) in sg.bonds]
+    spec = ed.singlet_spectrum(m, list(range(1, m.nsite + 1)), bonds, m.n_elec)
+    n = m.nsite
+    runs = {}
+    for carry in (False, True):
+        drv = Driver(ctx, kept_max=2000, tol=1e-12, kept_max_warm=2000, seed=11)
+        drv.set_levels(2)
+        drv.set_level_start(carry)
+        drv.warmup()
+        drv.sweep(0, n // 2 - 1, n - 2, 1e-12, driver.TRUN_ERR, 2000, fresh=True, phase=1)
+        drv.reset_transform()
+        drv.sweep(1, n - 2, 1, 1e-12, driver.TRUN_ERR, 2000, phase=1)
+        drv.reset_transform()
+        recs = drv.sweep(0, 1, n - 2, 1e-12, driver.TRUN_ERR, 2000, phase=2)
+        E = drv.level_energies()
+        for e1, e2 in E:
+            assert abs(e1 - spec[0]) < 1e-9 * abs(spec[0]) and abs(e2 - spec[1]) < 1e-9 * abs(spec[1])
+        runs[carry] = sum(r.hpsi_fp64 for r in recs)
+    print("H.psi over the sweep: seeded %d, carried %d" % (runs[False], runs[True]))
+    assert runs[True] < runs[False]
+
+
 def test_driver_levels_truncate_on_the_mixed_density_matrix(ctx):
     """A truncated t-J 4 x 4 cylinder sweep (flux 0.125, m = 40) with levels
     = 2: each step's kept count, truncation error and entropy are those of the
