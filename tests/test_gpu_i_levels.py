@@ -3,7 +3,9 @@ lanczos_memo_new with several levels (code/lanczos.f90:29-44,
 Gram_Schmidt_Lanczos code/space_info.f90:1222-1279) and the mixed density
 matrix of two levels (Get_density_matrix_sys_new / _env_new,
 code/get_den.f90:103-177) with its truncation, against the oracle on
-identical blocks and against exact diagonalisation."""
+identical blocks and against exact diagonalisation; and the sweep driver
+targeting two levels in every step (Get_Ground_State with levels,
+code/quantum.f90:179-288), with seeded or carried starts of level 2."""
 import numpy as np
 import pytest
 
